@@ -1,0 +1,174 @@
+"""Benchmark of the HMM classify path on one MI355X.
+
+Workload (one step): classify 100,000 simulated reads (3 channels, 12
+timesteps) against 1,000 dye-seqs (lengths 6-30, 0-5 dyes per channel) with a
+5-sigma pruning cutoff -- 10^8 read x dye-seq pairs -- on the FP32 rescaled
+path, inputs resident in HBM. Prints one JSON line.
+
+Also measured (untimed for `value`): top-1 agreement with the FP64 GPU path on
+the same reads, per-kernel HIP-event durations, algorithmic FLOPs of the
+pruned work, and the compiled reference's multithreaded CPU classify on a
+bounded sample of the same reads.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(model, strings, rads, cutoff, sample_reads):
+    """The reference's own HMMClassifier (oracle/_ref, compiled from its
+    sources) on a bounded sample of the same reads, OpenMP over reads; falls
+    back to the oracle's C restatement when the reference library is absent."""
+    import numpy as np
+    from tests._oracle import Oracle, Reference
+    o = Oracle()
+    threads = o.L.wpo_num_threads()
+    sample = np.ascontiguousarray(rads[:sample_reads])
+    counts = np.ones(len(strings), dtype=np.int32)
+    ids = np.arange(len(strings), dtype=np.int32)
+    ref = Reference.load()
+    kind = "reference" if ref is not None else "port"
+    impl = ref if ref is not None else o
+    t0 = time.perf_counter()
+    impl.classify(model, strings, counts, ids, sample, cutoff)
+    dt = time.perf_counter() - t0
+    pairs = sample.shape[0] * len(strings)
+    return {"value": pairs / dt, "unit": "pairs/s", "cores": int(threads), "kind": kind,
+            "sample": f"{sample.shape[0]} reads x {len(strings)} dye-seqs of the same workload, "
+                      f"{dt:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--reads", type=int, default=100000)
+    ap.add_argument("--dye-seqs", type=int, default=1000)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--cpu-sample-reads", type=int, default=3000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(reads shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import whatprot_amd as w
+
+    T, C, S, N, cutoff = 12, 3, args.dye_seqs, args.reads, 5.0
+    model = w.SequencingModel.uniform(C)  # seqparams_atto647n_x3 priors with mu = 1
+    log(f"generating {S} dye-seqs and {N} reads")
+    strings = w.generate_dye_seqs(C, S, 6, 30, 0, 5, 1001)
+    rads, true_idx = w.simulate_radiometries(model, T, strings, N, 2001)
+    entries = [w.DyeSeqEntry(s, 1, i) for i, s in enumerate(strings)]
+    prec = w.WP_PRECISION_FP32 if args.precision == "fp32" else w.WP_PRECISION_FP64
+    clf = w.HMMClassifier(T, C, model, w.SequencingSettings(cutoff), entries, prec)
+
+    dev = torch.device("cuda:0")
+    d_rads = torch.from_numpy(rads).to(dev)
+    d_id = torch.empty(N, dtype=torch.int32, device=dev)
+    d_adj = torch.empty(N, dtype=torch.float64, device=dev)
+    d_ls = torch.empty(N, dtype=torch.float64, device=dev)
+    d_lt = torch.empty(N, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        clf.classify_device(N, d_rads.data_ptr(), d_id.data_ptr(), d_adj.data_ptr(), d_ls.data_ptr(),
+                            d_lt.data_ptr(), stream.cuda_stream)
+
+    log("warmup")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    # timed region: K steps bracketed by synchronize
+    log(f"timing {args.steps} steps")
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = ev0.elapsed_time(ev1) / args.steps
+    ids = d_id.cpu().numpy()
+
+    # per-kernel durations over the same steps, HIP events on the same stream
+    clf.set_timing(True)
+    prep_ms, cls_ms = [], []
+    for _ in range(args.steps):
+        step()
+        a, b = clf.last_timing()
+        prep_ms.append(a)
+        cls_ms.append(b)
+    clf.set_timing(False)
+    torch.cuda.synchronize()
+    classify_ms = float(np.mean(cls_ms))
+    log(f"ms/step {ms_per_step:.3f}  prep {np.mean(prep_ms):.3f} ms  classify {classify_ms:.3f} ms")
+
+    pairs_nonempty, states, flops = clf.work(rads)
+    achieved = flops / (classify_ms * 1e-3) / 1e12
+
+    result = {
+        "metric": "HMM read x dye-seq pairs/s on 1 MI355X (FP32 rescaled, 5-sigma pruning)",
+        "value": N * S / (ms_per_step * 1e-3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if prec == w.WP_PRECISION_FP32 else "fp64",
+        "data": "synthetic (seeded simulator, seqparams_atto647n_x3 priors, mu = 1)",
+        "config": {"workload": "classify hmm: 100k reads x 1k dye-seqs, C=3, T=12, -p 5",
+                   "reads": N, "dye_seqs": S, "channels": C, "timesteps": T, "dist_cutoff": cutoff,
+                   "precision": args.precision},
+        "reads_per_s": N / (ms_per_step * 1e-3),
+        "wall_s": wall,
+        "nonempty_pairs": pairs_nonempty,
+        "state_timesteps": states,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "classify_kernel (all bins)", "kernel_ms": classify_ms,
+                     "flops_per_launch": flops},
+    }
+    if not args.no_fp64_check and prec == w.WP_PRECISION_FP32:
+        log("FP64 reference run for top-1 agreement")
+        clf64 = w.HMMClassifier(T, C, model, w.SequencingSettings(cutoff), entries, w.WP_PRECISION_FP64)
+        d_id64 = torch.empty(N, dtype=torch.int32, device=dev)
+        t64 = time.perf_counter()
+        clf64.classify_device(N, d_rads.data_ptr(), d_id64.data_ptr(), 0, 0, 0, stream.cuda_stream)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t64
+        ids64 = d_id64.cpu().numpy()
+        result["top1_vs_fp64"] = float((ids == ids64).mean())
+        result["fp64_gpu_pairs_per_s"] = N * S / t64
+        result["top1_vs_truth"] = float((ids == true_idx).mean())
+    if not args.no_cpu_baseline:
+        log("CPU baseline")
+        result["cpu_baseline"] = cpu_baseline(model, strings, rads, cutoff, args.cpu_sample_reads)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

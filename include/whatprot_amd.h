@@ -1,0 +1,222 @@
+/*
+ * whatprot_amd — C ABI of the MI355X (gfx950) implementation of whatprot's
+ * tensor-state HMM scoring path.
+ *
+ * Plain C, caller-owned host buffers unless a name ends in _device, no
+ * exceptions. Every entry point returns WP_OK (0) or a negative status; the
+ * text of the last error on the calling thread is wp_last_error().
+ *
+ * Reference interfaces replaced (paths relative to whatprot's cc_code/src):
+ *   SequencingModel / ChannelModel      parameterization/model/sequencing-model.h:22-49,
+ *                                       parameterization/model/channel-model.h:20-83
+ *   HMMClassifier(T, C, model, settings, dye_seqs)
+ *                                       classifiers/hmm-classifier.h:32-38
+ *   HMMClassifier::classify(vector<Radiometry>)
+ *                                       classifiers/hmm-classifier.h:42-43, hmm-classifier.cc:84-93
+ *   HMMClassifier::classify(Radiometry, vector<int> candidate_indices)
+ *                                       classifiers/hmm-classifier.h:40-41, hmm-classifier.cc:79-82
+ *   PeptideHMM(...).probability()       hmm/hmm/peptide-hmm.h:29-36, peptide-hmm.cc:94-100
+ */
+#ifndef WHATPROT_AMD_H
+#define WHATPROT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WP_MAX_CHANNELS 8
+#define WP_MAX_TIMESTEPS 64
+#define WP_MAX_DYES 15
+
+enum {
+    WP_OK = 0,
+    WP_ERR_ARG = -1,         /* invalid argument or shape */
+    WP_ERR_HIP = -2,         /* HIP runtime error */
+    WP_ERR_UNSUPPORTED = -3, /* shape beyond the limits above */
+    WP_ERR_NOMEM = -4,       /* allocation failed */
+    WP_ERR_NO_DEVICE = -5    /* no gfx950 device visible */
+};
+
+/* Arithmetic of the forward recursion.
+ *   WP_PRECISION_FP64: the reference's method, FP64 end to end, no rescaling.
+ *   WP_PRECISION_FP32: FP32 recursion with exact power-of-two rescaling after
+ *                      every timestep into an integer log2 accumulator; the
+ *                      per-pair log-likelihood, the normaliser over dye-seqs
+ *                      and the argmax are FP64. */
+enum { WP_PRECISION_FP64 = 0, WP_PRECISION_FP32 = 1 };
+
+/* One channel of the sequencing model; mirrors ChannelModel
+ * (parameterization/model/channel-model.h:76-83). interactions[c] and
+ * flat_interactions[c] hold the stored values, i.e. 1 - effect and
+ * 1 - flat_effect as read from the JSON (sequencing-model.cc:75-88); 1.0 means
+ * no interaction. */
+typedef struct wp_channel_model {
+    double p_bleach;
+    double p_dud;
+    double bg_sig;
+    double mu;
+    double sig;
+    double interactions[WP_MAX_CHANNELS];
+    double flat_interactions[WP_MAX_CHANNELS];
+} wp_channel_model;
+
+/* Mirrors SequencingModel (parameterization/model/sequencing-model.h:22-49)
+ * with DecayingRateModel p_detach (decaying-rate-model.h): the detach rate at
+ * step i is p_detach + p_initial_detach * exp(-i * p_initial_detach_decay). */
+typedef struct wp_model {
+    int num_channels;
+    double p_edman_failure;
+    double p_detach;
+    double p_initial_detach;
+    double p_initial_detach_decay;
+    double p_initial_block;
+    double p_cyclic_block;
+    wp_channel_model channels[WP_MAX_CHANNELS];
+} wp_model;
+
+/* Text of the last error on this thread ("" if none). */
+const char* wp_last_error(void);
+
+/* Mirrors SequencingModel::with_mu_as_one (sequencing-model.cc:140-151,
+ * channel-model.cc:35-45): bg_sig /= mu, sig /= mu, mu = 1. Host only. */
+void wp_model_with_mu_as_one(const wp_model* in, wp_model* out);
+
+/* Number of visible gfx950 devices (0 if none). */
+int wp_device_count(void);
+
+/* ---------------------------------------------------------------------------
+ * HMM classifier. Replaces HMMClassifier (classifiers/hmm-classifier.h:30-78).
+ *
+ * num_timesteps, num_channels: T and C of the reads.
+ * seq_model: the model with mu already set to one (run-classify-hmm.cc:60),
+ *            radiometries already divided by mu (radiometries-io.cc:50-51).
+ * dist_cutoff: SequencingSettings::dist_cutoff (the -p sigma cutoff);
+ *              DBL_MAX means no pruning (peptide-emission.cc:49-53).
+ * dye_strings[i]: the dye-seq text, e.g. "..0.1..0" (dye-seq.cc:23-40).
+ * dye_seq_counts[i], dye_seq_ids[i]: SourceCount<int>{id, count}
+ *              (dye-seqs-io.cc:96-103).
+ * precision: WP_PRECISION_FP64 or WP_PRECISION_FP32.
+ * The classifier copies everything it needs; the arrays may be freed after.
+ * One classifier is used by one host thread at a time.
+ * ------------------------------------------------------------------------- */
+typedef struct wp_hmm_classifier wp_hmm_classifier;
+
+int wp_hmm_classifier_create(unsigned int num_timesteps,
+                             unsigned int num_channels,
+                             const wp_model* seq_model,
+                             double dist_cutoff,
+                             int num_dye_seqs,
+                             const char* const* dye_strings,
+                             const int* dye_seq_counts,
+                             const int* dye_seq_ids,
+                             int precision,
+                             wp_hmm_classifier** out);
+
+void wp_hmm_classifier_destroy(wp_hmm_classifier* clf);
+
+/* max_num_dyes over the classifier's dye-seqs (hmm-classifier.cc:47-65). */
+int wp_hmm_classifier_max_num_dyes(const wp_hmm_classifier* clf);
+
+/* Classifies num_reads radiometries laid out [read][t][c] (t*C + c,
+ * radiometry.cc:39-44) against every dye-seq. Replaces
+ * HMMClassifier::classify(const vector<Radiometry>&)
+ * (hmm-classifier.cc:84-93, classify_helper hmm-classifier.h:45-78).
+ * Outputs per read (any pointer but out_id may be NULL):
+ *   out_id[r]        dye_seq_ids of the best dye-seq (strict >, first wins)
+ *   out_adjusted[r]  best / total (the value written to the results CSV by
+ *                    scored-classifications-io.cc:41-46), 0 when NaN
+ *                    (hmm-classifier.h:73-76)
+ *   out_log_score[r] ln P(best)   (-inf when P = 0)
+ *   out_log_total[r] ln sum_i P_i * count_i  (-inf when 0) */
+int wp_hmm_classify(wp_hmm_classifier* clf,
+                    int num_reads,
+                    const double* radiometries,
+                    int* out_id,
+                    double* out_adjusted,
+                    double* out_log_score,
+                    double* out_log_total);
+
+/* As wp_hmm_classify, restricted per read to candidate dye-seq indices
+ * cand_indices[cand_offsets[r] .. cand_offsets[r+1]) in that order (ties go
+ * to the earlier candidate). Replaces HMMClassifier::classify(rad,
+ * candidate_indices) as called by HybridClassifier (hybrid-classifier.cc:67). */
+int wp_hmm_classify_candidates(wp_hmm_classifier* clf,
+                               int num_reads,
+                               const double* radiometries,
+                               const int* cand_offsets,
+                               const int* cand_indices,
+                               int* out_id,
+                               double* out_adjusted,
+                               double* out_log_score,
+                               double* out_log_total);
+
+/* Device-resident form of wp_hmm_classify: every pointer is device memory,
+ * work is queued on `stream` (a hipStream_t, NULL = default stream) and the
+ * call returns without waiting. */
+int wp_hmm_classify_device(wp_hmm_classifier* clf,
+                           int num_reads,
+                           const double* d_radiometries,
+                           int* d_out_id,
+                           double* d_out_adjusted,
+                           double* d_out_log_score,
+                           double* d_out_log_total,
+                           void* stream);
+
+/* ln PeptideHMM::probability() for every (read, dye-seq) pair, row-major
+ * [read][dye_seq] (peptide-hmm.cc:94-100); -inf where P = 0. Validation and
+ * accuracy reporting. */
+int wp_hmm_pair_log_probability(wp_hmm_classifier* clf,
+                                int num_reads,
+                                const double* radiometries,
+                                double* out_log_p);
+
+/* Kernel timing (no reference counterpart; the reference prints wall times,
+ * cmd-line-out.cc:30-114). When enabled, the classifier records HIP events on
+ * the call's stream around its range/bounds kernels and its classify kernels;
+ * wp_hmm_last_timing waits for the last call and returns both in ms. */
+int wp_hmm_set_timing(wp_hmm_classifier* clf, int enable);
+int wp_hmm_last_timing(wp_hmm_classifier* clf, double* prep_ms, double* classify_ms);
+
+/* Algorithmic work of classifying these reads against every dye-seq (no
+ * reference counterpart; used for roofline accounting): out3[0] = non-empty
+ * pairs, out3[1] = pruned state-timesteps sum_t (t+1) prod_c (H-L),
+ * out3[2] = FLOPs by the per-state counts emission 2, block 3, detach 4,
+ * Edman 8, binomial 2(D_c+2) per channel. Host buffers; synchronous. */
+int wp_hmm_work(wp_hmm_classifier* clf, int num_reads, const double* radiometries,
+                double* out3);
+
+/* ---------------------------------------------------------------------------
+ * Synthetic inputs (host). Seeded and portable; the streams differ from the
+ * reference's clock-seeded std::default_random_engine, the distributions
+ * follow its simulator.
+ * ------------------------------------------------------------------------- */
+
+/* Random dye-seq strings, num_dye_seqs rows of `stride` bytes (NUL padded):
+ * length ~ U{len_min..len_max}, per-channel dye counts ~ U{dyes_min..dyes_max}
+ * capped by the length and redrawn while all are zero, positions uniform
+ * without replacement. No reference counterpart (the reference reads dye-seqs
+ * made by its Python tools, README.md:310-325). */
+int wp_generate_dye_seqs(int num_channels, int num_dye_seqs, int len_min, int len_max,
+                         int dyes_min, int dyes_max, uint64_t seed, char* out_strings,
+                         int stride);
+
+/* Replaces generate_radiometries (simulation/generate-radiometries.cc:30-64,
+ * generate-radiometry.cc:28-62, generate-dye-track.cc:29-110): draws a dye-seq
+ * by weight (NULL = uniform), simulates its dye track, drops invisible tracks
+ * (all zero at t = 0) and emits Normal(adjusted_mu, sigma) intensities rounded
+ * to float32, until num_reads reads exist. out_radiometries is [read][t][c] in
+ * the model's units (divide by mu for a model with mu = 1). */
+int wp_simulate_radiometries(const wp_model* seq_model, unsigned int num_timesteps,
+                             unsigned int num_channels, int num_dye_seqs,
+                             const char* const* dye_strings, const int* weights,
+                             int num_reads, uint64_t seed, double* out_radiometries,
+                             int* out_dye_seq_index);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WHATPROT_AMD_H */

@@ -1,0 +1,204 @@
+"""ctypes face of the oracle (oracle/liboracle.so) and of the compiled
+reference (oracle/_ref/libwpref.so) for the tests. Test infrastructure only."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+from whatprot_amd import SequencingModel
+from whatprot_amd._capi import ModelC
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libwpref.so")
+DBL_MAX = sys.float_info.max
+
+P = ctypes.POINTER
+c_double, c_int, c_uint, c_void_p = ctypes.c_double, ctypes.c_int, ctypes.c_uint, ctypes.c_void_p
+
+
+def _ptr(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+def pack_dye_seqs(strings):
+    cap = max([len(s) for s in strings] + [1])
+    seqs = np.full((len(strings), cap), -1, dtype=np.int16)
+    lens = np.zeros(len(strings), dtype=np.int32)
+    for i, s in enumerate(strings):
+        t = s.rstrip(".")
+        lens[i] = len(t)
+        for k, ch in enumerate(t):
+            seqs[i, k] = int(ch) if ch.isdigit() else -1
+    return np.ascontiguousarray(seqs), lens, cap
+
+
+class Oracle:
+    def __init__(self):
+        if not os.path.exists(ORACLE_SO):
+            import subprocess
+            subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
+                           stdout=subprocess.DEVNULL)
+        L = ctypes.CDLL(ORACLE_SO)
+        L.wpo_adjusted_mu.restype = c_double
+        L.wpo_adjusted_mu.argtypes = [P(ModelC), c_int, c_void_p]
+        L.wpo_sigma.restype = c_double
+        L.wpo_sigma.argtypes = [P(ModelC), c_int, c_double]
+        L.wpo_pdf.restype = c_double
+        L.wpo_pdf.argtypes = [P(ModelC), c_int, c_double, c_void_p]
+        L.wpo_p_detach.restype = c_double
+        L.wpo_p_detach.argtypes = [P(ModelC), c_uint]
+        L.wpo_binomial_table.restype = None
+        L.wpo_binomial_table.argtypes = [c_double, c_uint, c_void_p]
+        L.wpo_emission_range.restype = None
+        L.wpo_emission_range.argtypes = [P(ModelC), c_void_p, c_uint, c_uint, c_uint, c_double, c_void_p, c_void_p]
+        L.wpo_emission_boxes.restype = c_int
+        L.wpo_emission_boxes.argtypes = [P(ModelC), c_uint, c_uint, c_void_p, c_int, c_void_p, c_uint,
+                                         c_double, c_void_p, c_void_p]
+        L.wpo_pair_probability.restype = c_double
+        L.wpo_pair_probability.argtypes = [P(ModelC), c_uint, c_uint, c_void_p, c_int, c_void_p, c_uint, c_double]
+        L.wpo_classify.restype = None
+        L.wpo_classify.argtypes = [P(ModelC), c_uint, c_uint, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                                   c_void_p, c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p]
+        L.wpo_pair_probabilities.restype = None
+        L.wpo_pair_probabilities.argtypes = [P(ModelC), c_uint, c_uint, c_int, c_void_p, c_int, c_void_p,
+                                             c_int, c_void_p, c_double, c_void_p]
+        L.wpo_brute_force_probability.restype = c_double
+        L.wpo_brute_force_probability.argtypes = [P(ModelC), c_uint, c_uint, c_void_p, c_int, c_void_p]
+        L.wpo_num_threads.restype = c_int
+        self.L = L
+
+    @staticmethod
+    def _m(model):
+        return model.to_c() if isinstance(model, SequencingModel) else model
+
+    def pdf(self, model, channel, observed, counts):
+        m = self._m(model)
+        cnt = np.ascontiguousarray(counts, dtype=np.uint32)
+        return self.L.wpo_pdf(ctypes.byref(m), channel, observed, _ptr(cnt))
+
+    def binomial_table(self, q, max_n):
+        out = np.zeros((max_n + 1) * (max_n + 2) // 2)
+        self.L.wpo_binomial_table(q, max_n, _ptr(out))
+        return out
+
+    def emission_range(self, model, rad, t, M, cutoff):
+        m = self._m(model)
+        rad = np.ascontiguousarray(rad, dtype=np.float64)
+        C = rad.shape[-1]
+        lo = np.zeros(C, dtype=np.uint32)
+        hi = np.zeros(C, dtype=np.uint32)
+        self.L.wpo_emission_range(ctypes.byref(m), _ptr(rad), C, t, M, cutoff, _ptr(lo), _ptr(hi))
+        return lo, hi
+
+    def emission_boxes(self, model, dye_string, rad, M, cutoff):
+        m = self._m(model)
+        rad = np.ascontiguousarray(rad, dtype=np.float64)
+        T, C = rad.shape
+        seqs, lens, _ = pack_dye_seqs([dye_string])
+        lo = np.zeros(T * C, dtype=np.uint32)
+        hi = np.zeros(T * C, dtype=np.uint32)
+        ok = self.L.wpo_emission_boxes(ctypes.byref(m), T, C, _ptr(seqs), int(lens[0]), _ptr(rad), M, cutoff,
+                                       _ptr(lo), _ptr(hi))
+        return bool(ok), lo.reshape(T, C), hi.reshape(T, C)
+
+    def pair_probability(self, model, dye_string, rad, M, cutoff=DBL_MAX):
+        m = self._m(model)
+        rad = np.ascontiguousarray(rad, dtype=np.float64)
+        T, C = rad.shape
+        seqs, lens, _ = pack_dye_seqs([dye_string])
+        return self.L.wpo_pair_probability(ctypes.byref(m), T, C, _ptr(seqs), int(lens[0]), _ptr(rad), M, cutoff)
+
+    def brute_force(self, model, dye_string, rad):
+        m = self._m(model)
+        rad = np.ascontiguousarray(rad, dtype=np.float64)
+        T, C = rad.shape
+        seqs, lens, _ = pack_dye_seqs([dye_string])
+        return self.L.wpo_brute_force_probability(ctypes.byref(m), T, C, _ptr(seqs), int(lens[0]), _ptr(rad))
+
+    def pair_probabilities(self, model, dye_strings, rads, cutoff=DBL_MAX):
+        m = self._m(model)
+        rads = np.ascontiguousarray(rads, dtype=np.float64)
+        N, T, C = rads.shape
+        seqs, lens, cap = pack_dye_seqs(dye_strings)
+        out = np.zeros((N, len(dye_strings)))
+        self.L.wpo_pair_probabilities(ctypes.byref(m), T, C, len(dye_strings), _ptr(seqs), cap, _ptr(lens), N,
+                                      _ptr(rads), cutoff, _ptr(out))
+        return out
+
+    def classify(self, model, dye_strings, counts, ids, rads, cutoff=DBL_MAX, candidates=None):
+        m = self._m(model)
+        rads = np.ascontiguousarray(rads, dtype=np.float64)
+        N, T, C = rads.shape
+        seqs, lens, cap = pack_dye_seqs(dye_strings)
+        counts = np.ascontiguousarray(counts, dtype=np.int32)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        off = idx = None
+        if candidates is not None:
+            off = np.zeros(N + 1, dtype=np.int32)
+            off[1:] = np.cumsum([len(c) for c in candidates])
+            idx = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.int32) for c in candidates])
+                                       if off[-1] else np.zeros(1, np.int32), dtype=np.int32)
+        oid = np.zeros(N, dtype=np.int32)
+        osc = np.zeros(N)
+        otot = np.zeros(N)
+        self.L.wpo_classify(ctypes.byref(m), T, C, len(dye_strings), _ptr(seqs), cap, _ptr(lens), _ptr(counts),
+                            _ptr(ids), N, _ptr(rads), cutoff, _ptr(off) if off is not None else None,
+                            _ptr(idx) if idx is not None else None, _ptr(oid), _ptr(osc), _ptr(otot))
+        return oid, osc, otot
+
+
+class Reference:
+    """The reference's own classes, compiled from its sources (oracle/ref.mk)."""
+
+    def __init__(self, L):
+        L.wpref_pdf.restype = c_double
+        L.wpref_pdf.argtypes = [P(ModelC), c_int, c_double, c_void_p]
+        L.wpref_pair_probability.restype = c_double
+        L.wpref_pair_probability.argtypes = [P(ModelC), c_uint, c_uint, ctypes.c_char_p, c_void_p, c_uint, c_double]
+        L.wpref_classify.restype = None
+        L.wpref_classify.argtypes = [P(ModelC), c_uint, c_uint, c_int, P(ctypes.c_char_p), c_void_p, c_void_p,
+                                     c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p]
+        self.L = L
+
+    @classmethod
+    def load(cls):
+        if not os.path.exists(REF_SO):
+            return None
+        return cls(ctypes.CDLL(REF_SO))
+
+    def pdf(self, model, channel, observed, counts):
+        m = Oracle._m(model)
+        cnt = np.ascontiguousarray(counts, dtype=np.uint32)
+        return self.L.wpref_pdf(ctypes.byref(m), channel, observed, _ptr(cnt))
+
+    def pair_probability(self, model, dye_string, rad, M, cutoff=DBL_MAX):
+        m = Oracle._m(model)
+        rad = np.ascontiguousarray(rad, dtype=np.float64)
+        T, C = rad.shape
+        return self.L.wpref_pair_probability(ctypes.byref(m), T, C, dye_string.encode(), _ptr(rad), M, cutoff)
+
+    def classify(self, model, dye_strings, counts, ids, rads, cutoff=DBL_MAX):
+        m = Oracle._m(model)
+        rads = np.ascontiguousarray(rads, dtype=np.float64)
+        N, T, C = rads.shape
+        arr = (ctypes.c_char_p * len(dye_strings))(*[s.encode() for s in dye_strings])
+        counts = np.ascontiguousarray(counts, dtype=np.int32)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        oid = np.zeros(N, dtype=np.int32)
+        osc = np.zeros(N)
+        otot = np.zeros(N)
+        self.L.wpref_classify(ctypes.byref(m), T, C, len(dye_strings), arr, _ptr(counts), _ptr(ids), N,
+                              _ptr(rads), cutoff, _ptr(oid), _ptr(osc), _ptr(otot))
+        return oid, osc, otot
+
+
+def model_from_golden(g, C):
+    return SequencingModel.uniform(C, p_edman_failure=g["p_edman_failure"], p_detach=g["p_detach"],
+                                   p_initial_detach=g["p_initial_detach"],
+                                   p_initial_detach_decay=g["p_initial_detach_decay"],
+                                   p_initial_block=g["p_initial_block"], p_cyclic_block=g["p_cyclic_block"],
+                                   p_bleach=g["p_bleach"], p_dud=g["p_dud"], bg_sig=g["bg_sig"], mu=g["mu"],
+                                   sig=g["sig"])

@@ -1,0 +1,162 @@
+"""GPU parity of the HMM scoring path against the oracle, through the C ABI.
+
+Bars: FP64 path -- ln P within 1e-12 absolute of the oracle (P within ~1e-12
+relative), ids identical, adjusted scores within 1e-10 relative. FP32 path --
+|d ln P| / max(1, |ln P|) <= 1e-4 per pair, top-1 agreement >= 99.9%."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from tests._oracle import DBL_MAX, model_from_golden
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "reference_goldens.json")))
+FP32_LOGP_TOL = 1e-4
+
+
+def make_clf(w, model, T, strings, cutoff, precision, counts=None, ids=None):
+    entries = [w.DyeSeqEntry(s, int(counts[i]) if counts is not None else 1,
+                             int(ids[i]) if ids is not None else i) for i, s in enumerate(strings)]
+    return w.HMMClassifier(T, model.num_channels, model, w.SequencingSettings(cutoff), entries, precision)
+
+
+def log_err(a, b):
+    """|a - b| / max(1, |b|) with matching infinities counted as exact."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    both_inf = np.isinf(a) & np.isinf(b) & (np.sign(a) == np.sign(b))
+    d = np.where(both_inf, 0.0, np.abs(a - b) / np.maximum(1.0, np.abs(b)))
+    return d
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_peptide_hmm_goldens_on_gpu(gpu, precision):
+    w = gpu
+    g = GOLD["peptide_hmm"]
+    m = model_from_golden(g["model"], g["num_channels"])
+    for case in g["cases"]:
+        cut = DBL_MAX if case["cutoff"] is None else case["cutoff"]
+        clf = make_clf(w, m, g["num_timesteps"], [g["dye_seq"]], cut, precision)
+        assert clf.max_num_dyes == g["max_num_dyes"]
+        lp = clf.pair_log_probability(np.array([case["rad"]], dtype=np.float64))[0, 0]
+        exp = case["expected"]
+        print(case["source"], precision, lp, exp, flush=True)
+        if exp == 0.0:
+            assert lp == -math.inf
+        elif precision == 0:
+            assert abs(math.exp(lp) - exp) <= 1e-9 * exp
+        else:
+            assert abs(lp - math.log(exp)) <= FP32_LOGP_TOL * max(1.0, abs(math.log(exp)))
+
+
+def _workload(w, C, T, S, N, lens, dyes, seed):
+    m = w.SequencingModel.uniform(C)
+    strings = w.generate_dye_seqs(C, S, lens[0], lens[1], dyes[0], dyes[1], 1000 + seed)
+    rads, idx = w.simulate_radiometries(m, T, strings, N, 2000 + seed)
+    return m, strings, rads, idx
+
+
+CASES = {
+    # name: C, T, S, N, lens, dyes, cutoff
+    "c2t11_uncut": (2, 11, 200, 60, (8, 20), (0, 3), DBL_MAX),
+    "c3t12_cut5": (3, 12, 400, 60, (6, 30), (0, 5), 5.0),
+    "c1t8_cut3": (1, 8, 50, 80, (3, 12), (0, 4), 3.0),
+    "c3t20_cut5": (3, 20, 100, 30, (10, 30), (0, 3), 5.0),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("precision", [0, 1])
+def test_pair_log_probability_vs_oracle(gpu, oracle, name, precision):
+    C, T, S, N, lens, dyes, cut = CASES[name]
+    m, strings, rads, _ = _workload(gpu, C, T, S, N, lens, dyes, seed=len(name))
+    print(name, precision, flush=True)
+    clf = make_clf(gpu, m, T, strings, cut, precision)
+    got = clf.pair_log_probability(rads)
+    p = oracle.pair_probabilities(m, strings, rads, cut)
+    with np.errstate(divide="ignore"):
+        want = np.log(p)
+    assert np.array_equal(np.isinf(got), np.isinf(want)), "empty-pair pattern differs"
+    err = log_err(got, want)
+    if precision == 0:
+        assert err.max() <= 1e-12, err.max()
+    else:
+        assert err.max() <= FP32_LOGP_TOL, err.max()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("precision", [0, 1])
+def test_classify_vs_oracle(gpu, oracle, name, precision):
+    C, T, S, N, lens, dyes, cut = CASES[name]
+    m, strings, rads, _ = _workload(gpu, C, T, S, N, lens, dyes, seed=3 + len(name))
+    rng = np.random.default_rng(5)
+    counts = rng.integers(1, 4, size=S)
+    ids = rng.permutation(S) + 100
+    clf = make_clf(gpu, m, T, strings, cut, precision, counts, ids)
+    gid, adj, ls, lt = clf.classify_arrays(rads)
+    oid, osc, otot = oracle.classify(m, strings, counts, ids, rads, cut)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        oadj = np.where(np.isnan(osc / otot), 0.0, osc / otot)
+    if precision == 0:
+        assert (gid == oid).all()
+        np.testing.assert_allclose(adj, oadj, rtol=1e-10, atol=1e-300)
+    else:
+        agree = (gid == oid).mean()
+        assert agree >= 0.999, agree
+        np.testing.assert_allclose(adj, oadj, rtol=1e-3, atol=1e-6)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_candidates_vs_oracle(gpu, oracle, precision):
+    C, T, S, N = 3, 12, 300, 40
+    m, strings, rads, _ = _workload(gpu, C, T, S, N, (6, 30), (0, 5), seed=21)
+    rng = np.random.default_rng(9)
+    cands = [list(rng.choice(S, size=int(rng.integers(0, 40)), replace=False)) for _ in range(N)]
+    cands[0] = []
+    cands[1] = [5, 5, 3]  # repeated candidates are allowed
+    counts = np.ones(S, dtype=np.int32)
+    ids = np.arange(S)
+    clf = make_clf(gpu, m, T, strings, 5.0, precision, counts, ids)
+    gid, adj, _, _ = clf.classify_arrays(rads, candidates=cands)
+    oid, osc, otot = oracle.classify(m, strings, counts, ids, rads, 5.0, candidates=cands)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        oadj = np.where(np.isnan(osc / otot), 0.0, osc / otot)
+    # no candidates: the reference indexes dye_seqs[-1]; here id -1, score 0
+    assert gid[0] == -1 and oid[0] == -1 and adj[0] == 0.0
+    if precision == 0:
+        assert (gid == oid).all()
+        np.testing.assert_allclose(adj[1:], oadj[1:], rtol=1e-10, atol=1e-300)
+    else:
+        assert (gid == oid).mean() >= 0.95
+        np.testing.assert_allclose(adj[1:], oadj[1:], rtol=1e-3, atol=1e-6)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_edge_cases(gpu, oracle, precision):
+    w = gpu
+    m = w.SequencingModel.uniform(2)
+    strings = ["10.01", "0", "1", "..0.1.1.0", "11"]
+    # T = 1; reads that prune everything; an all-zero read
+    for T in (1, 2, 5):
+        rads = np.zeros((4, T, 2))
+        rads[0] = 1.0
+        rads[1] = -50.0   # below every count: every pair empty under a cutoff
+        rads[2, :, 0] = 7.0
+        for cut in (DBL_MAX, 5.0):
+            clf = make_clf(w, m, T, strings, cut, precision)
+            got = clf.pair_log_probability(rads)
+            p = oracle.pair_probabilities(m, strings, rads, cut)
+            with np.errstate(divide="ignore"):
+                want = np.log(p)
+            assert np.array_equal(np.isinf(got), np.isinf(want)), (T, cut)
+            tol = 1e-12 if precision == 0 else FP32_LOGP_TOL
+            assert log_err(got, want).max() <= tol, (T, cut)
+            gid, adj, ls, lt = clf.classify_arrays(rads)
+            oid, osc, otot = oracle.classify(m, strings, np.ones(5), np.arange(5), rads, cut)
+            assert (gid == oid).all() or precision == 1
+            if cut == 5.0:
+                assert gid[1] == 0 and adj[1] == 0.0  # all-empty: first candidate, NaN guard

@@ -1,0 +1,342 @@
+"""whatprot_amd: MI355X (gfx950) implementation of whatprot's tensor-state HMM
+scoring path behind the reference's interfaces.
+
+Host-side mirror of the reference classes this path touches (names and
+argument meaning follow whatprot's cc_code/src):
+  SequencingModel / ChannelModel  parameterization/model/sequencing-model.cc:48-151
+  SequencingSettings              parameterization/settings/sequencing-settings.h:14-19
+  DyeSeq sourced data             io/dye-seqs-io.cc:31-106
+  Radiometry files                io/radiometries-io.cc:33-82
+  ScoredClassification CSV        io/scored-classifications-io.cc:30-68
+  HMMClassifier                   classifiers/hmm-classifier.h:30-78
+All scoring runs in libwhatprot_amd.so on the GPU; there is no CPU path.
+"""
+import ctypes
+import json
+import math
+import sys
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _capi
+from ._capi import (WP_MAX_CHANNELS, WP_PRECISION_FP32, WP_PRECISION_FP64, ModelC,
+                    WhatprotError, check, lib)
+
+DBL_MAX = sys.float_info.max
+
+__all__ = [
+    "ChannelModel", "SequencingModel", "SequencingSettings", "DyeSeqEntry", "ScoredClassification",
+    "HMMClassifier", "read_dye_seqs", "write_dye_seqs", "read_radiometries", "write_radiometries",
+    "write_scored_classifications", "generate_dye_seqs", "simulate_radiometries",
+    "WP_PRECISION_FP32", "WP_PRECISION_FP64", "WhatprotError", "DBL_MAX", "lib",
+]
+
+
+@dataclass
+class ChannelModel:
+    """ChannelModel (channel-model.h:20-83); interactions hold 1 - effect."""
+    p_bleach: float = 0.0
+    p_dud: float = 0.0
+    bg_sig: float = 0.0
+    mu: float = 1.0
+    sig: float = 0.0
+    interactions: List[float] = field(default_factory=list)
+    flat_interactions: List[float] = field(default_factory=list)
+
+
+@dataclass
+class SequencingModel:
+    """SequencingModel (sequencing-model.h:22-49) with DecayingRateModel detach."""
+    p_edman_failure: float = 0.0
+    p_detach: float = 0.0
+    p_initial_detach: float = 0.0
+    p_initial_detach_decay: float = 0.0
+    p_initial_block: float = 0.0
+    p_cyclic_block: float = 0.0
+    channel_models: List[ChannelModel] = field(default_factory=list)
+
+    @property
+    def num_channels(self):
+        return len(self.channel_models)
+
+    @classmethod
+    def from_json(cls, path_or_dict):
+        """Parses the seqparams JSON exactly as sequencing-model.cc:48-92 reads it
+        (p_initial_detach and p_initial_detach_decay are required; optional
+        interactions store 1 - effect / 1 - flat_effect)."""
+        data = path_or_dict
+        if not isinstance(data, dict):
+            with open(path_or_dict) as f:
+                data = json.load(f)
+        chans = data["channel_models"]
+        C = len(chans)
+        m = cls(p_edman_failure=float(data["p_edman_failure"]), p_detach=float(data["p_detach"]),
+                p_initial_detach=float(data["p_initial_detach"]),
+                p_initial_detach_decay=float(data["p_initial_detach_decay"]),
+                p_initial_block=float(data["p_initial_block"]),
+                p_cyclic_block=float(data["p_cyclic_block"]))
+        for cd in chans:
+            cm = ChannelModel(p_bleach=float(cd["p_bleach"]), p_dud=float(cd["p_dud"]),
+                              bg_sig=float(cd["bg_sig"]), mu=float(cd["mu"]), sig=float(cd["sig"]),
+                              interactions=[1.0] * C, flat_interactions=[1.0] * C)
+            for it in cd.get("interactions", []) or []:
+                o = int(it["other_channel"])
+                if "effect" in it:
+                    cm.interactions[o] = 1.0 - float(it["effect"])
+                if "flat_effect" in it:
+                    cm.flat_interactions[o] = 1.0 - float(it["flat_effect"])
+            m.channel_models.append(cm)
+        return m
+
+    def with_mu_as_one(self):
+        """sequencing-model.cc:140-151 / channel-model.cc:35-45."""
+        out = SequencingModel(self.p_edman_failure, self.p_detach, self.p_initial_detach,
+                              self.p_initial_detach_decay, self.p_initial_block, self.p_cyclic_block)
+        for cm in self.channel_models:
+            out.channel_models.append(ChannelModel(cm.p_bleach, cm.p_dud, cm.bg_sig / cm.mu, 1.0,
+                                                   cm.sig / cm.mu, list(cm.interactions),
+                                                   list(cm.flat_interactions)))
+        return out
+
+    def p_detach_at(self, i):
+        """DecayingRateModel::operator[] (decaying-rate-model.cc:47-50)."""
+        return self.p_detach + self.p_initial_detach * math.exp(-float(i) * self.p_initial_detach_decay)
+
+    def to_c(self) -> ModelC:
+        m = ModelC()
+        C = self.num_channels
+        if C > WP_MAX_CHANNELS:
+            raise WhatprotError(f"at most {WP_MAX_CHANNELS} channels")
+        m.num_channels = C
+        m.p_edman_failure = self.p_edman_failure
+        m.p_detach = self.p_detach
+        m.p_initial_detach = self.p_initial_detach
+        m.p_initial_detach_decay = self.p_initial_detach_decay
+        m.p_initial_block = self.p_initial_block
+        m.p_cyclic_block = self.p_cyclic_block
+        for c, cm in enumerate(self.channel_models):
+            ch = m.channels[c]
+            ch.p_bleach, ch.p_dud, ch.bg_sig, ch.mu, ch.sig = cm.p_bleach, cm.p_dud, cm.bg_sig, cm.mu, cm.sig
+            for o in range(WP_MAX_CHANNELS):
+                ch.interactions[o] = cm.interactions[o] if o < len(cm.interactions) else 1.0
+                ch.flat_interactions[o] = cm.flat_interactions[o] if o < len(cm.flat_interactions) else 1.0
+        return m
+
+    @classmethod
+    def uniform(cls, C, p_edman_failure=0.06, p_detach=0.05, p_initial_detach=0.04,
+                p_initial_detach_decay=0.3, p_initial_block=0.07, p_cyclic_block=0.02,
+                p_bleach=0.05, p_dud=0.07, bg_sig=0.00667, mu=1.0, sig=0.16):
+        """A model with identical channels and no interactions."""
+        m = cls(p_edman_failure, p_detach, p_initial_detach, p_initial_detach_decay,
+                p_initial_block, p_cyclic_block)
+        for _ in range(C):
+            m.channel_models.append(ChannelModel(p_bleach, p_dud, bg_sig, mu, sig, [1.0] * C, [1.0] * C))
+        return m
+
+
+@dataclass
+class SequencingSettings:
+    """SequencingSettings (sequencing-settings.h:14-19): the -p sigma cutoff."""
+    dist_cutoff: float = DBL_MAX
+
+
+@dataclass
+class DyeSeqEntry:
+    """SourcedData<DyeSeq, SourceCount<int>> as read by dye-seqs-io.cc:96-103."""
+    dye_string: str
+    count: int = 1
+    id: int = 0
+
+
+@dataclass
+class ScoredClassification:
+    """scored-classification.h:16-27; adjusted_score is what the CSV holds."""
+    id: int
+    adjusted_score: float
+    log_score: float
+    log_total: float
+
+
+def read_dye_seqs(path):
+    """dye-seqs-io.cc:31-106: C, N, then rows 'dye_string count id'."""
+    with open(path) as f:
+        tok = f.read().split()
+    C, N = int(tok[0]), int(tok[1])
+    out = []
+    for i in range(N):
+        s, cnt, idv = tok[2 + 3 * i], int(tok[3 + 3 * i]), int(tok[4 + 3 * i])
+        out.append(DyeSeqEntry(s, cnt, idv))
+    return C, out
+
+
+def write_dye_seqs(path, C, entries: Sequence[DyeSeqEntry]):
+    with open(path, "w") as f:
+        f.write(f"{C}\n{len(entries)}\n")
+        for e in entries:
+            f.write(f"{e.dye_string}\t{e.count}\t{e.id}\n")
+
+
+def read_radiometries(path, seq_model: SequencingModel):
+    """radiometries-io.cc:33-56: T, C, N then N rows of T*C values (t outer,
+    c inner), each divided by that channel's mu. Returns (T, C, array[N,T,C])."""
+    with open(path) as f:
+        T = int(f.readline())
+        C = int(f.readline())
+        N = int(f.readline())
+        data = np.loadtxt(f, dtype=np.float64, ndmin=2) if N else np.zeros((0, T * C))
+    data = data.reshape(N, T, C)
+    mus = np.array([cm.mu for cm in seq_model.channel_models[:C]], dtype=np.float64)
+    return T, C, np.ascontiguousarray(data / mus[None, None, :])
+
+
+def write_radiometries(path, seq_model: SequencingModel, rads):
+    """radiometries-io.cc:58-82 (values multiplied back by mu, 17 digits)."""
+    N, T, C = rads.shape
+    mus = np.array([cm.mu for cm in seq_model.channel_models[:C]])
+    with open(path, "w") as f:
+        f.write(f"{T}\n{C}\n{N}\n")
+        for r in range(N):
+            vals = (rads[r] * mus[None, :]).reshape(-1)
+            f.write("\t".join(repr(float(v)) if False else f"{v:.17g}" for v in vals) + "\n")
+
+
+def write_scored_classifications(path, results: Sequence[ScoredClassification]):
+    """scored-classifications-io.cc:30-68."""
+    with open(path, "w") as f:
+        f.write("radmat_iz,best_pep_iz,best_pep_score\n")
+        for i, r in enumerate(results):
+            f.write(f"{i},{r.id},{r.adjusted_score:.17g}\n")
+
+
+def generate_dye_seqs(C, n, len_min, len_max, dyes_min, dyes_max, seed):
+    stride = len_max + 1
+    buf = ctypes.create_string_buffer(n * stride)
+    check(lib().wp_generate_dye_seqs(C, n, len_min, len_max, dyes_min, dyes_max, seed, buf, stride))
+    raw = buf.raw
+    return [raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode() for i in range(n)]
+
+
+def _cstrings(strings):
+    arr = (ctypes.c_char_p * max(1, len(strings)))()
+    enc = [s.encode() for s in strings]
+    for i, s in enumerate(enc):
+        arr[i] = s
+    return arr, enc
+
+
+def simulate_radiometries(seq_model: SequencingModel, T, dye_strings, n, seed, weights=None):
+    """Returns (rads[n,T,C] in the model's units, dye-seq index per read)."""
+    C = seq_model.num_channels
+    m = seq_model.to_c()
+    arr, keep = _cstrings(dye_strings)
+    out = np.zeros((n, T, C), dtype=np.float64)
+    idx = np.zeros(n, dtype=np.int32)
+    w = None
+    if weights is not None:
+        w = np.ascontiguousarray(weights, dtype=np.int32)
+    check(lib().wp_simulate_radiometries(ctypes.byref(m), T, C, len(dye_strings), arr,
+                                         w.ctypes.data if w is not None else None, n, seed,
+                                         out.ctypes.data, idx.ctypes.data))
+    return out, idx
+
+
+class HMMClassifier:
+    """HMMClassifier (classifiers/hmm-classifier.h:30-78) on the GPU.
+
+    HMMClassifier(num_timesteps, num_channels, seq_model, seq_settings,
+    dye_seqs) as in run-classify-hmm.cc:80-82, plus `precision`
+    (WP_PRECISION_FP64 reproduces the reference's arithmetic;
+    WP_PRECISION_FP32 is the rescaled batched path)."""
+
+    def __init__(self, num_timesteps, num_channels, seq_model: SequencingModel,
+                 seq_settings: SequencingSettings, dye_seqs: Sequence[DyeSeqEntry],
+                 precision=WP_PRECISION_FP64):
+        self.num_timesteps = int(num_timesteps)
+        self.num_channels = int(num_channels)
+        self.dye_seqs = list(dye_seqs)
+        self._m = seq_model.to_c()
+        arr, self._keep = _cstrings([d.dye_string for d in self.dye_seqs])
+        S = len(self.dye_seqs)
+        counts = (ctypes.c_int * max(1, S))(*[d.count for d in self.dye_seqs])
+        ids = (ctypes.c_int * max(1, S))(*[d.id for d in self.dye_seqs])
+        h = ctypes.c_void_p()
+        check(lib().wp_hmm_classifier_create(self.num_timesteps, self.num_channels, ctypes.byref(self._m),
+                                             float(seq_settings.dist_cutoff), S, arr, counts, ids,
+                                             int(precision), ctypes.byref(h)))
+        self._h = h
+        self.max_num_dyes = lib().wp_hmm_classifier_max_num_dyes(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().wp_hmm_classifier_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _rads(self, radiometries):
+        r = np.ascontiguousarray(radiometries, dtype=np.float64)
+        if r.ndim != 3 or r.shape[1] != self.num_timesteps or r.shape[2] != self.num_channels:
+            raise WhatprotError(f"radiometries must be [N, {self.num_timesteps}, {self.num_channels}]")
+        return r
+
+    def classify_arrays(self, radiometries, candidates: Optional[Sequence[Sequence[int]]] = None):
+        """Returns (ids, adjusted, log_score, log_total) arrays."""
+        r = self._rads(radiometries)
+        N = r.shape[0]
+        ids = np.zeros(N, dtype=np.int32)
+        adj = np.zeros(N)
+        ls = np.zeros(N)
+        lt = np.zeros(N)
+        if candidates is None:
+            check(lib().wp_hmm_classify(self._h, N, r.ctypes.data, ids.ctypes.data, adj.ctypes.data,
+                                        ls.ctypes.data, lt.ctypes.data))
+        else:
+            off = np.zeros(N + 1, dtype=np.int32)
+            off[1:] = np.cumsum([len(c) for c in candidates])
+            idx = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.int32) for c in candidates])
+                                       if off[-1] else np.zeros(1, dtype=np.int32), dtype=np.int32)
+            check(lib().wp_hmm_classify_candidates(self._h, N, r.ctypes.data, off.ctypes.data,
+                                                   idx.ctypes.data, ids.ctypes.data, adj.ctypes.data,
+                                                   ls.ctypes.data, lt.ctypes.data))
+        return ids, adj, ls, lt
+
+    def classify(self, radiometries, candidates=None) -> List[ScoredClassification]:
+        """HMMClassifier::classify(vector<Radiometry>) (hmm-classifier.cc:84-93),
+        or per-read candidate_indices (hmm-classifier.cc:79-82)."""
+        ids, adj, ls, lt = self.classify_arrays(radiometries, candidates)
+        return [ScoredClassification(int(i), float(a), float(s), float(t)) for i, a, s, t in zip(ids, adj, ls, lt)]
+
+    def pair_log_probability(self, radiometries):
+        """ln PeptideHMM::probability() for every (read, dye-seq) pair."""
+        r = self._rads(radiometries)
+        out = np.zeros((r.shape[0], len(self.dye_seqs)))
+        check(lib().wp_hmm_pair_log_probability(self._h, r.shape[0], r.ctypes.data, out.ctypes.data))
+        return out
+
+    def set_timing(self, enable=True):
+        check(lib().wp_hmm_set_timing(self._h, 1 if enable else 0))
+
+    def last_timing(self):
+        """(prep_ms, classify_ms) of the last call with timing enabled."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(lib().wp_hmm_last_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def work(self, radiometries):
+        """(non-empty pairs, state-timesteps, algorithmic FLOPs) of classifying these reads."""
+        r = self._rads(radiometries)
+        out = np.zeros(3)
+        check(lib().wp_hmm_work(self._h, r.shape[0], r.ctypes.data, out.ctypes.data))
+        return tuple(float(x) for x in out)
+
+    def classify_device(self, num_reads, d_rads, d_id, d_adjusted=0, d_log_score=0, d_log_total=0, stream=0):
+        """Device-resident classify: integer device addresses, work queued on
+        `stream` (a hipStream_t as an integer)."""
+        check(lib().wp_hmm_classify_device(self._h, int(num_reads), d_rads, d_id, d_adjusted or None,
+                                           d_log_score or None, d_log_total or None, stream or None))

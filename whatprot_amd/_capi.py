@@ -1,0 +1,108 @@
+"""ctypes view of the C ABI declared in include/whatprot_amd.h.
+
+Only layouts and prototypes live here; the host-side mirror of the reference's
+classes is in whatprot_amd/__init__.py.
+"""
+import ctypes
+import os
+
+WP_MAX_CHANNELS = 8
+WP_MAX_TIMESTEPS = 64
+WP_MAX_DYES = 15
+
+WP_OK = 0
+WP_PRECISION_FP64 = 0
+WP_PRECISION_FP32 = 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwhatprot_amd.so")
+
+
+class ChannelModelC(ctypes.Structure):
+    _fields_ = [
+        ("p_bleach", ctypes.c_double),
+        ("p_dud", ctypes.c_double),
+        ("bg_sig", ctypes.c_double),
+        ("mu", ctypes.c_double),
+        ("sig", ctypes.c_double),
+        ("interactions", ctypes.c_double * WP_MAX_CHANNELS),
+        ("flat_interactions", ctypes.c_double * WP_MAX_CHANNELS),
+    ]
+
+
+class ModelC(ctypes.Structure):
+    _fields_ = [
+        ("num_channels", ctypes.c_int),
+        ("p_edman_failure", ctypes.c_double),
+        ("p_detach", ctypes.c_double),
+        ("p_initial_detach", ctypes.c_double),
+        ("p_initial_detach_decay", ctypes.c_double),
+        ("p_initial_block", ctypes.c_double),
+        ("p_cyclic_block", ctypes.c_double),
+        ("channels", ChannelModelC * WP_MAX_CHANNELS),
+    ]
+
+
+_lib = None
+
+
+class WhatprotError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libwhatprot_amd.so. Fails loudly when it is missing: there is no
+    CPU fallback for the scoring path."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise WhatprotError(
+            "libwhatprot_amd.so is not built (run python whatprot_amd/build.py or "
+            "__graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    c_int, c_uint, c_double, c_void_p = ctypes.c_int, ctypes.c_uint, ctypes.c_double, ctypes.c_void_p
+    P = ctypes.POINTER
+    L.wp_last_error.restype = ctypes.c_char_p
+    L.wp_last_error.argtypes = []
+    L.wp_model_with_mu_as_one.restype = None
+    L.wp_model_with_mu_as_one.argtypes = [P(ModelC), P(ModelC)]
+    L.wp_device_count.restype = c_int
+    L.wp_device_count.argtypes = []
+    L.wp_hmm_classifier_create.restype = c_int
+    L.wp_hmm_classifier_create.argtypes = [c_uint, c_uint, P(ModelC), c_double, c_int,
+                                           P(ctypes.c_char_p), P(c_int), P(c_int), c_int, P(c_void_p)]
+    L.wp_hmm_classifier_destroy.restype = None
+    L.wp_hmm_classifier_destroy.argtypes = [c_void_p]
+    L.wp_hmm_classifier_max_num_dyes.restype = c_int
+    L.wp_hmm_classifier_max_num_dyes.argtypes = [c_void_p]
+    L.wp_hmm_classify.restype = c_int
+    L.wp_hmm_classify.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.wp_hmm_classify_candidates.restype = c_int
+    L.wp_hmm_classify_candidates.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                             c_void_p, c_void_p, c_void_p, c_void_p]
+    L.wp_hmm_classify_device.restype = c_int
+    L.wp_hmm_classify_device.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p]
+    L.wp_hmm_pair_log_probability.restype = c_int
+    L.wp_hmm_pair_log_probability.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
+    L.wp_hmm_set_timing.restype = c_int
+    L.wp_hmm_set_timing.argtypes = [c_void_p, c_int]
+    L.wp_hmm_last_timing.restype = c_int
+    L.wp_hmm_last_timing.argtypes = [c_void_p, P(c_double), P(c_double)]
+    L.wp_hmm_work.restype = c_int
+    L.wp_hmm_work.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
+    L.wp_generate_dye_seqs.restype = c_int
+    L.wp_generate_dye_seqs.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int, ctypes.c_uint64,
+                                       c_void_p, c_int]
+    L.wp_simulate_radiometries.restype = c_int
+    L.wp_simulate_radiometries.argtypes = [P(ModelC), c_uint, c_uint, c_int, P(ctypes.c_char_p),
+                                           c_void_p, c_int, ctypes.c_uint64, c_void_p, c_void_p]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != WP_OK:
+        msg = lib().wp_last_error().decode(errors="replace")
+        raise WhatprotError(f"whatprot_amd error {rc}: {msg}")

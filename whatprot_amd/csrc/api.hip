@@ -1,0 +1,560 @@
+// C ABI of libwhatprot_amd: model helpers, the HMM classifier context and the
+// synthetic-input generator. Declarations and reference citations are in
+// include/whatprot_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hmm_kernels.h"
+#include "wp_internal.h"
+
+namespace wp {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int parse_dye_seq(const char* s, short* seq, int cap) {
+    int len = (int)std::strlen(s);
+    while (len > 0 && s[len - 1] == '.') len--;
+    for (int i = 0; i < len && i < cap; i++) {
+        char ch = s[i];
+        seq[i] = (ch >= '0' && ch <= '9') ? (short)(ch - '0') : (short)-1;
+    }
+    return len;
+}
+
+}  // namespace wp
+
+#define WP_HIP(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            wp::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));              \
+            return WP_ERR_HIP;                                                             \
+        }                                                                                  \
+    } while (0)
+
+struct wp_hmm_classifier {
+    int T = 0, C = 0, S = 0, M = 0, precision = 0, G = 16, tri = 1;
+    wp::DevModel dm;
+    std::vector<int> ids, counts;
+    uint8_t* dD = nullptr;
+    int8_t* dSeq = nullptr;
+    uint8_t* dTrack = nullptr;
+    int* dCount = nullptr;
+    int* dId = nullptr;
+    double* dBinom = nullptr;
+    // per-read workspace of the device path
+    uint8_t* wL = nullptr;
+    uint8_t* wH = nullptr;
+    int* wNb = nullptr;
+    int wcap = 0;
+    int maxbox = 1;  // prod_c (M+1), bound on any read's box
+    // optional kernel timing: events around prep and classify launches
+    int timing = 0;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    int ev_recorded = 0;
+};
+
+namespace {
+
+struct Bin {
+    int lo, hi, ngroups;
+};
+
+constexpr size_t kLdsBudget = 48 * 1024;
+constexpr size_t kLdsHardLimit = 64 * 1024;
+
+// Bins of reads by box bound, each with its LDS sizing.
+std::vector<Bin> make_bins(const wp_hmm_classifier* c) {
+    std::vector<int> edges = {8, 32, 128, c->maxbox};
+    std::vector<Bin> bins;
+    int lo = 0;
+    for (int hi : edges) {
+        if (hi <= lo) continue;
+        if (hi > c->maxbox) hi = c->maxbox;
+        size_t fixed = wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, hi, 0, c->G);
+        size_t per = wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, hi, 1, c->G) - fixed;
+        int ng = (int)((kLdsBudget > fixed ? kLdsBudget - fixed : 0) / (per ? per : 1));
+        ng = std::max(1, std::min(ng, wp::kBlockThreads / c->G));
+        bins.push_back({lo, hi, ng});
+        lo = hi;
+        if (lo >= c->maxbox) break;
+    }
+    return bins;
+}
+
+int ensure_workspace(wp_hmm_classifier* c, int N) {
+    if (N <= c->wcap) return WP_OK;
+    hipFree(c->wL);
+    hipFree(c->wH);
+    hipFree(c->wNb);
+    c->wL = c->wH = nullptr;
+    c->wNb = nullptr;
+    c->wcap = 0;
+    size_t tc = (size_t)N * c->T * c->C;
+    WP_HIP(hipMalloc(&c->wL, tc));
+    WP_HIP(hipMalloc(&c->wH, tc));
+    WP_HIP(hipMalloc(&c->wNb, sizeof(int) * (size_t)N));
+    c->wcap = N;
+    return WP_OK;
+}
+
+int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_cand_off,
+               const int* d_cand_idx, int* d_id, double* d_adj, double* d_ls, double* d_lt,
+               double* d_pair, hipStream_t stream) {
+    if (N < 0) {
+        wp::set_error("num_reads < 0");
+        return WP_ERR_ARG;
+    }
+    if (N == 0) return WP_OK;
+    int rc = ensure_workspace(c, N);
+    if (rc) return rc;
+    if (c->timing) WP_HIP(hipEventRecord(c->ev[0], stream));
+    wp::launch_read_prep(c->dm, N, d_rads, c->wL, c->wH, c->wNb, stream);
+    WP_HIP(hipGetLastError());
+    if (c->timing) WP_HIP(hipEventRecord(c->ev[1], stream));
+    wp::ClassifyArgs a;
+    a.m = c->dm;
+    a.N = N;
+    a.S = c->S;
+    a.rads = d_rads;
+    a.Lr = c->wL;
+    a.Hr = c->wH;
+    a.nbmax = c->wNb;
+    a.dsD = c->dD;
+    a.dsSeq = c->dSeq;
+    a.dsTrack = c->dTrack;
+    a.dsCount = c->dCount;
+    a.dsId = c->dId;
+    a.binom = c->dBinom;
+    a.tri = c->tri;
+    a.cand_off = d_cand_off;
+    a.cand_idx = d_cand_idx;
+    a.out_id = d_id;
+    a.out_adjusted = d_adj;
+    a.out_log_score = d_ls;
+    a.out_log_total = d_lt;
+    a.out_pair_logp = d_pair;
+    for (const Bin& b : make_bins(c)) {
+        a.nb_lo = b.lo;
+        a.nb_hi = b.hi;
+        a.NB = b.hi;
+        a.ngroups = b.ngroups;
+        WP_HIP(wp::launch_classify(a, c->precision, c->G, stream));
+    }
+    if (c->timing) {
+        WP_HIP(hipEventRecord(c->ev[2], stream));
+        c->ev_recorded = 1;
+    }
+    return WP_OK;
+}
+
+template <typename T>
+int upload(T** d, const T* h, size_t n) {
+    WP_HIP(hipMalloc((void**)d, sizeof(T) * std::max<size_t>(n, 1)));
+    if (n) WP_HIP(hipMemcpy(*d, h, sizeof(T) * n, hipMemcpyHostToDevice));
+    return WP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* wp_last_error(void) { return wp::g_err.c_str(); }
+
+void wp_model_with_mu_as_one(const wp_model* in, wp_model* out) {
+    wp_model m = *in;
+    for (int c = 0; c < in->num_channels && c < WP_MAX_CHANNELS; c++) {
+        double mu = in->channels[c].mu;
+        m.channels[c].bg_sig = in->channels[c].bg_sig / mu;
+        m.channels[c].mu = 1.0;
+        m.channels[c].sig = in->channels[c].sig / mu;
+    }
+    *out = m;
+}
+
+int wp_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    int good = 0;
+    for (int d = 0; d < n; d++) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, d) == hipSuccess && std::strncmp(p.gcnArchName, "gfx950", 6) == 0)
+            good++;
+    }
+    return good;
+}
+
+int wp_hmm_classifier_create(unsigned int num_timesteps, unsigned int num_channels,
+                             const wp_model* seq_model, double dist_cutoff, int num_dye_seqs,
+                             const char* const* dye_strings, const int* dye_seq_counts,
+                             const int* dye_seq_ids, int precision, wp_hmm_classifier** out) {
+    if (!out || !seq_model || (num_dye_seqs > 0 && (!dye_strings || !dye_seq_counts || !dye_seq_ids))) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    *out = nullptr;
+    const int T = (int)num_timesteps, C = (int)num_channels;
+    if (T < 1 || T > WP_MAX_TIMESTEPS || C < 1 || C > 4 || seq_model->num_channels != C ||
+        num_dye_seqs < 0) {
+        wp::set_error("unsupported shape: need 1 <= T <= 64, 1 <= C <= 4, model channels == C");
+        return WP_ERR_UNSUPPORTED;
+    }
+    if (precision != WP_PRECISION_FP64 && precision != WP_PRECISION_FP32) {
+        wp::set_error("unknown precision");
+        return WP_ERR_ARG;
+    }
+    if (wp_device_count() < 1) {
+        wp::set_error("no gfx950 device visible");
+        return WP_ERR_NO_DEVICE;
+    }
+    const int S = num_dye_seqs;
+    std::vector<uint8_t> hD((size_t)S * C), hTrack((size_t)S * T * C);
+    std::vector<int8_t> hSeq((size_t)S * T);
+    int M = 0;
+    std::vector<short> seq(4096);
+    for (int i = 0; i < S; i++) {
+        const char* s = dye_strings[i];
+        int len = (int)std::strlen(s);
+        if (len > 4096) {
+            wp::set_error("dye-seq longer than 4096");
+            return WP_ERR_UNSUPPORTED;
+        }
+        len = wp::parse_dye_seq(s, seq.data(), 4096);
+        std::vector<int> cs(C, 0);
+        for (int k = 0; k < len; k++) {
+            if (seq[k] >= C) {
+                wp::set_error("dye-seq names a channel >= num_channels");
+                return WP_ERR_ARG;
+            }
+        }
+        // DyeTrack (dye-track.cc:26-43): dyes of each channel at positions >= e.
+        for (int e = len - 1; e >= 0; e--) {
+            if (seq[e] >= 0) cs[seq[e]]++;
+            if (e < T)
+                for (int c = 0; c < C; c++) hTrack[((size_t)i * T + e) * C + c] = (uint8_t)cs[c];
+        }
+        for (int c = 0; c < C; c++) {
+            if (cs[c] > WP_MAX_DYES) {
+                wp::set_error("more than 15 dyes of one channel in a dye-seq");
+                return WP_ERR_UNSUPPORTED;
+            }
+            hD[(size_t)i * C + c] = (uint8_t)cs[c];
+            M = std::max(M, cs[c]);
+        }
+        for (int e = 0; e < T; e++) hSeq[(size_t)i * T + e] = (int8_t)(e < len ? seq[e] : -1);
+    }
+    wp_hmm_classifier* c = new wp_hmm_classifier();
+    c->T = T;
+    c->C = C;
+    c->S = S;
+    c->M = M;
+    c->precision = precision;
+    c->G = T <= 16 ? 16 : (T <= 32 ? 32 : 64);
+    c->tri = (M + 1) * (M + 2) / 2;
+    c->ids.assign(dye_seq_ids, dye_seq_ids + S);
+    c->counts.assign(dye_seq_counts, dye_seq_counts + S);
+    c->maxbox = 1;
+    for (int k = 0; k < C; k++) c->maxbox *= (M + 1);
+    wp::DevModel& dm = c->dm;
+    std::memset(&dm, 0, sizeof(dm));
+    dm.T = T;
+    dm.C = C;
+    dm.M = M;
+    dm.uncut = dist_cutoff == DBL_MAX;
+    dm.cutoff = dist_cutoff;
+    dm.pf = seq_model->p_edman_failure;
+    dm.p_ib = seq_model->p_initial_block;
+    dm.p_cb = seq_model->p_cyclic_block;
+    for (int i = 0; i < T; i++)  // decaying-rate-model.cc:47-50
+        dm.p_detach[i] = seq_model->p_detach +
+                         seq_model->p_initial_detach * std::exp(-(double)i * seq_model->p_initial_detach_decay);
+    for (int k = 0; k < C; k++) {
+        dm.mu[k] = seq_model->channels[k].mu;
+        dm.sig[k] = seq_model->channels[k].sig;
+        dm.bg_sig[k] = seq_model->channels[k].bg_sig;
+        for (int o = 0; o < C; o++) {
+            dm.inter[k][o] = seq_model->channels[k].interactions[o];
+            dm.flat[k][o] = seq_model->channels[k].flat_interactions[o];
+        }
+    }
+    // BinomialTransition::reserve (binomial-transition.cc:51-67) for dud and bleach.
+    std::vector<double> binom((size_t)2 * C * c->tri);
+    for (int kind = 0; kind < 2; kind++) {
+        for (int k = 0; k < C; k++) {
+            double q = kind == 0 ? seq_model->channels[k].p_dud : seq_model->channels[k].p_bleach;
+            double p = 1.0 - q;
+            double* v = &binom[((size_t)kind * C + k) * c->tri];
+            v[0] = 1.0;
+            for (int i = 1; i <= M; i++) {
+                int row = i * (i + 1) / 2, prev = (i - 1) * i / 2;
+                v[row] = v[prev] * q;
+                for (int j = 1; j < i; j++) v[row + j] = v[prev + j] * q + v[prev + j - 1] * p;
+                v[row + i] = v[prev + i - 1] * p;
+            }
+        }
+    }
+    // every bin must fit at least one pair group
+    for (const Bin& b : make_bins(c)) {
+        size_t need = wp::classify_lds_bytes(precision, T, C, c->tri, b.hi, b.ngroups, c->G);
+        if (need > kLdsHardLimit) {
+            wp::set_error("state box too large for LDS at this precision (max_num_dyes/channels)");
+            delete c;
+            return WP_ERR_UNSUPPORTED;
+        }
+    }
+    int rc = WP_OK;
+    if (!rc) rc = upload(&c->dD, hD.data(), hD.size());
+    if (!rc) rc = upload(&c->dSeq, hSeq.data(), hSeq.size());
+    if (!rc) rc = upload(&c->dTrack, hTrack.data(), hTrack.size());
+    if (!rc) rc = upload(&c->dCount, dye_seq_counts, (size_t)S);
+    if (!rc) rc = upload(&c->dId, dye_seq_ids, (size_t)S);
+    if (!rc) rc = upload(&c->dBinom, binom.data(), binom.size());
+    if (rc) {
+        wp_hmm_classifier_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return WP_OK;
+}
+
+void wp_hmm_classifier_destroy(wp_hmm_classifier* c) {
+    if (!c) return;
+    for (int k = 0; k < 3; k++)
+        if (c->ev[k]) hipEventDestroy(c->ev[k]);
+    hipFree(c->dD);
+    hipFree(c->dSeq);
+    hipFree(c->dTrack);
+    hipFree(c->dCount);
+    hipFree(c->dId);
+    hipFree(c->dBinom);
+    hipFree(c->wL);
+    hipFree(c->wH);
+    hipFree(c->wNb);
+    delete c;
+}
+
+int wp_hmm_classifier_max_num_dyes(const wp_hmm_classifier* c) { return c ? c->M : -1; }
+
+int wp_hmm_classify_device(wp_hmm_classifier* c, int N, const double* d_rads, int* d_id, double* d_adj,
+                           double* d_ls, double* d_lt, void* stream) {
+    if (!c || (N > 0 && (!d_rads || !d_id))) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    return run_device(c, N, d_rads, nullptr, nullptr, d_id, d_adj, d_ls, d_lt, nullptr,
+                      (hipStream_t)stream);
+}
+
+static int classify_host(wp_hmm_classifier* c, int N, const double* rads, const int* cand_off,
+                         const int* cand_idx, int* out_id, double* out_adj, double* out_ls,
+                         double* out_lt, double* out_pair) {
+    if (!c || N < 0 || (N > 0 && !rads)) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    if (N == 0) return WP_OK;
+    if (cand_off) {
+        for (int r = 0; r < N; r++) {
+            if (cand_off[r + 1] < cand_off[r]) {
+                wp::set_error("cand_offsets not monotone");
+                return WP_ERR_ARG;
+            }
+        }
+        for (int k = cand_off[0]; k < cand_off[N]; k++) {
+            if (cand_idx[k] < 0 || cand_idx[k] >= c->S) {
+                wp::set_error("candidate index out of range");
+                return WP_ERR_ARG;
+            }
+        }
+    }
+    size_t tc = (size_t)N * c->T * c->C;
+    double* d_rads = nullptr;
+    int* d_id = nullptr;
+    double* d_out = nullptr;
+    double* d_pair = nullptr;
+    int* d_off = nullptr;
+    int* d_idx = nullptr;
+    int rc = WP_OK;
+    hipStream_t s = nullptr;
+    auto cleanup = [&]() {
+        hipFree(d_rads);
+        hipFree(d_id);
+        hipFree(d_out);
+        hipFree(d_pair);
+        hipFree(d_off);
+        hipFree(d_idx);
+    };
+#define WP_HIPC(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            wp::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));      \
+            cleanup();                                                             \
+            return WP_ERR_HIP;                                                     \
+        }                                                                          \
+    } while (0)
+    WP_HIPC(hipMalloc(&d_rads, sizeof(double) * tc));
+    WP_HIPC(hipMalloc(&d_id, sizeof(int) * (size_t)N));
+    WP_HIPC(hipMalloc(&d_out, sizeof(double) * 3 * (size_t)N));
+    if (out_pair) WP_HIPC(hipMalloc(&d_pair, sizeof(double) * (size_t)N * std::max(c->S, 1)));
+    if (cand_off) {
+        size_t nc = (size_t)(cand_off[N] - cand_off[0]);
+        std::vector<int> off(N + 1);
+        for (int r = 0; r <= N; r++) off[r] = cand_off[r] - cand_off[0];
+        WP_HIPC(hipMalloc(&d_off, sizeof(int) * (size_t)(N + 1)));
+        WP_HIPC(hipMalloc(&d_idx, sizeof(int) * std::max<size_t>(nc, 1)));
+        WP_HIPC(hipMemcpy(d_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+        if (nc) WP_HIPC(hipMemcpy(d_idx, cand_idx + cand_off[0], sizeof(int) * nc, hipMemcpyHostToDevice));
+    }
+    WP_HIPC(hipMemcpy(d_rads, rads, sizeof(double) * tc, hipMemcpyHostToDevice));
+    rc = run_device(c, N, d_rads, d_off, d_idx, d_id, d_out, d_out + N, d_out + 2 * (size_t)N, d_pair, s);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    WP_HIPC(hipStreamSynchronize(s));
+    std::vector<double> hout(3 * (size_t)N);
+    if (out_id) WP_HIPC(hipMemcpy(out_id, d_id, sizeof(int) * N, hipMemcpyDeviceToHost));
+    WP_HIPC(hipMemcpy(hout.data(), d_out, sizeof(double) * 3 * N, hipMemcpyDeviceToHost));
+    if (out_pair) WP_HIPC(hipMemcpy(out_pair, d_pair, sizeof(double) * (size_t)N * c->S, hipMemcpyDeviceToHost));
+#undef WP_HIPC
+    for (int r = 0; r < N; r++) {
+        if (out_adj) out_adj[r] = hout[r];
+        if (out_ls) out_ls[r] = hout[N + r];
+        if (out_lt) out_lt[r] = hout[2 * (size_t)N + r];
+    }
+    cleanup();
+    return WP_OK;
+}
+
+int wp_hmm_classify(wp_hmm_classifier* c, int N, const double* rads, int* out_id, double* out_adj,
+                    double* out_ls, double* out_lt) {
+    if (!out_id) {
+        wp::set_error("out_id is required");
+        return WP_ERR_ARG;
+    }
+    return classify_host(c, N, rads, nullptr, nullptr, out_id, out_adj, out_ls, out_lt, nullptr);
+}
+
+int wp_hmm_classify_candidates(wp_hmm_classifier* c, int N, const double* rads, const int* cand_off,
+                               const int* cand_idx, int* out_id, double* out_adj, double* out_ls,
+                               double* out_lt) {
+    if (!out_id || !cand_off || !cand_idx) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    return classify_host(c, N, rads, cand_off, cand_idx, out_id, out_adj, out_ls, out_lt, nullptr);
+}
+
+int wp_hmm_pair_log_probability(wp_hmm_classifier* c, int N, const double* rads, double* out_log_p) {
+    if (!out_log_p) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    std::vector<int> ids(std::max(N, 1));
+    return classify_host(c, N, rads, nullptr, nullptr, ids.data(), nullptr, nullptr, nullptr, out_log_p);
+}
+
+int wp_hmm_set_timing(wp_hmm_classifier* c, int enable) {
+    if (!c) {
+        wp::set_error("null classifier");
+        return WP_ERR_ARG;
+    }
+    if (enable && !c->ev[0]) {
+        for (int k = 0; k < 3; k++) WP_HIP(hipEventCreate(&c->ev[k]));
+    }
+    c->timing = enable ? 1 : 0;
+    c->ev_recorded = 0;
+    return WP_OK;
+}
+
+int wp_hmm_last_timing(wp_hmm_classifier* c, double* prep_ms, double* classify_ms) {
+    if (!c || !c->timing || !c->ev_recorded) {
+        wp::set_error("timing not enabled or nothing recorded");
+        return WP_ERR_ARG;
+    }
+    WP_HIP(hipEventSynchronize(c->ev[2]));
+    float a = 0.f, b = 0.f;
+    WP_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    WP_HIP(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    if (prep_ms) *prep_ms = a;
+    if (classify_ms) *classify_ms = b;
+    return WP_OK;
+}
+
+int wp_hmm_work(wp_hmm_classifier* c, int N, const double* rads, double* out3) {
+    if (!c || N < 0 || (N > 0 && (!rads || !out3))) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    out3[0] = out3[1] = out3[2] = 0.0;
+    if (N == 0) return WP_OK;
+    int rc = ensure_workspace(c, N);
+    if (rc) return rc;
+    double* d_rads = nullptr;
+    double* d_out = nullptr;
+    size_t tc = (size_t)N * c->T * c->C;
+    hipError_t e = hipMalloc(&d_rads, sizeof(double) * tc);
+    if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * 3 * (size_t)N);
+    if (e == hipSuccess) e = hipMemcpy(d_rads, rads, sizeof(double) * tc, hipMemcpyHostToDevice);
+    std::vector<double> h(3 * (size_t)N);
+    if (e == hipSuccess) {
+        wp::launch_read_prep(c->dm, N, d_rads, c->wL, c->wH, c->wNb, nullptr);
+        wp::launch_work(c->dm, N, c->S, c->wL, c->wH, c->wNb, c->dD, d_out, nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(h.data(), d_out, sizeof(double) * 3 * N, hipMemcpyDeviceToHost);
+    hipFree(d_rads);
+    hipFree(d_out);
+    if (e != hipSuccess) {
+        wp::set_error(std::string("wp_hmm_work: ") + hipGetErrorString(e));
+        return WP_ERR_HIP;
+    }
+    for (int r = 0; r < N; r++)
+        for (int k = 0; k < 3; k++) out3[k] += h[(size_t)r * 3 + k];
+    return WP_OK;
+}
+
+int wp_generate_dye_seqs(int num_channels, int num_dye_seqs, int len_min, int len_max, int dyes_min,
+                         int dyes_max, uint64_t seed, char* out_strings, int stride) {
+    if (num_channels < 1 || num_channels > 10 || num_dye_seqs < 0 || len_min < 1 || len_max < len_min ||
+        dyes_min < 0 || dyes_max < dyes_min || !out_strings || stride < len_max + 1) {
+        wp::set_error("bad dye-seq generator arguments");
+        return WP_ERR_ARG;
+    }
+    wp::generate_dye_seqs(num_channels, num_dye_seqs, len_min, len_max, dyes_min, dyes_max, seed,
+                          out_strings, stride);
+    return WP_OK;
+}
+
+int wp_simulate_radiometries(const wp_model* seq_model, unsigned int num_timesteps,
+                             unsigned int num_channels, int num_dye_seqs,
+                             const char* const* dye_strings, const int* weights, int num_reads,
+                             uint64_t seed, double* out_radiometries, int* out_dye_seq_index) {
+    if (!seq_model || num_dye_seqs < 1 || !dye_strings || num_reads < 0 || !out_radiometries ||
+        num_timesteps < 1 || num_timesteps > WP_MAX_TIMESTEPS || num_channels < 1 ||
+        num_channels > WP_MAX_CHANNELS) {
+        wp::set_error("bad simulator arguments");
+        return WP_ERR_ARG;
+    }
+    int cap = 0;
+    for (int i = 0; i < num_dye_seqs; i++) cap = std::max(cap, (int)std::strlen(dye_strings[i]));
+    cap = std::max(cap, 1);
+    std::vector<short> seqs((size_t)num_dye_seqs * cap, -1);
+    std::vector<int> lens(num_dye_seqs);
+    for (int i = 0; i < num_dye_seqs; i++) lens[i] = wp::parse_dye_seq(dye_strings[i], &seqs[(size_t)i * cap], cap);
+    wp::simulate_radiometries(*seq_model, (int)num_timesteps, (int)num_channels, num_dye_seqs, seqs.data(), cap,
+                              lens.data(), weights, num_reads, seed, out_radiometries, out_dye_seq_index,
+                              nullptr);
+    return WP_OK;
+}
+
+}  // extern "C"

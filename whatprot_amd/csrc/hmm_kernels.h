@@ -1,0 +1,58 @@
+// Launch interface of the HMM scoring kernels (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hmm_device.h"
+
+namespace wp {
+
+// Arguments of the fused classify kernel. Device pointers unless noted.
+struct ClassifyArgs {
+    DevModel m;
+    int N;                    // reads
+    int S;                    // dye-seqs in the table
+    const double* rads;       // [N][T][C], intensities / mu
+    const uint8_t* Lr;        // [N][T][C] lower box bound (suffix max of range lows)
+    const uint8_t* Hr;        // [N][T][C] upper box bound (prefix min of range highs)
+    const int* nbmax;         // [N] max_t prod_c (Hr - Lr); 0 = every pair empty
+    int nb_lo, nb_hi;         // this launch takes reads with nb_lo < nbmax <= nb_hi
+    int NB;                   // per-group state capacity in count-box entries
+    int ngroups;              // pair groups per workgroup
+    const uint8_t* dsD;       // [S][C] dyes per channel
+    const int8_t* dsSeq;      // [S][T] channel of position e (-1 = none)
+    const uint8_t* dsTrack;   // [S][T][C] dyes of channel c at positions >= e
+    const int* dsCount;       // [S]
+    const int* dsId;          // [S]
+    const double* binom;      // [2][C][tri]: dud tables then bleach tables
+    int tri;                  // (M+1)(M+2)/2
+    const int* cand_off;      // [N+1] or null (every dye-seq)
+    const int* cand_idx;      // candidate dye-seq indices
+    int* out_id;
+    double* out_adjusted;
+    double* out_log_score;
+    double* out_log_total;
+    double* out_pair_logp;    // [N][S] or null
+};
+
+// Per-timestep emission ranges then pruned boxes for every read.
+void launch_read_prep(const DevModel& m, int N, const double* d_rads, uint8_t* d_Lr, uint8_t* d_Hr,
+                      int* d_nbmax, hipStream_t stream);
+
+// Fused classify over the reads of one box-size bin. precision: WP_PRECISION_*.
+// Returns hipSuccess or the launch error.
+hipError_t launch_classify(const ClassifyArgs& a, int precision, int group_lanes, hipStream_t stream);
+
+// Per-read algorithmic work (non-empty pairs, state-timesteps, FLOPs) -> [N][3].
+void launch_work(const DevModel& m, int N, int S, const uint8_t* d_Lr, const uint8_t* d_Hr,
+                 const int* d_nbmax, const uint8_t* d_D, double* d_out, hipStream_t stream);
+
+// Bytes of dynamic LDS one classify workgroup needs.
+size_t classify_lds_bytes(int precision, int T, int C, int tri, int NB, int ngroups, int group_lanes);
+
+// Chunk of candidates compacted per pass.
+constexpr int kCandChunk = 1024;
+constexpr int kBlockThreads = 256;
+
+}  // namespace wp

@@ -1,0 +1,809 @@
+// Fused PeptideHMM forward scoring for gfx950.
+//
+// Reference path (whatprot cc_code/src): HMMClassifier::classify_helper
+// (classifiers/hmm-classifier.h:45-78) builds a PeptideHMM per (read,
+// dye-seq) pair (hmm/hmm/peptide-hmm.cc:33-84) and runs the step chain
+// InitialBlock, Dud_c, Emit(0), then per timestep t>=1: CyclicBlock,
+// Detach(t-1), Bleach_c, Edman, Emit(t) (generic-hmm.h:41-55).
+//
+// Pruning in closed form. Running the per-step prune rules of
+// peptide-hmm.cc:62-83 (block-transition.cc:22-29, binomial-transition.cc:77-89,
+// detach-transition.cc:24-34, edman-transition.cc:28-60,
+// peptide-emission.cc:198-210) symbolically gives, for the emission at
+// timestep t, the box [0, t+1) x prod_c [L[t][c], H[t][c]) with
+//   L[t][c] = max over t' >= t of lo_t'(c)          (read only)
+//   H[t][c] = min(D_c + 1, min over t' <= t of hi_t'(c))
+// where [lo_t(c), hi_t(c)) is PeptideEmission's own range. Bleach_c at
+// timestep t maps channel c from [L[t-1][c], H[t-1][c]) onto
+// [L[t][c], U[t][c]), U = min(H[t] + 1, H[t-1]); Edman reads
+// [0,t) x [L[t], U[t]) and the emission keeps [0,t+1) x [L[t], H[t]). The pair
+// is empty (P = 0) iff some L >= H. tests/test_boxes.py checks this against
+// the oracle's literal step-by-step box algebra.
+//
+// Mapping. A workgroup owns one read; its emission table, binomial tables and
+// pruned bounds sit in LDS. Pairs are scored by groups of G lanes (G >= T):
+// lane e owns Edman count e and keeps its column of the state tensors A
+// (normal) and B (blocked N-terminus) over the count box in LDS, laid out
+// [count index][lane] so a group touches G consecutive words. Block, Detach,
+// the Bleach binomials and the Emission touch only the lane's own column; the
+// Edman shift reads column e-1, the only cross-lane access, fenced at wave
+// scope. Non-empty candidates are compacted first (73% of pairs are empty at a
+// 5-sigma cutoff), so groups of one wave work on live pairs together.
+//
+// Precision. FP64 reproduces the reference's arithmetic (same operation order
+// per state). FP32 keeps the state in float and, after every timestep,
+// rescales it by an exact power of two 2^-k, accumulating k in an integer;
+// emissions are stored per timestep scaled by the power of two of their
+// maximum over the read's box. ln P, the argmax and the normaliser over
+// dye-seqs are FP64.
+#include <float.h>
+#include <math.h>
+
+#include "hmm_kernels.h"
+
+namespace wp {
+
+namespace {
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename Real, int G>
+__device__ inline Real group_sum(Real v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+    return v;
+}
+
+template <typename Real, int G>
+__device__ inline Real group_max(Real v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, G));
+    return v;
+}
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// LDS carve-up shared by host sizing and the kernel.
+struct Layout {
+    size_t rad, L, H, eoff, estr, ek, emax, etab, binom, list, scan, gres, state, total;
+};
+
+template <typename Real>
+__host__ __device__ inline Layout make_layout(int T, int C, int tri, int NB, int ngroups, int G,
+                                              int nthreads) {
+    Layout l;
+    size_t o = 0;
+    l.rad = o;   o = align16(o + sizeof(double) * T * C);
+    l.L = o;     o = align16(o + sizeof(int) * T * C);
+    l.H = o;     o = align16(o + sizeof(int) * T * C);
+    l.eoff = o;  o = align16(o + sizeof(int) * (T + 1));
+    l.estr = o;  o = align16(o + sizeof(int) * T * C);
+    l.ek = o;    o = align16(o + sizeof(int) * (T + 1));
+    l.emax = o;  o = align16(o + sizeof(unsigned long long) * T);
+    l.etab = o;  o = align16(o + sizeof(Real) * (size_t)T * NB);
+    l.binom = o; o = align16(o + sizeof(Real) * 2 * C * tri);
+    l.list = o;  o = align16(o + sizeof(int) * nthreads * 4);
+    l.scan = o;  o = align16(o + sizeof(int) * (nthreads / 64 + 2));
+    l.gres = o;  o = align16(o + sizeof(double) * 4 * ngroups);
+    l.state = o; o = align16(o + sizeof(Real) * 4 * (size_t)NB * G * ngroups);
+    l.total = o;
+    return l;
+}
+
+// Ranges of one (read, timestep).
+__global__ void range_kernel(DevModel m, int N, const double* __restrict__ rads,
+                             uint8_t* __restrict__ lo_out, uint8_t* __restrict__ hi_out) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int T = m.T, C = m.C;
+    if (idx >= N * T) return;
+    double x[WP_MAX_CHANNELS];
+    for (int c = 0; c < C; c++) x[c] = rads[(size_t)idx * C + c];
+    unsigned int lo[WP_MAX_CHANNELS], hi[WP_MAX_CHANNELS];
+    dev_emission_range(m, x, lo, hi);
+    for (int c = 0; c < C; c++) {
+        lo_out[(size_t)idx * C + c] = (uint8_t)lo[c];
+        hi_out[(size_t)idx * C + c] = (uint8_t)hi[c];
+    }
+}
+
+// Turns per-timestep ranges into the pruned bounds in place (suffix max of
+// lows, prefix min of highs capped at M+1) and the read's box bound.
+__global__ void bounds_kernel(DevModel m, int N, uint8_t* __restrict__ L, uint8_t* __restrict__ H,
+                              int* __restrict__ nbmax) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const int T = m.T, C = m.C;
+    uint8_t* l = L + (size_t)r * T * C;
+    uint8_t* h = H + (size_t)r * T * C;
+    for (int c = 0; c < C; c++) {
+        for (int t = T - 2; t >= 0; t--) {
+            uint8_t a = l[t * C + c], b = l[(t + 1) * C + c];
+            l[t * C + c] = a > b ? a : b;
+        }
+        uint8_t cap = (uint8_t)(m.M + 1);
+        if (h[c] > cap) h[c] = cap;
+        for (int t = 1; t < T; t++) {
+            uint8_t a = h[t * C + c], b = h[(t - 1) * C + c];
+            h[t * C + c] = a < b ? a : b;
+        }
+    }
+    int best = 0;
+    bool empty = false;
+    for (int t = 0; t < T; t++) {
+        int nb = 1;
+        for (int c = 0; c < C; c++) {
+            int w = (int)h[t * C + c] - (int)l[t * C + c];
+            if (w <= 0) empty = true;
+            nb *= w > 0 ? w : 0;
+        }
+        if (nb > best) best = nb;
+    }
+    nbmax[r] = empty ? 0 : best;
+}
+
+template <typename Real>
+struct PairResult {
+    double logp;  // ln P (-inf when P == 0)
+    double p;     // P itself (FP64 path; FP32 fills exp(logp))
+};
+
+// Incremental walk over a C-dim box, last channel fastest.
+template <int C>
+__device__ inline void box_step(int* n, const int* lo, const int* w) {
+#pragma unroll
+    for (int c = C - 1; c >= 0; c--) {
+        if (++n[c] < lo[c] + w[c]) return;
+        n[c] = lo[c];
+    }
+}
+
+template <int C>
+__device__ inline int box_strides(const int* w, int* str) {
+    int s = 1;
+#pragma unroll
+    for (int c = C - 1; c >= 0; c--) {
+        str[c] = s;
+        s *= w[c];
+    }
+    return s;
+}
+
+// Forward algorithm of one (read, dye-seq) pair on one group of G lanes.
+template <typename Real, int C, int G, bool kRescale>
+__device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, Real* st, int NB,
+                                       const int* sL, const int* sH, const int* eoff,
+                                       const int* estr, const Real* etab, const Real* bin,
+                                       int kread) {
+    const int T = a.m.T;
+    const int tri = a.tri;
+    int D[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) D[c] = a.dsD[(size_t)i * C + c];
+    // Edman source of this lane as a target (e' = lane, source e'-1):
+    // edman-transition.cc:97-125 gathered.
+    int ce = -1, Ke = 1;
+    if (lane >= 1 && lane < T) {
+        ce = a.dsSeq[(size_t)i * T + lane - 1];
+        if (ce >= 0) Ke = a.dsTrack[((size_t)i * T + lane - 1) * C + ce];
+    }
+    const Real pf = (Real)a.m.pf;
+    const Real one_m_pf = (Real)(1.0 - a.m.pf);
+    const Real* dud = bin;
+    const Real* ble = bin + C * tri;
+    Real* Acur = st;
+    Real* Bcur = st + NB * G;
+    Real* Anx = st + 2 * NB * G;
+    Real* Bnx = st + 3 * NB * G;
+
+    // --- t = 0: InitialBlock, Dud_c, Emit(0) -------------------------------
+    int lo[C], hi[C], w[C], str[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        lo[c] = sL[c];
+        hi[c] = min(sH[c], D[c] + 1);
+        w[c] = hi[c] - lo[c];
+    }
+    int nb = box_strides<C>(w, str);
+    {
+        int n[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) n[c] = lo[c];
+        const Real pib = (Real)a.m.p_ib;
+        for (int f = 0; f < nb; f++) {
+            Real va = (Real)1 - pib, vb = pib;
+            int ei = eoff[0];
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                Real p = dud[c * tri + D[c] * (D[c] + 1) / 2 + n[c]];
+                va = p * va;
+                vb = p * vb;
+                ei += (n[c] - sL[c]) * estr[c];
+            }
+            Real em = etab[ei];
+            Acur[f * G + lane] = lane == 0 ? va * em : (Real)0;
+            Bcur[f * G + lane] = lane == 0 ? vb * em : (Real)0;
+            box_step<C>(n, lo, w);
+        }
+    }
+    Real delta = 0;
+    int logacc = 0;
+    Real sc = 1;
+    if (kRescale && T > 1) {
+        Real mx = 0;
+        for (int f = 0; f < nb; f++) mx = fmax(mx, fmax(Acur[f * G + lane], Bcur[f * G + lane]));
+        mx = group_max<Real, G>(mx);
+        if (mx > 0) {
+            int k;
+            frexp((double)mx, &k);
+            k = max(k, -120);
+            sc = (Real)ldexp(1.0, -k);
+            logacc += k;
+        }
+    }
+
+    for (int t = 1; t < T; t++) {
+        const int* Lt = sL + t * C;
+        const int* Ht = sH + t * C;
+        int lo2[C], hi2[C], U[C], wu[C], ustr[C], w2[C], str2[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            lo2[c] = Lt[c];
+            hi2[c] = min(Ht[c], D[c] + 1);
+            U[c] = min(hi2[c] + 1, hi[c]);
+            wu[c] = U[c] - lo2[c];
+            w2[c] = hi2[c] - lo2[c];
+        }
+        box_strides<C>(wu, ustr);
+        int nb2 = box_strides<C>(w2, str2);
+
+        // CyclicBlock + Detach(t-1) in place over the previous box
+        // (block-transition.cc:31-60, detach-transition.cc:36-68).
+        {
+            const Real pcb = (Real)a.m.p_cb;
+            const Real pd = (Real)a.m.p_detach[t - 1];
+            Real sa = 0, sb = 0;
+            for (int f = 0; f < nb; f++) {
+                Real av = Acur[f * G + lane], bv = Bcur[f * G + lane];
+                if (kRescale) {
+                    av *= sc;
+                    bv *= sc;
+                }
+                Real a1 = ((Real)1 - pcb) * av;
+                Real b1 = bv + pcb * av;
+                sa += a1;
+                sb += b1;
+                Acur[f * G + lane] = a1 * ((Real)1 - pd);
+                Bcur[f * G + lane] = b1 * ((Real)1 - pd);
+            }
+            sa = group_sum<Real, G>(sa);
+            sb = group_sum<Real, G>(sb);
+            if (kRescale) delta *= sc;
+            delta = delta + pd * (sa + sb);
+        }
+
+        // Bleach_c, c = 0..C-1 (binomial-transition.cc:91-136), lane-local.
+        {
+            int ilo[C], iw[C];
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                ilo[c] = lo[c];
+                iw[c] = w[c];
+            }
+            Real* Ain = Acur;
+            Real* Bin = Bcur;
+            Real* Aout = Anx;
+            Real* Bout = Bnx;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                int istr[C], olo[C], ow[C], ostr[C];
+                box_strides<C>(iw, istr);
+#pragma unroll
+                for (int k = 0; k < C; k++) {
+                    olo[k] = ilo[k];
+                    ow[k] = iw[k];
+                }
+                olo[c] = lo2[c];
+                ow[c] = wu[c];
+                int onb = box_strides<C>(ow, ostr);
+                int n[C];
+#pragma unroll
+                for (int k = 0; k < C; k++) n[k] = olo[k];
+                const Real* P = ble + c * tri;
+                const int ihi = ilo[c] + iw[c];
+                for (int f = 0; f < onb; f++) {
+                    int base = 0;
+#pragma unroll
+                    for (int k = 0; k < C; k++)
+                        if (k != c) base += (n[k] - ilo[k]) * istr[k];
+                    const int j = n[c];
+                    Real va = 0, vb = 0;
+                    for (int from = max(j, ilo[c]); from < ihi; from++) {
+                        int idx = (base + (from - ilo[c]) * istr[c]) * G + lane;
+                        Real p = P[from * (from + 1) / 2 + j];
+                        va += p * Ain[idx];
+                        vb += p * Bin[idx];
+                    }
+                    Aout[f * G + lane] = va;
+                    Bout[f * G + lane] = vb;
+                    box_step<C>(n, olo, ow);
+                }
+#pragma unroll
+                for (int k = 0; k < C; k++) {
+                    ilo[k] = olo[k];
+                    iw[k] = ow[k];
+                }
+                Real* tA = Ain; Ain = Aout; Aout = tA;
+                Real* tB = Bin; Bin = Bout; Bout = tB;
+            }
+            Acur = Ain;
+            Bcur = Bin;
+            Anx = Aout;
+            Bnx = Bout;
+        }
+
+        // Edman (edman-transition.cc:62-162) as a gather, fused with Emit(t)
+        // (peptide-emission.cc:212-243). Reads column lane-1.
+        wave_sync();
+        {
+            int n[C];
+#pragma unroll
+            for (int c = 0; c < C; c++) n[c] = lo2[c];
+            int kc = 0, Uc = 0, sc_str = 0;
+#pragma unroll
+            for (int c = 0; c < C; c++)
+                if (c == ce) {
+                    Uc = U[c];
+                    sc_str = ustr[c];
+                }
+            const int eo = eoff[t];
+            const int* es = estr + t * C;
+            Real mx = 0;
+            for (int f = 0; f < nb2; f++) {
+                int idx0 = 0, ei = eo;
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    idx0 += (n[c] - lo2[c]) * ustr[c];
+                    ei += (n[c] - Lt[c]) * es[c];
+                    if (c == ce) kc = n[c];
+                }
+                Real acc = 0;
+                if (lane >= 1 && lane <= t) {
+                    Real a1 = Acur[idx0 * G + lane - 1];
+                    if (ce < 0) {
+                        acc += one_m_pf * a1;
+                    } else {
+                        if (kc < Ke) {
+                            Real ratio = (Real)kc / (Real)Ke;
+                            acc += one_m_pf * ((Real)1 - ratio) * a1;
+                        }
+                        if (kc + 1 < Uc) {
+                            Real a2 = Acur[(idx0 + sc_str) * G + lane - 1];
+                            Real ratio2 = (Real)(kc + 1) / (Real)Ke;
+                            acc += one_m_pf * ratio2 * a2;
+                        }
+                    }
+                }
+                Real a0 = lane < t ? Acur[idx0 * G + lane] : (Real)0;
+                acc += pf * a0;
+                Real b0 = lane < t ? Bcur[idx0 * G + lane] : (Real)0;
+                if (lane > t) acc = 0;
+                Real em = etab[ei];
+                Real ao = acc * em, bo = b0 * em;
+                Anx[f * G + lane] = ao;
+                Bnx[f * G + lane] = bo;
+                if (kRescale) mx = fmax(mx, fmax(ao, bo));
+                box_step<C>(n, lo2, w2);
+            }
+            bool allow = true;
+#pragma unroll
+            for (int c = 0; c < C; c++) allow = allow && lo2[c] == 0;
+            if (allow) {
+                int ei0 = eo;
+#pragma unroll
+                for (int c = 0; c < C; c++) ei0 += (0 - Lt[c]) * es[c];
+                delta = delta * etab[ei0];
+            } else {
+                delta = 0;
+            }
+            if (kRescale && t + 1 < T) {
+                mx = group_max<Real, G>(mx);
+                mx = fmax(mx, delta);
+                sc = 1;
+                if (mx > 0) {
+                    int k;
+                    frexp((double)mx, &k);
+                    k = max(k, -120);
+                    sc = (Real)ldexp(1.0, -k);
+                    logacc += k;
+                }
+            }
+        }
+        wave_sync();
+        Real* tA = Acur; Acur = Anx; Anx = tA;
+        Real* tB = Bcur; Bcur = Bnx; Bnx = tB;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            lo[c] = lo2[c];
+            hi[c] = hi2[c];
+            w[c] = w2[c];
+            str[c] = str2[c];
+        }
+        nb = nb2;
+    }
+
+    // PeptideStateVector::sum (peptide-state-vector.cc:52-54).
+    Real sa = 0, sb = 0;
+    for (int f = 0; f < nb; f++) {
+        sa += Acur[f * G + lane];
+        sb += Bcur[f * G + lane];
+    }
+    sa = group_sum<Real, G>(sa);
+    sb = group_sum<Real, G>(sb);
+    PairResult<Real> res;
+    if (kRescale) {
+        double s = (double)sa + (double)sb + (double)delta;
+        res.logp = s > 0 ? log(s) + (double)(logacc + kread) * 0.69314718055994530942 : -INFINITY;
+        res.p = exp(res.logp);
+    } else {
+        double s = ((double)sa + (double)sb) + (double)delta;
+        res.p = s;
+        res.logp = log(s);
+    }
+    return res;
+}
+
+template <typename Real, int C, int G>
+__global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyArgs a) {
+    constexpr bool kRescale = sizeof(Real) == 4;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int r = blockIdx.x;
+    if (r >= a.N) return;
+    const int nbr = a.nbmax[r];
+    // Empty reads go with the first bin (nb_lo == 0 there).
+    if (!((nbr > a.nb_lo && nbr <= a.nb_hi) || (nbr == 0 && a.nb_lo == 0))) return;
+
+    const int T = a.m.T;
+    const int M = a.m.M;
+    const int tid = threadIdx.x;
+    const int nthreads = blockDim.x;
+    const int ngroups = a.ngroups;
+    const int group = tid / G;
+    const int lane = tid % G;
+    const Layout lay = make_layout<Real>(T, C, a.tri, a.NB, ngroups, G, nthreads);
+    double* s_rad = (double*)(smem + lay.rad);
+    int* sL = (int*)(smem + lay.L);
+    int* sH = (int*)(smem + lay.H);
+    int* eoff = (int*)(smem + lay.eoff);
+    int* estr = (int*)(smem + lay.estr);
+    int* ek = (int*)(smem + lay.ek);
+    unsigned long long* emax = (unsigned long long*)(smem + lay.emax);
+    Real* etab = (Real*)(smem + lay.etab);
+    Real* bin = (Real*)(smem + lay.binom);
+    int* list = (int*)(smem + lay.list);
+    int* scan = (int*)(smem + lay.scan);
+    double* gres = (double*)(smem + lay.gres);
+    Real* state = (Real*)(smem + lay.state);
+
+    const int ncand = a.cand_off ? a.cand_off[r + 1] - a.cand_off[r] : a.S;
+    const int* cidx = a.cand_off ? a.cand_idx + a.cand_off[r] : nullptr;
+
+    // ---- prologue: read data, bounds, emission table, binomial tables ----
+    for (int k = tid; k < T * C; k += nthreads) {
+        s_rad[k] = a.rads[(size_t)r * T * C + k];
+        sL[k] = a.Lr[(size_t)r * T * C + k];
+        sH[k] = a.Hr[(size_t)r * T * C + k];
+    }
+    for (int k = tid; k < 2 * C * a.tri; k += nthreads) bin[k] = (Real)a.binom[k];
+    for (int k = tid; k < T; k += nthreads) emax[k] = 0ull;
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int t = 0; t < T; t++) {
+            eoff[t] = acc;
+            int s = 1;
+            for (int c = C - 1; c >= 0; c--) {
+                estr[t * C + c] = s;
+                int wdt = sH[t * C + c] - sL[t * C + c];
+                s *= wdt > 0 ? wdt : 0;
+            }
+            acc += s;
+        }
+        eoff[T] = acc;
+    }
+    __syncthreads();
+    const int nent = eoff[T];
+    // emission densities over each timestep's box (peptide-emission.cc:165-180)
+    for (int pass = 0; pass < (kRescale ? 2 : 1); pass++) {
+        for (int k = tid; k < nent; k += nthreads) {
+            int t = 0;
+            while (eoff[t + 1] <= k) t++;
+            int rem = k - eoff[t];
+            unsigned int n[WP_MAX_CHANNELS];
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                int s = estr[t * C + c];
+                n[c] = (unsigned int)(sL[t * C + c] + rem / s);
+                rem %= s;
+            }
+            double v = 1.0;
+#pragma unroll
+            for (int c = 0; c < C; c++) v *= dev_pdf(a.m, c, s_rad[t * C + c], n);
+            if (!kRescale) {
+                etab[k] = (Real)v;
+            } else if (pass == 0) {
+                atomicMax(&emax[t], (unsigned long long)__double_as_longlong(v));
+            } else {
+                etab[k] = (Real)ldexp(v, -ek[t]);
+            }
+        }
+        if (kRescale && pass == 0) {
+            __syncthreads();
+            if (tid == 0) {
+                int tot = 0;
+                for (int t = 0; t < T; t++) {
+                    double mxv = __longlong_as_double((long long)emax[t]);
+                    int k = 0;
+                    if (mxv > 0) frexp(mxv, &k);
+                    ek[t] = k;
+                    tot += k;
+                }
+                ek[T] = tot;
+            }
+        }
+        __syncthreads();
+    }
+    const int kread = kRescale ? ek[T] : 0;
+    bool read_empty = (nbr == 0);
+    // The pair is non-empty iff D_c >= L[0][c] for every channel.
+
+    // ---- per-group running results ----
+    double best_v = kRescale ? -INFINITY : 0.0;  // ln P (FP32) or P (FP64)
+    int best_k = 0;
+    double tot_a = kRescale ? -INFINITY : 0.0;  // FP64: total; FP32: running max
+    double tot_b = 0.0;                           // FP32: running scaled sum
+    Real* st = state + (size_t)group * 4 * a.NB * G;
+
+    for (int base = 0; base < ncand; base += nthreads * 4) {
+        // compact the non-empty candidates of this chunk, keeping order
+        int cnt = 0;
+        int kk0 = base + tid * 4;
+        int flags = 0;
+        for (int q = 0; q < 4; q++) {
+            int kk = kk0 + q;
+            if (kk >= ncand) break;
+            int i = cidx ? cidx[kk] : kk;
+            bool ok = !read_empty;
+            if (ok) {
+                for (int c = 0; c < C; c++) ok = ok && (int)a.dsD[(size_t)i * C + c] >= sL[c];
+            }
+            if (ok) {
+                flags |= 1 << q;
+                cnt++;
+            } else if (a.out_pair_logp && !cidx) {
+                a.out_pair_logp[(size_t)r * a.S + i] = -INFINITY;
+            }
+        }
+        // block exclusive scan of cnt
+        int incl = cnt;
+        const int wl = tid & 63, wv = tid >> 6;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(incl, off, 64);
+            if (wl >= off) incl += y;
+        }
+        if (wl == 63) scan[wv] = incl;
+        __syncthreads();
+        if (tid == 0) {
+            int acc = 0;
+            for (int q = 0; q < nthreads / 64; q++) {
+                int v = scan[q];
+                scan[q] = acc;
+                acc += v;
+            }
+            scan[nthreads / 64] = acc;
+        }
+        __syncthreads();
+        int pos = scan[wv] + incl - cnt;
+        for (int q = 0; q < 4; q++)
+            if (flags & (1 << q)) list[pos++] = kk0 + q;
+        const int nlive = scan[nthreads / 64];
+        __syncthreads();
+        if (group < ngroups) {
+            for (int q = group; q < nlive; q += ngroups) {
+                const int kk = list[q];
+                const int i = cidx ? cidx[kk] : kk;
+                PairResult<Real> pr = score_pair<Real, C, G, kRescale>(a, i, lane, st, a.NB, sL, sH, eoff,
+                                                                        estr, etab, bin, kread);
+                if (a.out_pair_logp && !cidx && lane == 0) a.out_pair_logp[(size_t)r * a.S + i] = pr.logp;
+                const int cntw = a.dsCount[i];
+                if (kRescale) {
+                    if (pr.logp > best_v || (pr.logp == best_v && kk < best_k)) {
+                        best_v = pr.logp;
+                        best_k = kk;
+                    }
+                    if (cntw > 0 && pr.logp > -INFINITY) {
+                        double x = pr.logp + log((double)cntw);
+                        if (x > tot_a) {
+                            tot_b = tot_b * exp(tot_a - x) + 1.0;
+                            tot_a = x;
+                        } else {
+                            tot_b += exp(x - tot_a);
+                        }
+                    }
+                } else {
+                    // hmm-classifier.h:59-65 (strict >, first candidate wins)
+                    tot_a += pr.p * cntw;
+                    if (pr.p > best_v || (pr.p == best_v && kk < best_k)) {
+                        best_v = pr.p;
+                        best_k = kk;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (group < ngroups && lane == 0) {
+        gres[group * 4 + 0] = best_v;
+        gres[group * 4 + 1] = (double)best_k;
+        gres[group * 4 + 2] = tot_a;
+        gres[group * 4 + 3] = tot_b;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double bv = gres[0];
+        int bk = (int)gres[1];
+        double ta = gres[2], tb = gres[3];
+        for (int g = 1; g < ngroups; g++) {
+            double v = gres[g * 4 + 0];
+            int k = (int)gres[g * 4 + 1];
+            if (v > bv || (v == bv && k < bk)) {
+                bv = v;
+                bk = k;
+            }
+            if (kRescale) {
+                double xa = gres[g * 4 + 2], xb = gres[g * 4 + 3];
+                if (xa > ta) {
+                    tb = tb * exp(ta - xa) + xb;
+                    ta = xa;
+                } else if (xa > -INFINITY) {
+                    tb += xb * exp(xa - ta);
+                }
+            } else {
+                ta += gres[g * 4 + 2];
+            }
+        }
+        double log_score, log_total, adjusted;
+        if (kRescale) {
+            log_score = bv;
+            log_total = tb > 0 ? ta + log(tb) : -INFINITY;
+            adjusted = exp(log_score - log_total);
+        } else {
+            log_score = log(bv);
+            log_total = log(ta);
+            adjusted = bv / ta;
+        }
+        if (isnan(adjusted)) {  // hmm-classifier.h:73-76
+            adjusted = 0.0;
+            log_score = -INFINITY;
+            log_total = 0.0;
+        }
+        int id = -1;
+        if (ncand > 0) id = a.dsId[cidx ? cidx[bk] : bk];
+        a.out_id[r] = id;
+        if (a.out_adjusted) a.out_adjusted[r] = adjusted;
+        if (a.out_log_score) a.out_log_score[r] = log_score;
+        if (a.out_log_total) a.out_log_total[r] = log_total;
+    }
+}
+
+template <typename Real, int C, int G>
+hipError_t launch_t(const ClassifyArgs& a, hipStream_t stream) {
+    int nthreads = a.ngroups * G;
+    if (nthreads < 64) nthreads = 64;
+    nthreads = (nthreads + 63) / 64 * 64;
+    Layout l = make_layout<Real>(a.m.T, C, a.tri, a.NB, a.ngroups, G, nthreads);
+    hipLaunchKernelGGL((classify_kernel<Real, C, G>), dim3(a.N), dim3(nthreads), l.total, stream, a);
+    return hipGetLastError();
+}
+
+template <typename Real, int C>
+hipError_t launch_c(const ClassifyArgs& a, int G, hipStream_t stream) {
+    switch (G) {
+    case 16: return launch_t<Real, C, 16>(a, stream);
+    case 32: return launch_t<Real, C, 32>(a, stream);
+    case 64: return launch_t<Real, C, 64>(a, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename Real>
+hipError_t launch_r(const ClassifyArgs& a, int G, hipStream_t stream) {
+    switch (a.m.C) {
+    case 1: return launch_c<Real, 1>(a, G, stream);
+    case 2: return launch_c<Real, 2>(a, G, stream);
+    case 3: return launch_c<Real, 3>(a, G, stream);
+    case 4: return launch_c<Real, 4>(a, G, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Algorithmic work of a batch: per read, the number of non-empty pairs, the
+// pruned state-timesteps sum_t S_t with S_t = (t+1) prod_c (H - L), and the
+// FLOP count F = S_0 (5 + sum_c 2(D_c+2)) + sum_{t>=1} S_t (17 + sum_c 2(D_c+2))
+// (emission 2, block 3, detach 4, Edman 8, binomial 2(D_c+2) per state).
+__global__ void work_kernel(DevModel m, int N, int S, const uint8_t* __restrict__ Lr,
+                            const uint8_t* __restrict__ Hr, const int* __restrict__ nbmax,
+                            const uint8_t* __restrict__ dsD, double* __restrict__ out) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const int T = m.T, C = m.C;
+    double pairs = 0, states = 0, flops = 0;
+    if (nbmax[r] > 0) {
+        const uint8_t* l = Lr + (size_t)r * T * C;
+        const uint8_t* h = Hr + (size_t)r * T * C;
+        for (int i = 0; i < S; i++) {
+            bool ok = true;
+            int bino = 0;
+            for (int c = 0; c < C; c++) {
+                int D = dsD[(size_t)i * C + c];
+                ok = ok && D >= l[c];
+                bino += 2 * (D + 2);
+            }
+            if (!ok) continue;
+            pairs += 1;
+            for (int t = 0; t < T; t++) {
+                double nb = t + 1;
+                for (int c = 0; c < C; c++) {
+                    int D = dsD[(size_t)i * C + c];
+                    int hh = min((int)h[t * C + c], D + 1);
+                    nb *= (double)(hh - (int)l[t * C + c]);
+                }
+                states += nb;
+                flops += nb * (double)((t == 0 ? 5 : 17) + bino);
+            }
+        }
+    }
+    out[(size_t)r * 3 + 0] = pairs;
+    out[(size_t)r * 3 + 1] = states;
+    out[(size_t)r * 3 + 2] = flops;
+}
+
+}  // namespace
+
+void launch_read_prep(const DevModel& m, int N, const double* d_rads, uint8_t* d_Lr, uint8_t* d_Hr,
+                      int* d_nbmax, hipStream_t stream) {
+    int total = N * m.T;
+    if (total > 0) {
+        hipLaunchKernelGGL(range_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, m, N, d_rads,
+                           d_Lr, d_Hr);
+        hipLaunchKernelGGL(bounds_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, m, N, d_Lr, d_Hr,
+                           d_nbmax);
+    }
+}
+
+hipError_t launch_classify(const ClassifyArgs& a, int precision, int group_lanes, hipStream_t stream) {
+    if (a.N <= 0) return hipSuccess;
+    if (precision == WP_PRECISION_FP32) return launch_r<float>(a, group_lanes, stream);
+    return launch_r<double>(a, group_lanes, stream);
+}
+
+void launch_work(const DevModel& m, int N, int S, const uint8_t* d_Lr, const uint8_t* d_Hr,
+                 const int* d_nbmax, const uint8_t* d_D, double* d_out, hipStream_t stream) {
+    if (N > 0)
+        hipLaunchKernelGGL(work_kernel, dim3((N + 127) / 128), dim3(128), 0, stream, m, N, S, d_Lr, d_Hr,
+                           d_nbmax, d_D, d_out);
+}
+
+size_t classify_lds_bytes(int precision, int T, int C, int tri, int NB, int ngroups, int G) {
+    int nthreads = ngroups * G;
+    if (nthreads < 64) nthreads = 64;
+    nthreads = (nthreads + 63) / 64 * 64;
+    if (precision == WP_PRECISION_FP32) return make_layout<float>(T, C, tri, NB, ngroups, G, nthreads).total;
+    return make_layout<double>(T, C, tri, NB, ngroups, G, nthreads).total;
+}
+
+}  // namespace wp

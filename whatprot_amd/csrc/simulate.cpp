@@ -1,0 +1,174 @@
+// Seeded synthetic-input generator: random dye-seqs and radiometries drawn
+// from the sequencing model's forward process. Restates the generative model
+// of whatprot's simulator (simulation/generate-dye-track.cc:29-110,
+// generate-radiometry.cc:28-62, generate-radiometries.cc:30-64) with a
+// portable counter-free RNG (xoshiro256** seeded by splitmix64) instead of the
+// reference's clock-seeded std::default_random_engine, so runs are
+// reproducible across machines. Streams therefore differ from the reference;
+// the distribution is the same.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "wp_internal.h"
+
+namespace wp {
+
+namespace {
+
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed) {
+        uint64_t x = seed;
+        for (int i = 0; i < 4; i++) {
+            x += 0x9e3779b97f4a7c15ull;
+            uint64_t z = x;
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+            s[i] = z ^ (z >> 31);
+        }
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next() {
+        uint64_t r = rotl(s[1] * 5, 7) * 9;
+        uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+    bool bernoulli(double p) { return uniform() < p; }
+    // uniform integer in [lo, hi]
+    int range(int lo, int hi) { return lo + (int)(next() % (uint64_t)(hi - lo + 1)); }
+    double normal() {  // Marsaglia polar method
+        for (;;) {
+            double u = 2.0 * uniform() - 1.0, v = 2.0 * uniform() - 1.0;
+            double s2 = u * u + v * v;
+            if (s2 > 0.0 && s2 < 1.0) return u * std::sqrt(-2.0 * std::log(s2) / s2);
+        }
+    }
+};
+
+}  // namespace
+
+// Dye-seq generator: length ~ U{len_min..len_max}; per-channel dye counts
+// k_c ~ U{k_min..k_max} capped so the total fits the length and redrawn while
+// the total is zero; dye positions uniform without replacement.
+void generate_dye_seqs(int C, int n, int len_min, int len_max, int k_min, int k_max,
+                       uint64_t seed, char* out, int stride) {
+    Rng rng(seed);
+    for (int i = 0; i < n; i++) {
+        int len = rng.range(len_min, len_max);
+        std::vector<int> k(C);
+        int total;
+        do {
+            total = 0;
+            for (int c = 0; c < C; c++) {
+                k[c] = rng.range(k_min, k_max);
+                if (total + k[c] > len) k[c] = len - total;
+                total += k[c];
+            }
+        } while (total == 0);
+        std::vector<char> s(len, '.');
+        std::vector<int> free_pos(len);
+        for (int p = 0; p < len; p++) free_pos[p] = p;
+        int nfree = len;
+        for (int c = 0; c < C; c++) {
+            for (int j = 0; j < k[c]; j++) {
+                int pick = rng.range(0, nfree - 1);
+                s[free_pos[pick]] = (char)('0' + c);
+                free_pos[pick] = free_pos[--nfree];
+            }
+        }
+        char* dst = out + (size_t)i * stride;
+        std::memset(dst, 0, stride);
+        std::memcpy(dst, s.data(), std::min(len, stride - 1));
+    }
+}
+
+// One dye track following generate-dye-track.cc:29-110 (duds; initial block;
+// per cycle: record, cyclic block, detach, Edman unless blocked, bleach).
+static void dye_track(const wp_model& m, const short* seq0, int len, int T, int C, Rng& rng,
+                      unsigned int* track) {
+    std::vector<short> ds(seq0, seq0 + len);
+    for (int i = 0; i < len; i++)
+        if (ds[i] != -1 && rng.bernoulli(m.channels[ds[i]].p_dud)) ds[i] = -1;
+    int counts[WP_MAX_CHANNELS] = {0};
+    for (int i = 0; i < len; i++)
+        if (ds[i] != -1) counts[ds[i]]++;
+    int e = 0;
+    bool block = rng.bernoulli(m.p_initial_block);
+    for (int t = 0; t < T; t++) {
+        if (e < len) {
+            for (int c = 0; c < C; c++) track[t * C + c] = (unsigned int)counts[c];
+            if (rng.bernoulli(m.p_cyclic_block)) block = true;
+            double pd = m.p_detach + m.p_initial_detach * std::exp(-(double)t * m.p_initial_detach_decay);
+            if (rng.bernoulli(pd)) {
+                e = len;
+                continue;
+            }
+            if (!block) {
+                if (!rng.bernoulli(m.p_edman_failure)) {
+                    if (ds[e] != -1) counts[ds[e]]--;
+                    e++;
+                }
+            }
+            for (int i = e; i < len; i++) {
+                if (ds[i] != -1 && rng.bernoulli(m.channels[ds[i]].p_bleach)) {
+                    counts[ds[i]]--;
+                    ds[i] = -1;
+                }
+            }
+        } else {
+            for (int c = 0; c < C; c++) track[t * C + c] = 0;
+        }
+    }
+}
+
+// generate-radiometries.cc:30-64 with the visibility filter of
+// generate-radiometry.cc:41-53, repeated until exactly n visible reads exist.
+// Values are rounded to float32 (the precision of the synthetic files).
+void simulate_radiometries(const wp_model& m, int T, int C, int nds, const short* seqs, int cap,
+                           const int* lens, const int* weights, int n, uint64_t seed,
+                           double* out, int* out_idx, unsigned int* out_tracks) {
+    Rng rng(seed);
+    std::vector<double> cum(nds);
+    double acc = 0.0;
+    for (int i = 0; i < nds; i++) {
+        acc += weights ? (double)weights[i] : 1.0;
+        cum[i] = acc;
+    }
+    std::vector<unsigned int> track((size_t)T * C);
+    int made = 0;
+    while (made < n) {
+        double u = rng.uniform() * acc;
+        int i = (int)(std::upper_bound(cum.begin(), cum.end(), u) - cum.begin());
+        if (i >= nds) i = nds - 1;
+        dye_track(m, seqs + (size_t)i * cap, lens[i], T, C, rng, track.data());
+        bool trivial = true;
+        for (int c = 0; c < C; c++)
+            if (track[c] != 0) trivial = false;
+        if (trivial) continue;
+        double* r = out + (size_t)made * T * C;
+        for (int t = 0; t < T; t++) {
+            for (int c = 0; c < C; c++) {
+                unsigned int cnt[WP_MAX_CHANNELS];
+                for (int k = 0; k < C; k++) cnt[k] = track[t * C + k];
+                double mu = adjusted_mu(m, c, cnt);
+                double sd = sigma(m, c, mu);
+                r[t * C + c] = (double)(float)(mu + sd * rng.normal());
+            }
+        }
+        if (out_idx) out_idx[made] = i;
+        if (out_tracks) std::memcpy(out_tracks + (size_t)made * T * C, track.data(), sizeof(unsigned int) * T * C);
+        made++;
+    }
+}
+
+}  // namespace wp

@@ -1,0 +1,50 @@
+// Internal declarations shared by the host-side translation units of
+// libwhatprot_amd. Not part of the C ABI.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "../../include/whatprot_amd.h"
+
+namespace wp {
+
+void set_error(const std::string& msg);
+
+// ChannelModel::adjusted_mu (channel-model.h:35-55), host side.
+inline double adjusted_mu(const wp_model& m, int channel, const unsigned int* counts) {
+    const wp_channel_model& cm = m.channels[channel];
+    double amu = cm.mu;
+    amu *= (double)counts[channel];
+    for (int c = 0; c < m.num_channels; c++) {
+        if (c == channel) {
+            if (counts[c] > 1) {
+                amu *= cm.flat_interactions[c];
+                amu *= std::pow(cm.interactions[c], (double)(counts[c] - 1));
+            }
+        } else if (counts[c] > 0) {
+            amu *= cm.flat_interactions[c];
+            amu *= std::pow(cm.interactions[c], (double)counts[c]);
+        }
+    }
+    return amu;
+}
+
+// ChannelModel::sigma (channel-model.cc:47-49), host side.
+inline double sigma(const wp_model& m, int channel, double amu) {
+    const wp_channel_model& cm = m.channels[channel];
+    return std::sqrt(cm.bg_sig * cm.bg_sig + amu * cm.sig * cm.sig);
+}
+
+// DyeSeq(C, string) (dye-seq.cc:23-40): returns the length after trimming
+// trailing '.'; seq[i] = channel digit or -1.
+int parse_dye_seq(const char* s, short* seq, int cap);
+
+void generate_dye_seqs(int C, int n, int len_min, int len_max, int k_min, int k_max,
+                       uint64_t seed, char* out, int stride);
+void simulate_radiometries(const wp_model& m, int T, int C, int nds, const short* seqs, int cap,
+                           const int* lens, const int* weights, int n, uint64_t seed,
+                           double* out, int* out_idx, unsigned int* out_tracks);
+
+}  // namespace wp
