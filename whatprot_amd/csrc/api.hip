@@ -69,7 +69,21 @@ struct Bin {
 };
 
 constexpr size_t kLdsBudget = 48 * 1024;
-constexpr size_t kLdsHardLimit = 64 * 1024;
+
+// Largest dynamic LDS one workgroup may take on the current device (160 KB
+// on gfx950), queried once.
+size_t lds_hard_limit() {
+    static size_t lim = 0;
+    if (!lim) {
+        int dev = 0;
+        hipDeviceProp_t p;
+        lim = 64 * 1024;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess &&
+            p.sharedMemPerBlock > lim)
+            lim = p.sharedMemPerBlock;
+    }
+    return lim;
+}
 
 // Bins of reads by box bound, each with its LDS sizing.
 std::vector<Bin> make_bins(const wp_hmm_classifier* c) {
@@ -304,7 +318,7 @@ int wp_hmm_classifier_create(unsigned int num_timesteps, unsigned int num_channe
     // every bin must fit at least one pair group
     for (const Bin& b : make_bins(c)) {
         size_t need = wp::classify_lds_bytes(precision, T, C, c->tri, b.hi, b.ngroups, c->G);
-        if (need > kLdsHardLimit) {
+        if (need > lds_hard_limit()) {
             wp::set_error("state box too large for LDS at this precision (max_num_dyes/channels)");
             delete c;
             return WP_ERR_UNSUPPORTED;

@@ -65,11 +65,31 @@ __device__ inline Real group_max(Real v) {
     return v;
 }
 
+template <int G>
+__device__ inline int group_max_int(int v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, G));
+    return v;
+}
+
+// Exponent e with x = m 2^e, m in [0.5, 1); kNoExp for zero.
+constexpr int kNoExp = -100000;
+__device__ inline int exp_of(float x) {
+    int e;
+    frexpf(x, &e);
+    return x > 0.0f ? e : kNoExp;
+}
+__device__ inline int exp_of(double x) {
+    int e;
+    frexp(x, &e);
+    return x > 0.0 ? e : kNoExp;
+}
+
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // LDS carve-up shared by host sizing and the kernel.
 struct Layout {
-    size_t rad, L, H, eoff, estr, ek, emax, etab, binom, list, scan, gres, state, total;
+    size_t rad, L, H, eoff, estr, etab, eexp, binom, list, scan, gres, state, total;
 };
 
 template <typename Real>
@@ -82,9 +102,8 @@ __host__ __device__ inline Layout make_layout(int T, int C, int tri, int NB, int
     l.H = o;     o = align16(o + sizeof(int) * T * C);
     l.eoff = o;  o = align16(o + sizeof(int) * (T + 1));
     l.estr = o;  o = align16(o + sizeof(int) * T * C);
-    l.ek = o;    o = align16(o + sizeof(int) * (T + 1));
-    l.emax = o;  o = align16(o + sizeof(unsigned long long) * T);
     l.etab = o;  o = align16(o + sizeof(Real) * (size_t)T * NB);
+    l.eexp = o;  o = align16(o + (sizeof(Real) == 4 ? sizeof(short) * (size_t)T * NB : 0));
     l.binom = o; o = align16(o + sizeof(Real) * 2 * C * tri);
     l.list = o;  o = align16(o + sizeof(int) * nthreads * 4);
     l.scan = o;  o = align16(o + sizeof(int) * (nthreads / 64 + 2));
@@ -131,16 +150,21 @@ __global__ void bounds_kernel(DevModel m, int N, uint8_t* __restrict__ L, uint8_
             h[t * C + c] = a < b ? a : b;
         }
     }
+    // Capacity per pair: the emission boxes, and for t >= 1 the union box
+    // prod_c (H[t-1] - L[t]) that contains the Bleach intermediates and the
+    // Edman input [L[t], U[t]) (U <= H[t-1]).
     int best = 0;
     bool empty = false;
     for (int t = 0; t < T; t++) {
-        int nb = 1;
+        int nb = 1, nu = 1;
         for (int c = 0; c < C; c++) {
             int w = (int)h[t * C + c] - (int)l[t * C + c];
             if (w <= 0) empty = true;
             nb *= w > 0 ? w : 0;
+            if (t > 0) nu *= (int)h[(t - 1) * C + c] - (int)l[t * C + c];
         }
         if (nb > best) best = nb;
+        if (t > 0 && nu > best) best = nu;
     }
     nbmax[r] = empty ? 0 : best;
 }
@@ -172,12 +196,22 @@ __device__ inline int box_strides(const int* w, int* str) {
     return s;
 }
 
+// Emission density of table entry ei. FP64 keeps the reference's value; FP32
+// stores mantissa and exponent separately and returns the density times
+// 2^-kmax, kmax being the largest exponent over the pair's count box at that
+// timestep, so no candidate's box underflows whatever the read's range.
+template <typename Real, bool kRescale>
+__device__ inline Real emission(const Real* etab, const short* eexp, int ei, int kmax) {
+    if constexpr (kRescale) return ldexpf(etab[ei], (int)eexp[ei] - kmax);
+    else return etab[ei];
+}
+
 // Forward algorithm of one (read, dye-seq) pair on one group of G lanes.
 template <typename Real, int C, int G, bool kRescale>
 __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, Real* st, int NB,
                                        const int* sL, const int* sH, const int* eoff,
-                                       const int* estr, const Real* etab, const Real* bin,
-                                       int kread) {
+                                       const int* estr, const Real* etab, const short* eexp,
+                                       const Real* bin) {
     const int T = a.m.T;
     const int tri = a.tri;
     int D[C];
@@ -208,11 +242,33 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
         w[c] = hi[c] - lo[c];
     }
     int nb = box_strides<C>(w, str);
+    int logacc = 0;
     {
+        const Real pib = (Real)a.m.p_ib;
+        int K0 = 0;
+        if (kRescale) {
+            // largest exponent of mass x emission over the box
+            int n[C];
+#pragma unroll
+            for (int c = 0; c < C; c++) n[c] = lo[c];
+            int km = kNoExp;
+            for (int f = 0; f < nb; f++) {
+                Real v = 1;
+                int ei = eoff[0];
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    v = dud[c * tri + D[c] * (D[c] + 1) / 2 + n[c]] * v;
+                    ei += (n[c] - sL[c]) * estr[c];
+                }
+                km = max(km, exp_of(v) + (int)eexp[ei]);
+                box_step<C>(n, lo, w);
+            }
+            K0 = km == kNoExp ? 0 : km;
+            logacc += K0;
+        }
         int n[C];
 #pragma unroll
         for (int c = 0; c < C; c++) n[c] = lo[c];
-        const Real pib = (Real)a.m.p_ib;
         for (int f = 0; f < nb; f++) {
             Real va = (Real)1 - pib, vb = pib;
             int ei = eoff[0];
@@ -223,27 +279,13 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
                 vb = p * vb;
                 ei += (n[c] - sL[c]) * estr[c];
             }
-            Real em = etab[ei];
+            Real em = emission<Real, kRescale>(etab, eexp, ei, K0);
             Acur[f * G + lane] = lane == 0 ? va * em : (Real)0;
             Bcur[f * G + lane] = lane == 0 ? vb * em : (Real)0;
             box_step<C>(n, lo, w);
         }
     }
     Real delta = 0;
-    int logacc = 0;
-    Real sc = 1;
-    if (kRescale && T > 1) {
-        Real mx = 0;
-        for (int f = 0; f < nb; f++) mx = fmax(mx, fmax(Acur[f * G + lane], Bcur[f * G + lane]));
-        mx = group_max<Real, G>(mx);
-        if (mx > 0) {
-            int k;
-            frexp((double)mx, &k);
-            k = max(k, -120);
-            sc = (Real)ldexp(1.0, -k);
-            logacc += k;
-        }
-    }
 
     for (int t = 1; t < T; t++) {
         const int* Lt = sL + t * C;
@@ -268,10 +310,6 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
             Real sa = 0, sb = 0;
             for (int f = 0; f < nb; f++) {
                 Real av = Acur[f * G + lane], bv = Bcur[f * G + lane];
-                if (kRescale) {
-                    av *= sc;
-                    bv *= sc;
-                }
                 Real a1 = ((Real)1 - pcb) * av;
                 Real b1 = bv + pcb * av;
                 sa += a1;
@@ -281,7 +319,6 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
             }
             sa = group_sum<Real, G>(sa);
             sb = group_sum<Real, G>(sb);
-            if (kRescale) delta *= sc;
             delta = delta + pd * (sa + sb);
         }
 
@@ -361,7 +398,7 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
                 }
             const int eo = eoff[t];
             const int* es = estr + t * C;
-            Real mx = 0;
+            int km = kNoExp;  // FP32: largest exponent of mass x emission
             for (int f = 0; f < nb2; f++) {
                 int idx0 = 0, ei = eo;
 #pragma unroll
@@ -391,35 +428,47 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
                 acc += pf * a0;
                 Real b0 = lane < t ? Bcur[idx0 * G + lane] : (Real)0;
                 if (lane > t) acc = 0;
-                Real em = etab[ei];
-                Real ao = acc * em, bo = b0 * em;
-                Anx[f * G + lane] = ao;
-                Bnx[f * G + lane] = bo;
-                if (kRescale) mx = fmax(mx, fmax(ao, bo));
+                if constexpr (kRescale) {
+                    // store the mass; the emission is applied below once the
+                    // common exponent is known
+                    Anx[f * G + lane] = acc;
+                    Bnx[f * G + lane] = b0;
+                    km = max(km, max(exp_of(acc), exp_of(b0)) + (int)eexp[ei]);
+                } else {
+                    Real em = etab[ei];
+                    Anx[f * G + lane] = acc * em;
+                    Bnx[f * G + lane] = b0 * em;
+                }
                 box_step<C>(n, lo2, w2);
             }
             bool allow = true;
 #pragma unroll
             for (int c = 0; c < C; c++) allow = allow && lo2[c] == 0;
-            if (allow) {
-                int ei0 = eo;
+            int ei0 = eo;
 #pragma unroll
-                for (int c = 0; c < C; c++) ei0 += (0 - Lt[c]) * es[c];
-                delta = delta * etab[ei0];
-            } else {
-                delta = 0;
-            }
-            if (kRescale && t + 1 < T) {
-                mx = group_max<Real, G>(mx);
-                mx = fmax(mx, delta);
-                sc = 1;
-                if (mx > 0) {
-                    int k;
-                    frexp((double)mx, &k);
-                    k = max(k, -120);
-                    sc = (Real)ldexp(1.0, -k);
-                    logacc += k;
+            for (int c = 0; c < C; c++) ei0 += (0 - Lt[c]) * es[c];
+            if constexpr (kRescale) {
+                // Emit(t) with exact power-of-two normalisation: K is the
+                // largest exponent of (mass x density) over the state and the
+                // detached scalar; every product is multiplied by 2^-K.
+                km = group_max_int<G>(km);
+                if (allow) km = max(km, exp_of(delta) + (int)eexp[ei0]);
+                const int K = km == kNoExp ? 0 : km;
+                logacc += K;
+#pragma unroll
+                for (int c = 0; c < C; c++) n[c] = lo2[c];
+                for (int f = 0; f < nb2; f++) {
+                    int ei = eo;
+#pragma unroll
+                    for (int c = 0; c < C; c++) ei += (n[c] - Lt[c]) * es[c];
+                    Real em = emission<Real, kRescale>(etab, eexp, ei, K);
+                    Anx[f * G + lane] *= em;
+                    Bnx[f * G + lane] *= em;
+                    box_step<C>(n, lo2, w2);
                 }
+                delta = allow ? delta * emission<Real, kRescale>(etab, eexp, ei0, K) : (Real)0;
+            } else {
+                delta = allow ? delta * etab[ei0] : (Real)0;
             }
         }
         wave_sync();
@@ -446,7 +495,7 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
     PairResult<Real> res;
     if (kRescale) {
         double s = (double)sa + (double)sb + (double)delta;
-        res.logp = s > 0 ? log(s) + (double)(logacc + kread) * 0.69314718055994530942 : -INFINITY;
+        res.logp = s > 0 ? log(s) + (double)logacc * 0.69314718055994530942 : -INFINITY;
         res.p = exp(res.logp);
     } else {
         double s = ((double)sa + (double)sb) + (double)delta;
@@ -467,7 +516,6 @@ __global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyA
     if (!((nbr > a.nb_lo && nbr <= a.nb_hi) || (nbr == 0 && a.nb_lo == 0))) return;
 
     const int T = a.m.T;
-    const int M = a.m.M;
     const int tid = threadIdx.x;
     const int nthreads = blockDim.x;
     const int ngroups = a.ngroups;
@@ -479,9 +527,8 @@ __global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyA
     int* sH = (int*)(smem + lay.H);
     int* eoff = (int*)(smem + lay.eoff);
     int* estr = (int*)(smem + lay.estr);
-    int* ek = (int*)(smem + lay.ek);
-    unsigned long long* emax = (unsigned long long*)(smem + lay.emax);
     Real* etab = (Real*)(smem + lay.etab);
+    short* eexp = (short*)(smem + lay.eexp);
     Real* bin = (Real*)(smem + lay.binom);
     int* list = (int*)(smem + lay.list);
     int* scan = (int*)(smem + lay.scan);
@@ -498,7 +545,6 @@ __global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyA
         sH[k] = a.Hr[(size_t)r * T * C + k];
     }
     for (int k = tid; k < 2 * C * a.tri; k += nthreads) bin[k] = (Real)a.binom[k];
-    for (int k = tid; k < T; k += nthreads) emax[k] = 0ull;
     __syncthreads();
     if (tid == 0) {
         int acc = 0;
@@ -517,46 +563,30 @@ __global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyA
     __syncthreads();
     const int nent = eoff[T];
     // emission densities over each timestep's box (peptide-emission.cc:165-180)
-    for (int pass = 0; pass < (kRescale ? 2 : 1); pass++) {
-        for (int k = tid; k < nent; k += nthreads) {
-            int t = 0;
-            while (eoff[t + 1] <= k) t++;
-            int rem = k - eoff[t];
-            unsigned int n[WP_MAX_CHANNELS];
+    for (int k = tid; k < nent; k += nthreads) {
+        int t = 0;
+        while (eoff[t + 1] <= k) t++;
+        int rem = k - eoff[t];
+        unsigned int n[WP_MAX_CHANNELS];
 #pragma unroll
-            for (int c = 0; c < C; c++) {
-                int s = estr[t * C + c];
-                n[c] = (unsigned int)(sL[t * C + c] + rem / s);
-                rem %= s;
-            }
-            double v = 1.0;
+        for (int c = 0; c < C; c++) {
+            int s = estr[t * C + c];
+            n[c] = (unsigned int)(sL[t * C + c] + rem / s);
+            rem %= s;
+        }
+        double v = 1.0;
 #pragma unroll
-            for (int c = 0; c < C; c++) v *= dev_pdf(a.m, c, s_rad[t * C + c], n);
-            if (!kRescale) {
-                etab[k] = (Real)v;
-            } else if (pass == 0) {
-                atomicMax(&emax[t], (unsigned long long)__double_as_longlong(v));
-            } else {
-                etab[k] = (Real)ldexp(v, -ek[t]);
-            }
+        for (int c = 0; c < C; c++) v *= dev_pdf(a.m, c, s_rad[t * C + c], n);
+        if constexpr (kRescale) {
+            int ex = 0;
+            double mant = frexp(v, &ex);
+            etab[k] = (Real)mant;
+            eexp[k] = v > 0.0 ? (short)ex : (short)-30000;
+        } else {
+            etab[k] = (Real)v;
         }
-        if (kRescale && pass == 0) {
-            __syncthreads();
-            if (tid == 0) {
-                int tot = 0;
-                for (int t = 0; t < T; t++) {
-                    double mxv = __longlong_as_double((long long)emax[t]);
-                    int k = 0;
-                    if (mxv > 0) frexp(mxv, &k);
-                    ek[t] = k;
-                    tot += k;
-                }
-                ek[T] = tot;
-            }
-        }
-        __syncthreads();
     }
-    const int kread = kRescale ? ek[T] : 0;
+    __syncthreads();
     bool read_empty = (nbr == 0);
     // The pair is non-empty iff D_c >= L[0][c] for every channel.
 
@@ -617,7 +647,7 @@ __global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyA
                 const int kk = list[q];
                 const int i = cidx ? cidx[kk] : kk;
                 PairResult<Real> pr = score_pair<Real, C, G, kRescale>(a, i, lane, st, a.NB, sL, sH, eoff,
-                                                                        estr, etab, bin, kread);
+                                                                        estr, etab, eexp, bin);
                 if (a.out_pair_logp && !cidx && lane == 0) a.out_pair_logp[(size_t)r * a.S + i] = pr.logp;
                 const int cntw = a.dsCount[i];
                 if (kRescale) {
@@ -706,6 +736,11 @@ hipError_t launch_t(const ClassifyArgs& a, hipStream_t stream) {
     if (nthreads < 64) nthreads = 64;
     nthreads = (nthreads + 63) / 64 * 64;
     Layout l = make_layout<Real>(a.m.T, C, a.tri, a.NB, a.ngroups, G, nthreads);
+    if (l.total > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)classify_kernel<Real, C, G>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL((classify_kernel<Real, C, G>), dim3(a.N), dim3(nthreads), l.total, stream, a);
     return hipGetLastError();
 }
