@@ -87,7 +87,7 @@ size_t lds_hard_limit() {
 
 // Bins of reads by box bound, each with its LDS sizing.
 std::vector<Bin> make_bins(const wp_hmm_classifier* c) {
-    std::vector<int> edges = {8, 32, 128, c->maxbox};
+    std::vector<int> edges = {4, 8, 16, 32, 64, 128, c->maxbox};
     std::vector<Bin> bins;
     int lo = 0;
     for (int hi : edges) {
@@ -271,7 +271,7 @@ int wp_hmm_classifier_create(unsigned int num_timesteps, unsigned int num_channe
     c->S = S;
     c->M = M;
     c->precision = precision;
-    c->G = T <= 16 ? 16 : (T <= 32 ? 32 : 64);
+    c->G = wp::kGroupLanes;
     c->tri = (M + 1) * (M + 2) / 2;
     c->ids.assign(dye_seq_ids, dye_seq_ids + S);
     c->counts.assign(dye_seq_counts, dye_seq_counts + S);

@@ -54,5 +54,6 @@ size_t classify_lds_bytes(int precision, int T, int C, int tri, int NB, int ngro
 // Chunk of candidates compacted per pass.
 constexpr int kCandChunk = 1024;
 constexpr int kBlockThreads = 256;
+constexpr int kGroupLanes = 16;  // lanes scoring one pair
 
 }  // namespace wp

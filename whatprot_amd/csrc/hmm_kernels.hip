@@ -20,22 +20,32 @@
 // is empty (P = 0) iff some L >= H. tests/test_boxes.py checks this against
 // the oracle's literal step-by-step box algebra.
 //
-// Mapping. A workgroup owns one read; its emission table, binomial tables and
-// pruned bounds sit in LDS. Pairs are scored by groups of G lanes (G >= T):
-// lane e owns Edman count e and keeps its column of the state tensors A
-// (normal) and B (blocked N-terminus) over the count box in LDS, laid out
-// [count index][lane] so a group touches G consecutive words. Block, Detach,
-// the Bleach binomials and the Emission touch only the lane's own column; the
-// Edman shift reads column e-1, the only cross-lane access, fenced at wave
-// scope. Non-empty candidates are compacted first (73% of pairs are empty at a
-// 5-sigma cutoff), so groups of one wave work on live pairs together.
+// One gather pass per timestep. Block, Detach, Bleach and Edman are linear, so
+// for t >= 1 every output state (e', n') of the emission box X_t is computed
+// directly from the previous state X_{t-1}:
+//   Bl(X)[e, m] = sum_{n >= m, n in X_{t-1} box} prod_c Pbleach_c(n_c, m_c) X[e, n]
+//   A_t[e', n'] = (1-pd)(1-pcb) [ pf Bl(A)[e', n']
+//                 + (1-pf)(1 - n'_c/K) Bl(A)[e'-1, n']            (c = dye at e'-1)
+//                 + (1-pf)((n'_c+1)/K) Bl(A)[e'-1, n'+1_c] ]       (if n'_c+1 < U_c)
+//   B_t[e', n'] = (1-pd) (Bl(B)[e', n'] + pcb Bl(A)[e', n'])      (e' < t)
+//   then times the emission density E_t(n'); the detached mass gets
+//   pd * sum(A + B) of X_{t-1} (the previous output total) and E_t(0).
+// No intermediate tensor is materialised, so a pair needs only its previous
+// and current state, sized by the emission boxes.
 //
-// Precision. FP64 reproduces the reference's arithmetic (same operation order
-// per state). FP32 keeps the state in float and, after every timestep,
-// rescales it by an exact power of two 2^-k, accumulating k in an integer;
-// emissions are stored per timestep scaled by the power of two of their
-// maximum over the read's box. ln P, the argmax and the normaliser over
-// dye-seqs are FP64.
+// Mapping. A workgroup owns one read; its emission table, binomial tables and
+// pruned bounds sit in LDS. Pairs are scored by groups of G = 16 lanes (four
+// per wave); the lanes of a group stride over the flattened output states of
+// each timestep and read the previous state from the group's LDS buffer. One
+// wave-scope fence per timestep orders the group's writes before the next
+// timestep's reads. Non-empty candidates are compacted first (73% of pairs are
+// empty at a 5-sigma cutoff) so the groups of a wave work on live pairs.
+//
+// Precision. FP64 keeps the reference's arithmetic (doubles, no scaling).
+// FP32 keeps the state in float; the emission table holds mantissa and
+// exponent separately, and at every timestep the products (mass x density) are
+// normalised by 2^-K with K the largest exponent among them, K accumulated in
+// an integer. ln P, the argmax and the normaliser over dye-seqs are FP64.
 #include <float.h>
 #include <math.h>
 
@@ -58,13 +68,6 @@ __device__ inline Real group_sum(Real v) {
     return v;
 }
 
-template <typename Real, int G>
-__device__ inline Real group_max(Real v) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, G));
-    return v;
-}
-
 template <int G>
 __device__ inline int group_max_int(int v) {
 #pragma unroll
@@ -79,17 +82,12 @@ __device__ inline int exp_of(float x) {
     frexpf(x, &e);
     return x > 0.0f ? e : kNoExp;
 }
-__device__ inline int exp_of(double x) {
-    int e;
-    frexp(x, &e);
-    return x > 0.0 ? e : kNoExp;
-}
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // LDS carve-up shared by host sizing and the kernel.
 struct Layout {
-    size_t rad, L, H, eoff, estr, etab, eexp, binom, list, scan, gres, state, total;
+    size_t rad, L, H, eoff, estr, etab, eexp, binom, list, scan, gres, gseq, state, total;
 };
 
 template <typename Real>
@@ -108,7 +106,8 @@ __host__ __device__ inline Layout make_layout(int T, int C, int tri, int NB, int
     l.list = o;  o = align16(o + sizeof(int) * nthreads * 4);
     l.scan = o;  o = align16(o + sizeof(int) * (nthreads / 64 + 2));
     l.gres = o;  o = align16(o + sizeof(double) * 4 * ngroups);
-    l.state = o; o = align16(o + sizeof(Real) * 4 * (size_t)NB * G * ngroups);
+    l.gseq = o;  o = align16(o + sizeof(int) * 2 * T * ngroups);
+    l.state = o; o = align16(o + sizeof(Real) * 4 * (size_t)T * NB * ngroups);
     l.total = o;
     return l;
 }
@@ -150,21 +149,17 @@ __global__ void bounds_kernel(DevModel m, int N, uint8_t* __restrict__ L, uint8_
             h[t * C + c] = a < b ? a : b;
         }
     }
-    // Capacity per pair: the emission boxes, and for t >= 1 the union box
-    // prod_c (H[t-1] - L[t]) that contains the Bleach intermediates and the
-    // Edman input [L[t], U[t]) (U <= H[t-1]).
+    // Capacity per pair: the largest emission box (the state of one timestep).
     int best = 0;
     bool empty = false;
     for (int t = 0; t < T; t++) {
-        int nb = 1, nu = 1;
+        int nb = 1;
         for (int c = 0; c < C; c++) {
             int w = (int)h[t * C + c] - (int)l[t * C + c];
             if (w <= 0) empty = true;
             nb *= w > 0 ? w : 0;
-            if (t > 0) nu *= (int)h[(t - 1) * C + c] - (int)l[t * C + c];
         }
         if (nb > best) best = nb;
-        if (t > 0 && nu > best) best = nu;
     }
     nbmax[r] = empty ? 0 : best;
 }
@@ -174,16 +169,6 @@ struct PairResult {
     double logp;  // ln P (-inf when P == 0)
     double p;     // P itself (FP64 path; FP32 fills exp(logp))
 };
-
-// Incremental walk over a C-dim box, last channel fastest.
-template <int C>
-__device__ inline void box_step(int* n, const int* lo, const int* w) {
-#pragma unroll
-    for (int c = C - 1; c >= 0; c--) {
-        if (++n[c] < lo[c] + w[c]) return;
-        n[c] = lo[c];
-    }
-}
 
 template <int C>
 __device__ inline int box_strides(const int* w, int* str) {
@@ -206,34 +191,68 @@ __device__ inline Real emission(const Real* etab, const short* eexp, int ei, int
     else return etab[ei];
 }
 
+// q / d for 0 <= q < 2^16 and 1 <= d <= 64 with a per-divisor multiplier.
+struct Divider {
+    unsigned int m;
+    __device__ explicit Divider(int d) : m(((1u << 22) + (unsigned int)d - 1u) / (unsigned int)d) {}
+    __device__ int div(int q) const { return (int)(((unsigned int)q * m) >> 22); }
+};
+
+// Composite Bleach over channels c..C-1 evaluated at counts m: accumulates
+// coef * prod P_c(n_c, m_c) * X[e, n] over n >= m inside the previous box
+// (binomial-transition.cc:124-136 applied channel after channel).
+template <typename Real, int C, int c, bool kWithB>
+struct CompositeBleach {
+    __device__ static inline void eval(const Real* A, const Real* B, int e, int es, const int* m,
+                                       const int* lo, const int* hi, const int* str, const Real* ble,
+                                       int tri, int idx, Real coef, Real& sa, Real& sb) {
+        const int mc = m[c];
+        for (int n = max(mc, lo[c]); n < hi[c]; n++) {
+            Real p = ble[c * tri + n * (n + 1) / 2 + mc] * coef;
+            CompositeBleach<Real, C, c + 1, kWithB>::eval(A, B, e, es, m, lo, hi, str, ble, tri,
+                                                          idx + (n - lo[c]) * str[c], p, sa, sb);
+        }
+    }
+};
+template <typename Real, int C, bool kWithB>
+struct CompositeBleach<Real, C, C, kWithB> {
+    __device__ static inline void eval(const Real* A, const Real* B, int e, int es, const int*,
+                                       const int*, const int*, const int*, const Real*, int, int idx,
+                                       Real coef, Real& sa, Real& sb) {
+        sa += coef * A[idx * es + e];
+        if (kWithB) sb += coef * B[idx * es + e];
+    }
+};
+
 // Forward algorithm of one (read, dye-seq) pair on one group of G lanes.
+// st holds four buffers of cap = T * NB entries (A, B in; A, B out); gseq/gK
+// hold the dye-seq's channel at each position e < T and K = dyes of that
+// channel at positions >= e.
 template <typename Real, int C, int G, bool kRescale>
-__device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, Real* st, int NB,
+__device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, Real* st, int cap,
                                        const int* sL, const int* sH, const int* eoff,
                                        const int* estr, const Real* etab, const short* eexp,
-                                       const Real* bin) {
+                                       const Real* bin, int* gseq, int* gK) {
     const int T = a.m.T;
     const int tri = a.tri;
     int D[C];
 #pragma unroll
     for (int c = 0; c < C; c++) D[c] = a.dsD[(size_t)i * C + c];
-    // Edman source of this lane as a target (e' = lane, source e'-1):
-    // edman-transition.cc:97-125 gathered.
-    int ce = -1, Ke = 1;
-    if (lane >= 1 && lane < T) {
-        ce = a.dsSeq[(size_t)i * T + lane - 1];
-        if (ce >= 0) Ke = a.dsTrack[((size_t)i * T + lane - 1) * C + ce];
+    for (int e = lane; e < T; e += G) {
+        int ch = a.dsSeq[(size_t)i * T + e];
+        gseq[e] = ch;
+        gK[e] = ch >= 0 ? (int)a.dsTrack[((size_t)i * T + e) * C + ch] : 1;
     }
     const Real pf = (Real)a.m.pf;
     const Real one_m_pf = (Real)(1.0 - a.m.pf);
     const Real* dud = bin;
     const Real* ble = bin + C * tri;
-    Real* Acur = st;
-    Real* Bcur = st + NB * G;
-    Real* Anx = st + 2 * NB * G;
-    Real* Bnx = st + 3 * NB * G;
+    Real* Ain = st;
+    Real* Bin = st + cap;
+    Real* Aout = st + 2 * cap;
+    Real* Bout = st + 3 * cap;
 
-    // --- t = 0: InitialBlock, Dud_c, Emit(0) -------------------------------
+    // --- t = 0: InitialBlock, Dud_c, Emit(0) over [0,1) x [L0, H0) ---------
     int lo[C], hi[C], w[C], str[C];
 #pragma unroll
     for (int c = 0; c < C; c++) {
@@ -243,237 +262,170 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
     }
     int nb = box_strides<C>(w, str);
     int logacc = 0;
+    Real S = 0;  // sum of A + B over the current state
     {
         const Real pib = (Real)a.m.p_ib;
         int K0 = 0;
-        if (kRescale) {
-            // largest exponent of mass x emission over the box
-            int n[C];
-#pragma unroll
-            for (int c = 0; c < C; c++) n[c] = lo[c];
+        if constexpr (kRescale) {
             int km = kNoExp;
-            for (int f = 0; f < nb; f++) {
+            for (int q = lane; q < nb; q += G) {
+                int rem = q, ei = eoff[0];
                 Real v = 1;
-                int ei = eoff[0];
 #pragma unroll
                 for (int c = 0; c < C; c++) {
-                    v = dud[c * tri + D[c] * (D[c] + 1) / 2 + n[c]] * v;
-                    ei += (n[c] - sL[c]) * estr[c];
+                    int n = lo[c] + rem / str[c];
+                    rem -= (n - lo[c]) * str[c];
+                    v = dud[c * tri + D[c] * (D[c] + 1) / 2 + n] * v;
+                    ei += (n - sL[c]) * estr[c];
                 }
                 km = max(km, exp_of(v) + (int)eexp[ei]);
-                box_step<C>(n, lo, w);
             }
+            km = group_max_int<G>(km);
             K0 = km == kNoExp ? 0 : km;
             logacc += K0;
         }
-        int n[C];
-#pragma unroll
-        for (int c = 0; c < C; c++) n[c] = lo[c];
-        for (int f = 0; f < nb; f++) {
+        for (int q = lane; q < nb; q += G) {
+            int rem = q, ei = eoff[0];
             Real va = (Real)1 - pib, vb = pib;
-            int ei = eoff[0];
 #pragma unroll
             for (int c = 0; c < C; c++) {
-                Real p = dud[c * tri + D[c] * (D[c] + 1) / 2 + n[c]];
+                int n = lo[c] + rem / str[c];
+                rem -= (n - lo[c]) * str[c];
+                Real p = dud[c * tri + D[c] * (D[c] + 1) / 2 + n];
                 va = p * va;
                 vb = p * vb;
-                ei += (n[c] - sL[c]) * estr[c];
+                ei += (n - sL[c]) * estr[c];
             }
             Real em = emission<Real, kRescale>(etab, eexp, ei, K0);
-            Acur[f * G + lane] = lane == 0 ? va * em : (Real)0;
-            Bcur[f * G + lane] = lane == 0 ? vb * em : (Real)0;
-            box_step<C>(n, lo, w);
+            va *= em;
+            vb *= em;
+            Ain[q] = va;
+            Bin[q] = vb;
+            S += va + vb;
         }
+        S = group_sum<Real, G>(S);
     }
     Real delta = 0;
+    wave_sync();
 
     for (int t = 1; t < T; t++) {
         const int* Lt = sL + t * C;
         const int* Ht = sH + t * C;
-        int lo2[C], hi2[C], U[C], wu[C], ustr[C], w2[C], str2[C];
+        int lo2[C], hi2[C], U[C], w2[C], str2[C];
 #pragma unroll
         for (int c = 0; c < C; c++) {
             lo2[c] = Lt[c];
             hi2[c] = min(Ht[c], D[c] + 1);
             U[c] = min(hi2[c] + 1, hi[c]);
-            wu[c] = U[c] - lo2[c];
             w2[c] = hi2[c] - lo2[c];
         }
-        box_strides<C>(wu, ustr);
-        int nb2 = box_strides<C>(w2, str2);
-
-        // CyclicBlock + Detach(t-1) in place over the previous box
-        // (block-transition.cc:31-60, detach-transition.cc:36-68).
-        {
-            const Real pcb = (Real)a.m.p_cb;
-            const Real pd = (Real)a.m.p_detach[t - 1];
-            Real sa = 0, sb = 0;
-            for (int f = 0; f < nb; f++) {
-                Real av = Acur[f * G + lane], bv = Bcur[f * G + lane];
-                Real a1 = ((Real)1 - pcb) * av;
-                Real b1 = bv + pcb * av;
-                sa += a1;
-                sb += b1;
-                Acur[f * G + lane] = a1 * ((Real)1 - pd);
-                Bcur[f * G + lane] = b1 * ((Real)1 - pd);
-            }
-            sa = group_sum<Real, G>(sa);
-            sb = group_sum<Real, G>(sb);
-            delta = delta + pd * (sa + sb);
-        }
-
-        // Bleach_c, c = 0..C-1 (binomial-transition.cc:91-136), lane-local.
-        {
-            int ilo[C], iw[C];
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                ilo[c] = lo[c];
-                iw[c] = w[c];
-            }
-            Real* Ain = Acur;
-            Real* Bin = Bcur;
-            Real* Aout = Anx;
-            Real* Bout = Bnx;
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                int istr[C], olo[C], ow[C], ostr[C];
-                box_strides<C>(iw, istr);
-#pragma unroll
-                for (int k = 0; k < C; k++) {
-                    olo[k] = ilo[k];
-                    ow[k] = iw[k];
-                }
-                olo[c] = lo2[c];
-                ow[c] = wu[c];
-                int onb = box_strides<C>(ow, ostr);
-                int n[C];
-#pragma unroll
-                for (int k = 0; k < C; k++) n[k] = olo[k];
-                const Real* P = ble + c * tri;
-                const int ihi = ilo[c] + iw[c];
-                for (int f = 0; f < onb; f++) {
-                    int base = 0;
-#pragma unroll
-                    for (int k = 0; k < C; k++)
-                        if (k != c) base += (n[k] - ilo[k]) * istr[k];
-                    const int j = n[c];
-                    Real va = 0, vb = 0;
-                    for (int from = max(j, ilo[c]); from < ihi; from++) {
-                        int idx = (base + (from - ilo[c]) * istr[c]) * G + lane;
-                        Real p = P[from * (from + 1) / 2 + j];
-                        va += p * Ain[idx];
-                        vb += p * Bin[idx];
-                    }
-                    Aout[f * G + lane] = va;
-                    Bout[f * G + lane] = vb;
-                    box_step<C>(n, olo, ow);
-                }
-#pragma unroll
-                for (int k = 0; k < C; k++) {
-                    ilo[k] = olo[k];
-                    iw[k] = ow[k];
-                }
-                Real* tA = Ain; Ain = Aout; Aout = tA;
-                Real* tB = Bin; Bin = Bout; Bout = tB;
-            }
-            Acur = Ain;
-            Bcur = Bin;
-            Anx = Aout;
-            Bnx = Bout;
-        }
-
-        // Edman (edman-transition.cc:62-162) as a gather, fused with Emit(t)
-        // (peptide-emission.cc:212-243). Reads column lane-1.
-        wave_sync();
-        {
+        const int nb2 = box_strides<C>(w2, str2);
+        const int nout = (t + 1) * nb2;
+        const Divider by_e(t + 1);
+        const int eo = eoff[t];
+        const int* es = estr + t * C;
+        // Detach(t-1) adds pd times the previous total (block keeps A + B).
+        const Real pd = (Real)a.m.p_detach[t - 1];
+        const Real pcb = (Real)a.m.p_cb;
+        delta = delta + pd * S;
+        const Real cAA = ((Real)1 - pd) * ((Real)1 - pcb);
+        const Real cB = (Real)1 - pd;
+        int km = kNoExp;
+        Real Snew = 0;
+        for (int q = lane; q < nout; q += G) {
+            const int ni = by_e.div(q);
+            const int e = q - ni * (t + 1);
             int n[C];
+            int rem = ni, ei = eo;
 #pragma unroll
-            for (int c = 0; c < C; c++) n[c] = lo2[c];
-            int kc = 0, Uc = 0, sc_str = 0;
+            for (int c = 0; c < C; c++) {
+                n[c] = lo2[c] + rem / str2[c];
+                rem -= (n[c] - lo2[c]) * str2[c];
+                ei += (n[c] - Lt[c]) * es[c];
+            }
+            Real blA = 0, blB = 0;
+            if (e < t)
+                CompositeBleach<Real, C, 0, true>::eval(Ain, Bin, e, t, n, lo, hi, str, ble, tri, 0,
+                                                        (Real)1, blA, blB);
+            Real acc = 0, dummy = 0;
+            if (e >= 1) {
+                const int ce = gseq[e - 1];
+                Real p1 = 0;
+                CompositeBleach<Real, C, 0, false>::eval(Ain, Bin, e - 1, t, n, lo, hi, str, ble, tri, 0,
+                                                         (Real)1, p1, dummy);
+                if (ce < 0) {
+                    acc += one_m_pf * p1;
+                } else {
+                    const int Ke = gK[e - 1];
+                    int kc = 0, Uc = 0;
 #pragma unroll
-            for (int c = 0; c < C; c++)
-                if (c == ce) {
-                    Uc = U[c];
-                    sc_str = ustr[c];
+                    for (int c = 0; c < C; c++)
+                        if (c == ce) {
+                            kc = n[c];
+                            Uc = U[c];
+                        }
+                    if (kc < Ke) acc += one_m_pf * ((Real)1 - (Real)kc / (Real)Ke) * p1;
+                    if (kc + 1 < Uc) {
+                        int n1[C];
+#pragma unroll
+                        for (int c = 0; c < C; c++) n1[c] = n[c] + (c == ce ? 1 : 0);
+                        Real p2 = 0;
+                        CompositeBleach<Real, C, 0, false>::eval(Ain, Bin, e - 1, t, n1, lo, hi, str, ble,
+                                                                 tri, 0, (Real)1, p2, dummy);
+                        acc += one_m_pf * ((Real)(kc + 1) / (Real)Ke) * p2;
+                    }
                 }
-            const int eo = eoff[t];
-            const int* es = estr + t * C;
-            int km = kNoExp;  // FP32: largest exponent of mass x emission
-            for (int f = 0; f < nb2; f++) {
-                int idx0 = 0, ei = eo;
+            }
+            acc += pf * blA;
+            Real ao = cAA * acc;
+            Real bo = e < t ? cB * (blB + pcb * blA) : (Real)0;
+            if constexpr (kRescale) {
+                Aout[q] = ao;
+                Bout[q] = bo;
+                km = max(km, max(exp_of(ao), exp_of(bo)) + (int)eexp[ei]);
+            } else {
+                Real em = etab[ei];
+                ao *= em;
+                bo *= em;
+                Aout[q] = ao;
+                Bout[q] = bo;
+                Snew += ao + bo;
+            }
+        }
+        bool allow = true;
+#pragma unroll
+        for (int c = 0; c < C; c++) allow = allow && lo2[c] == 0;
+        int ei0 = eo;
+#pragma unroll
+        for (int c = 0; c < C; c++) ei0 -= Lt[c] * es[c];
+        if constexpr (kRescale) {
+            km = group_max_int<G>(km);
+            if (allow) km = max(km, exp_of(delta) + (int)eexp[ei0]);
+            const int K = km == kNoExp ? 0 : km;
+            logacc += K;
+            for (int q = lane; q < nout; q += G) {
+                const int ni = by_e.div(q);
+                int rem = ni, ei = eo;
 #pragma unroll
                 for (int c = 0; c < C; c++) {
-                    idx0 += (n[c] - lo2[c]) * ustr[c];
-                    ei += (n[c] - Lt[c]) * es[c];
-                    if (c == ce) kc = n[c];
+                    int nc = rem / str2[c];
+                    rem -= nc * str2[c];
+                    ei += (nc + lo2[c] - Lt[c]) * es[c];
                 }
-                Real acc = 0;
-                if (lane >= 1 && lane <= t) {
-                    Real a1 = Acur[idx0 * G + lane - 1];
-                    if (ce < 0) {
-                        acc += one_m_pf * a1;
-                    } else {
-                        if (kc < Ke) {
-                            Real ratio = (Real)kc / (Real)Ke;
-                            acc += one_m_pf * ((Real)1 - ratio) * a1;
-                        }
-                        if (kc + 1 < Uc) {
-                            Real a2 = Acur[(idx0 + sc_str) * G + lane - 1];
-                            Real ratio2 = (Real)(kc + 1) / (Real)Ke;
-                            acc += one_m_pf * ratio2 * a2;
-                        }
-                    }
-                }
-                Real a0 = lane < t ? Acur[idx0 * G + lane] : (Real)0;
-                acc += pf * a0;
-                Real b0 = lane < t ? Bcur[idx0 * G + lane] : (Real)0;
-                if (lane > t) acc = 0;
-                if constexpr (kRescale) {
-                    // store the mass; the emission is applied below once the
-                    // common exponent is known
-                    Anx[f * G + lane] = acc;
-                    Bnx[f * G + lane] = b0;
-                    km = max(km, max(exp_of(acc), exp_of(b0)) + (int)eexp[ei]);
-                } else {
-                    Real em = etab[ei];
-                    Anx[f * G + lane] = acc * em;
-                    Bnx[f * G + lane] = b0 * em;
-                }
-                box_step<C>(n, lo2, w2);
+                Real em = emission<Real, kRescale>(etab, eexp, ei, K);
+                Real ao = Aout[q] * em, bo = Bout[q] * em;
+                Aout[q] = ao;
+                Bout[q] = bo;
+                Snew += ao + bo;
             }
-            bool allow = true;
-#pragma unroll
-            for (int c = 0; c < C; c++) allow = allow && lo2[c] == 0;
-            int ei0 = eo;
-#pragma unroll
-            for (int c = 0; c < C; c++) ei0 += (0 - Lt[c]) * es[c];
-            if constexpr (kRescale) {
-                // Emit(t) with exact power-of-two normalisation: K is the
-                // largest exponent of (mass x density) over the state and the
-                // detached scalar; every product is multiplied by 2^-K.
-                km = group_max_int<G>(km);
-                if (allow) km = max(km, exp_of(delta) + (int)eexp[ei0]);
-                const int K = km == kNoExp ? 0 : km;
-                logacc += K;
-#pragma unroll
-                for (int c = 0; c < C; c++) n[c] = lo2[c];
-                for (int f = 0; f < nb2; f++) {
-                    int ei = eo;
-#pragma unroll
-                    for (int c = 0; c < C; c++) ei += (n[c] - Lt[c]) * es[c];
-                    Real em = emission<Real, kRescale>(etab, eexp, ei, K);
-                    Anx[f * G + lane] *= em;
-                    Bnx[f * G + lane] *= em;
-                    box_step<C>(n, lo2, w2);
-                }
-                delta = allow ? delta * emission<Real, kRescale>(etab, eexp, ei0, K) : (Real)0;
-            } else {
-                delta = allow ? delta * etab[ei0] : (Real)0;
-            }
+            delta = allow ? delta * emission<Real, kRescale>(etab, eexp, ei0, K) : (Real)0;
+        } else {
+            delta = allow ? delta * etab[ei0] : (Real)0;
         }
+        S = group_sum<Real, G>(Snew);
         wave_sync();
-        Real* tA = Acur; Acur = Anx; Anx = tA;
-        Real* tB = Bcur; Bcur = Bnx; Bnx = tB;
+        Real* tA = Ain; Ain = Aout; Aout = tA;
+        Real* tB = Bin; Bin = Bout; Bout = tB;
 #pragma unroll
         for (int c = 0; c < C; c++) {
             lo[c] = lo2[c];
@@ -484,24 +436,18 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
         nb = nb2;
     }
 
-    // PeptideStateVector::sum (peptide-state-vector.cc:52-54).
-    Real sa = 0, sb = 0;
-    for (int f = 0; f < nb; f++) {
-        sa += Acur[f * G + lane];
-        sb += Bcur[f * G + lane];
-    }
-    sa = group_sum<Real, G>(sa);
-    sb = group_sum<Real, G>(sb);
+    // PeptideStateVector::sum (peptide-state-vector.cc:52-54): S + delta.
     PairResult<Real> res;
     if (kRescale) {
-        double s = (double)sa + (double)sb + (double)delta;
+        double s = (double)S + (double)delta;
         res.logp = s > 0 ? log(s) + (double)logacc * 0.69314718055994530942 : -INFINITY;
         res.p = exp(res.logp);
     } else {
-        double s = ((double)sa + (double)sb) + (double)delta;
+        double s = (double)S + (double)delta;
         res.p = s;
         res.logp = log(s);
     }
+    wave_sync();
     return res;
 }
 
@@ -533,6 +479,7 @@ __global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyA
     int* list = (int*)(smem + lay.list);
     int* scan = (int*)(smem + lay.scan);
     double* gres = (double*)(smem + lay.gres);
+    int* gseq_all = (int*)(smem + lay.gseq);
     Real* state = (Real*)(smem + lay.state);
 
     const int ncand = a.cand_off ? a.cand_off[r + 1] - a.cand_off[r] : a.S;
@@ -595,7 +542,10 @@ __global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyA
     int best_k = 0;
     double tot_a = kRescale ? -INFINITY : 0.0;  // FP64: total; FP32: running max
     double tot_b = 0.0;                           // FP32: running scaled sum
-    Real* st = state + (size_t)group * 4 * a.NB * G;
+    const int cap = T * a.NB;
+    Real* st = state + (size_t)group * 4 * cap;
+    int* gseq = gseq_all + group * 2 * T;
+    int* gK = gseq + T;
 
     for (int base = 0; base < ncand; base += nthreads * 4) {
         // compact the non-empty candidates of this chunk, keeping order
@@ -646,8 +596,8 @@ __global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyA
             for (int q = group; q < nlive; q += ngroups) {
                 const int kk = list[q];
                 const int i = cidx ? cidx[kk] : kk;
-                PairResult<Real> pr = score_pair<Real, C, G, kRescale>(a, i, lane, st, a.NB, sL, sH, eoff,
-                                                                        estr, etab, eexp, bin);
+                PairResult<Real> pr = score_pair<Real, C, G, kRescale>(a, i, lane, st, cap, sL, sH, eoff,
+                                                                        estr, etab, eexp, bin, gseq, gK);
                 if (a.out_pair_logp && !cidx && lane == 0) a.out_pair_logp[(size_t)r * a.S + i] = pr.logp;
                 const int cntw = a.dsCount[i];
                 if (kRescale) {
@@ -747,12 +697,8 @@ hipError_t launch_t(const ClassifyArgs& a, hipStream_t stream) {
 
 template <typename Real, int C>
 hipError_t launch_c(const ClassifyArgs& a, int G, hipStream_t stream) {
-    switch (G) {
-    case 16: return launch_t<Real, C, 16>(a, stream);
-    case 32: return launch_t<Real, C, 32>(a, stream);
-    case 64: return launch_t<Real, C, 64>(a, stream);
-    default: return hipErrorInvalidValue;
-    }
+    if (G != kGroupLanes) return hipErrorInvalidValue;
+    return launch_t<Real, C, kGroupLanes>(a, stream);
 }
 
 template <typename Real>
