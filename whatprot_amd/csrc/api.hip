@@ -66,6 +66,7 @@ namespace {
 
 struct Bin {
     int lo, hi, ngroups;
+    bool rows;  // row-per-lane kernel (small boxes) instead of the gather kernel
 };
 
 constexpr size_t kLdsBudget = 48 * 1024;
@@ -97,7 +98,9 @@ std::vector<Bin> make_bins(const wp_hmm_classifier* c) {
         size_t per = wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, hi, 1, c->G) - fixed;
         int ng = (int)((kLdsBudget > fixed ? kLdsBudget - fixed : 0) / (per ? per : 1));
         ng = std::max(1, std::min(ng, wp::kBlockThreads / c->G));
-        bins.push_back({lo, hi, ng});
+        bool rows = wp::v3_supports(c->precision, c->T, hi) &&
+                    wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, hi) <= lds_hard_limit();
+        bins.push_back({lo, hi, ng, rows});
         lo = hi;
         if (lo >= c->maxbox) break;
     }
@@ -161,7 +164,8 @@ int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_c
         a.nb_hi = b.hi;
         a.NB = b.hi;
         a.ngroups = b.ngroups;
-        WP_HIP(wp::launch_classify(a, c->precision, c->G, stream));
+        if (b.rows) WP_HIP(wp::launch_classify_v3(a, c->precision, stream));
+        else WP_HIP(wp::launch_classify(a, c->precision, c->G, stream));
     }
     if (c->timing) {
         WP_HIP(hipEventRecord(c->ev[2], stream));
@@ -317,7 +321,8 @@ int wp_hmm_classifier_create(unsigned int num_timesteps, unsigned int num_channe
     }
     // every bin must fit at least one pair group
     for (const Bin& b : make_bins(c)) {
-        size_t need = wp::classify_lds_bytes(precision, T, C, c->tri, b.hi, b.ngroups, c->G);
+        size_t need = b.rows ? wp::v3_lds_bytes(precision, T, C, c->tri, b.hi)
+                             : wp::classify_lds_bytes(precision, T, C, c->tri, b.hi, b.ngroups, c->G);
         if (need > lds_hard_limit()) {
             wp::set_error("state box too large for LDS at this precision (max_num_dyes/channels)");
             delete c;

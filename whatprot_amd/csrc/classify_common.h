@@ -1,0 +1,201 @@
+// Device helpers shared by the classify kernels: wave-scope fences, group
+// reductions, exponent handling, the order-preserving compaction of a read's
+// non-empty candidates, and the per-read argmax / normaliser
+// (classifiers/hmm-classifier.h:45-78).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "hmm_kernels.h"
+
+namespace wp {
+namespace cc {
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename Real, int G>
+__device__ inline Real group_sum(Real v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+    return v;
+}
+
+template <int G>
+__device__ inline int group_max_int(int v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, G));
+    return v;
+}
+
+// Exponent e with x = m 2^e, m in [0.5, 1); kNoExp for zero.
+constexpr int kNoExp = -100000;
+__device__ inline int exp_of(float x) {
+    int e;
+    frexpf(x, &e);
+    return x > 0.0f ? e : kNoExp;
+}
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// q / d for 0 <= q < 2^16 and 1 <= d <= 64 with a per-divisor multiplier.
+struct Divider {
+    unsigned int m;
+    __device__ Divider() : m(0) {}
+    __device__ explicit Divider(int d) : m(((1u << 22) + (unsigned int)d - 1u) / (unsigned int)d) {}
+    __device__ int div(int q) const { return (int)(((unsigned int)q * m) >> 22); }
+};
+
+// Running argmax and normaliser of one group over the candidates it scored.
+// FP64 (kRescale false): v = P, total = sum P * count, strict > with the
+// earlier candidate winning ties (hmm-classifier.h:59-65). FP32: v = ln P and
+// the total is an online log-sum-exp (tot_a = running max, tot_b = scaled sum).
+template <bool kRescale>
+struct CandAcc {
+    double best_v;
+    int best_k;
+    double tot_a, tot_b;
+    __device__ CandAcc() : best_v(kRescale ? -INFINITY : 0.0), best_k(0), tot_a(kRescale ? -INFINITY : 0.0), tot_b(0.0) {}
+    __device__ void add(double logp, double p, int kk, int count) {
+        if (kRescale) {
+            if (logp > best_v || (logp == best_v && kk < best_k)) {
+                best_v = logp;
+                best_k = kk;
+            }
+            if (count > 0 && logp > -INFINITY) {
+                double x = logp + log((double)count);
+                if (x > tot_a) {
+                    tot_b = tot_b * exp(tot_a - x) + 1.0;
+                    tot_a = x;
+                } else {
+                    tot_b += exp(x - tot_a);
+                }
+            }
+        } else {
+            tot_a += p * count;
+            if (p > best_v || (p == best_v && kk < best_k)) {
+                best_v = p;
+                best_k = kk;
+            }
+        }
+    }
+};
+
+// Compacts, in candidate order, the non-empty candidates among
+// [base, base + 4 * blockDim.x) of read r into list[]; returns how many. A
+// pair is non-empty iff the read is and D_c >= L[0][c] for every channel.
+// Empty pairs get ln P = -inf in the optional pair output. Block-wide; every
+// thread must call it.
+template <int C>
+__device__ int compact_chunk(const ClassifyArgs& a, int r, int ncand, const int* cidx, int base,
+                             bool read_empty, const int* sL, int* list, int* scan) {
+    const int tid = threadIdx.x, nthreads = blockDim.x;
+    int cnt = 0, flags = 0;
+    const int kk0 = base + tid * 4;
+    for (int q = 0; q < 4; q++) {
+        int kk = kk0 + q;
+        if (kk >= ncand) break;
+        int i = cidx ? cidx[kk] : kk;
+        bool ok = !read_empty;
+        if (ok) {
+#pragma unroll
+            for (int c = 0; c < C; c++) ok = ok && (int)a.dsD[(size_t)i * C + c] >= sL[c];
+        }
+        if (ok) {
+            flags |= 1 << q;
+            cnt++;
+        } else if (a.out_pair_logp && !cidx) {
+            a.out_pair_logp[(size_t)r * a.S + i] = -INFINITY;
+        }
+    }
+    int incl = cnt;
+    const int wl = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(incl, off, 64);
+        if (wl >= off) incl += y;
+    }
+    if (wl == 63) scan[wv] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int q = 0; q < nthreads / 64; q++) {
+            int v = scan[q];
+            scan[q] = acc;
+            acc += v;
+        }
+        scan[nthreads / 64] = acc;
+    }
+    __syncthreads();
+    int pos = scan[wv] + incl - cnt;
+    for (int q = 0; q < 4; q++)
+        if (flags & (1 << q)) list[pos++] = kk0 + q;
+    const int nlive = scan[nthreads / 64];
+    __syncthreads();
+    return nlive;
+}
+
+// Merges the groups' accumulators (gres: 4 doubles per group) and writes the
+// read's outputs, with the NaN guard of hmm-classifier.h:73-76. Block-wide.
+template <bool kRescale>
+__device__ void merge_and_store(const ClassifyArgs& a, int r, int ncand, const int* cidx, int ngroups,
+                                int group, int lane, const CandAcc<kRescale>& acc, double* gres) {
+    if (group < ngroups && lane == 0) {
+        gres[group * 4 + 0] = acc.best_v;
+        gres[group * 4 + 1] = (double)acc.best_k;
+        gres[group * 4 + 2] = acc.tot_a;
+        gres[group * 4 + 3] = acc.tot_b;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double bv = gres[0];
+    int bk = (int)gres[1];
+    double ta = gres[2], tb = gres[3];
+    for (int g = 1; g < ngroups; g++) {
+        double v = gres[g * 4 + 0];
+        int k = (int)gres[g * 4 + 1];
+        if (v > bv || (v == bv && k < bk)) {
+            bv = v;
+            bk = k;
+        }
+        if (kRescale) {
+            double xa = gres[g * 4 + 2], xb = gres[g * 4 + 3];
+            if (xa > ta) {
+                tb = tb * exp(ta - xa) + xb;
+                ta = xa;
+            } else if (xa > -INFINITY) {
+                tb += xb * exp(xa - ta);
+            }
+        } else {
+            ta += gres[g * 4 + 2];
+        }
+    }
+    double log_score, log_total, adjusted;
+    if (kRescale) {
+        log_score = bv;
+        log_total = tb > 0 ? ta + log(tb) : -INFINITY;
+        adjusted = exp(log_score - log_total);
+    } else {
+        log_score = log(bv);
+        log_total = log(ta);
+        adjusted = bv / ta;
+    }
+    if (isnan(adjusted)) {
+        adjusted = 0.0;
+        log_score = -INFINITY;
+        log_total = 0.0;
+    }
+    int id = -1;
+    if (ncand > 0) id = a.dsId[cidx ? cidx[bk] : bk];
+    a.out_id[r] = id;
+    if (a.out_adjusted) a.out_adjusted[r] = adjusted;
+    if (a.out_log_score) a.out_log_score[r] = log_score;
+    if (a.out_log_total) a.out_log_total[r] = log_total;
+}
+
+}  // namespace cc
+}  // namespace wp

@@ -44,6 +44,12 @@ void launch_read_prep(const DevModel& m, int N, const double* d_rads, uint8_t* d
 // Returns hipSuccess or the launch error.
 hipError_t launch_classify(const ClassifyArgs& a, int precision, int group_lanes, hipStream_t stream);
 
+// Row-per-lane classify for reads whose count box is at most NB in {4, 8}
+// states, T <= 32.
+bool v3_supports(int precision, int T, int NB);
+size_t v3_lds_bytes(int precision, int T, int C, int tri, int NB);
+hipError_t launch_classify_v3(const ClassifyArgs& a, int precision, hipStream_t stream);
+
 // Per-read algorithmic work (non-empty pairs, state-timesteps, FLOPs) -> [N][3].
 void launch_work(const DevModel& m, int N, int S, const uint8_t* d_Lr, const uint8_t* d_Hr,
                  const int* d_nbmax, const uint8_t* d_D, double* d_out, hipStream_t stream);

@@ -49,41 +49,24 @@
 #include <float.h>
 #include <math.h>
 
+#include "classify_common.h"
 #include "hmm_kernels.h"
+
+// Register budget of the classify kernel: 4 waves per SIMD unless overridden.
+#ifndef WP_WAVES_PER_EU
+#define WP_WAVES_PER_EU 4
+#endif
+#if WP_WAVES_PER_EU > 0
+#define WP_CLASSIFY_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WP_WAVES_PER_EU, 8)))
+#else
+#define WP_CLASSIFY_OCCUPANCY
+#endif
 
 namespace wp {
 
 namespace {
 
-__device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <typename Real, int G>
-__device__ inline Real group_sum(Real v) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
-    return v;
-}
-
-template <int G>
-__device__ inline int group_max_int(int v) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, G));
-    return v;
-}
-
-// Exponent e with x = m 2^e, m in [0.5, 1); kNoExp for zero.
-constexpr int kNoExp = -100000;
-__device__ inline int exp_of(float x) {
-    int e;
-    frexpf(x, &e);
-    return x > 0.0f ? e : kNoExp;
-}
-
-__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+using namespace cc;
 
 // LDS carve-up shared by host sizing and the kernel.
 struct Layout {
@@ -106,7 +89,7 @@ __host__ __device__ inline Layout make_layout(int T, int C, int tri, int NB, int
     l.list = o;  o = align16(o + sizeof(int) * nthreads * 4);
     l.scan = o;  o = align16(o + sizeof(int) * (nthreads / 64 + 2));
     l.gres = o;  o = align16(o + sizeof(double) * 4 * ngroups);
-    l.gseq = o;  o = align16(o + sizeof(int) * 2 * T * ngroups);
+    l.gseq = o;  o = align16(o + sizeof(int) * 3 * T * ngroups);
     l.state = o; o = align16(o + sizeof(Real) * 4 * (size_t)T * NB * ngroups);
     l.total = o;
     return l;
@@ -191,13 +174,6 @@ __device__ inline Real emission(const Real* etab, const short* eexp, int ei, int
     else return etab[ei];
 }
 
-// q / d for 0 <= q < 2^16 and 1 <= d <= 64 with a per-divisor multiplier.
-struct Divider {
-    unsigned int m;
-    __device__ explicit Divider(int d) : m(((1u << 22) + (unsigned int)d - 1u) / (unsigned int)d) {}
-    __device__ int div(int q) const { return (int)(((unsigned int)q * m) >> 22); }
-};
-
 // Composite Bleach over channels c..C-1 evaluated at counts m: accumulates
 // coef * prod P_c(n_c, m_c) * X[e, n] over n >= m inside the previous box
 // (binomial-transition.cc:124-136 applied channel after channel).
@@ -241,8 +217,11 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
     for (int e = lane; e < T; e += G) {
         int ch = a.dsSeq[(size_t)i * T + e];
         gseq[e] = ch;
-        gK[e] = ch >= 0 ? (int)a.dsTrack[((size_t)i * T + e) * C + ch] : 1;
+        int K = ch >= 0 ? (int)a.dsTrack[((size_t)i * T + e) * C + ch] : 1;
+        gK[e] = K;
+        ((float*)(gK + T))[e] = 1.0f / (float)K;
     }
+    const float* ginvK = (const float*)(gK + T);  // 1/K per position (FP32 path)
     const Real pf = (Real)a.m.pf;
     const Real one_m_pf = (Real)(1.0 - a.m.pf);
     const Real* dud = bin;
@@ -322,6 +301,9 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
         const int nb2 = box_strides<C>(w2, str2);
         const int nout = (t + 1) * nb2;
         const Divider by_e(t + 1);
+        Divider by_str[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) by_str[c] = Divider(str2[c]);
         const int eo = eoff[t];
         const int* es = estr + t * C;
         // Detach(t-1) adds pd times the previous total (block keeps A + B).
@@ -339,8 +321,9 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
             int rem = ni, ei = eo;
 #pragma unroll
             for (int c = 0; c < C; c++) {
-                n[c] = lo2[c] + rem / str2[c];
-                rem -= (n[c] - lo2[c]) * str2[c];
+                const int qc = by_str[c].div(rem);
+                n[c] = lo2[c] + qc;
+                rem -= qc * str2[c];
                 ei += (n[c] - Lt[c]) * es[c];
             }
             Real blA = 0, blB = 0;
@@ -364,7 +347,16 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
                             kc = n[c];
                             Uc = U[c];
                         }
-                    if (kc < Ke) acc += one_m_pf * ((Real)1 - (Real)kc / (Real)Ke) * p1;
+                    Real ratio, ratio2;
+                    if constexpr (kRescale) {
+                        const float inv = ginvK[e - 1];
+                        ratio = (Real)kc * inv;
+                        ratio2 = (Real)(kc + 1) * inv;
+                    } else {
+                        ratio = (Real)kc / (Real)Ke;
+                        ratio2 = (Real)(kc + 1) / (Real)Ke;
+                    }
+                    if (kc < Ke) acc += one_m_pf * ((Real)1 - ratio) * p1;
                     if (kc + 1 < Uc) {
                         int n1[C];
 #pragma unroll
@@ -372,7 +364,7 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
                         Real p2 = 0;
                         CompositeBleach<Real, C, 0, false>::eval(Ain, Bin, e - 1, t, n1, lo, hi, str, ble,
                                                                  tri, 0, (Real)1, p2, dummy);
-                        acc += one_m_pf * ((Real)(kc + 1) / (Real)Ke) * p2;
+                        acc += one_m_pf * ratio2 * p2;
                     }
                 }
             }
@@ -408,7 +400,7 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
                 int rem = ni, ei = eo;
 #pragma unroll
                 for (int c = 0; c < C; c++) {
-                    int nc = rem / str2[c];
+                    const int nc = by_str[c].div(rem);
                     rem -= nc * str2[c];
                     ei += (nc + lo2[c] - Lt[c]) * es[c];
                 }
@@ -452,7 +444,8 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
 }
 
 template <typename Real, int C, int G>
-__global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyArgs a) {
+__global__ void __launch_bounds__(kBlockThreads) WP_CLASSIFY_OCCUPANCY
+classify_kernel(const ClassifyArgs a) {
     constexpr bool kRescale = sizeof(Real) == 4;
     extern __shared__ __align__(16) unsigned char smem[];
     const int r = blockIdx.x;
@@ -479,7 +472,7 @@ __global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyA
     int* list = (int*)(smem + lay.list);
     int* scan = (int*)(smem + lay.scan);
     double* gres = (double*)(smem + lay.gres);
-    int* gseq_all = (int*)(smem + lay.gseq);
+    int* gseq_all = (int*)(smem + lay.gseq);  // per group: seq[T], K[T], 1/K[T]
     Real* state = (Real*)(smem + lay.state);
 
     const int ncand = a.cand_off ? a.cand_off[r + 1] - a.cand_off[r] : a.S;
@@ -537,61 +530,13 @@ __global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyA
     bool read_empty = (nbr == 0);
     // The pair is non-empty iff D_c >= L[0][c] for every channel.
 
-    // ---- per-group running results ----
-    double best_v = kRescale ? -INFINITY : 0.0;  // ln P (FP32) or P (FP64)
-    int best_k = 0;
-    double tot_a = kRescale ? -INFINITY : 0.0;  // FP64: total; FP32: running max
-    double tot_b = 0.0;                           // FP32: running scaled sum
+    CandAcc<kRescale> acc;
     const int cap = T * a.NB;
     Real* st = state + (size_t)group * 4 * cap;
-    int* gseq = gseq_all + group * 2 * T;
+    int* gseq = gseq_all + group * 3 * T;
     int* gK = gseq + T;
-
     for (int base = 0; base < ncand; base += nthreads * 4) {
-        // compact the non-empty candidates of this chunk, keeping order
-        int cnt = 0;
-        int kk0 = base + tid * 4;
-        int flags = 0;
-        for (int q = 0; q < 4; q++) {
-            int kk = kk0 + q;
-            if (kk >= ncand) break;
-            int i = cidx ? cidx[kk] : kk;
-            bool ok = !read_empty;
-            if (ok) {
-                for (int c = 0; c < C; c++) ok = ok && (int)a.dsD[(size_t)i * C + c] >= sL[c];
-            }
-            if (ok) {
-                flags |= 1 << q;
-                cnt++;
-            } else if (a.out_pair_logp && !cidx) {
-                a.out_pair_logp[(size_t)r * a.S + i] = -INFINITY;
-            }
-        }
-        // block exclusive scan of cnt
-        int incl = cnt;
-        const int wl = tid & 63, wv = tid >> 6;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            int y = __shfl_up(incl, off, 64);
-            if (wl >= off) incl += y;
-        }
-        if (wl == 63) scan[wv] = incl;
-        __syncthreads();
-        if (tid == 0) {
-            int acc = 0;
-            for (int q = 0; q < nthreads / 64; q++) {
-                int v = scan[q];
-                scan[q] = acc;
-                acc += v;
-            }
-            scan[nthreads / 64] = acc;
-        }
-        __syncthreads();
-        int pos = scan[wv] + incl - cnt;
-        for (int q = 0; q < 4; q++)
-            if (flags & (1 << q)) list[pos++] = kk0 + q;
-        const int nlive = scan[nthreads / 64];
-        __syncthreads();
+        const int nlive = compact_chunk<C>(a, r, ncand, cidx, base, read_empty, sL, list, scan);
         if (group < ngroups) {
             for (int q = group; q < nlive; q += ngroups) {
                 const int kk = list[q];
@@ -599,85 +544,12 @@ __global__ void __launch_bounds__(kBlockThreads) classify_kernel(const ClassifyA
                 PairResult<Real> pr = score_pair<Real, C, G, kRescale>(a, i, lane, st, cap, sL, sH, eoff,
                                                                         estr, etab, eexp, bin, gseq, gK);
                 if (a.out_pair_logp && !cidx && lane == 0) a.out_pair_logp[(size_t)r * a.S + i] = pr.logp;
-                const int cntw = a.dsCount[i];
-                if (kRescale) {
-                    if (pr.logp > best_v || (pr.logp == best_v && kk < best_k)) {
-                        best_v = pr.logp;
-                        best_k = kk;
-                    }
-                    if (cntw > 0 && pr.logp > -INFINITY) {
-                        double x = pr.logp + log((double)cntw);
-                        if (x > tot_a) {
-                            tot_b = tot_b * exp(tot_a - x) + 1.0;
-                            tot_a = x;
-                        } else {
-                            tot_b += exp(x - tot_a);
-                        }
-                    }
-                } else {
-                    // hmm-classifier.h:59-65 (strict >, first candidate wins)
-                    tot_a += pr.p * cntw;
-                    if (pr.p > best_v || (pr.p == best_v && kk < best_k)) {
-                        best_v = pr.p;
-                        best_k = kk;
-                    }
-                }
+                acc.add(pr.logp, pr.p, kk, a.dsCount[i]);
             }
         }
         __syncthreads();
     }
-    if (group < ngroups && lane == 0) {
-        gres[group * 4 + 0] = best_v;
-        gres[group * 4 + 1] = (double)best_k;
-        gres[group * 4 + 2] = tot_a;
-        gres[group * 4 + 3] = tot_b;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double bv = gres[0];
-        int bk = (int)gres[1];
-        double ta = gres[2], tb = gres[3];
-        for (int g = 1; g < ngroups; g++) {
-            double v = gres[g * 4 + 0];
-            int k = (int)gres[g * 4 + 1];
-            if (v > bv || (v == bv && k < bk)) {
-                bv = v;
-                bk = k;
-            }
-            if (kRescale) {
-                double xa = gres[g * 4 + 2], xb = gres[g * 4 + 3];
-                if (xa > ta) {
-                    tb = tb * exp(ta - xa) + xb;
-                    ta = xa;
-                } else if (xa > -INFINITY) {
-                    tb += xb * exp(xa - ta);
-                }
-            } else {
-                ta += gres[g * 4 + 2];
-            }
-        }
-        double log_score, log_total, adjusted;
-        if (kRescale) {
-            log_score = bv;
-            log_total = tb > 0 ? ta + log(tb) : -INFINITY;
-            adjusted = exp(log_score - log_total);
-        } else {
-            log_score = log(bv);
-            log_total = log(ta);
-            adjusted = bv / ta;
-        }
-        if (isnan(adjusted)) {  // hmm-classifier.h:73-76
-            adjusted = 0.0;
-            log_score = -INFINITY;
-            log_total = 0.0;
-        }
-        int id = -1;
-        if (ncand > 0) id = a.dsId[cidx ? cidx[bk] : bk];
-        a.out_id[r] = id;
-        if (a.out_adjusted) a.out_adjusted[r] = adjusted;
-        if (a.out_log_score) a.out_log_score[r] = log_score;
-        if (a.out_log_total) a.out_log_total[r] = log_total;
-    }
+    merge_and_store<kRescale>(a, r, ncand, cidx, ngroups, group, lane, acc, gres);
 }
 
 template <typename Real, int C, int G>
