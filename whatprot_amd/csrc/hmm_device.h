@@ -20,12 +20,19 @@ struct DevModel {
     double flat[WP_MAX_CHANNELS][WP_MAX_CHANNELS];
 };
 
-// ChannelModel::adjusted_mu (channel-model.h:35-55). counts[] holds C values.
+// ChannelModel::adjusted_mu (channel-model.h:35-55). counts[] holds C values;
+// C is a compile-time bound so the count arrays stay in registers.
+template <int C>
 __device__ inline double dev_adjusted_mu(const DevModel& m, int channel, const unsigned int* counts) {
 #pragma clang fp contract(off)
     double amu = m.mu[channel];
-    amu *= (double)counts[channel];
-    for (int c = 0; c < m.C; c++) {
+    unsigned int own = 0;
+#pragma unroll
+    for (int c = 0; c < C; c++)
+        if (c == channel) own = counts[c];
+    amu *= (double)own;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
         if (c == channel) {
             if (counts[c] > 1) {
                 amu *= m.flat[channel][c];
@@ -46,10 +53,11 @@ __device__ inline double dev_sigma(const DevModel& m, int channel, double amu) {
 }
 
 // ChannelModel::pdf_helper (channel-model.h:62-70).
+template <int C>
 __device__ inline double dev_pdf(const DevModel& m, int channel, double observed, const unsigned int* counts) {
 #pragma clang fp contract(off)
     const double pi = 3.141592653589793238;
-    double amu = dev_adjusted_mu(m, channel, counts);
+    double amu = dev_adjusted_mu<C>(m, channel, counts);
     double offset = observed - amu;
     double s = dev_sigma(m, channel, amu);
     return (1.0 / (s * sqrt(2.0 * pi))) * exp(-offset * offset / (2.0 * s * s));
@@ -59,20 +67,22 @@ __device__ inline double dev_pdf(const DevModel& m, int channel, double observed
 // (peptide-emission.cc:49-164). x[] holds the C intensities of that timestep.
 // FMA contraction is off so every comparison sees the same doubles as the
 // reference's x86-64 build.
+template <int C>
 __device__ inline void dev_emission_range(const DevModel& m, const double* x, unsigned int* lo,
                                           unsigned int* hi) {
 #pragma clang fp contract(off)
     const unsigned int M = (unsigned int)m.M;
-    const int C = m.C;
     if (m.uncut) {
+#pragma unroll
         for (int c = 0; c < C; c++) {
             lo[c] = 0;
             hi[c] = M + 1;
         }
         return;
     }
-    unsigned int counts_lo[WP_MAX_CHANNELS];
-    unsigned int counts_hi[WP_MAX_CHANNELS];
+    unsigned int counts_lo[C];
+    unsigned int counts_hi[C];
+#pragma unroll
     for (int c = 0; c < C; c++) {
         counts_lo[c] = 0;
         counts_hi[c] = 0;
@@ -82,12 +92,13 @@ __device__ inline void dev_emission_range(const DevModel& m, const double* x, un
     while (did_change && guard < 4096) {
         did_change = false;
         guard++;
+#pragma unroll
         for (int c = 0; c < C; c++) {
             unsigned int old = counts_hi[c];
             counts_hi[c] = 1;
-            double mu1 = dev_adjusted_mu(m, c, counts_hi);
+            double mu1 = dev_adjusted_mu<C>(m, c, counts_hi);
             counts_hi[c] = old;
-            double mu = dev_adjusted_mu(m, c, counts_hi);
+            double mu = dev_adjusted_mu<C>(m, c, counts_hi);
             double s = m.cutoff * dev_sigma(m, c, mu);
             long rounded = lround((x[c] + s) / mu1);
             unsigned int new_cts = (unsigned int)rounded;
@@ -98,12 +109,12 @@ __device__ inline void dev_emission_range(const DevModel& m, const double* x, un
             }
         }
     }
+#pragma unroll
     for (int c = 0; c < C; c++) {
         unsigned int cmin = M + 1;
-        unsigned int old_lo = counts_lo[c];
         for (unsigned int d = 0; d < M + 1; d++) {
             counts_lo[c] = d;
-            double mu = dev_adjusted_mu(m, c, counts_lo);
+            double mu = dev_adjusted_mu<C>(m, c, counts_lo);
             double s = m.cutoff * dev_sigma(m, c, mu);
             if (mu + s > x[c]) {
                 cmin = d;
@@ -111,12 +122,12 @@ __device__ inline void dev_emission_range(const DevModel& m, const double* x, un
             }
         }
         lo[c] = cmin;
-        counts_lo[c] = old_lo;
+        counts_lo[c] = 0;
         unsigned int cmax = M + 1;
         unsigned int old_hi = counts_hi[c];
         for (unsigned int d = cmin; d < M + 1; d++) {
             counts_hi[c] = d;
-            double mu = dev_adjusted_mu(m, c, counts_hi);
+            double mu = dev_adjusted_mu<C>(m, c, counts_hi);
             double s = m.cutoff * dev_sigma(m, c, mu);
             if (mu - s > x[c]) {
                 cmax = d;

@@ -52,9 +52,9 @@
 #include "classify_common.h"
 #include "hmm_kernels.h"
 
-// Register budget of the classify kernel: 4 waves per SIMD unless overridden.
+// Register budget of the gather kernel: 3 waves per SIMD (no spills) unless overridden.
 #ifndef WP_WAVES_PER_EU
-#define WP_WAVES_PER_EU 4
+#define WP_WAVES_PER_EU 3
 #endif
 #if WP_WAVES_PER_EU > 0
 #define WP_CLASSIFY_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WP_WAVES_PER_EU, 8)))
@@ -96,15 +96,18 @@ __host__ __device__ inline Layout make_layout(int T, int C, int tri, int NB, int
 }
 
 // Ranges of one (read, timestep).
+template <int C>
 __global__ void range_kernel(DevModel m, int N, const double* __restrict__ rads,
                              uint8_t* __restrict__ lo_out, uint8_t* __restrict__ hi_out) {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int T = m.T, C = m.C;
+    const int T = m.T;
     if (idx >= N * T) return;
-    double x[WP_MAX_CHANNELS];
+    double x[C];
+#pragma unroll
     for (int c = 0; c < C; c++) x[c] = rads[(size_t)idx * C + c];
-    unsigned int lo[WP_MAX_CHANNELS], hi[WP_MAX_CHANNELS];
-    dev_emission_range(m, x, lo, hi);
+    unsigned int lo[C], hi[C];
+    dev_emission_range<C>(m, x, lo, hi);
+#pragma unroll
     for (int c = 0; c < C; c++) {
         lo_out[(size_t)idx * C + c] = (uint8_t)lo[c];
         hi_out[(size_t)idx * C + c] = (uint8_t)hi[c];
@@ -516,7 +519,7 @@ classify_kernel(const ClassifyArgs a) {
         }
         double v = 1.0;
 #pragma unroll
-        for (int c = 0; c < C; c++) v *= dev_pdf(a.m, c, s_rad[t * C + c], n);
+        for (int c = 0; c < C; c++) v *= dev_pdf<C>(a.m, c, s_rad[t * C + c], n);
         if constexpr (kRescale) {
             int ex = 0;
             double mant = frexp(v, &ex);
@@ -631,8 +634,14 @@ void launch_read_prep(const DevModel& m, int N, const double* d_rads, uint8_t* d
                       int* d_nbmax, hipStream_t stream) {
     int total = N * m.T;
     if (total > 0) {
-        hipLaunchKernelGGL(range_kernel, dim3((total + 255) / 256), dim3(256), 0, stream, m, N, d_rads,
-                           d_Lr, d_Hr);
+        const dim3 grid((total + 255) / 256), block(256);
+        switch (m.C) {
+        case 1: hipLaunchKernelGGL(range_kernel<1>, grid, block, 0, stream, m, N, d_rads, d_Lr, d_Hr); break;
+        case 2: hipLaunchKernelGGL(range_kernel<2>, grid, block, 0, stream, m, N, d_rads, d_Lr, d_Hr); break;
+        case 3: hipLaunchKernelGGL(range_kernel<3>, grid, block, 0, stream, m, N, d_rads, d_Lr, d_Hr); break;
+        case 4: hipLaunchKernelGGL(range_kernel<4>, grid, block, 0, stream, m, N, d_rads, d_Lr, d_Hr); break;
+        default: break;
+        }
         hipLaunchKernelGGL(bounds_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, m, N, d_Lr, d_Hr,
                            d_nbmax);
     }

@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(kBlockThreads) classify_v3_kernel(const Classi
             }
             double v = 1.0;
 #pragma unroll
-            for (int c = 0; c < C; c++) v *= dev_pdf(a.m, c, s_rad[t * C + c], n);
+            for (int c = 0; c < C; c++) v *= dev_pdf<C>(a.m, c, s_rad[t * C + c], n);
             if constexpr (kRescale) {
                 int ex = 0;
                 double mant = frexp(v, &ex);
