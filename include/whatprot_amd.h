@@ -189,6 +189,90 @@ int wp_hmm_work(wp_hmm_classifier* clf, int num_reads, const double* radiometrie
                 double* out3);
 
 /* ---------------------------------------------------------------------------
+ * Baum-Welch fit. Replaces HMMFitter (fitters/hmm-fitter.h:30-151) as run by
+ * run_fit (main/run-fit.cc:37-140): one dye-seq, uncut ranges (the reference
+ * sets dist_cutoff = DBL_MAX for fitting, run-fit.cc:64-65).
+ *
+ * Sufficient statistics of one E-step, the fields of SequencingModelFitter
+ * (parameterization/fit/sequencing-model-fitter.h:33-38) summed over reads,
+ * as one FP64 array of WP_FIT_STATS_LEN(T, C):
+ *   [WP_FIT_P_EDMAN_FAILURE + 0/1]  p_edman_failure numerator, denominator
+ *   [WP_FIT_P_INITIAL_BLOCK + 0/1]  p_initial_block numerator, denominator
+ *   [WP_FIT_P_CYCLIC_BLOCK + 0/1]   p_cyclic_block numerator, denominator
+ *   [WP_FIT_DETACH_X + t]           p_detach x_t, t = 0..T-2
+ *   [WP_FIT_DETACH_X + T-1 + t]     p_detach n_t
+ *                                   (DecayingParameterFitter::add_timestep)
+ *   [WP_FIT_CHANNEL(T, c) + 0/1]    channel c p_bleach numerator, denominator
+ *   [WP_FIT_CHANNEL(T, c) + 2/3]    channel c p_dud numerator, denominator
+ * ------------------------------------------------------------------------- */
+#define WP_FIT_STATS_LEN(T, C) (6 + 2 * ((T) - 1) + 4 * (C))
+#define WP_FIT_P_EDMAN_FAILURE 0
+#define WP_FIT_P_INITIAL_BLOCK 2
+#define WP_FIT_P_CYCLIC_BLOCK 4
+#define WP_FIT_DETACH_X 6
+#define WP_FIT_CHANNEL(T, c) (6 + 2 * ((T) - 1) + 4 * (c))
+
+/* FitSettings (parameterization/settings/fit-settings.h:20-37,
+ * channel-fit-settings.h:20-26): non-zero holds a parameter at its starting
+ * value. bg_sig, mu and sig are never fitted (channel-model-fitter.cc:39-50). */
+typedef struct wp_fit_settings {
+    int hold_p_edman_failure;
+    int hold_p_detach;
+    int hold_p_initial_detach;
+    int hold_p_initial_detach_decay;
+    int hold_p_initial_block;
+    int hold_p_cyclic_block;
+    int hold_p_bleach[WP_MAX_CHANNELS];
+    int hold_p_dud[WP_MAX_CHANNELS];
+} wp_fit_settings;
+
+typedef struct wp_hmm_fitter wp_hmm_fitter;
+
+/* Replaces HMMFitter(T, C, ..., dye_seq) (hmm-fitter.h:35-43,
+ * hmm-fitter.cc:30-61) with its reads: parses the dye-seq, copies
+ * num_reads radiometries [read][t][c] (already divided by mu,
+ * radiometries-io.cc:50-51) to the device and sizes the E-step. precision:
+ * WP_PRECISION_FP64 (the reference's arithmetic) or WP_PRECISION_FP32 (FP32
+ * forward/backward with power-of-two rescaling, FP64 statistics). */
+int wp_hmm_fitter_create(unsigned int num_timesteps, unsigned int num_channels,
+                         const char* dye_string, int num_reads, const double* radiometries,
+                         int precision, wp_hmm_fitter** out);
+
+void wp_hmm_fitter_destroy(wp_hmm_fitter* fitter);
+
+/* One E-step under seq_model (mu = 1): the sufficient statistics of every
+ * read's PeptideHMM::improve_fit summed over reads (hmm-fitter.h:63-78) and
+ * sum_r ln P_r (HMMFitter::log_likelihood, hmm-fitter.h:120-140). Either
+ * output may be NULL. A read with P = 0 adds no statistics and -inf to the
+ * sum (generic-hmm.h:77-87). */
+int wp_hmm_fitter_estep(wp_hmm_fitter* fitter, const wp_model* seq_model, double* out_stats,
+                        double* out_sum_log_p);
+
+/* The M-step of HMMFitter::fit (hmm-fitter.h:80-104): the dud correction for
+ * reads hidden by all-dud dye-seqs, SequencingModelFitter::get
+ * (sequencing-model-fitter.cc:68-78, decaying-parameter-fitter.cc:42-66) and
+ * update_with_holds (hmm-fitter.cc:63-97). out_step_size receives
+ * SequencingModel::distance (sequencing-model.cc:153-166). settings may be
+ * NULL (nothing held). Host only. */
+int wp_fit_mstep(unsigned int num_timesteps, unsigned int num_channels, const char* dye_string,
+                 const wp_model* seq_model, int num_reads, const double* stats,
+                 const wp_fit_settings* settings, wp_model* out_model, double* out_step_size);
+
+/* HMMFitter::fit (hmm-fitter.h:50-116): EM until the step size falls below
+ * stopping_threshold or max_runtime seconds have passed, returning in
+ * out_log_l the log-likelihood under the fitted model. max_iterations > 0
+ * also stops after that many iterations (no reference counterpart; the
+ * reference stops on the threshold or the clock only). */
+int wp_hmm_fit(wp_hmm_fitter* fitter, const wp_model* seq_model, const wp_fit_settings* settings,
+               double stopping_threshold, double max_runtime, int max_iterations,
+               wp_model* out_model, double* out_step_size, double* out_log_l, int* out_iterations);
+
+/* least_squares_fit_of_offset_exponential (util/levmar-helper.cc:44-85):
+ * weighted fit of y_t ~ p[0] + p[1] exp(-p[2] t) for t = 0..n-1 within
+ * [0, 1]^3, hold[i] != 0 pinning p[i]; p is updated in place. Host only. */
+int wp_fit_offset_exponential(int n, const double* y, const double* w, const int* hold, double* p);
+
+/* ---------------------------------------------------------------------------
  * Synthetic inputs (host). Seeded and portable; the streams differ from the
  * reference's clock-seeded std::default_random_engine, the distributions
  * follow its simulator.

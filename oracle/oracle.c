@@ -215,6 +215,7 @@ typedef struct {
     obox tb; /* edman: true_backward */
     int allow;   /* emission allow_detached */
     int det_fwd; /* detach detached_forward */
+    int det_bwd; /* detach detached_backward */
 } ostep;
 
 typedef struct {
@@ -332,9 +333,13 @@ static void plan_build(oplan* p, const wp_model* m, unsigned int T, unsigned int
         switch (s->kind) {
         case ST_BLOCK_INITIAL:
         case ST_BLOCK_CYCLIC: /* block-transition.cc:26-29 */
-        case ST_DETACH:       /* detach-transition.cc:30-34 */
             s->a = box_intersect(&s->a, &r, nd);
             r = s->a;
+            break;
+        case ST_DETACH: /* detach-transition.cc:30-34 */
+            s->a = box_intersect(&s->a, &r, nd);
+            r = s->a;
+            s->det_bwd = allow;
             break;
         case ST_DUD:
         case ST_BLEACH: /* binomial-transition.cc:83-89 */
@@ -391,7 +396,7 @@ int wpo_emission_boxes(const wp_model* m, unsigned int T, unsigned int C,
     return ok;
 }
 
-/* ---- forward (generic-hmm.h:41-55 over the steps' forward operators) ------- */
+/* ---- step operators over dense tensors ------------------------------------ */
 
 static size_t flat(const oplan* p, const unsigned int* loc) {
     size_t i = 0;
@@ -399,197 +404,575 @@ static size_t flat(const oplan* p, const unsigned int* loc) {
     return i;
 }
 
+/* A PeptideStateVector (peptide-state-vector.cc:20-58): A = tensor,
+ * B = broken_n_tensor, delta = p_detached, allow = allow_detached. */
+typedef struct {
+    double *A, *B;
+    double delta;
+    int allow;
+} ostate;
+
+#define WPO_TRI ((WP_MAX_DYES + 1) * (WP_MAX_DYES + 2) / 2)
+
+/* Everything one (dye-seq, radiometry) pair needs besides its plan. */
+typedef struct {
+    const wp_model* m;
+    const oplan* p;
+    const short* seq;
+    int len;
+    const unsigned int* track;
+    unsigned int C, M;
+    size_t esz;  /* (M+1)^C */
+    double* E;   /* [T][esz] emission densities on each timestep's own range */
+    double qdud[WP_MAX_CHANNELS][WPO_TRI];
+    double qble[WP_MAX_CHANNELS][WPO_TRI];
+} octx;
+
+static void ctx_init(octx* x, const wp_model* m, const oplan* p, const short* seq, int len,
+                     const unsigned int* track, const double* rad, unsigned int M) {
+    unsigned int C = p->C;
+    x->m = m;
+    x->p = p;
+    x->seq = seq;
+    x->len = len;
+    x->track = track;
+    x->C = C;
+    x->M = M;
+    x->esz = 1;
+    for (unsigned int c = 0; c < C; c++) x->esz *= (M + 1);
+    x->E = (double*)calloc(x->esz * p->T, sizeof(double));
+    for (unsigned int c = 0; c < C; c++) { /* universal-precomputations.cc:23-59 */
+        wpo_binomial_table(m->channels[c].p_dud, M, x->qdud[c]);
+        wpo_binomial_table(m->channels[c].p_bleach, M, x->qble[c]);
+    }
+    /* PeptideEmission's tensor over its own range (peptide-emission.cc:165-181) */
+    for (unsigned int t = 0; t < p->T; t++) {
+        const obox* own = &p->own[t];
+        unsigned int n[WP_MAX_CHANNELS];
+        obox tr;
+        for (unsigned int c = 0; c < C; c++) {
+            tr.lo[c] = own->lo[1 + c];
+            tr.hi[c] = own->hi[1 + c];
+        }
+        if (box_empty(&tr, (int)C)) continue;
+        memcpy(n, tr.lo, sizeof(n));
+        do {
+            size_t ei = 0;
+            for (unsigned int c = 0; c < C; c++) ei = ei * (M + 1) + n[c];
+            double v = 1.0;
+            for (unsigned int c = 0; c < C; c++) v *= wpo_pdf(m, (int)c, rad[t * C + c], n);
+            x->E[t * x->esz + ei] = v;
+        } while (box_next(&tr, (int)C, n));
+    }
+}
+
+static double emis(const octx* x, unsigned int t, const unsigned int* loc) {
+    size_t ei = 0;
+    for (unsigned int c = 0; c < x->C; c++) ei = ei * (x->M + 1) + loc[1 + c];
+    return x->E[t * x->esz + ei];
+}
+
+static int dye_at(const octx* x, unsigned int e) { /* dye-seq.cc:52-58 */
+    return (int)e < x->len ? x->seq[e] : -1;
+}
+
+/* One step forward: out = step(in). out's tensors arrive zeroed. */
+static void fwd_step(const octx* x, const ostep* s, const ostate* in, ostate* out) {
+    const oplan* p = x->p;
+    const wp_model* m = x->m;
+    int nd = (int)p->nd;
+    unsigned int loc[WPO_MAXD];
+    out->allow = in->allow;
+    out->delta = in->allow ? in->delta : 0.0;
+    switch (s->kind) {
+    case ST_BLOCK_INITIAL:
+    case ST_BLOCK_CYCLIC: { /* block-transition.cc:31-60 */
+        double pb = s->kind == ST_BLOCK_INITIAL ? m->p_initial_block : m->p_cyclic_block;
+        memcpy(loc, s->a.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            out->A[i] = (1 - pb) * in->A[i];
+            out->B[i] = in->B[i] + pb * in->A[i];
+        } while (box_next(&s->a, nd, loc));
+        break;
+    }
+    case ST_DUD:
+    case ST_BLEACH: { /* binomial-transition.cc:91-136 */
+        int c = s->channel;
+        const double* P = s->kind == ST_DUD ? x->qdud[c] : x->qble[c];
+        obox fib = s->b; /* fibers: dim 1+c fixed at its low end */
+        fib.hi[1 + c] = fib.lo[1 + c] + 1;
+        memcpy(loc, fib.lo, sizeof(loc));
+        do {
+            unsigned int l2[WPO_MAXD];
+            memcpy(l2, loc, sizeof(l2));
+            for (unsigned int to = s->b.lo[1 + c]; to < s->b.hi[1 + c]; to++) {
+                double va = 0.0, vb = 0.0;
+                unsigned int from0 = to > s->a.lo[1 + c] ? to : s->a.lo[1 + c];
+                for (unsigned int from = from0; from < s->a.hi[1 + c]; from++) {
+                    l2[1 + c] = from;
+                    size_t i = flat(p, l2);
+                    double pr = P[from * (from + 1) / 2 + to];
+                    va += pr * in->A[i];
+                    vb += pr * in->B[i];
+                }
+                l2[1 + c] = to;
+                size_t o = flat(p, l2);
+                out->A[o] = va;
+                out->B[o] = vb;
+            }
+        } while (box_next(&fib, nd, loc));
+        break;
+    }
+    case ST_DETACH: { /* detach-transition.cc:36-68 */
+        double pd = wpo_p_detach(m, (unsigned int)s->t);
+        double sa = 0.0, sb = 0.0;
+        memcpy(loc, s->a.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            out->A[i] = in->A[i] * (1 - pd);
+            sa += in->A[i];
+        } while (box_next(&s->a, nd, loc));
+        memcpy(loc, s->a.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            out->B[i] = in->B[i] * (1 - pd);
+            sb += in->B[i];
+        } while (box_next(&s->a, nd, loc));
+        double sum = sa + sb;
+        out->delta = 0.0;
+        if (s->det_bwd) out->delta = s->det_fwd ? in->delta + pd * sum : pd * sum;
+        out->allow = s->det_bwd;
+        break;
+    }
+    case ST_EDMAN: { /* edman-transition.cc:62-162 */
+        double pf = m->p_edman_failure;
+        size_t ts = p->strides[0];
+        memcpy(loc, s->a.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            double f = in->A[i];
+            unsigned int e = loc[0];
+            int c = dye_at(x, e);
+            out->A[i] += pf * f;
+            if (c == -1) {
+                out->A[i + ts] += (1 - pf) * f;
+            } else {
+                unsigned int k = loc[1 + c];
+                unsigned int K = x->track[e * x->C + (unsigned int)c];
+                double ratio = (double)k / (double)K;
+                size_t cs = p->strides[1 + c];
+                if (k < K) out->A[i + ts] += (1 - pf) * (1 - ratio) * f;
+                if (k > 0) out->A[i + ts - cs] += (1 - pf) * ratio * f;
+            }
+        } while (box_next(&s->a, nd, loc));
+        memcpy(loc, s->a.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            out->B[i] = in->B[i];
+        } while (box_next(&s->a, nd, loc));
+        break;
+    }
+    case ST_EMIT: { /* peptide-emission.cc:212-243 */
+        unsigned int t = (unsigned int)s->t;
+        memcpy(loc, s->a.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            double e = emis(x, t, loc);
+            out->A[i] = in->A[i] * e;
+            out->B[i] = in->B[i] * e;
+        } while (box_next(&s->a, nd, loc));
+        unsigned int zero[WPO_MAXD] = {0};
+        out->delta = s->allow ? in->delta * emis(x, t, zero) : 0.0;
+        out->allow = s->allow;
+        break;
+    }
+    }
+}
+
+/* One step backward: out = step.backward(in). out's tensors arrive zeroed. */
+static void bwd_step(const octx* x, const ostep* s, const ostate* in, ostate* out) {
+    const oplan* p = x->p;
+    const wp_model* m = x->m;
+    int nd = (int)p->nd;
+    unsigned int loc[WPO_MAXD];
+    out->allow = in->allow;
+    out->delta = in->allow ? in->delta : 0.0;
+    switch (s->kind) {
+    case ST_BLOCK_INITIAL:
+    case ST_BLOCK_CYCLIC: { /* block-transition.cc:62-91 */
+        double pb = s->kind == ST_BLOCK_INITIAL ? m->p_initial_block : m->p_cyclic_block;
+        memcpy(loc, s->a.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            out->A[i] = pb * in->B[i] + (1 - pb) * in->A[i];
+            out->B[i] = in->B[i];
+        } while (box_next(&s->a, nd, loc));
+        break;
+    }
+    case ST_DUD:
+    case ST_BLEACH: { /* binomial-transition.cc:138-183 */
+        int c = s->channel;
+        const double* P = s->kind == ST_DUD ? x->qdud[c] : x->qble[c];
+        obox fib = s->a;
+        fib.hi[1 + c] = fib.lo[1 + c] + 1;
+        memcpy(loc, fib.lo, sizeof(loc));
+        do {
+            unsigned int l2[WPO_MAXD];
+            memcpy(l2, loc, sizeof(l2));
+            for (int from = (int)s->a.hi[1 + c] - 1; from >= (int)s->a.lo[1 + c]; from--) {
+                double va = 0.0, vb = 0.0;
+                int to_min = (int)s->b.lo[1 + c];
+                int to_max = from < (int)s->b.hi[1 + c] - 1 ? from : (int)s->b.hi[1 + c] - 1;
+                for (int to = to_min; to <= to_max; to++) {
+                    l2[1 + c] = (unsigned int)to;
+                    size_t i = flat(p, l2);
+                    double pr = P[from * (from + 1) / 2 + to];
+                    va += pr * in->A[i];
+                    vb += pr * in->B[i];
+                }
+                l2[1 + c] = (unsigned int)from;
+                size_t o = flat(p, l2);
+                out->A[o] = va;
+                out->B[o] = vb;
+            }
+        } while (box_next(&fib, nd, loc));
+        break;
+    }
+    case ST_DETACH: { /* detach-transition.cc:70-103 */
+        double pd = wpo_p_detach(m, (unsigned int)s->t);
+        memcpy(loc, s->a.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            out->A[i] = (1 - pd) * in->A[i];
+            if (s->det_bwd) out->A[i] += pd * in->delta;
+        } while (box_next(&s->a, nd, loc));
+        memcpy(loc, s->a.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            out->B[i] = (1 - pd) * in->B[i];
+            if (s->det_bwd) out->B[i] += pd * in->delta;
+        } while (box_next(&s->a, nd, loc));
+        out->delta = (s->det_fwd && s->det_bwd) ? in->delta : 0.0;
+        out->allow = s->det_fwd;
+        break;
+    }
+    case ST_EDMAN: { /* edman-transition.cc:164-277 */
+        double pf = m->p_edman_failure;
+        size_t ts = p->strides[0];
+        memcpy(loc, s->tb.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            double f = in->A[i];
+            unsigned int e = loc[0];
+            out->A[i] += pf * f;
+            if (e > 0) {
+                int c = dye_at(x, e - 1);
+                if (c == -1) {
+                    out->A[i - ts] += (1 - pf) * f;
+                } else {
+                    unsigned int K = x->track[(e - 1) * x->C + (unsigned int)c];
+                    size_t cs = p->strides[1 + c];
+                    unsigned int k = loc[1 + c];
+                    if (k < K) out->A[i - ts] += (1 - pf) * (1 - (double)k / (double)K) * f;
+                    /* the reference's tensor here spans safe_forward_range, one
+                     * wider than the dye-seq's shape; entries past the shape
+                     * are never read again, so they are not kept */
+                    if (k + 1 < p->shape[1 + c])
+                        out->A[i - ts + cs] += (1 - pf) * ((double)(k + 1) / (double)K) * f;
+                }
+            }
+        } while (box_next(&s->tb, nd, loc));
+        memcpy(loc, s->tb.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            out->B[i] = in->B[i];
+        } while (box_next(&s->tb, nd, loc));
+        break;
+    }
+    case ST_EMIT: { /* peptide-emission.cc:212-248 */
+        unsigned int t = (unsigned int)s->t;
+        memcpy(loc, s->a.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            double e = emis(x, t, loc);
+            out->A[i] = in->A[i] * e;
+            out->B[i] = in->B[i] * e;
+        } while (box_next(&s->a, nd, loc));
+        unsigned int zero[WPO_MAXD] = {0};
+        out->delta = s->allow ? in->delta * emis(x, t, zero) : 0.0;
+        out->allow = s->allow;
+        break;
+    }
+    }
+}
+
+/* Source entry: the last element of the first Edman slice of a tensor over
+ * forward_range (peptide-state-vector.cc:38-40, 63-65). */
+static size_t source_index(const oplan* p) {
+    unsigned int loc[WPO_MAXD];
+    loc[0] = p->forward_range.lo[0];
+    for (unsigned int o = 1; o < p->nd; o++) loc[o] = p->forward_range.hi[o] - 1;
+    return flat(p, loc);
+}
+
+static void state_alloc(ostate* s, size_t n) {
+    s->A = (double*)calloc(n, sizeof(double));
+    s->B = (double*)calloc(n, sizeof(double));
+    s->delta = 0.0;
+    s->allow = 1; /* PeptideStateVector ctor (peptide-state-vector.cc:33-36) */
+}
+
+static void state_clear(ostate* s, size_t n) {
+    memset(s->A, 0, n * sizeof(double));
+    memset(s->B, 0, n * sizeof(double));
+}
+
+static void state_free(ostate* s) {
+    free(s->A);
+    free(s->B);
+}
+
+static int pair_setup(oplan* p, octx* x, const wp_model* m, unsigned int T, unsigned int C,
+                      const short* seq, int len, unsigned int* track, const double* rad,
+                      unsigned int M, double cutoff) {
+    dye_track(T, C, seq, len, track);
+    plan_build(p, m, T, C, track, rad, M, cutoff);
+    if (p->empty) return 0;
+    ctx_init(x, m, p, seq, len, track, rad, M);
+    return 1;
+}
+
+/* PeptideHMM::probability (peptide-hmm.cc:94-100, generic-hmm.h:41-55). */
 double wpo_pair_probability(const wp_model* m, unsigned int T, unsigned int C,
                             const short* seq, int len, const double* rad,
                             unsigned int M, double cutoff) {
     unsigned int track[WP_MAX_TIMESTEPS * WP_MAX_CHANNELS];
-    dye_track(T, C, seq, len, track);
     oplan p;
-    plan_build(&p, m, T, C, track, rad, M, cutoff);
-    if (p.empty) { /* peptide-hmm.cc:94-97 */
+    octx x;
+    if (!pair_setup(&p, &x, m, T, C, seq, len, track, rad, M, cutoff)) {
         free(p.steps);
         return 0.0;
     }
-    int nd = (int)p.nd;
-    size_t n = p.size;
-    double* A = (double*)calloc(n, sizeof(double));
-    double* B = (double*)calloc(n, sizeof(double));
-    double* A2 = (double*)calloc(n, sizeof(double));
-    double* B2 = (double*)calloc(n, sizeof(double));
-    /* emission density table over [0, M+1)^C, filled on the own range only */
-    size_t esz = 1;
-    for (unsigned int c = 0; c < C; c++) esz *= (M + 1);
-    double* E = (double*)malloc(esz * sizeof(double));
-    double delta = 0.0;
-    int allow = 1; /* PeptideStateVector ctor (peptide-state-vector.cc:33-36) */
-    double qdud[WP_MAX_CHANNELS][(WP_MAX_DYES + 1) * (WP_MAX_DYES + 2) / 2];
-    double qble[WP_MAX_CHANNELS][(WP_MAX_DYES + 1) * (WP_MAX_DYES + 2) / 2];
-    for (unsigned int c = 0; c < C; c++) {
-        wpo_binomial_table(m->channels[c].p_dud, M, qdud[c]);
-        wpo_binomial_table(m->channels[c].p_bleach, M, qble[c]);
-    }
-    /* initialize_from_start: last element of the first Edman slice of the
-     * forward_range tensor (peptide-state-vector.cc:38-40). */
-    {
-        unsigned int loc[WPO_MAXD];
-        loc[0] = p.forward_range.lo[0];
-        for (int o = 1; o < nd; o++) loc[o] = p.forward_range.hi[o] - 1;
-        A[flat(&p, loc)] = 1.0;
-    }
-    unsigned int loc[WPO_MAXD];
+    ostate a, b;
+    state_alloc(&a, p.size);
+    state_alloc(&b, p.size);
+    a.A[source_index(&p)] = 1.0;
     for (int si = 0; si < p.nsteps; si++) {
-        const ostep* s = &p.steps[si];
-        memset(A2, 0, n * sizeof(double));
-        memset(B2, 0, n * sizeof(double));
-        switch (s->kind) {
-        case ST_BLOCK_INITIAL:
-        case ST_BLOCK_CYCLIC: { /* block-transition.cc:31-60 */
-            double pb = s->kind == ST_BLOCK_INITIAL ? m->p_initial_block : m->p_cyclic_block;
-            memcpy(loc, s->a.lo, sizeof(loc));
-            do {
-                size_t i = flat(&p, loc);
-                A2[i] = (1 - pb) * A[i];
-                B2[i] = B[i] + pb * A[i];
-            } while (box_next(&s->a, nd, loc));
-            if (!allow) delta = 0.0;
-            break;
-        }
-        case ST_DUD:
-        case ST_BLEACH: { /* binomial-transition.cc:91-136 */
-            int c = s->channel;
-            const double* P = s->kind == ST_DUD ? qdud[c] : qble[c];
-            obox fib = s->b; /* iterate fibers: fix dim 1+c at its low end */
+        state_clear(&b, p.size);
+        fwd_step(&x, &p.steps[si], &a, &b);
+        ostate tmp = a;
+        a = b;
+        b = tmp;
+    }
+    /* PeptideStateVector::sum over the last range (peptide-state-vector.cc:52-54) */
+    const obox* fin = &p.steps[p.nsteps - 1].a;
+    unsigned int loc[WPO_MAXD];
+    double sa = 0.0, sb = 0.0;
+    memcpy(loc, fin->lo, sizeof(loc));
+    do {
+        sa += a.A[flat(&p, loc)];
+    } while (box_next(fin, (int)p.nd, loc));
+    memcpy(loc, fin->lo, sizeof(loc));
+    do {
+        sb += a.B[flat(&p, loc)];
+    } while (box_next(fin, (int)p.nd, loc));
+    double result = sa + sb + a.delta;
+    state_free(&a);
+    state_free(&b);
+    free(x.E);
+    free(p.steps);
+    return result;
+}
+
+/* ---- Baum-Welch E-step (generic-hmm.h:60-109) ----------------------------- */
+
+/* Per-step improve_fit into the flat statistics array (layout of
+ * WP_FIT_STATS_LEN in whatprot_amd.h). f = forward state entering the step,
+ * b = backward state entering it, nb = backward state leaving it. */
+static void fit_step(const octx* x, const ostep* s, const ostate* f, const ostate* b,
+                     const ostate* nb, double P, double* st) {
+    const oplan* p = x->p;
+    const wp_model* m = x->m;
+    unsigned int T = p->T;
+    int nd = (int)p->nd;
+    unsigned int loc[WPO_MAXD];
+    switch (s->kind) {
+    case ST_BLOCK_INITIAL:
+    case ST_BLOCK_CYCLIC: { /* block-transition.cc:93-116 */
+        double pb = s->kind == ST_BLOCK_INITIAL ? m->p_initial_block : m->p_cyclic_block;
+        double* acc = s->kind == ST_BLOCK_INITIAL ? &st[WP_FIT_P_INITIAL_BLOCK]
+                                                  : &st[WP_FIT_P_CYCLIC_BLOCK];
+        memcpy(loc, s->a.lo, sizeof(loc));
+        do {
+            size_t i = flat(p, loc);
+            acc[0] += f->A[i] * pb * nb->B[i] / P;
+            acc[1] += f->A[i] * b->A[i] / P;
+        } while (box_next(&s->a, nd, loc));
+        break;
+    }
+    case ST_DUD:
+    case ST_BLEACH: { /* binomial-transition.cc:185-256, dud/bleach-transition.cc:25-37 */
+        int c = s->channel;
+        const double* Pq = s->kind == ST_DUD ? x->qdud[c] : x->qble[c];
+        double* acc = &st[WP_FIT_CHANNEL(T, c) + (s->kind == ST_DUD ? 2 : 0)];
+        for (int tensor = 0; tensor < 2; tensor++) {
+            const double* fv = tensor ? f->B : f->A;
+            const double* bv = tensor ? b->B : b->A;
+            const double* nbv = tensor ? nb->B : nb->A;
+            obox fib = s->a;
             fib.hi[1 + c] = fib.lo[1 + c] + 1;
             memcpy(loc, fib.lo, sizeof(loc));
             do {
                 unsigned int l2[WPO_MAXD];
                 memcpy(l2, loc, sizeof(l2));
-                for (unsigned int to = s->b.lo[1 + c]; to < s->b.hi[1 + c]; to++) {
-                    double va = 0.0, vb = 0.0;
-                    unsigned int from0 = to > s->a.lo[1 + c] ? to : s->a.lo[1 + c];
-                    for (unsigned int from = from0; from < s->a.hi[1 + c]; from++) {
-                        l2[1 + c] = from;
-                        size_t i = flat(&p, l2);
-                        double pr = P[from * (from + 1) / 2 + to];
-                        va += pr * A[i];
-                        vb += pr * B[i];
+                int from_min = (int)s->a.lo[1 + c];
+                if (from_min < 1) from_min = 1;
+                for (int from = (int)s->a.hi[1 + c] - 1; from >= from_min; from--) {
+                    l2[1 + c] = (unsigned int)from;
+                    size_t fi = flat(p, l2);
+                    double p_state = fv[fi] * bv[fi] / P;
+                    acc[1] += p_state * (double)from;
+                    int to_max = (int)s->b.hi[1 + c] < from ? (int)s->b.hi[1 + c] : from;
+                    for (int to = (int)s->b.lo[1 + c]; to < to_max; to++) {
+                        l2[1 + c] = (unsigned int)to;
+                        size_t ti = flat(p, l2);
+                        double p_tr = fv[fi] * Pq[from * (from + 1) / 2 + to] * nbv[ti] / P;
+                        acc[0] += p_tr * (double)(from - to);
                     }
-                    l2[1 + c] = to;
-                    size_t o = flat(&p, l2);
-                    A2[o] = va;
-                    B2[o] = vb;
                 }
             } while (box_next(&fib, nd, loc));
-            if (!allow) delta = 0.0;
-            break;
         }
-        case ST_DETACH: { /* detach-transition.cc:36-68 */
-            double pd = wpo_p_detach(m, (unsigned int)s->t);
-            double sa = 0.0, sb = 0.0;
-            memcpy(loc, s->a.lo, sizeof(loc));
-            do {
-                size_t i = flat(&p, loc);
-                A2[i] = A[i] * (1 - pd);
-                sa += A[i];
-            } while (box_next(&s->a, nd, loc));
-            memcpy(loc, s->a.lo, sizeof(loc));
-            do {
-                size_t i = flat(&p, loc);
-                B2[i] = B[i] * (1 - pd);
-                sb += B[i];
-            } while (box_next(&s->a, nd, loc));
-            double sum = sa + sb;
-            /* detached_backward is always true (detach-transition.cc:33) */
-            delta = s->det_fwd ? delta + pd * sum : pd * sum;
-            allow = 1;
-            break;
-        }
-        case ST_EDMAN: { /* edman-transition.cc:62-162 */
-            double pf = m->p_edman_failure;
-            size_t ts = p.strides[0];
-            memcpy(loc, s->a.lo, sizeof(loc));
-            do {
-                size_t i = flat(&p, loc);
-                double f = A[i];
-                unsigned int e = loc[0];
-                int c = (int)e < len ? seq[e] : -1;
-                A2[i] += pf * f;
-                if (c == -1) {
-                    A2[i + ts] += (1 - pf) * f;
-                } else {
-                    unsigned int k = loc[1 + c];
-                    unsigned int K = track[e * C + (unsigned int)c];
-                    double ratio = (double)k / (double)K;
-                    size_t cs = p.strides[1 + c];
-                    if (k < K) A2[i + ts] += (1 - pf) * (1 - ratio) * f;
-                    if (k > 0) A2[i + ts - cs] += (1 - pf) * ratio * f;
-                }
-            } while (box_next(&s->a, nd, loc));
-            memcpy(loc, s->a.lo, sizeof(loc));
-            do {
-                size_t i = flat(&p, loc);
-                B2[i] = B[i];
-            } while (box_next(&s->a, nd, loc));
-            if (!allow) delta = 0.0;
-            break;
-        }
-        case ST_EMIT: { /* peptide-emission.cc:39-181, 212-243 */
-            unsigned int t = (unsigned int)s->t;
-            const obox* own = &p.own[t];
-            unsigned int n[WP_MAX_CHANNELS];
-            obox tr; /* own range without the Edman axis */
-            for (unsigned int c = 0; c < C; c++) {
-                tr.lo[c] = own->lo[1 + c];
-                tr.hi[c] = own->hi[1 + c];
-            }
-            memcpy(n, tr.lo, sizeof(n));
-            do {
-                size_t ei = 0;
-                for (unsigned int c = 0; c < C; c++) ei = ei * (M + 1) + n[c];
-                double v = 1.0;
-                for (unsigned int c = 0; c < C; c++) v *= wpo_pdf(m, (int)c, rad[t * C + c], n);
-                E[ei] = v;
-            } while (box_next(&tr, (int)C, n));
-            memcpy(loc, s->a.lo, sizeof(loc));
-            do {
-                size_t i = flat(&p, loc);
-                size_t ei = 0;
-                for (unsigned int c = 0; c < C; c++) ei = ei * (M + 1) + loc[1 + c];
-                A2[i] = A[i] * E[ei];
-                B2[i] = B[i] * E[ei];
-            } while (box_next(&s->a, nd, loc));
-            if (s->allow) delta = delta * E[0];
-            else delta = 0.0;
-            allow = s->allow;
-            break;
-        }
-        }
-        double* tmp = A;
-        A = A2;
-        A2 = tmp;
-        tmp = B;
-        B = B2;
-        B2 = tmp;
+        break;
     }
-    /* PeptideStateVector::sum over the last range (peptide-state-vector.cc:52-54) */
-    const obox* fin = &p.steps[p.nsteps - 1].a;
-    double sa = 0.0, sb = 0.0;
-    memcpy(loc, fin->lo, sizeof(loc));
-    do {
-        sa += A[flat(&p, loc)];
-    } while (box_next(fin, nd, loc));
-    memcpy(loc, fin->lo, sizeof(loc));
-    do {
-        sb += B[flat(&p, loc)];
-    } while (box_next(fin, nd, loc));
-    double result = sa + sb + delta;
-    free(A);
-    free(B);
-    free(A2);
-    free(B2);
-    free(E);
+    case ST_DETACH: { /* detach-transition.cc:105-166 */
+        double pd = wpo_p_detach(m, (unsigned int)s->t);
+        int skip = box_includes_zero(&s->a, nd);
+        for (int tensor = 0; tensor < 2; tensor++) {
+            const double* fv = tensor ? f->B : f->A;
+            const double* bv = tensor ? b->B : b->A;
+            double fs = 0.0, fbs = 0.0;
+            unsigned int old_t = 0xffffffffu;
+            memcpy(loc, s->a.lo, sizeof(loc));
+            do {
+                if (skip && loc[0] != old_t) { /* first entry of each Edman slice */
+                    old_t = loc[0];
+                    continue;
+                }
+                size_t i = flat(p, loc);
+                fs += fv[i];
+                fbs += fv[i] * bv[i];
+            } while (box_next(&s->a, nd, loc));
+            st[WP_FIT_DETACH_X + s->t] += fs * pd * nb->delta / P;
+            st[WP_FIT_DETACH_X + (T - 1) + s->t] += fbs / P;
+        }
+        break;
+    }
+    case ST_EDMAN: { /* edman-transition.cc:279-325 */
+        double pf = m->p_edman_failure;
+        int skip = box_includes_zero(&s->a, nd);
+        unsigned int old_t = 0xffffffffu;
+        memcpy(loc, s->a.lo, sizeof(loc));
+        do {
+            if (skip && loc[0] != old_t) {
+                old_t = loc[0];
+                continue;
+            }
+            size_t i = flat(p, loc);
+            st[WP_FIT_P_EDMAN_FAILURE] += f->A[i] * pf * nb->A[i] / P;
+            st[WP_FIT_P_EDMAN_FAILURE + 1] += f->A[i] * b->A[i] / P;
+        } while (box_next(&s->a, nd, loc));
+        break;
+    }
+    case ST_EMIT: /* peptide-emission.cc:255-260: no statistics */
+        break;
+    }
+}
+
+double wpo_improve_fit(const wp_model* m, unsigned int T, unsigned int C,
+                       const short* seq, int len, const double* rad,
+                       unsigned int M, double cutoff, double* stats) {
+    unsigned int track[WP_MAX_TIMESTEPS * WP_MAX_CHANNELS];
+    oplan p;
+    octx x;
+    if (!pair_setup(&p, &x, m, T, C, seq, len, track, rad, M, cutoff)) {
+        free(p.steps);
+        return 0.0;
+    }
+    int K = p.nsteps;
+    /* backward states: bs[k] enters step k from the right side's view, bs[K]
+     * is the finish state (peptide-state-vector.cc:42-50) */
+    ostate* bs = (ostate*)malloc(sizeof(ostate) * (size_t)(K + 1));
+    for (int k = 0; k <= K; k++) state_alloc(&bs[k], p.size);
+    {
+        const obox* r = &p.backward_range;
+        unsigned int loc[WPO_MAXD];
+        memcpy(loc, r->lo, sizeof(loc));
+        do {
+            size_t i = flat(&p, loc);
+            bs[K].A[i] = 1.0;
+            bs[K].B[i] = 1.0;
+        } while (box_next(r, (int)p.nd, loc));
+        bs[K].delta = 1.0;
+        bs[K].allow = 1;
+    }
+    for (int k = K - 1; k >= 0; k--) bwd_step(&x, &p.steps[k], &bs[k + 1], &bs[k]);
+    double P = bs[0].A[source_index(&p)];
+    if (P != 0.0) { /* generic-hmm.h:77-87 */
+        ostate a, b;
+        state_alloc(&a, p.size);
+        state_alloc(&b, p.size);
+        a.A[source_index(&p)] = 1.0;
+        for (int k = 0; k < K; k++) {
+            fit_step(&x, &p.steps[k], &a, &bs[k], &bs[k + 1], P, stats);
+            state_clear(&b, p.size);
+            fwd_step(&x, &p.steps[k], &a, &b);
+            ostate tmp = a;
+            a = b;
+            b = tmp;
+        }
+        state_free(&a);
+        state_free(&b);
+    }
+    for (int k = 0; k <= K; k++) state_free(&bs[k]);
+    free(bs);
+    free(x.E);
     free(p.steps);
-    return result;
+    return P;
+}
+
+void wpo_fit_estep(const wp_model* m, unsigned int T, unsigned int C,
+                   const short* seq, int len, int num_reads, const double* rads,
+                   unsigned int M, double* stats, double* sum_log_p) {
+    int L = WP_FIT_STATS_LEN(T, C);
+    int nt = wpo_num_threads();
+    double* part = (double*)calloc((size_t)nt * (L + 1), sizeof(double));
+#pragma omp parallel
+    {
+#ifdef _OPENMP
+        int tid = omp_get_thread_num();
+#else
+        int tid = 0;
+#endif
+        double* mine = &part[(size_t)tid * (L + 1)];
+        double* one = (double*)malloc(sizeof(double) * (size_t)L);
+#pragma omp for schedule(static)
+        for (int r = 0; r < num_reads; r++) {
+            /* each read fills its own fitter, then fitter += it (hmm-fitter.h:63-78) */
+            memset(one, 0, sizeof(double) * (size_t)L);
+            double P = wpo_improve_fit(m, T, C, seq, len, &rads[(size_t)r * T * C], M, DBL_MAX, one);
+            for (int i = 0; i < L; i++) mine[i] += one[i];
+            mine[L] += log(P);
+        }
+        free(one);
+    }
+    for (int i = 0; i <= L; i++) {
+        double s = 0.0;
+        for (int t = 0; t < nt; t++) s += part[(size_t)t * (L + 1) + i];
+        if (i < L) stats[i] = s;
+        else *sum_log_p = s;
+    }
+    free(part);
 }
 
 /* ---- classifier (classifiers/hmm-classifier.h:45-78) ---------------------- */

@@ -88,6 +88,21 @@ double wpo_brute_force_probability(const wp_model* m, unsigned int T,
                                    unsigned int C, const short* seq, int len,
                                    const double* rad);
 
+/* GenericHMM::improve_fit for one (dye-seq, radiometry) pair
+ * (generic-hmm.h:60-109 with every step's backward and improve_fit): adds the
+ * pair's sufficient statistics into stats (layout WP_FIT_STATS_LEN in
+ * whatprot_amd.h) and returns P. Nothing is added when P == 0. */
+double wpo_improve_fit(const wp_model* m, unsigned int T, unsigned int C,
+                       const short* seq, int len, const double* rad,
+                       unsigned int M, double cutoff, double* stats);
+
+/* One E-step of HMMFitter::fit (hmm-fitter.h:63-78): the sum over reads of
+ * the per-read statistics (uncut ranges, as run-fit.cc:64-65 sets) and the
+ * sum of ln P. OpenMP over reads, partial sums combined in thread order. */
+void wpo_fit_estep(const wp_model* m, unsigned int T, unsigned int C,
+                   const short* seq, int len, int num_reads, const double* rads,
+                   unsigned int M, double* stats, double* sum_log_p);
+
 int wpo_num_threads(void);
 
 #ifdef __cplusplus

@@ -15,10 +15,15 @@
 #include "common/radiometry.h"
 #include "common/scored-classification.h"
 #include "common/sourced-data.h"
+#include "fitters/hmm-fitter.h"
 #include "hmm/hmm/peptide-hmm.h"
 #include "hmm/precomputations/dye-seq-precomputations.h"
 #include "hmm/precomputations/radiometry-precomputations.h"
 #include "hmm/precomputations/universal-precomputations.h"
+#include "parameterization/fit/sequencing-model-fitter.h"
+#include "parameterization/settings/fit-settings.h"
+#include "util/levmar-helper.h"
+#include "levmar.h"
 #include "parameterization/model/sequencing-model.h"
 #include "parameterization/settings/sequencing-settings.h"
 #include "whatprot_amd.h"
@@ -46,6 +51,63 @@ void fill_model(const wp_model* m, unsigned int C, SequencingModel* sm) {
             cm->flat_interactions[o] = m->channels[c].flat_interactions[o];
         }
         sm->channel_models.push_back(cm);
+    }
+}
+
+// SequencingModelFitter fields into the flat layout of WP_FIT_STATS_LEN.
+void add_stats(const SequencingModelFitter& f, unsigned int T, unsigned int C, double* st) {
+    st[WP_FIT_P_EDMAN_FAILURE] += f.p_edman_failure_fit.numerator;
+    st[WP_FIT_P_EDMAN_FAILURE + 1] += f.p_edman_failure_fit.denominator;
+    st[WP_FIT_P_INITIAL_BLOCK] += f.p_initial_block_fit.numerator;
+    st[WP_FIT_P_INITIAL_BLOCK + 1] += f.p_initial_block_fit.denominator;
+    st[WP_FIT_P_CYCLIC_BLOCK] += f.p_cyclic_block_fit.numerator;
+    st[WP_FIT_P_CYCLIC_BLOCK + 1] += f.p_cyclic_block_fit.denominator;
+    for (unsigned int t = 0; t + 1 < T; t++) {
+        st[WP_FIT_DETACH_X + t] += f.p_detach_fit.xvec[t];
+        st[WP_FIT_DETACH_X + (T - 1) + t] += f.p_detach_fit.nvec[t];
+    }
+    for (unsigned int c = 0; c < C; c++) {
+        st[WP_FIT_CHANNEL(T, c) + 0] += f.channel_fits[c]->p_bleach_fit.numerator;
+        st[WP_FIT_CHANNEL(T, c) + 1] += f.channel_fits[c]->p_bleach_fit.denominator;
+        st[WP_FIT_CHANNEL(T, c) + 2] += f.channel_fits[c]->p_dud_fit.numerator;
+        st[WP_FIT_CHANNEL(T, c) + 3] += f.channel_fits[c]->p_dud_fit.denominator;
+    }
+}
+
+void read_model(const SequencingModel& sm, unsigned int C, wp_model* m) {
+    m->num_channels = (int)C;
+    m->p_edman_failure = sm.p_edman_failure;
+    m->p_detach = sm.p_detach.base;
+    m->p_initial_detach = sm.p_detach.initial;
+    m->p_initial_detach_decay = sm.p_detach.initial_decay;
+    m->p_initial_block = sm.p_initial_block;
+    m->p_cyclic_block = sm.p_cyclic_block;
+    for (unsigned int c = 0; c < C; c++) {
+        const ChannelModel& cm = *sm.channel_models[c];
+        m->channels[c].p_bleach = cm.p_bleach;
+        m->channels[c].p_dud = cm.p_dud;
+        m->channels[c].bg_sig = cm.bg_sig;
+        m->channels[c].mu = cm.mu;
+        m->channels[c].sig = cm.sig;
+        for (unsigned int o = 0; o < C; o++) {
+            m->channels[c].interactions[o] = cm.interactions[o];
+            m->channels[c].flat_interactions[o] = cm.flat_interactions[o];
+        }
+    }
+}
+
+// holds[0..5]: edman, detach, initial detach, initial detach decay, initial
+// block, cyclic block; holds[6 + 2c], holds[7 + 2c]: bleach, dud of channel c.
+void fill_fit_settings(const int* holds, unsigned int C, FitSettings* fs) {
+    fs->hold_p_edman_failure = holds && holds[0];
+    fs->hold_p_detach = holds && holds[1];
+    fs->hold_p_initial_detach = holds && holds[2];
+    fs->hold_p_initial_detach_decay = holds && holds[3];
+    fs->hold_p_initial_block = holds && holds[4];
+    fs->hold_p_cyclic_block = holds && holds[5];
+    for (unsigned int c = 0; c < C; c++) {
+        fs->channel_fit_settings[c]->hold_p_bleach = holds && holds[6 + 2 * c];
+        fs->channel_fit_settings[c]->hold_p_dud = holds && holds[7 + 2 * c];
     }
 }
 
@@ -109,6 +171,127 @@ void wpref_classify(const wp_model* m, unsigned int T, unsigned int C,
         out_score[r] = res[r].score;
         out_total[r] = res[r].total;
     }
+}
+
+// GenericHMM::improve_fit of one pair with uncut ranges, as HMMFitter::fit
+// runs it (hmm-fitter.h:63-78); adds the per-read fitter into stats.
+double wpref_improve_fit(const wp_model* m, unsigned int T, unsigned int C,
+                         const char* dye_string, const double* rad,
+                         unsigned int max_num_dyes, double* stats) {
+    SequencingModel sm;
+    fill_model(m, C, &sm);
+    SequencingSettings ss;
+    ss.dist_cutoff = DBL_MAX;
+    FitSettings fs(C);
+    UniversalPrecomputations up(sm, T, C);
+    up.set_max_num_dyes(max_num_dyes);
+    DyeSeq ds(C, std::string(dye_string));
+    DyeSeqPrecomputations dsp(ds, sm, T, C);
+    Radiometry r(T, C);
+    for (unsigned int i = 0; i < T * C; i++) r.intensities[i] = rad[i];
+    RadiometryPrecomputations rp(r, sm, ss, max_num_dyes);
+    PeptideHMM hmm(T, C, dsp, rp, up);
+    SequencingModelFitter f(T, C, sm, fs);
+    double P = hmm.improve_fit(&f);
+    add_stats(f, T, C, stats);
+    return P;
+}
+
+// least_squares_fit_of_offset_exponential (util/levmar-helper.cc:44-85).
+void wpref_offset_exponential_fit(int n, const double* y, const double* w, const int* hold,
+                                  double* p3) {
+    std::vector<double> yv(y, y + n), wv(w, w + n), p(p3, p3 + 3);
+    std::vector<bool> h(3);
+    for (int i = 0; i < 3; i++) h[i] = hold[i] != 0;
+    least_squares_fit_of_offset_exponential(yv, wv, h, &p);
+    for (int i = 0; i < 3; i++) p3[i] = p[i];
+}
+
+// The M-step alone: SequencingModelFitter::get from flat statistics plus the
+// dud correction and update_with_holds of HMMFitter::fit
+// (hmm-fitter.h:80-104, hmm-fitter.cc:63-97). Returns the step size.
+double wpref_mstep(const wp_model* m, unsigned int T, unsigned int C, const char* dye_string,
+                   int num_reads, const double* stats, const int* holds, wp_model* out) {
+    SequencingModel sm;
+    fill_model(m, C, &sm);
+    FitSettings fs(C);
+    fill_fit_settings(holds, C, &fs);
+    SequencingModelFitter f(T, C, sm, fs);
+    f.p_edman_failure_fit.numerator = stats[WP_FIT_P_EDMAN_FAILURE];
+    f.p_edman_failure_fit.denominator = stats[WP_FIT_P_EDMAN_FAILURE + 1];
+    f.p_initial_block_fit.numerator = stats[WP_FIT_P_INITIAL_BLOCK];
+    f.p_initial_block_fit.denominator = stats[WP_FIT_P_INITIAL_BLOCK + 1];
+    f.p_cyclic_block_fit.numerator = stats[WP_FIT_P_CYCLIC_BLOCK];
+    f.p_cyclic_block_fit.denominator = stats[WP_FIT_P_CYCLIC_BLOCK + 1];
+    for (unsigned int t = 0; t + 1 < T; t++) {
+        f.p_detach_fit.xvec[t] = stats[WP_FIT_DETACH_X + t];
+        f.p_detach_fit.nvec[t] = stats[WP_FIT_DETACH_X + (T - 1) + t];
+    }
+    for (unsigned int c = 0; c < C; c++) {
+        f.channel_fits[c]->p_bleach_fit.numerator = stats[WP_FIT_CHANNEL(T, c) + 0];
+        f.channel_fits[c]->p_bleach_fit.denominator = stats[WP_FIT_CHANNEL(T, c) + 1];
+        f.channel_fits[c]->p_dud_fit.numerator = stats[WP_FIT_CHANNEL(T, c) + 2];
+        f.channel_fits[c]->p_dud_fit.denominator = stats[WP_FIT_CHANNEL(T, c) + 3];
+    }
+    DyeSeq ds(C, std::string(dye_string));
+    double ratio_hidden = 1.0;
+    for (unsigned int i = 0; i < ds.length; i++)
+        if (ds[i] != -1) ratio_hidden *= sm.channel_models[ds[i]]->p_dud;
+    double magic_ratio = 1.0 / (1.0 - ratio_hidden) - 1.0;
+    double expected_hidden_count = magic_ratio * num_reads;
+    for (unsigned int i = 0; i < ds.length; i++) {
+        if (ds[i] != -1) {
+            f.channel_fits[ds[i]]->p_dud_fit.numerator += expected_hidden_count;
+            f.channel_fits[ds[i]]->p_dud_fit.denominator += expected_hidden_count;
+        }
+    }
+    SequencingModel next = sm;
+    SequencingSettings ss;
+    ss.dist_cutoff = DBL_MAX;
+    HMMFitter helper(T, C, 0.0, 0.0, sm, ss, fs, ds);
+    helper.update_with_holds(f.get(), &next);
+    read_model(next, C, out);
+    return sm.distance(next);
+}
+
+// HMMFitter::fit (hmm-fitter.h:50-116) as run_fit calls it; returns log L
+// under the fitted model.
+double wpref_fit(const wp_model* m, unsigned int T, unsigned int C, const char* dye_string,
+                 int num_reads, const double* rads, double stopping_threshold,
+                 double max_runtime, const int* holds, wp_model* out, double* step_size) {
+    SequencingModel sm;
+    fill_model(m, C, &sm);
+    SequencingSettings ss;
+    ss.dist_cutoff = DBL_MAX;
+    FitSettings fs(C);
+    fill_fit_settings(holds, C, &fs);
+    DyeSeq ds(C, std::string(dye_string));
+    std::vector<Radiometry> radiometries;
+    radiometries.reserve(num_reads);
+    for (int r = 0; r < num_reads; r++) {
+        radiometries.push_back(Radiometry(T, C));
+        for (unsigned int i = 0; i < T * C; i++)
+            radiometries.back().intensities[i] = rads[(size_t)r * T * C + i];
+    }
+    HMMFitter fitter(T, C, stopping_threshold, max_runtime, sm, ss, fs, ds);
+    SequencingModel fitted;
+    double log_l = fitter.fit(radiometries, &fitted, step_size);
+    read_model(fitted, C, out);
+    return log_l;
+}
+
+// dlevmar_bc_der exactly as least_squares_fit_of_offset_exponential calls
+// it (util/levmar-helper.cc:44-85) but with a chosen iteration cap and the
+// info vector returned, for step-by-step comparison.
+void wpref_levmar_bc(int n, const double* y, const double* w, const int* hold, double* p3, int itmax,
+                     double* info) {
+    std::vector<double> lb(3, 0), ub(3, 1), sw(n), wy(n);
+    for (int i = 0; i < 3; i++)
+        if (hold[i]) lb[i] = ub[i] = p3[i];
+    for (int i = 0; i < n; i++) sw[i] = std::sqrt(w[i]);
+    for (int i = 0; i < n; i++) wy[i] = sw[i] * y[i];
+    dlevmar_bc_der(&offset_exponential, &jacobian_of_offset_exponential, p3, wy.data(), 3, n, lb.data(),
+                   ub.data(), NULL, itmax, NULL, info, NULL, NULL, sw.data());
 }
 
 }  // extern "C"

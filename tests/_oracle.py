@@ -68,6 +68,12 @@ class Oracle:
         L.wpo_brute_force_probability.restype = c_double
         L.wpo_brute_force_probability.argtypes = [P(ModelC), c_uint, c_uint, c_void_p, c_int, c_void_p]
         L.wpo_num_threads.restype = c_int
+        L.wpo_improve_fit.restype = c_double
+        L.wpo_improve_fit.argtypes = [P(ModelC), c_uint, c_uint, c_void_p, c_int, c_void_p, c_uint, c_double,
+                                      c_void_p]
+        L.wpo_fit_estep.restype = None
+        L.wpo_fit_estep.argtypes = [P(ModelC), c_uint, c_uint, c_void_p, c_int, c_int, c_void_p, c_uint,
+                                    c_void_p, c_void_p]
         self.L = L
 
     @staticmethod
@@ -118,6 +124,28 @@ class Oracle:
         seqs, lens, _ = pack_dye_seqs([dye_string])
         return self.L.wpo_brute_force_probability(ctypes.byref(m), T, C, _ptr(seqs), int(lens[0]), _ptr(rad))
 
+    def improve_fit(self, model, dye_string, rad, M, cutoff=DBL_MAX):
+        """(P, statistics) of one pair; layout WP_FIT_STATS_LEN."""
+        m = self._m(model)
+        rad = np.ascontiguousarray(rad, dtype=np.float64)
+        T, C = rad.shape
+        seqs, lens, _ = pack_dye_seqs([dye_string])
+        st = np.zeros(fit_stats_len(T, C))
+        p = self.L.wpo_improve_fit(ctypes.byref(m), T, C, _ptr(seqs), int(lens[0]), _ptr(rad), M, cutoff, _ptr(st))
+        return p, st
+
+    def fit_estep(self, model, dye_string, rads, M):
+        """(summed statistics, sum of ln P) over reads, uncut."""
+        m = self._m(model)
+        rads = np.ascontiguousarray(rads, dtype=np.float64)
+        N, T, C = rads.shape
+        seqs, lens, _ = pack_dye_seqs([dye_string])
+        st = np.zeros(fit_stats_len(T, C))
+        slp = np.zeros(1)
+        self.L.wpo_fit_estep(ctypes.byref(m), T, C, _ptr(seqs), int(lens[0]), N, _ptr(rads), M, _ptr(st),
+                             _ptr(slp))
+        return st, float(slp[0])
+
     def pair_probabilities(self, model, dye_strings, rads, cutoff=DBL_MAX):
         m = self._m(model)
         rads = np.ascontiguousarray(rads, dtype=np.float64)
@@ -158,6 +186,16 @@ class Reference:
         L.wpref_pdf.argtypes = [P(ModelC), c_int, c_double, c_void_p]
         L.wpref_pair_probability.restype = c_double
         L.wpref_pair_probability.argtypes = [P(ModelC), c_uint, c_uint, ctypes.c_char_p, c_void_p, c_uint, c_double]
+        L.wpref_improve_fit.restype = c_double
+        L.wpref_improve_fit.argtypes = [P(ModelC), c_uint, c_uint, ctypes.c_char_p, c_void_p, c_uint, c_void_p]
+        L.wpref_offset_exponential_fit.restype = None
+        L.wpref_offset_exponential_fit.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+        L.wpref_mstep.restype = c_double
+        L.wpref_mstep.argtypes = [P(ModelC), c_uint, c_uint, ctypes.c_char_p, c_int, c_void_p, c_void_p,
+                                  P(ModelC)]
+        L.wpref_fit.restype = c_double
+        L.wpref_fit.argtypes = [P(ModelC), c_uint, c_uint, ctypes.c_char_p, c_int, c_void_p, c_double, c_double,
+                                c_void_p, P(ModelC), c_void_p]
         L.wpref_classify.restype = None
         L.wpref_classify.argtypes = [P(ModelC), c_uint, c_uint, c_int, P(ctypes.c_char_p), c_void_p, c_void_p,
                                      c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p]
@@ -180,6 +218,42 @@ class Reference:
         T, C = rad.shape
         return self.L.wpref_pair_probability(ctypes.byref(m), T, C, dye_string.encode(), _ptr(rad), M, cutoff)
 
+    def improve_fit(self, model, dye_string, rad, M):
+        m = Oracle._m(model)
+        rad = np.ascontiguousarray(rad, dtype=np.float64)
+        T, C = rad.shape
+        st = np.zeros(fit_stats_len(T, C))
+        p = self.L.wpref_improve_fit(ctypes.byref(m), T, C, dye_string.encode(), _ptr(rad), M, _ptr(st))
+        return p, st
+
+    def offset_exponential_fit(self, y, w, hold, p):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        h = np.ascontiguousarray(hold, dtype=np.int32)
+        out = np.ascontiguousarray(p, dtype=np.float64).copy()
+        self.L.wpref_offset_exponential_fit(len(y), _ptr(y), _ptr(w), _ptr(h), _ptr(out))
+        return out
+
+    def mstep(self, model, T, C, dye_string, num_reads, stats, holds=None):
+        m = Oracle._m(model)
+        st = np.ascontiguousarray(stats, dtype=np.float64)
+        h = np.ascontiguousarray(holds if holds is not None else np.zeros(6 + 2 * C), dtype=np.int32)
+        out = ModelC()
+        step = self.L.wpref_mstep(ctypes.byref(m), T, C, dye_string.encode(), num_reads, _ptr(st), _ptr(h),
+                                  ctypes.byref(out))
+        return out, step
+
+    def fit(self, model, dye_string, rads, stopping_threshold, max_runtime=1e30, holds=None):
+        m = Oracle._m(model)
+        rads = np.ascontiguousarray(rads, dtype=np.float64)
+        N, T, C = rads.shape
+        h = np.ascontiguousarray(holds if holds is not None else np.zeros(6 + 2 * C), dtype=np.int32)
+        out = ModelC()
+        step = np.zeros(1)
+        log_l = self.L.wpref_fit(ctypes.byref(m), T, C, dye_string.encode(), N, _ptr(rads), stopping_threshold,
+                                 max_runtime, _ptr(h), ctypes.byref(out), _ptr(step))
+        return out, float(step[0]), log_l
+
     def classify(self, model, dye_strings, counts, ids, rads, cutoff=DBL_MAX):
         m = Oracle._m(model)
         rads = np.ascontiguousarray(rads, dtype=np.float64)
@@ -193,6 +267,15 @@ class Reference:
         self.L.wpref_classify(ctypes.byref(m), T, C, len(dye_strings), arr, _ptr(counts), _ptr(ids), N,
                               _ptr(rads), cutoff, _ptr(oid), _ptr(osc), _ptr(otot))
         return oid, osc, otot
+
+
+def fit_stats_len(T, C):
+    return 6 + 2 * (T - 1) + 4 * C
+
+
+def max_dyes_of(dye_string, C):
+    """HMMFitter's max_num_dyes (hmm-fitter.cc:44-60)."""
+    return max(dye_string.count(str(c)) for c in range(C))
 
 
 def model_from_golden(g, C):

@@ -9,6 +9,8 @@ argument meaning follow whatprot's cc_code/src):
   Radiometry files                io/radiometries-io.cc:33-82
   ScoredClassification CSV        io/scored-classifications-io.cc:30-68
   HMMClassifier                   classifiers/hmm-classifier.h:30-78
+  FitSettings                     parameterization/settings/fit-settings.cc:28-100
+  HMMFitter                       fitters/hmm-fitter.h:30-151
 All scoring runs in libwhatprot_amd.so on the GPU; there is no CPU path.
 """
 import ctypes
@@ -21,14 +23,15 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _capi
-from ._capi import (WP_MAX_CHANNELS, WP_PRECISION_FP32, WP_PRECISION_FP64, ModelC,
-                    WhatprotError, check, lib)
+from ._capi import (WP_MAX_CHANNELS, WP_PRECISION_FP32, WP_PRECISION_FP64, FitSettingsC, ModelC,
+                    WhatprotError, check, fit_stats_len, lib)
 
 DBL_MAX = sys.float_info.max
 
 __all__ = [
     "ChannelModel", "SequencingModel", "SequencingSettings", "DyeSeqEntry", "ScoredClassification",
-    "HMMClassifier", "read_dye_seqs", "write_dye_seqs", "read_radiometries", "write_radiometries",
+    "HMMClassifier", "HMMFitter", "FitSettings", "fit_mstep", "offset_exponential_fit", "fit_stats_len",
+    "read_dye_seqs", "write_dye_seqs", "read_radiometries", "write_radiometries",
     "write_scored_classifications", "generate_dye_seqs", "simulate_radiometries",
     "WP_PRECISION_FP32", "WP_PRECISION_FP64", "WhatprotError", "DBL_MAX", "lib",
 ]
@@ -123,6 +126,29 @@ class SequencingModel:
                 ch.interactions[o] = cm.interactions[o] if o < len(cm.interactions) else 1.0
                 ch.flat_interactions[o] = cm.flat_interactions[o] if o < len(cm.flat_interactions) else 1.0
         return m
+
+    @classmethod
+    def from_c(cls, m: ModelC, like: Optional["SequencingModel"] = None):
+        """Inverse of to_c for m.num_channels channels."""
+        C = m.num_channels
+        out = cls(m.p_edman_failure, m.p_detach, m.p_initial_detach, m.p_initial_detach_decay,
+                  m.p_initial_block, m.p_cyclic_block)
+        for c in range(C):
+            ch = m.channels[c]
+            out.channel_models.append(ChannelModel(ch.p_bleach, ch.p_dud, ch.bg_sig, ch.mu, ch.sig,
+                                                   [ch.interactions[o] for o in range(C)],
+                                                   [ch.flat_interactions[o] for o in range(C)]))
+        return out
+
+    def distance(self, other: "SequencingModel"):
+        """SequencingModel::distance (sequencing-model.cc:153-166)."""
+        d = max(abs(self.p_edman_failure - other.p_edman_failure), abs(self.p_detach - other.p_detach),
+                abs(self.p_initial_detach - other.p_initial_detach),
+                abs(self.p_initial_detach_decay - other.p_initial_detach_decay),
+                abs(self.p_initial_block - other.p_initial_block), abs(self.p_cyclic_block - other.p_cyclic_block))
+        for a, b in zip(self.channel_models, other.channel_models):
+            d = max(d, abs(a.p_bleach - b.p_bleach), abs(a.p_dud - b.p_dud))
+        return d
 
     @classmethod
     def uniform(cls, C, p_edman_failure=0.06, p_detach=0.05, p_initial_detach=0.04,
@@ -340,3 +366,130 @@ class HMMClassifier:
         `stream` (a hipStream_t as an integer)."""
         check(lib().wp_hmm_classify_device(self._h, int(num_reads), d_rads, d_id, d_adjusted or None,
                                            d_log_score or None, d_log_total or None, stream or None))
+
+
+@dataclass
+class FitSettings:
+    """FitSettings (parameterization/settings/fit-settings.h:20-37): which
+    parameters are held at their starting values."""
+    hold_p_edman_failure: bool = False
+    hold_p_detach: bool = False
+    hold_p_initial_detach: bool = False
+    hold_p_initial_detach_decay: bool = False
+    hold_p_initial_block: bool = False
+    hold_p_cyclic_block: bool = False
+    hold_p_bleach: List[bool] = field(default_factory=list)
+    hold_p_dud: List[bool] = field(default_factory=list)
+
+    @classmethod
+    def from_json(cls, num_channels, path_or_dict):
+        """Parses a fit settings JSON as fit-settings.cc:44-100 does: every key
+        optional (false when absent); per-channel holds under
+        "channel_settings" (one entry per channel, in order)."""
+        data = path_or_dict
+        if not isinstance(data, dict):
+            with open(path_or_dict) as f:
+                data = json.load(f)
+        fs = cls(**{k: bool(data.get(k, False)) for k in (
+            "hold_p_edman_failure", "hold_p_detach", "hold_p_initial_detach", "hold_p_initial_detach_decay",
+            "hold_p_initial_block", "hold_p_cyclic_block")})
+        if "channel_settings" in data:
+            for cd in data["channel_settings"]:
+                fs.hold_p_bleach.append(bool(cd.get("hold_p_bleach", False)))
+                fs.hold_p_dud.append(bool(cd.get("hold_p_dud", False)))
+        else:
+            fs.hold_p_bleach = [False] * num_channels
+            fs.hold_p_dud = [False] * num_channels
+        return fs
+
+    def to_c(self) -> FitSettingsC:
+        c = FitSettingsC()
+        for k in ("hold_p_edman_failure", "hold_p_detach", "hold_p_initial_detach", "hold_p_initial_detach_decay",
+                  "hold_p_initial_block", "hold_p_cyclic_block"):
+            setattr(c, k, 1 if getattr(self, k) else 0)
+        for i, h in enumerate(self.hold_p_bleach[:WP_MAX_CHANNELS]):
+            c.hold_p_bleach[i] = 1 if h else 0
+        for i, h in enumerate(self.hold_p_dud[:WP_MAX_CHANNELS]):
+            c.hold_p_dud[i] = 1 if h else 0
+        return c
+
+
+def offset_exponential_fit(y, w, hold, p):
+    """least_squares_fit_of_offset_exponential (util/levmar-helper.cc:44-85);
+    returns the fitted (base, initial, initial_decay)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    h = np.ascontiguousarray([1 if x else 0 for x in hold], dtype=np.int32)
+    out = np.ascontiguousarray(p, dtype=np.float64).copy()
+    check(lib().wp_fit_offset_exponential(len(y), y.ctypes.data, w.ctypes.data, h.ctypes.data, out.ctypes.data))
+    return out
+
+
+def fit_mstep(num_timesteps, num_channels, dye_string, seq_model: SequencingModel, num_reads, stats,
+              settings: Optional[FitSettings] = None):
+    """M-step of HMMFitter::fit (hmm-fitter.h:80-104): returns (next model, step size)."""
+    st = np.ascontiguousarray(stats, dtype=np.float64)
+    m = seq_model.to_c()
+    out = ModelC()
+    step = ctypes.c_double()
+    fs = settings.to_c() if settings is not None else None
+    check(lib().wp_fit_mstep(num_timesteps, num_channels, dye_string.encode(), ctypes.byref(m), int(num_reads),
+                             st.ctypes.data, ctypes.byref(fs) if fs is not None else None, ctypes.byref(out),
+                             ctypes.byref(step)))
+    return SequencingModel.from_c(out), step.value
+
+
+class HMMFitter:
+    """HMMFitter (fitters/hmm-fitter.h:30-151) on the GPU, holding its reads.
+
+    HMMFitter(num_timesteps, num_channels, dye_string, radiometries) as
+    run-fit.cc:94-103 builds it (uncut ranges), plus `precision`. fit()
+    mirrors HMMFitter::fit with an optional iteration cap."""
+
+    def __init__(self, num_timesteps, num_channels, dye_string, radiometries, precision=WP_PRECISION_FP64):
+        self.num_timesteps = int(num_timesteps)
+        self.num_channels = int(num_channels)
+        self.dye_string = dye_string
+        r = np.ascontiguousarray(radiometries, dtype=np.float64)
+        if r.ndim != 3 or r.shape[1] != self.num_timesteps or r.shape[2] != self.num_channels:
+            raise WhatprotError(f"radiometries must be [N, {self.num_timesteps}, {self.num_channels}]")
+        self.num_reads = r.shape[0]
+        h = ctypes.c_void_p()
+        check(lib().wp_hmm_fitter_create(self.num_timesteps, self.num_channels, dye_string.encode(),
+                                         self.num_reads, r.ctypes.data, int(precision), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().wp_hmm_fitter_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def estep(self, seq_model: SequencingModel):
+        """(statistics [WP_FIT_STATS_LEN], sum of ln P) under seq_model."""
+        st = np.zeros(fit_stats_len(self.num_timesteps, self.num_channels))
+        slp = ctypes.c_double()
+        m = seq_model.to_c()
+        check(lib().wp_hmm_fitter_estep(self._h, ctypes.byref(m), st.ctypes.data, ctypes.byref(slp)))
+        return st, slp.value
+
+    def log_likelihood(self, seq_model: SequencingModel):
+        """HMMFitter::log_likelihood (hmm-fitter.h:120-140)."""
+        return self.estep(seq_model)[1]
+
+    def fit(self, seq_model: SequencingModel, stopping_threshold, max_runtime=float("inf"),
+            settings: Optional[FitSettings] = None, max_iterations=0):
+        """HMMFitter::fit: returns (fitted model, step size, log L, iterations)."""
+        m = seq_model.to_c()
+        out = ModelC()
+        step, log_l, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        fs = settings.to_c() if settings is not None else None
+        check(lib().wp_hmm_fit(self._h, ctypes.byref(m), ctypes.byref(fs) if fs is not None else None,
+                               float(stopping_threshold), float(min(max_runtime, 1e300)), int(max_iterations),
+                               ctypes.byref(out), ctypes.byref(step), ctypes.byref(log_l), ctypes.byref(it)))
+        return SequencingModel.from_c(out), step.value, log_l.value, it.value

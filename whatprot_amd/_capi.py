@@ -43,6 +43,24 @@ class ModelC(ctypes.Structure):
     ]
 
 
+class FitSettingsC(ctypes.Structure):
+    _fields_ = [
+        ("hold_p_edman_failure", ctypes.c_int),
+        ("hold_p_detach", ctypes.c_int),
+        ("hold_p_initial_detach", ctypes.c_int),
+        ("hold_p_initial_detach_decay", ctypes.c_int),
+        ("hold_p_initial_block", ctypes.c_int),
+        ("hold_p_cyclic_block", ctypes.c_int),
+        ("hold_p_bleach", ctypes.c_int * WP_MAX_CHANNELS),
+        ("hold_p_dud", ctypes.c_int * WP_MAX_CHANNELS),
+    ]
+
+
+def fit_stats_len(T, C):
+    """WP_FIT_STATS_LEN."""
+    return 6 + 2 * (T - 1) + 4 * C
+
+
 _lib = None
 
 
@@ -98,6 +116,20 @@ def lib():
     L.wp_simulate_radiometries.restype = c_int
     L.wp_simulate_radiometries.argtypes = [P(ModelC), c_uint, c_uint, c_int, P(ctypes.c_char_p),
                                            c_void_p, c_int, ctypes.c_uint64, c_void_p, c_void_p]
+    L.wp_hmm_fitter_create.restype = c_int
+    L.wp_hmm_fitter_create.argtypes = [c_uint, c_uint, ctypes.c_char_p, c_int, c_void_p, c_int, P(c_void_p)]
+    L.wp_hmm_fitter_destroy.restype = None
+    L.wp_hmm_fitter_destroy.argtypes = [c_void_p]
+    L.wp_hmm_fitter_estep.restype = c_int
+    L.wp_hmm_fitter_estep.argtypes = [c_void_p, P(ModelC), c_void_p, c_void_p]
+    L.wp_fit_mstep.restype = c_int
+    L.wp_fit_mstep.argtypes = [c_uint, c_uint, ctypes.c_char_p, P(ModelC), c_int, c_void_p,
+                               P(FitSettingsC), P(ModelC), P(c_double)]
+    L.wp_hmm_fit.restype = c_int
+    L.wp_hmm_fit.argtypes = [c_void_p, P(ModelC), P(FitSettingsC), c_double, c_double, c_int, P(ModelC),
+                             P(c_double), P(c_double), P(c_int)]
+    L.wp_fit_offset_exponential.restype = c_int
+    L.wp_fit_offset_exponential.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_void_p]
     _lib = L
     return L
 

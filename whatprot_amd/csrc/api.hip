@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "hmm_kernels.h"
+#include "model_tables.h"
 #include "wp_internal.h"
 
 namespace wp {
@@ -29,16 +30,57 @@ int parse_dye_seq(const char* s, short* seq, int cap) {
     return len;
 }
 
-}  // namespace wp
+void fill_dev_model(const wp_model& m, int T, int C, int M, double cutoff, DevModel* out) {
+    DevModel& dm = *out;
+    std::memset(&dm, 0, sizeof(dm));
+    dm.T = T;
+    dm.C = C;
+    dm.M = M;
+    dm.uncut = cutoff == DBL_MAX;
+    dm.cutoff = cutoff;
+    dm.pf = m.p_edman_failure;
+    dm.p_ib = m.p_initial_block;
+    dm.p_cb = m.p_cyclic_block;
+    for (int i = 0; i < T; i++)
+        dm.p_detach[i] = m.p_detach + m.p_initial_detach * std::exp(-(double)i * m.p_initial_detach_decay);
+    for (int k = 0; k < C; k++) {
+        dm.mu[k] = m.channels[k].mu;
+        dm.sig[k] = m.channels[k].sig;
+        dm.bg_sig[k] = m.channels[k].bg_sig;
+        for (int o = 0; o < C; o++) {
+            dm.inter[k][o] = m.channels[k].interactions[o];
+            dm.flat[k][o] = m.channels[k].flat_interactions[o];
+        }
+    }
+}
 
-#define WP_HIP(expr)                                                                       \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess) {                                                            \
-            wp::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));              \
-            return WP_ERR_HIP;                                                             \
-        }                                                                                  \
-    } while (0)
+void binomial_tables(const wp_model& m, int C, int M, double* out) {
+    const int tri = (M + 1) * (M + 2) / 2;
+    for (int kind = 0; kind < 2; kind++) {
+        for (int k = 0; k < C; k++) {
+            double q = kind == 0 ? m.channels[k].p_dud : m.channels[k].p_bleach;
+            double p = 1.0 - q;
+            double* v = &out[((size_t)kind * C + k) * tri];
+            v[0] = 1.0;
+            for (int i = 1; i <= M; i++) {
+                int row = i * (i + 1) / 2, prev = (i - 1) * i / 2;
+                v[row] = v[prev] * q;
+                for (int j = 1; j < i; j++) v[row + j] = v[prev + j] * q + v[prev + j - 1] * p;
+                v[row + i] = v[prev + i - 1] * p;
+            }
+        }
+    }
+}
+
+int require_device() {
+    if (wp_device_count() < 1) {
+        set_error("no gfx950 device visible");
+        return WP_ERR_NO_DEVICE;
+    }
+    return WP_OK;
+}
+
+}  // namespace wp
 
 struct wp_hmm_classifier {
     int T = 0, C = 0, S = 0, M = 0, precision = 0, G = 16, tri = 1;
@@ -281,44 +323,9 @@ int wp_hmm_classifier_create(unsigned int num_timesteps, unsigned int num_channe
     c->counts.assign(dye_seq_counts, dye_seq_counts + S);
     c->maxbox = 1;
     for (int k = 0; k < C; k++) c->maxbox *= (M + 1);
-    wp::DevModel& dm = c->dm;
-    std::memset(&dm, 0, sizeof(dm));
-    dm.T = T;
-    dm.C = C;
-    dm.M = M;
-    dm.uncut = dist_cutoff == DBL_MAX;
-    dm.cutoff = dist_cutoff;
-    dm.pf = seq_model->p_edman_failure;
-    dm.p_ib = seq_model->p_initial_block;
-    dm.p_cb = seq_model->p_cyclic_block;
-    for (int i = 0; i < T; i++)  // decaying-rate-model.cc:47-50
-        dm.p_detach[i] = seq_model->p_detach +
-                         seq_model->p_initial_detach * std::exp(-(double)i * seq_model->p_initial_detach_decay);
-    for (int k = 0; k < C; k++) {
-        dm.mu[k] = seq_model->channels[k].mu;
-        dm.sig[k] = seq_model->channels[k].sig;
-        dm.bg_sig[k] = seq_model->channels[k].bg_sig;
-        for (int o = 0; o < C; o++) {
-            dm.inter[k][o] = seq_model->channels[k].interactions[o];
-            dm.flat[k][o] = seq_model->channels[k].flat_interactions[o];
-        }
-    }
-    // BinomialTransition::reserve (binomial-transition.cc:51-67) for dud and bleach.
+    wp::fill_dev_model(*seq_model, T, C, M, dist_cutoff, &c->dm);
     std::vector<double> binom((size_t)2 * C * c->tri);
-    for (int kind = 0; kind < 2; kind++) {
-        for (int k = 0; k < C; k++) {
-            double q = kind == 0 ? seq_model->channels[k].p_dud : seq_model->channels[k].p_bleach;
-            double p = 1.0 - q;
-            double* v = &binom[((size_t)kind * C + k) * c->tri];
-            v[0] = 1.0;
-            for (int i = 1; i <= M; i++) {
-                int row = i * (i + 1) / 2, prev = (i - 1) * i / 2;
-                v[row] = v[prev] * q;
-                for (int j = 1; j < i; j++) v[row + j] = v[prev + j] * q + v[prev + j - 1] * p;
-                v[row + i] = v[prev + i - 1] * p;
-            }
-        }
-    }
+    wp::binomial_tables(*seq_model, C, M, binom.data());
     // every bin must fit at least one pair group
     for (const Bin& b : make_bins(c)) {
         size_t need = b.rows ? wp::v3_lds_bytes(precision, T, C, c->tri, b.hi)

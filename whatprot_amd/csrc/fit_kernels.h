@@ -1,0 +1,40 @@
+// Launch interface of the Baum-Welch E-step kernel (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "hmm_device.h"
+
+namespace wp {
+
+// Arguments of the E-step kernel. Device pointers unless noted.
+struct FitArgs {
+    DevModel m;                   // model of this iteration, uncut
+    int N;                        // reads
+    const double* rads;           // [N][T][C], intensities / mu
+    int NB;                       // prod_c (D_c + 1): count states per Edman row
+    int D[WP_MAX_CHANNELS];       // dyes of each channel in the dye-seq
+    const unsigned char* crd;     // [NB][C] per-channel counts of count index q
+    const unsigned short* fbase;  // per channel c, NB / (D_c + 1) fiber base indices
+    int nfib;                     // total fiber bases
+    const signed char* seq;       // [T] channel of position e (-1 = none)
+    const unsigned char* track;   // [T][C] dyes of channel c at positions >= e
+    const double* binom;          // [2][C][tri] dud then bleach tables
+    int tri;
+    void* hist;                   // per workgroup: T x 2 x T*NB states (Real)
+    size_t hist_stride;           // elements per workgroup
+    double* partial;              // [grid][L + 1] per-workgroup totals
+    int L;                        // WP_FIT_STATS_LEN(T, C)
+};
+
+// Dynamic LDS of one workgroup ((size_t)-1 for an unsupported C).
+size_t fit_lds_bytes(int precision, int T, int C, int NB, int tri, int nfib);
+int fit_threads();
+
+// E-step over all reads; d_out[0..L) receives the summed statistics and
+// d_out[L] the sum of ln P. grid workgroups, each owning hist and partial
+// slots.
+hipError_t launch_fit_estep(const FitArgs& a, int precision, int grid, double* d_out, hipStream_t stream);
+
+}  // namespace wp

@@ -47,4 +47,11 @@ void simulate_radiometries(const wp_model& m, int T, int C, int nds, const short
                            const int* lens, const int* weights, int n, uint64_t seed,
                            double* out, int* out_idx, unsigned int* out_tracks);
 
+// least_squares_fit_of_offset_exponential (util/levmar-helper.cc:44-85):
+// weighted fit of y_t ~ p[0] + p[1] exp(-p[2] t), t = 0..n-1, bounds [0, 1],
+// hold[i] != 0 pins p[i]; p is updated in place. Own restatement of levmar
+// 2.6's box-constrained method (levmar_bc.cpp).
+void offset_exponential_fit(int n, const double* y, const double* w, const int* hold, double* p,
+                            int itmax = 10);
+
 }  // namespace wp
