@@ -1,4 +1,6 @@
-"""Benchmark of the HMM classify path on one MI355X.
+"""Benchmarks of the HMM scoring path on one MI355X.
+
+Default (--workload classify), the headline:
 
 Workload (one step): classify 100,000 simulated reads (3 channels, 12
 timesteps) against 1,000 dye-seqs (lengths 6-30, 0-5 dyes per channel) with a
@@ -9,6 +11,11 @@ Also measured (untimed for `value`): top-1 agreement with the FP64 GPU path on
 the same reads, per-kernel HIP-event durations, algorithmic FLOPs of the
 pruned work, and the compiled reference's multithreaded CPU classify on a
 bounded sample of the same reads.
+
+--workload fit: Baum-Welch fitting of one dye-seq ("..0..1..2..0..1..2..",
+2 dyes in each of 3 channels, T = 30) to 10^6 simulated reads; one step is one
+EM iteration (GPU E-step over every read + host M-step). CPU baseline: the
+reference's serial E-step loop on a bounded sample.
 """
 import argparse
 import json
@@ -60,6 +67,7 @@ def main():
     ap.add_argument("--cpu-sample-reads", type=int, default=3000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--workload", choices=["classify", "fit"], default="classify")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -67,6 +75,9 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(reads shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "fit":
+        return bench_fit(args)
 
     import numpy as np
     import torch
@@ -188,6 +199,94 @@ def main():
     if not args.no_cpu_baseline:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(model, strings, rads, cutoff, args.cpu_sample_reads)
+    print(json.dumps(result), flush=True)
+
+
+def fit_flops_per_read(T, D):
+    """Algorithmic FLOPs of one read's E-step: the forward pass by the
+    per-state counts of the classify roofline over full (uncut) boxes, the
+    backward pass (the same operators transposed) and the statistics (taken
+    as one more pass of the same size)."""
+    nb = 1
+    for d in D:
+        nb *= d + 1
+    binom = sum(2 * (d + 2) for d in D)
+    fwd = nb * (3 + 2 + binom)  # t = 0: initial block, duds, emission
+    for t in range(1, T):
+        fwd += (t + 1) * nb * (3 + 4 + 8 + 2 + binom)
+    return 3 * fwd
+
+
+def bench_fit(args):
+    import numpy as np
+    import torch
+
+    import whatprot_amd as w
+
+    T, C, ds = 30, 3, "..0..1..2..0..1..2.."
+    N = args.reads if args.reads != 100000 else 1000000
+    D = [ds.count(str(c)) for c in range(C)]
+    truth = w.SequencingModel.uniform(C, p_edman_failure=0.07, p_bleach=0.04, p_dud=0.08, p_detach=0.04)
+    start = w.SequencingModel.uniform(C)
+    log(f"simulating {N} reads of {ds}")
+    rads, _ = w.simulate_radiometries(truth, T, [ds], N, 3004)
+    prec = w.WP_PRECISION_FP32 if args.precision == "fp32" else w.WP_PRECISION_FP64
+    f = w.HMMFitter(T, C, ds, rads, prec)
+    log("warmup")
+    m = start
+    for _ in range(args.warmup):
+        st, _ = f.estep(m)
+        m, _ = w.fit_mstep(T, C, ds, m, N, st)
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} EM iterations")
+    t0 = time.perf_counter()
+    m = start
+    for _ in range(args.steps):
+        st, _ = f.estep(m)
+        m, _ = w.fit_mstep(T, C, ds, m, N, st)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt * 1e3 / args.steps
+    flops = fit_flops_per_read(T, D) * N
+    achieved = flops / (ms * 1e-3) / 1e12
+    peak = PEAK_FP32_TFLOPS if prec == w.WP_PRECISION_FP32 else 78.6
+    result = {
+        "metric": "Baum-Welch E-step reads/s on 1 MI355X (fit, uncut)",
+        "value": N / (ms * 1e-3),
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic (seeded simulator, seqparams_atto647n_x3 priors with mu = 1)",
+        "config": {"workload": "fit: 1 dye-seq L=20, 2 dyes x 3 channels, T=30, 1e6 reads, one EM iteration per step",
+                   "reads": N, "timesteps": T, "channels": C, "dye_seq": ds, "precision": args.precision},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None, "kernel": "fit_estep_kernel",
+                     "flops_per_launch": flops},
+    }
+    if prec == w.WP_PRECISION_FP32:
+        f64 = w.HMMFitter(T, C, ds, rads[:20000], w.WP_PRECISION_FP64)
+        f32 = w.HMMFitter(T, C, ds, rads[:20000], w.WP_PRECISION_FP32)
+        a, la = f64.estep(start)
+        b, lb = f32.estep(start)
+        result["fp32_vs_fp64_max_rel_stat"] = float(np.max(np.abs(b - a) / np.maximum(np.abs(a), 1e-300)))
+        result["fp32_vs_fp64_rel_logL"] = float(abs(lb - la) / abs(la))
+    if not args.no_cpu_baseline:
+        from tests._oracle import Reference
+        ref = Reference.load()
+        if ref is not None:
+            n = min(N, 1000)
+            t1 = time.perf_counter()
+            ref.fit_estep(start, ds, rads[:n])
+            d1 = time.perf_counter() - t1
+            result["cpu_baseline"] = {"value": n / d1, "unit": "reads/s", "cores": 1, "kind": "reference",
+                                      "sample": f"{n} reads of the same workload, one serial E-step "
+                                                f"(HMMFitter::fit's loop), {d1:.1f} s wall"}
     print(json.dumps(result), flush=True)
 
 

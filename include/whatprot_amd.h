@@ -189,6 +189,77 @@ int wp_hmm_work(wp_hmm_classifier* clf, int num_reads, const double* radiometrie
                 double* out3);
 
 /* ---------------------------------------------------------------------------
+ * kNN classifier. Replaces NNClassifier (classifiers/nn-classifier.h:44-82)
+ * over the KDTree of kd-tree/kd-tree.h:22-63, and HybridClassifier
+ * (classifiers/hybrid-classifier.h:25-52).
+ *
+ * Dye-tracks as read_dye_tracks gives them (io/dye-tracks-io.cc:32-62):
+ *   track_counts[i][t][c]  dyes of channel c at timestep t of track i (short)
+ *   sources of track i: [source_offsets[i], source_offsets[i+1]) into
+ *   source_ids / source_counts / source_hits (SourceCountHits: dye-seq id,
+ *   its count, simulated hits).
+ * seq_model has mu = 1 (run-classify-nn.cc), radiometries are divided by mu.
+ * ------------------------------------------------------------------------- */
+typedef struct wp_nn_classifier wp_nn_classifier;
+typedef struct wp_hybrid_classifier wp_hybrid_classifier;
+
+/* The point order the kd-tree construction leaves (kd-tree.h:33-37 with
+ * make-node.h:28-80): out_order[p] = input index of the point at position p;
+ * out_depth (may be NULL) the tree depth. n points of dimension d, row-major.
+ * Host only; the search breaks distance ties by this order. */
+int wp_kdtree_order(int k, int d, int n, const double* points, int* out_order, int* out_depth);
+
+/* NNClassifier(T, C, seq_model, k, sig, dye_tracks) (nn-classifier.cc:89-108):
+ * builds the kd-tree on the host (make-node.h:28-80) and copies it to the
+ * device. k >= 1. */
+int wp_nn_classifier_create(unsigned int num_timesteps, unsigned int num_channels,
+                            const wp_model* seq_model, int k, double sig, int num_dye_tracks,
+                            const short* track_counts, const int* source_offsets,
+                            const int* source_ids, const int* source_counts,
+                            const int* source_hits, wp_nn_classifier** out);
+
+void wp_nn_classifier_destroy(wp_nn_classifier* clf);
+
+/* KDTree::search for every read (kd-tree.h:40-46): out_count[r] <= k
+ * neighbours in KBest::fill order (farthest first, k-best.h:45-55),
+ * out_track[r * k + i] the dye-track index, out_dist_sq[r * k + i] its squared
+ * distance. */
+int wp_nn_search(wp_nn_classifier* clf, int num_reads, const double* radiometries,
+                 int* out_count, int* out_track, double* out_dist_sq);
+
+/* NNClassifier::classify(vector<Radiometry>) (nn-classifier.cc:153-177,
+ * 206-215): best id, its score and the total (adjusted = score / total; the
+ * NaN guard sets score 0, total 1). */
+int wp_nn_classify(wp_nn_classifier* clf, int num_reads, const double* radiometries,
+                   int* out_id, double* out_score, double* out_total);
+
+/* NNClassifier::classify(radiometry, h) (nn-classifier.cc:179-204): up to h
+ * ids per read in ascending score order at out_ids[r * h + i], their scores,
+ * and the read's total. */
+int wp_nn_classify_top(wp_nn_classifier* clf, int num_reads, const double* radiometries, int h,
+                       int* out_count, int* out_ids, double* out_scores, double* out_total);
+
+/* HybridClassifier(T, C, seq_model, seq_settings, k, sig, dye_tracks, h,
+ * dye_seqs) (hybrid-classifier.cc:25-41): an NN classifier and an HMM
+ * classifier (precision as wp_hmm_classifier_create). */
+int wp_hybrid_classifier_create(unsigned int num_timesteps, unsigned int num_channels,
+                                const wp_model* seq_model, double dist_cutoff, int k, double sig,
+                                int num_dye_tracks, const short* track_counts,
+                                const int* source_offsets, const int* source_ids,
+                                const int* source_counts, const int* source_hits, int h,
+                                int num_dye_seqs, const char* const* dye_strings,
+                                const int* dye_seq_counts, const int* dye_seq_ids, int precision,
+                                wp_hybrid_classifier** out);
+
+void wp_hybrid_classifier_destroy(wp_hybrid_classifier* clf);
+
+/* HybridClassifier::classify (hybrid-classifier.cc:43-95): the top-h kNN ids
+ * as candidates of the HMM, the HMM score scaled by the candidates' kNN
+ * share. out_adjusted is the written score (score / total). */
+int wp_hybrid_classify(wp_hybrid_classifier* clf, int num_reads, const double* radiometries,
+                       int* out_id, double* out_adjusted);
+
+/* ---------------------------------------------------------------------------
  * Baum-Welch fit. Replaces HMMFitter (fitters/hmm-fitter.h:30-151) as run by
  * run_fit (main/run-fit.cc:37-140): one dye-seq, uncut ranges (the reference
  * sets dist_cutoff = DBL_MAX for fitting, run-fit.cc:64-65).
@@ -298,6 +369,22 @@ int wp_simulate_radiometries(const wp_model* seq_model, unsigned int num_timeste
                              const char* const* dye_strings, const int* weights,
                              int num_reads, uint64_t seed, double* out_radiometries,
                              int* out_dye_seq_index);
+
+/* generate_dye_tracks (simulation/generate-dye-tracks.cc:28-63) followed by
+ * the deduplication of dedup-dye-tracks.cc:35-148: per_peptide tracks for each
+ * of dye_seq_counts[i] peptides of every dye-seq, all-zero tracks dropped,
+ * identical tracks merged with per source dye-seq id its count and hits.
+ * Output in the layout wp_nn_classifier_create takes, tracks ordered by their
+ * counts, sources by id; each source array needs room for max_tracks *
+ * num_dye_seqs entries at most (one per distinct track and dye-seq). Fails
+ * with WP_ERR_ARG (and *out_num_tracks set) when more than max_tracks
+ * distinct tracks result. */
+int wp_simulate_dye_tracks(const wp_model* seq_model, unsigned int num_timesteps,
+                           unsigned int num_channels, int num_dye_seqs,
+                           const char* const* dye_strings, const int* dye_seq_counts,
+                           const int* dye_seq_ids, int per_peptide, uint64_t seed, int max_tracks,
+                           int* out_num_tracks, short* out_track_counts, int* out_source_offsets,
+                           int* out_source_ids, int* out_source_counts, int* out_source_hits);
 
 #ifdef __cplusplus
 }

@@ -5,12 +5,18 @@
 // hmm-classifier.h:45-78, peptide-hmm.test.cc:152-189) and returns their
 // results, so the tests can pin the C restatement and the HIP path against the
 // reference itself.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <string>
 #include <vector>
 
+#include <map>
+
 #include "classifiers/hmm-classifier.h"
+#include "classifiers/hybrid-classifier.h"
+#include "classifiers/nn-classifier.h"
+#include "common/dye-track.h"
 #include "common/dye-seq.h"
 #include "common/radiometry.h"
 #include "common/scored-classification.h"
@@ -109,6 +115,33 @@ void fill_fit_settings(const int* holds, unsigned int C, FitSettings* fs) {
         fs->channel_fit_settings[c]->hold_p_bleach = holds && holds[6 + 2 * c];
         fs->channel_fit_settings[c]->hold_p_dud = holds && holds[7 + 2 * c];
     }
+}
+
+std::vector<SourcedData<DyeTrack, SourceCountHitsList<int>>> make_tracks(
+        unsigned int T, unsigned int C, int n, const short* counts, const int* off, const int* ids,
+        const int* cnts, const int* hits) {
+    std::vector<SourcedData<DyeTrack, SourceCountHitsList<int>>> v;
+    v.reserve(n);
+    for (int i = 0; i < n; i++) {
+        DyeTrack dt(T, C, const_cast<short*>(counts + (size_t)i * T * C));
+        int ns = off[i + 1] - off[i];
+        SourceCountHits<int>** src = new SourceCountHits<int>*[ns];
+        for (int j = 0; j < ns; j++)
+            src[j] = new SourceCountHits<int>(ids[off[i] + j], cnts[off[i] + j], hits[off[i] + j]);
+        v.push_back(SourcedData<DyeTrack, SourceCountHitsList<int>>(std::move(dt),
+                                                                    SourceCountHitsList<int>(ns, src)));
+    }
+    return v;
+}
+
+std::vector<Radiometry> make_rads(unsigned int T, unsigned int C, int N, const double* rads) {
+    std::vector<Radiometry> out;
+    out.reserve(N);
+    for (int r = 0; r < N; r++) {
+        out.push_back(Radiometry(T, C));
+        for (unsigned int i = 0; i < T * C; i++) out.back().intensities[i] = rads[(size_t)r * T * C + i];
+    }
+    return out;
 }
 
 }  // namespace
@@ -292,6 +325,146 @@ void wpref_levmar_bc(int n, const double* y, const double* w, const int* hold, d
     for (int i = 0; i < n; i++) wy[i] = sw[i] * y[i];
     dlevmar_bc_der(&offset_exponential, &jacobian_of_offset_exponential, p3, wy.data(), 3, n, lb.data(),
                    ub.data(), NULL, itmax, NULL, info, NULL, NULL, sw.data());
+}
+
+// One E-step of HMMFitter::fit exactly as its loop runs it (hmm-fitter.h:57-79):
+// serial over reads, each read's PeptideHMM::improve_fit into a fresh fitter
+// added to the total. Returns the sum of ln P.
+double wpref_fit_estep(const wp_model* m, unsigned int T, unsigned int C, const char* dye_string,
+                       int num_reads, const double* rads, double* stats) {
+    SequencingModel sm;
+    fill_model(m, C, &sm);
+    SequencingSettings ss;
+    ss.dist_cutoff = DBL_MAX;
+    FitSettings fs(C);
+    DyeSeq ds(C, std::string(dye_string));
+    unsigned int max_num_dyes = 0;
+    for (unsigned int c = 0; c < C; c++) {
+        unsigned int n = 0;
+        for (unsigned int i = 0; i < ds.length; i++)
+            if (ds[i] == (int)c) n++;
+        max_num_dyes = std::max(max_num_dyes, n);
+    }
+    SequencingModelFitter total(T, C, sm, fs);
+    DyeSeqPrecomputations dsp(ds, sm, T, C);
+    UniversalPrecomputations up(sm, T, C);
+    up.set_max_num_dyes(max_num_dyes);
+    double log_p = 0.0;
+    for (int r = 0; r < num_reads; r++) {
+        Radiometry rad(T, C);
+        for (unsigned int i = 0; i < T * C; i++) rad.intensities[i] = rads[(size_t)r * T * C + i];
+        RadiometryPrecomputations rp(rad, sm, ss, max_num_dyes);
+        PeptideHMM hmm(T, C, dsp, rp, up);
+        SequencingModelFitter one(T, C, sm, fs);
+        log_p += std::log(hmm.improve_fit(&one));
+        total += one;
+    }
+    add_stats(total, T, C, stats);
+    return log_p;
+}
+
+// KDTree::search of NNClassifier's tree for every read (kd-tree.h:40-46):
+// neighbours farthest first, identified by the index of their (distinct)
+// dye-track in the input. out arrays are [N][k].
+void wpref_nn_search(const wp_model* m, unsigned int T, unsigned int C, int k, double sig, int n,
+                     const short* counts, const int* off, const int* ids, const int* cnts, const int* hits,
+                     int N, const double* rads, int* out_count, int* out_track, double* out_dist) {
+    SequencingModel sm;
+    fill_model(m, C, &sm);
+    std::map<std::vector<short>, int> index;
+    for (int i = 0; i < n; i++)
+        index[std::vector<short>(counts + (size_t)i * T * C, counts + (size_t)(i + 1) * T * C)] = i;
+    auto tracks = make_tracks(T, C, n, counts, off, ids, cnts, hits);
+    NNClassifier nn(T, C, sm, k, sig, &tracks);
+    auto radv = make_rads(T, C, N, rads);
+    for (int r = 0; r < N; r++) {
+        KDTQuery q(radv[r]);
+        std::vector<KDTEntry*> kn;
+        std::vector<double> ds;
+        nn.kd_tree->search(q, &kn, &ds);
+        out_count[r] = (int)kn.size();
+        for (size_t i = 0; i < kn.size(); i++) {
+            out_track[(size_t)r * k + i] = index[kn[i]->dye_track.value.counts];
+            out_dist[(size_t)r * k + i] = ds[i];
+        }
+    }
+}
+
+// NNClassifier::classify(vector<Radiometry>) (nn-classifier.cc:206-215).
+void wpref_nn_classify(const wp_model* m, unsigned int T, unsigned int C, int k, double sig, int n,
+                       const short* counts, const int* off, const int* ids, const int* cnts, const int* hits,
+                       int N, const double* rads, int* out_id, double* out_score, double* out_total) {
+    SequencingModel sm;
+    fill_model(m, C, &sm);
+    auto tracks = make_tracks(T, C, n, counts, off, ids, cnts, hits);
+    NNClassifier nn(T, C, sm, k, sig, &tracks);
+    auto res = nn.classify(make_rads(T, C, N, rads));
+    for (int r = 0; r < N; r++) {
+        out_id[r] = res[r].id;
+        out_score[r] = res[r].score;
+        out_total[r] = res[r].total;
+    }
+}
+
+// NNClassifier::classify(radiometry, h) (nn-classifier.cc:179-204).
+void wpref_nn_classify_top(const wp_model* m, unsigned int T, unsigned int C, int k, double sig, int n,
+                           const short* counts, const int* off, const int* ids, const int* cnts,
+                           const int* hits, int N, const double* rads, int h, int* out_count, int* out_ids,
+                           double* out_scores, double* out_total) {
+    SequencingModel sm;
+    fill_model(m, C, &sm);
+    auto tracks = make_tracks(T, C, n, counts, off, ids, cnts, hits);
+    NNClassifier nn(T, C, sm, k, sig, &tracks);
+    auto radv = make_rads(T, C, N, rads);
+    for (int r = 0; r < N; r++) {
+        auto top = nn.classify(radv[r], (unsigned int)h);
+        out_count[r] = (int)top.size();
+        out_total[r] = top.empty() ? 0.0 : top[0].total;
+        for (size_t i = 0; i < top.size(); i++) {
+            out_ids[(size_t)r * h + i] = top[i].id;
+            out_scores[(size_t)r * h + i] = top[i].score;
+        }
+    }
+}
+
+// HybridClassifier::classify(vector<Radiometry>) (hybrid-classifier.cc:84-95).
+void wpref_hybrid_classify(const wp_model* m, unsigned int T, unsigned int C, double dist_cutoff, int k,
+                           double sig, int n, const short* counts, const int* off, const int* ids,
+                           const int* cnts, const int* hits, int h, int num_dye_seqs,
+                           const char* const* dye_strings, const int* ds_counts, const int* ds_ids, int N,
+                           const double* rads, int* out_id, double* out_score, double* out_total) {
+    SequencingModel sm;
+    fill_model(m, C, &sm);
+    SequencingSettings ss;
+    ss.dist_cutoff = dist_cutoff;
+    auto tracks = make_tracks(T, C, n, counts, off, ids, cnts, hits);
+    std::vector<SourcedData<DyeSeq, SourceCount<int>>> dye_seqs;
+    for (int i = 0; i < num_dye_seqs; i++)
+        dye_seqs.push_back(SourcedData<DyeSeq, SourceCount<int>>(DyeSeq(C, std::string(dye_strings[i])),
+                                                                SourceCount<int>(ds_ids[i], ds_counts[i])));
+    HybridClassifier hc(T, C, sm, ss, k, sig, &tracks, h, dye_seqs);
+    auto res = hc.classify(make_rads(T, C, N, rads));
+    for (int r = 0; r < N; r++) {
+        out_id[r] = res[r].id;
+        out_score[r] = res[r].score;
+        out_total[r] = res[r].total;
+    }
+}
+
+// The permuted point order of KDTree's construction (kd-tree.h:33-37): for
+// each position, the index of the point given at that position in the input.
+// Points are rows of X (n x d) with one hit each.
+void wpref_kdtree_order(int k, int d, int n, const double* X, int* out_order) {
+    struct P {
+        std::vector<double> v;
+        int hits;
+        int orig;
+        double operator[](int i) const { return v[i]; }
+    };
+    std::vector<P> pts;
+    for (int i = 0; i < n; i++) pts.push_back(P{std::vector<double>(X + (size_t)i * d, X + (size_t)(i + 1) * d), 1, i});
+    KDTree<P, std::vector<double>> tree(k, d, std::move(pts));
+    for (int i = 0; i < n; i++) out_order[i] = tree.values[i].orig;
 }
 
 }  // extern "C"

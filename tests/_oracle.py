@@ -196,6 +196,23 @@ class Reference:
         L.wpref_fit.restype = c_double
         L.wpref_fit.argtypes = [P(ModelC), c_uint, c_uint, ctypes.c_char_p, c_int, c_void_p, c_double, c_double,
                                 c_void_p, P(ModelC), c_void_p]
+        L.wpref_fit_estep.restype = c_double
+        L.wpref_fit_estep.argtypes = [P(ModelC), c_uint, c_uint, ctypes.c_char_p, c_int, c_void_p, c_void_p]
+        L.wpref_nn_search.restype = None
+        L.wpref_nn_search.argtypes = [P(ModelC), c_uint, c_uint, c_int, c_double, c_int] + [c_void_p] * 5 + \
+            [c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+        L.wpref_nn_classify.restype = None
+        L.wpref_nn_classify.argtypes = [P(ModelC), c_uint, c_uint, c_int, c_double, c_int] + [c_void_p] * 5 + \
+            [c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+        L.wpref_nn_classify_top.restype = None
+        L.wpref_nn_classify_top.argtypes = [P(ModelC), c_uint, c_uint, c_int, c_double, c_int] + [c_void_p] * 5 + \
+            [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+        L.wpref_hybrid_classify.restype = None
+        L.wpref_hybrid_classify.argtypes = [P(ModelC), c_uint, c_uint, c_double, c_int, c_double, c_int] + \
+            [c_void_p] * 5 + [c_int, c_int, P(ctypes.c_char_p), c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                              c_void_p, c_void_p]
+        L.wpref_kdtree_order.restype = None
+        L.wpref_kdtree_order.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p]
         L.wpref_classify.restype = None
         L.wpref_classify.argtypes = [P(ModelC), c_uint, c_uint, c_int, P(ctypes.c_char_p), c_void_p, c_void_p,
                                      c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p]
@@ -253,6 +270,79 @@ class Reference:
         log_l = self.L.wpref_fit(ctypes.byref(m), T, C, dye_string.encode(), N, _ptr(rads), stopping_threshold,
                                  max_runtime, _ptr(h), ctypes.byref(out), _ptr(step))
         return out, float(step[0]), log_l
+
+    def fit_estep(self, model, dye_string, rads):
+        """One serial E-step of HMMFitter::fit: (statistics, sum of ln P)."""
+        m = Oracle._m(model)
+        rads = np.ascontiguousarray(rads, dtype=np.float64)
+        N, T, C = rads.shape
+        st = np.zeros(fit_stats_len(T, C))
+        lp = self.L.wpref_fit_estep(ctypes.byref(m), T, C, dye_string.encode(), N, _ptr(rads), _ptr(st))
+        return st, lp
+
+    def kdtree_order(self, points, k):
+        X = np.ascontiguousarray(points, dtype=np.float64)
+        n, d = X.shape
+        out = np.zeros(n, dtype=np.int32)
+        self.L.wpref_kdtree_order(k, d, n, _ptr(X), _ptr(out))
+        return out
+
+    @staticmethod
+    def _tracks(tracks):
+        return [np.ascontiguousarray(a) for a in tracks._arrays()]
+
+    def nn_search(self, model, tracks, k, sig, rads):
+        m = Oracle._m(model)
+        rads = np.ascontiguousarray(rads, dtype=np.float64)
+        N, T, C = rads.shape
+        ta = self._tracks(tracks)
+        cnt = np.zeros(N, dtype=np.int32)
+        idx = np.full((N, k), -1, dtype=np.int32)
+        dist = np.zeros((N, k))
+        self.L.wpref_nn_search(ctypes.byref(m), T, C, k, sig, len(tracks), *[_ptr(a) for a in ta], N, _ptr(rads),
+                               _ptr(cnt), _ptr(idx), _ptr(dist))
+        return cnt, idx, dist
+
+    def nn_classify(self, model, tracks, k, sig, rads):
+        m = Oracle._m(model)
+        rads = np.ascontiguousarray(rads, dtype=np.float64)
+        N, T, C = rads.shape
+        ta = self._tracks(tracks)
+        ids = np.zeros(N, dtype=np.int32)
+        sc = np.zeros(N)
+        tot = np.zeros(N)
+        self.L.wpref_nn_classify(ctypes.byref(m), T, C, k, sig, len(tracks), *[_ptr(a) for a in ta], N, _ptr(rads),
+                                 _ptr(ids), _ptr(sc), _ptr(tot))
+        return ids, sc, tot
+
+    def nn_classify_top(self, model, tracks, k, sig, rads, h):
+        m = Oracle._m(model)
+        rads = np.ascontiguousarray(rads, dtype=np.float64)
+        N, T, C = rads.shape
+        ta = self._tracks(tracks)
+        cnt = np.zeros(N, dtype=np.int32)
+        ids = np.zeros((N, max(h, 1)), dtype=np.int32)
+        sc = np.zeros((N, max(h, 1)))
+        tot = np.zeros(N)
+        self.L.wpref_nn_classify_top(ctypes.byref(m), T, C, k, sig, len(tracks), *[_ptr(a) for a in ta], N,
+                                     _ptr(rads), h, _ptr(cnt), _ptr(ids), _ptr(sc), _ptr(tot))
+        return [list(zip(ids[i, :cnt[i]].tolist(), sc[i, :cnt[i]].tolist())) for i in range(N)], tot
+
+    def hybrid_classify(self, model, tracks, k, sig, h, dye_strings, counts, ids, rads, cutoff=DBL_MAX):
+        m = Oracle._m(model)
+        rads = np.ascontiguousarray(rads, dtype=np.float64)
+        N, T, C = rads.shape
+        ta = self._tracks(tracks)
+        arr = (ctypes.c_char_p * len(dye_strings))(*[s.encode() for s in dye_strings])
+        counts = np.ascontiguousarray(counts, dtype=np.int32)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        oid = np.zeros(N, dtype=np.int32)
+        osc = np.zeros(N)
+        otot = np.zeros(N)
+        self.L.wpref_hybrid_classify(ctypes.byref(m), T, C, cutoff, k, sig, len(tracks), *[_ptr(a) for a in ta], h,
+                                     len(dye_strings), arr, _ptr(counts), _ptr(ids), N, _ptr(rads), _ptr(oid),
+                                     _ptr(osc), _ptr(otot))
+        return oid, osc, otot
 
     def classify(self, model, dye_strings, counts, ids, rads, cutoff=DBL_MAX):
         m = Oracle._m(model)

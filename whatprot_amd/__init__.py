@@ -10,6 +10,8 @@ argument meaning follow whatprot's cc_code/src):
   ScoredClassification CSV        io/scored-classifications-io.cc:30-68
   HMMClassifier                   classifiers/hmm-classifier.h:30-78
   FitSettings                     parameterization/settings/fit-settings.cc:28-100
+  Dye-track files                 io/dye-tracks-io.cc:32-103
+  NNClassifier, HybridClassifier  classifiers/nn-classifier.h:44-82, hybrid-classifier.h:25-52
   HMMFitter                       fitters/hmm-fitter.h:30-151
 All scoring runs in libwhatprot_amd.so on the GPU; there is no CPU path.
 """
@@ -31,7 +33,8 @@ DBL_MAX = sys.float_info.max
 __all__ = [
     "ChannelModel", "SequencingModel", "SequencingSettings", "DyeSeqEntry", "ScoredClassification",
     "HMMClassifier", "HMMFitter", "FitSettings", "fit_mstep", "offset_exponential_fit", "fit_stats_len",
-    "read_dye_seqs", "write_dye_seqs", "read_radiometries", "write_radiometries",
+    "DyeTracks", "read_dye_tracks", "write_dye_tracks", "simulate_dye_tracks", "kdtree_order", "NNClassifier",
+    "HybridClassifier", "read_dye_seqs", "write_dye_seqs", "read_radiometries", "write_radiometries",
     "write_scored_classifications", "generate_dye_seqs", "simulate_radiometries",
     "WP_PRECISION_FP32", "WP_PRECISION_FP64", "WhatprotError", "DBL_MAX", "lib",
 ]
@@ -493,3 +496,222 @@ class HMMFitter:
                                float(stopping_threshold), float(min(max_runtime, 1e300)), int(max_iterations),
                                ctypes.byref(out), ctypes.byref(step), ctypes.byref(log_l), ctypes.byref(it)))
         return SequencingModel.from_c(out), step.value, log_l.value, it.value
+
+
+@dataclass
+class DyeTracks:
+    """Deduplicated dye-tracks with their sources (SourcedData<DyeTrack,
+    SourceCountHitsList<int>>, io/dye-tracks-io.cc:32-62): counts[i, t, c],
+    and for track i the sources offsets[i]:offsets[i+1] of ids / src_counts /
+    hits."""
+    num_timesteps: int
+    num_channels: int
+    counts: np.ndarray
+    offsets: np.ndarray
+    ids: np.ndarray
+    src_counts: np.ndarray
+    hits: np.ndarray
+
+    def __len__(self):
+        return self.counts.shape[0]
+
+    def _arrays(self):
+        return (np.ascontiguousarray(self.counts, dtype=np.int16), np.ascontiguousarray(self.offsets, dtype=np.int32),
+                np.ascontiguousarray(self.ids, dtype=np.int32), np.ascontiguousarray(self.src_counts, dtype=np.int32),
+                np.ascontiguousarray(self.hits, dtype=np.int32))
+
+
+def read_dye_tracks(path) -> DyeTracks:
+    """read_dye_tracks (io/dye-tracks-io.cc:32-62): T, C, n, then per track
+    T*C counts, the number of sources and (id, count, hits) per source."""
+    with open(path) as f:
+        tok = f.read().split()
+    pos = 0
+
+    def nxt():
+        nonlocal pos
+        pos += 1
+        return int(tok[pos - 1])
+
+    T, C, n = nxt(), nxt(), nxt()
+    counts = np.zeros((n, T, C), dtype=np.int16)
+    offsets = [0]
+    ids, cnts, hits = [], [], []
+    for i in range(n):
+        for j in range(T * C):
+            counts[i].flat[j] = nxt()
+        for _ in range(nxt()):
+            ids.append(nxt())
+            cnts.append(nxt())
+            hits.append(nxt())
+        offsets.append(len(ids))
+    return DyeTracks(T, C, counts, np.array(offsets, np.int32), np.array(ids, np.int32), np.array(cnts, np.int32),
+                     np.array(hits, np.int32))
+
+
+def write_dye_tracks(path, tracks: DyeTracks):
+    """write_dye_tracks (io/dye-tracks-io.cc:64-103), tab separated."""
+    with open(path, "w") as f:
+        f.write(f"{tracks.num_timesteps}\n{tracks.num_channels}\n{len(tracks)}\n")
+        for i in range(len(tracks)):
+            parts = [str(int(x)) for x in tracks.counts[i].ravel()]
+            b, e = int(tracks.offsets[i]), int(tracks.offsets[i + 1])
+            src = []
+            for j in range(b, e):
+                src.append(f"{int(tracks.ids[j])}\t{int(tracks.src_counts[j])}\t{int(tracks.hits[j])}")
+            f.write("\t".join(parts) + "\t" + str(e - b) + "\t" + "\t".join(src) + "\n")
+
+
+def simulate_dye_tracks(seq_model: SequencingModel, T, dye_seqs: Sequence[DyeSeqEntry], per_peptide, seed) -> DyeTracks:
+    """generate_dye_tracks + dedup (generate-dye-tracks.cc:28-63,
+    dedup-dye-tracks.cc:35-148) with this library's seeded generator."""
+    m = seq_model.to_c()
+    C = seq_model.num_channels
+    arr, keep = _cstrings([d.dye_string for d in dye_seqs])
+    n = len(dye_seqs)
+    cnt = np.ascontiguousarray([d.count for d in dye_seqs], dtype=np.int32)
+    ids = np.ascontiguousarray([d.id for d in dye_seqs], dtype=np.int32)
+    cap = int(max(1, cnt.sum() * per_peptide))
+    counts = np.zeros((cap, T, C), dtype=np.int16)
+    off = np.zeros(cap + 1, dtype=np.int32)
+    sid = np.zeros(cap, dtype=np.int32)
+    scnt = np.zeros(cap, dtype=np.int32)
+    shits = np.zeros(cap, dtype=np.int32)
+    nt = ctypes.c_int()
+    check(lib().wp_simulate_dye_tracks(ctypes.byref(m), T, C, n, arr, cnt.ctypes.data, ids.ctypes.data,
+                                       int(per_peptide), int(seed), cap, ctypes.byref(nt), counts.ctypes.data,
+                                       off.ctypes.data, sid.ctypes.data, scnt.ctypes.data, shits.ctypes.data))
+    k = nt.value
+    ns = int(off[k])
+    return DyeTracks(T, C, counts[:k].copy(), off[:k + 1].copy(), sid[:ns].copy(), scnt[:ns].copy(),
+                     shits[:ns].copy())
+
+
+def kdtree_order(points, k):
+    """The point order the kd-tree construction leaves (make-node.h:28-80):
+    (order, depth)."""
+    X = np.ascontiguousarray(points, dtype=np.float64)
+    n, d = X.shape
+    out = np.zeros(max(n, 1), dtype=np.int32)
+    depth = ctypes.c_int()
+    check(lib().wp_kdtree_order(int(k), d, n, X.ctypes.data, out.ctypes.data, ctypes.byref(depth)))
+    return out[:n], depth.value
+
+
+class NNClassifier:
+    """NNClassifier (classifiers/nn-classifier.h:44-82): kd-tree kNN over
+    simulated dye-tracks, searched on the GPU."""
+
+    def __init__(self, num_timesteps, num_channels, seq_model: SequencingModel, k, sig, dye_tracks: DyeTracks):
+        self.num_timesteps, self.num_channels, self.k = int(num_timesteps), int(num_channels), int(k)
+        self._m = seq_model.to_c()
+        counts, off, ids, cnts, hits = dye_tracks._arrays()
+        h = ctypes.c_void_p()
+        check(lib().wp_nn_classifier_create(self.num_timesteps, self.num_channels, ctypes.byref(self._m), self.k,
+                                            float(sig), len(dye_tracks), counts.ctypes.data, off.ctypes.data,
+                                            ids.ctypes.data, cnts.ctypes.data, hits.ctypes.data, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().wp_nn_classifier_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _rads(self, radiometries):
+        r = np.ascontiguousarray(radiometries, dtype=np.float64)
+        if r.ndim != 3 or r.shape[1] != self.num_timesteps or r.shape[2] != self.num_channels:
+            raise WhatprotError(f"radiometries must be [N, {self.num_timesteps}, {self.num_channels}]")
+        return r
+
+    def search(self, radiometries):
+        """KDTree::search per read: (counts [N], tracks [N, k], dists_sq [N, k]),
+        farthest first."""
+        r = self._rads(radiometries)
+        N = r.shape[0]
+        cnt = np.zeros(N, dtype=np.int32)
+        idx = np.zeros((N, self.k), dtype=np.int32)
+        dist = np.zeros((N, self.k))
+        check(lib().wp_nn_search(self._h, N, r.ctypes.data, cnt.ctypes.data, idx.ctypes.data, dist.ctypes.data))
+        return cnt, idx, dist
+
+    def classify_arrays(self, radiometries):
+        """(ids, scores, totals) of NNClassifier::classify."""
+        r = self._rads(radiometries)
+        N = r.shape[0]
+        ids = np.zeros(N, dtype=np.int32)
+        sc = np.zeros(N)
+        tot = np.zeros(N)
+        check(lib().wp_nn_classify(self._h, N, r.ctypes.data, ids.ctypes.data, sc.ctypes.data, tot.ctypes.data))
+        return ids, sc, tot
+
+    def classify(self, radiometries) -> List[ScoredClassification]:
+        ids, sc, tot = self.classify_arrays(radiometries)
+        return [ScoredClassification(int(i), float(s / t) if t else 0.0, math.log(s) if s > 0 else -math.inf,
+                                     math.log(t) if t > 0 else -math.inf) for i, s, t in zip(ids, sc, tot)]
+
+    def classify_top(self, radiometries, h):
+        """NNClassifier::classify(radiometry, h): per read a list of (id, score)
+        in ascending score order, and the totals."""
+        r = self._rads(radiometries)
+        N = r.shape[0]
+        cnt = np.zeros(N, dtype=np.int32)
+        ids = np.zeros((N, max(h, 1)), dtype=np.int32)
+        sc = np.zeros((N, max(h, 1)))
+        tot = np.zeros(N)
+        check(lib().wp_nn_classify_top(self._h, N, r.ctypes.data, int(h), cnt.ctypes.data, ids.ctypes.data,
+                                       sc.ctypes.data, tot.ctypes.data))
+        return [list(zip(ids[i, :cnt[i]].tolist(), sc[i, :cnt[i]].tolist())) for i in range(N)], tot
+
+
+class HybridClassifier:
+    """HybridClassifier (classifiers/hybrid-classifier.h:25-52): kNN top-h
+    candidates rescored by the HMM, both on the GPU."""
+
+    def __init__(self, num_timesteps, num_channels, seq_model: SequencingModel, seq_settings: SequencingSettings,
+                 k, sig, dye_tracks: DyeTracks, h, dye_seqs: Sequence[DyeSeqEntry], precision=WP_PRECISION_FP64):
+        self.num_timesteps, self.num_channels = int(num_timesteps), int(num_channels)
+        self._m = seq_model.to_c()
+        counts, off, ids, cnts, hits = dye_tracks._arrays()
+        arr, self._keep = _cstrings([d.dye_string for d in dye_seqs])
+        S = len(dye_seqs)
+        dcounts = (ctypes.c_int * max(1, S))(*[d.count for d in dye_seqs])
+        dids = (ctypes.c_int * max(1, S))(*[d.id for d in dye_seqs])
+        hnd = ctypes.c_void_p()
+        check(lib().wp_hybrid_classifier_create(self.num_timesteps, self.num_channels, ctypes.byref(self._m),
+                                                float(seq_settings.dist_cutoff), int(k), float(sig), len(dye_tracks),
+                                                counts.ctypes.data, off.ctypes.data, ids.ctypes.data,
+                                                cnts.ctypes.data, hits.ctypes.data, int(h), S, arr, dcounts, dids,
+                                                int(precision), ctypes.byref(hnd)))
+        self._h = hnd
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().wp_hybrid_classifier_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def classify_arrays(self, radiometries):
+        """(ids, adjusted scores) of HybridClassifier::classify."""
+        r = np.ascontiguousarray(radiometries, dtype=np.float64)
+        if r.ndim != 3 or r.shape[1] != self.num_timesteps or r.shape[2] != self.num_channels:
+            raise WhatprotError(f"radiometries must be [N, {self.num_timesteps}, {self.num_channels}]")
+        N = r.shape[0]
+        ids = np.zeros(N, dtype=np.int32)
+        adj = np.zeros(N)
+        check(lib().wp_hybrid_classify(self._h, N, r.ctypes.data, ids.ctypes.data, adj.ctypes.data))
+        return ids, adj
+
+    def classify(self, radiometries) -> List[ScoredClassification]:
+        ids, adj = self.classify_arrays(radiometries)
+        return [ScoredClassification(int(i), float(a), math.nan, math.nan) for i, a in zip(ids, adj)]

@@ -128,6 +128,31 @@ def lib():
     L.wp_hmm_fit.restype = c_int
     L.wp_hmm_fit.argtypes = [c_void_p, P(ModelC), P(FitSettingsC), c_double, c_double, c_int, P(ModelC),
                              P(c_double), P(c_double), P(c_int)]
+    L.wp_kdtree_order.restype = c_int
+    L.wp_kdtree_order.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p, P(c_int)]
+    L.wp_nn_classifier_create.restype = c_int
+    L.wp_nn_classifier_create.argtypes = [c_uint, c_uint, P(ModelC), c_int, c_double, c_int, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p, P(c_void_p)]
+    L.wp_nn_classifier_destroy.restype = None
+    L.wp_nn_classifier_destroy.argtypes = [c_void_p]
+    L.wp_nn_search.restype = c_int
+    L.wp_nn_search.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.wp_nn_classify.restype = c_int
+    L.wp_nn_classify.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.wp_nn_classify_top.restype = c_int
+    L.wp_nn_classify_top.argtypes = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.wp_hybrid_classifier_create.restype = c_int
+    L.wp_hybrid_classifier_create.argtypes = [c_uint, c_uint, P(ModelC), c_double, c_int, c_double, c_int,
+                                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                              P(ctypes.c_char_p), c_void_p, c_void_p, c_int, P(c_void_p)]
+    L.wp_hybrid_classifier_destroy.restype = None
+    L.wp_hybrid_classifier_destroy.argtypes = [c_void_p]
+    L.wp_hybrid_classify.restype = c_int
+    L.wp_hybrid_classify.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p]
+    L.wp_simulate_dye_tracks.restype = c_int
+    L.wp_simulate_dye_tracks.argtypes = [P(ModelC), c_uint, c_uint, c_int, P(ctypes.c_char_p), c_void_p, c_void_p,
+                                         c_int, ctypes.c_uint64, c_int, P(c_int), c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p]
     L.wp_fit_offset_exponential.restype = c_int
     L.wp_fit_offset_exponential.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_void_p]
     _lib = L

@@ -583,4 +583,43 @@ int wp_simulate_radiometries(const wp_model* seq_model, unsigned int num_timeste
     return WP_OK;
 }
 
+int wp_simulate_dye_tracks(const wp_model* seq_model, unsigned int num_timesteps, unsigned int num_channels,
+                           int num_dye_seqs, const char* const* dye_strings, const int* dye_seq_counts,
+                           const int* dye_seq_ids, int per_peptide, uint64_t seed, int max_tracks,
+                           int* out_num_tracks, short* out_track_counts, int* out_source_offsets,
+                           int* out_source_ids, int* out_source_counts, int* out_source_hits) {
+    if (!seq_model || num_dye_seqs < 1 || !dye_strings || !dye_seq_counts || !dye_seq_ids || per_peptide < 0 ||
+        !out_num_tracks || num_timesteps < 1 || num_timesteps > WP_MAX_TIMESTEPS || num_channels < 1 ||
+        num_channels > WP_MAX_CHANNELS) {
+        wp::set_error("bad simulator arguments");
+        return WP_ERR_ARG;
+    }
+    int cap = 0;
+    for (int i = 0; i < num_dye_seqs; i++) cap = std::max(cap, (int)std::strlen(dye_strings[i]));
+    cap = std::max(cap, 1);
+    std::vector<short> seqs((size_t)num_dye_seqs * cap, -1);
+    std::vector<int> lens(num_dye_seqs);
+    for (int i = 0; i < num_dye_seqs; i++) lens[i] = wp::parse_dye_seq(dye_strings[i], &seqs[(size_t)i * cap], cap);
+    std::vector<short> tc;
+    std::vector<int> off, id, cnt, hits;
+    int n = wp::simulate_dye_tracks(*seq_model, (int)num_timesteps, (int)num_channels, num_dye_seqs, seqs.data(),
+                                    cap, lens.data(), dye_seq_counts, dye_seq_ids, per_peptide, seed, &tc, &off, &id,
+                                    &cnt, &hits);
+    *out_num_tracks = n;
+    if (n > max_tracks) {
+        wp::set_error("more distinct dye-tracks than max_tracks");
+        return WP_ERR_ARG;
+    }
+    if (!out_track_counts || !out_source_offsets || !out_source_ids || !out_source_counts || !out_source_hits) {
+        wp::set_error("null output");
+        return WP_ERR_ARG;
+    }
+    std::memcpy(out_track_counts, tc.data(), sizeof(short) * tc.size());
+    std::memcpy(out_source_offsets, off.data(), sizeof(int) * off.size());
+    std::memcpy(out_source_ids, id.data(), sizeof(int) * id.size());
+    std::memcpy(out_source_counts, cnt.data(), sizeof(int) * cnt.size());
+    std::memcpy(out_source_hits, hits.data(), sizeof(int) * hits.size());
+    return WP_OK;
+}
+
 }  // extern "C"

@@ -1,0 +1,243 @@
+// kd-tree k-nearest-neighbour search on the GPU (KDTree::search,
+// kd-tree/kd-tree.h:40-46) with the reference's exact semantics.
+//
+// One wavefront per query walks the tree depth first in the reference's
+// order (internal-node.h:33-56: the near side of the split first, the far
+// side afterwards only if the current k-th distance still exceeds the squared
+// gap, strictly). At a leaf the 64 lanes compute the squared distances of up
+// to 64 points at once, summed as leaf-node.h:38-60 sums them (groups of four
+// dimensions, ((x1^2 + x2^2) + x3^2) + x4^2 added to the running total, the
+// rest one by one, no contraction); the partial-sum early exit there only
+// skips points whose full distance is >= the k-th distance, so evaluating
+// every point and then admitting, in leaf order, those strictly below the
+// current k-th distance admits the same points. Admission is KBest::insert
+// (k-best.h:26-43): a max-heap on (distance, position) where position is the
+// point's place in the permuted array (the reference's pointer order), one
+// pop when the farthest entry is not needed to keep k hits, and the k-th
+// distance set once the hits reach k. The heap lives in LDS and is updated
+// by lane 0.
+#include <hip/hip_runtime.h>
+#include <float.h>
+
+#include "kdtree.h"
+
+namespace wp {
+namespace {
+
+constexpr int kWaves = 4;
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct WaveLds {
+    double* q;       // [d]
+    double* hd;      // heap distances [k]
+    int* hp;         // heap positions [k]
+    int* sn;         // stack nodes [depth]
+    double* sg;      // stack squared gaps [depth]
+    double* cd;      // chunk distances [64]
+    int* st;         // hsize, hits
+    double* kth;
+};
+
+__host__ __device__ inline size_t align8(size_t x) { return (x + 7) & ~(size_t)7; }
+
+// The heap holds k + 1 entries between KBest::insert's push and its pop; the
+// stack holds one entry per internal node on the current path (< depth).
+__host__ __device__ inline size_t wave_bytes(int d, int k, int depth) {
+    return align8(8 * (size_t)d) + align8(8 * (size_t)(k + 1)) + align8(4 * (size_t)(k + 1)) +
+           align8(4 * (size_t)(depth + 1)) + align8(8 * (size_t)(depth + 1)) + 8 * 64 + 16 + 8;
+}
+
+__device__ inline WaveLds carve(unsigned char* base, int d, int k, int depth) {
+    WaveLds w;
+    size_t o = 0;
+    w.q = (double*)(base + o);
+    o += align8(8 * (size_t)d);
+    w.hd = (double*)(base + o);
+    o += align8(8 * (size_t)(k + 1));
+    w.hp = (int*)(base + o);
+    o += align8(4 * (size_t)(k + 1));
+    w.sn = (int*)(base + o);
+    o += align8(4 * (size_t)(depth + 1));
+    w.sg = (double*)(base + o);
+    o += align8(8 * (size_t)(depth + 1));
+    w.cd = (double*)(base + o);
+    o += 8 * 64;
+    w.st = (int*)(base + o);
+    o += 16;
+    w.kth = (double*)(base + o);
+    return w;
+}
+
+// (d1, p1) < (d2, p2) lexicographically (std::pair ordering).
+__device__ inline bool less_dp(double d1, int p1, double d2, int p2) { return d1 < d2 || (d1 == d2 && p1 < p2); }
+
+__device__ void heap_push(double* hd, int* hp, int& n, double d, int p) {
+    int i = n++;
+    while (i > 0) {
+        int parent = (i - 1) >> 1;
+        if (!less_dp(hd[parent], hp[parent], d, p)) break;
+        hd[i] = hd[parent];
+        hp[i] = hp[parent];
+        i = parent;
+    }
+    hd[i] = d;
+    hp[i] = p;
+}
+
+__device__ void heap_pop(double* hd, int* hp, int& n) {
+    n--;
+    double d = hd[n];
+    int p = hp[n];
+    int i = 0;
+    while (true) {
+        int c = 2 * i + 1;
+        if (c >= n) break;
+        if (c + 1 < n && less_dp(hd[c], hp[c], hd[c + 1], hp[c + 1])) c++;
+        if (!less_dp(d, p, hd[c], hp[c])) break;
+        hd[i] = hd[c];
+        hp[i] = hp[c];
+        i = c;
+    }
+    if (n > 0) {
+        hd[i] = d;
+        hp[i] = p;
+    }
+}
+
+__global__ void __launch_bounds__(64 * kWaves) knn_kernel(const KnnArgs a) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int qi = blockIdx.x * kWaves + wv;
+    if (qi >= a.N) return;  // whole wavefront
+    const int d = a.d, k = a.k;
+    WaveLds L = carve(smem + (size_t)wv * wave_bytes(d, k, a.depth), d, k, a.depth);
+    for (int i = lane; i < d; i += 64) L.q[i] = a.queries[(size_t)qi * d + i];
+    if (lane == 0) {
+        L.st[0] = 0;  // heap size
+        L.st[1] = 0;  // hits
+        L.kth[0] = DBL_MAX;
+    }
+    wave_sync();
+    int sp = 0;
+    int node = 0;
+    bool done = false;
+    while (!done) {
+        // descend to a leaf, stacking the far sides
+        KdNode nd = a.nodes[node];
+        while (nd.left >= 0) {
+            double qv = L.q[nd.s];
+            int near, far;
+            double gap;
+            if (qv < nd.split) {
+                near = nd.left;
+                far = nd.right;
+                gap = nd.min_right - qv;
+            } else {
+                near = nd.right;
+                far = nd.left;
+                gap = qv - nd.max_left;
+            }
+            if (lane == 0) {
+                L.sn[sp] = far;
+                L.sg[sp] = gap * gap;
+            }
+            sp++;
+            node = near;
+            nd = a.nodes[node];
+        }
+        // the leaf's points, 64 at a time
+        for (int base = nd.begin; base < nd.end; base += 64) {
+            int p = base + lane;
+            if (p < nd.end) {
+                const double* e = a.points + (size_t)p * d;
+                double s = 0.0;
+                int i = 0;
+                while (i < d - 3) {
+                    double x1 = L.q[i] - e[i];
+                    double x2 = L.q[i + 1] - e[i + 1];
+                    double x3 = L.q[i + 2] - e[i + 2];
+                    double x4 = L.q[i + 3] - e[i + 3];
+                    s += x1 * x1 + x2 * x2 + x3 * x3 + x4 * x4;
+                    i += 4;
+                }
+                while (i < d) {
+                    double x = L.q[i] - e[i];
+                    s += x * x;
+                    i++;
+                }
+                L.cd[lane] = s;
+            }
+            wave_sync();
+            if (lane == 0) {
+                int cnt = min(64, nd.end - base);
+                int hn = L.st[0], hits = L.st[1];
+                double kth = L.kth[0];
+                for (int j = 0; j < cnt; j++) {
+                    double dist = L.cd[j];
+                    if (dist >= kth) continue;
+                    int pos = base + j;
+                    hits += a.hits[pos];
+                    heap_push(L.hd, L.hp, hn, dist, pos);
+                    int top_hits = a.hits[L.hp[0]];
+                    if (hits - top_hits >= k) {
+                        hits -= top_hits;
+                        heap_pop(L.hd, L.hp, hn);
+                    }
+                    if (hits >= k) kth = L.hd[0];
+                }
+                L.st[0] = hn;
+                L.st[1] = hits;
+                L.kth[0] = kth;
+            }
+            wave_sync();
+        }
+        // next far side worth visiting
+        done = true;
+        while (sp > 0) {
+            sp--;
+            double g = L.sg[sp];
+            int far = L.sn[sp];
+            if (L.kth[0] > g) {
+                node = far;
+                done = false;
+                break;
+            }
+        }
+    }
+    // KBest::fill: farthest first
+    if (lane == 0) {
+        int hn = L.st[0];
+        int* oi = a.out_index + (size_t)qi * k;
+        double* od = a.out_dist_sq + (size_t)qi * k;
+        a.out_count[qi] = hn;
+        int j = 0;
+        while (hn > 0) {
+            od[j] = L.hd[0];
+            oi[j] = a.perm[L.hp[0]];
+            j++;
+            heap_pop(L.hd, L.hp, hn);
+        }
+    }
+}
+
+}  // namespace
+
+size_t knn_lds_bytes(int d, int k, int depth) { return kWaves * wave_bytes(d, k, depth); }
+
+hipError_t launch_knn(const KnnArgs& a, hipStream_t stream) {
+    if (a.N <= 0) return hipSuccess;
+    size_t lds = knn_lds_bytes(a.d, a.k, a.depth);
+    hipError_t e = hipFuncSetAttribute((const void*)knn_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int grid = (a.N + kWaves - 1) / kWaves;
+    hipLaunchKernelGGL(knn_kernel, dim3(grid), dim3(64 * kWaves), lds, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace wp

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "wp_internal.h"
@@ -169,6 +170,51 @@ void simulate_radiometries(const wp_model& m, int T, int C, int nds, const short
         if (out_tracks) std::memcpy(out_tracks + (size_t)made * T * C, track.data(), sizeof(unsigned int) * T * C);
         made++;
     }
+}
+
+// generate_dye_tracks (simulation/generate-dye-tracks.cc:28-63): per_peptide
+// tracks for every peptide (count) of every dye-seq, all-zero tracks dropped;
+// then deduplicated as dedup-dye-tracks.cc does (one entry per distinct track
+// with, per source dye-seq id, its count and the number of hits). Entries are
+// ordered by their counts (lexicographically), sources by id.
+int simulate_dye_tracks(const wp_model& m, int T, int C, int nds, const short* seqs, int cap, const int* lens,
+                        const int* counts, const int* ids, int per_peptide, uint64_t seed,
+                        std::vector<short>* track_counts, std::vector<int>* src_off, std::vector<int>* src_id,
+                        std::vector<int>* src_count, std::vector<int>* src_hits) {
+    Rng rng(seed);
+    std::map<std::vector<short>, std::map<int, std::pair<int, int>>> table;
+    std::vector<unsigned int> track((size_t)T * C);
+    std::vector<short> key((size_t)T * C);
+    for (int i = 0; i < nds; i++) {
+        for (int copy = 0; copy < counts[i]; copy++) {
+            for (int j = 0; j < per_peptide; j++) {
+                dye_track(m, seqs + (size_t)i * cap, lens[i], T, C, rng, track.data());
+                bool trivial = true;
+                for (int c = 0; c < C; c++)
+                    if (track[c] != 0) trivial = false;
+                if (trivial) continue;
+                for (size_t q = 0; q < key.size(); q++) key[q] = (short)track[q];
+                auto& src = table[key][ids[i]];
+                src.first = counts[i];
+                src.second++;
+            }
+        }
+    }
+    track_counts->clear();
+    src_off->assign(1, 0);
+    src_id->clear();
+    src_count->clear();
+    src_hits->clear();
+    for (const auto& kv : table) {
+        track_counts->insert(track_counts->end(), kv.first.begin(), kv.first.end());
+        for (const auto& s : kv.second) {
+            src_id->push_back(s.first);
+            src_count->push_back(s.second.first);
+            src_hits->push_back(s.second.second);
+        }
+        src_off->push_back((int)src_id->size());
+    }
+    return (int)table.size();
 }
 
 }  // namespace wp

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/whatprot_amd.h"
 
@@ -46,6 +47,11 @@ void generate_dye_seqs(int C, int n, int len_min, int len_max, int k_min, int k_
 void simulate_radiometries(const wp_model& m, int T, int C, int nds, const short* seqs, int cap,
                            const int* lens, const int* weights, int n, uint64_t seed,
                            double* out, int* out_idx, unsigned int* out_tracks);
+
+int simulate_dye_tracks(const wp_model& m, int T, int C, int nds, const short* seqs, int cap, const int* lens,
+                        const int* counts, const int* ids, int per_peptide, uint64_t seed,
+                        std::vector<short>* track_counts, std::vector<int>* src_off, std::vector<int>* src_id,
+                        std::vector<int>* src_count, std::vector<int>* src_hits);
 
 // least_squares_fit_of_offset_exponential (util/levmar-helper.cc:44-85):
 // weighted fit of y_t ~ p[0] + p[1] exp(-p[2] t), t = 0..n-1, bounds [0, 1],
