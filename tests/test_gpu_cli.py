@@ -1,0 +1,91 @@
+"""Command line end to end on the GPU: files in the reference's formats in,
+results CSV out, against the reference's classifiers and fitter compiled from
+its sources (the model read with mu = 10000, radiometries divided by it)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import whatprot_amd as w
+from tests._files import read_results, write_radiometries_scaled, write_seqparams
+
+pytestmark = pytest.mark.gpu
+CLI = os.path.join(os.path.dirname(w.__file__), "whatprot-amd")
+MU = 10000.0
+
+
+def run(*args):
+    r = subprocess.run([CLI, *[str(a) for a in args]], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+@pytest.fixture(scope="module")
+def files(tmp_path_factory):
+    d = tmp_path_factory.mktemp("cli")
+    C, T = 2, 8
+    sp = d / "sp.json"
+    write_seqparams(sp, C, MU)
+    truth = w.SequencingModel.from_json(str(sp))
+    model = truth.with_mu_as_one()
+    strings = w.generate_dye_seqs(C, 40, 6, 12, 0, 3, 71)
+    seqs = [w.DyeSeqEntry(s, 1 + i % 2, 500 + i) for i, s in enumerate(strings)]
+    ds = d / "ds.tsv"
+    w.write_dye_seqs(str(ds), C, seqs)
+    rads, _ = w.simulate_radiometries(model, T, strings, 120, 72)
+    rp = d / "r.tsv"
+    write_radiometries_scaled(rp, rads, MU)
+    _, _, back = w.read_radiometries(str(rp), truth)  # what the reference's reader yields
+    tracks = w.simulate_dye_tracks(model, T, seqs, 25, 73)
+    tp = d / "dt.tsv"
+    w.write_dye_tracks(str(tp), tracks)
+    return dict(d=d, sp=sp, ds=ds, rp=rp, tp=tp, model=model, seqs=seqs, rads=back, tracks=tracks, T=T, C=C)
+
+
+def test_classify_hmm(gpu, ref, files):
+    out = files["d"] / "hmm.csv"
+    run("classify", "hmm", "-P", files["sp"], "-S", files["ds"], "-R", files["rp"], "-Y", out, "-p", "5")
+    ids, scores = read_results(out)
+    seqs = files["seqs"]
+    rid, rs, rt = ref.classify(files["model"], [s.dye_string for s in seqs], [s.count for s in seqs],
+                               [s.id for s in seqs], files["rads"], 5.0)
+    np.testing.assert_array_equal(ids, rid)
+    np.testing.assert_allclose(scores, rs / rt, rtol=1e-12)
+
+
+def test_classify_nn(gpu, ref, files):
+    out = files["d"] / "nn.csv"
+    run("classify", "nn", "-P", files["sp"], "-k", 10, "-s", 0.5, "-T", files["tp"], "-R", files["rp"], "-Y", out)
+    ids, scores = read_results(out)
+    rid, rs, rt = ref.nn_classify(files["model"], files["tracks"], 10, 0.5, files["rads"])
+    np.testing.assert_array_equal(ids, rid)
+    np.testing.assert_allclose(scores, rs / rt, rtol=1e-15)
+
+
+def test_classify_hybrid(gpu, ref, files):
+    out = files["d"] / "hyb.csv"
+    run("classify", "hybrid", "-P", files["sp"], "-k", 10, "-s", 0.5, "-H", 8, "-S", files["ds"], "-T", files["tp"],
+        "-R", files["rp"], "-Y", out, "-p", 5)
+    ids, scores = read_results(out)
+    seqs = files["seqs"]
+    rid, rs, rt = ref.hybrid_classify(files["model"], files["tracks"], 10, 0.5, 8, [s.dye_string for s in seqs],
+                                      [s.count for s in seqs], [s.id for s in seqs], files["rads"], 5.0)
+    np.testing.assert_array_equal(ids, rid)
+    np.testing.assert_allclose(scores, rs / rt, rtol=1e-12)
+
+
+def test_fit(gpu, ref, files):
+    d, C, T = files["d"], files["C"], 10
+    ds = ".0.1.1.0"
+    model = files["model"]
+    rads, _ = w.simulate_radiometries(model, T, [ds], 80, 74)
+    rp = d / "fit_r.tsv"
+    write_radiometries_scaled(rp, rads, MU)
+    _, _, back = w.read_radiometries(str(rp), w.SequencingModel.from_json(str(files["sp"])))
+    outp = run("fit", "-P", files["sp"], "-x", ds, "-L", 1e-3, "-R", rp)
+    log_l = float(re.search(r"log\(L\): (\S+)", outp).group(1))
+    _, _, want = ref.fit(model, ds, back, 1e-3)
+    assert log_l == pytest.approx(want, rel=1e-8)
+    assert "Edman failure rate: " in outp and "Channel 1 info: (Bleach rate: " in outp

@@ -109,3 +109,19 @@ def test_hybrid_matches_reference(gpu, ref, workload, cutoff):
                                          [s.id for s in seqs], rads, cutoff)
     np.testing.assert_array_equal(ids, rid)
     np.testing.assert_allclose(adj, rsc / rtot, rtol=1e-12, atol=1e-300)
+
+
+def test_search_matches_reference_larger_tree(gpu, ref):
+    """Tens of thousands of tracks in 36 dimensions, k = 100: deep trees,
+    early exits and many visited leaves."""
+    C, T = 3, 12
+    model = w.SequencingModel.uniform(C)
+    strings = w.generate_dye_seqs(C, 1500, 6, 30, 0, 5, 61)
+    seqs = [w.DyeSeqEntry(s, 1, i) for i, s in enumerate(strings)]
+    tracks = w.simulate_dye_tracks(model, T, seqs, 20, 62)
+    rads, _ = w.simulate_radiometries(model, T, strings, 60, 63)
+    got = w.NNClassifier(T, C, model, 100, 0.5, tracks).search(rads)
+    want = ref.nn_search(model, tracks, 100, 0.5, rads)
+    np.testing.assert_array_equal(got[0], want[0])
+    np.testing.assert_array_equal(got[1], want[1])
+    np.testing.assert_array_equal(got[2], want[2])

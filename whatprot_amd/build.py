@@ -1,4 +1,5 @@
-"""Builds libwhatprot_amd.so for gfx950 with hipcc (in tree, no JIT cache)."""
+"""Builds libwhatprot_amd.so for gfx950 with hipcc (in tree, no JIT cache) and
+the whatprot-amd command line over its C ABI."""
 import glob
 import os
 import subprocess
@@ -7,6 +8,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libwhatprot_amd.so")
+CLI_OUT = os.path.join(HERE, "whatprot-amd")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -30,6 +33,13 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(OUT + ".tmp", OUT)
+    # host-only executable: plain g++ against the library's C ABI
+    cli = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, os.path.join(HERE, "cli", "main.cpp"),
+           "-L" + HERE, "-lwhatprot_amd", "-Wl,-rpath,$ORIGIN", "-o", CLI_OUT + ".tmp"]
+    if verbose:
+        print(" ".join(cli), flush=True)
+    subprocess.run(cli, check=True)
+    os.replace(CLI_OUT + ".tmp", CLI_OUT)
     return OUT
 
 

@@ -25,7 +25,7 @@ struct KnnArgs {
     int N, d, k, depth;
     const double* queries;   // [N][d]
     const KdNode* nodes;
-    const double* points;    // [n][d] in permuted order
+    const double* points;    // permuted order, each leaf's block [begin*d, end*d) dimension-major
     const int* hits;         // [n] in permuted order
     const int* perm;         // position -> original index
     int* out_count;          // [N]

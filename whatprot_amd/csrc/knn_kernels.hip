@@ -151,37 +151,48 @@ __global__ void __launch_bounds__(64 * kWaves) knn_kernel(const KnnArgs a) {
             node = near;
             nd = a.nodes[node];
         }
-        // the leaf's points, 64 at a time
-        for (int base = nd.begin; base < nd.end; base += 64) {
-            int p = base + lane;
-            if (p < nd.end) {
-                const double* e = a.points + (size_t)p * d;
-                double s = 0.0;
-                int i = 0;
-                while (i < d - 3) {
-                    double x1 = L.q[i] - e[i];
-                    double x2 = L.q[i + 1] - e[i + 1];
-                    double x3 = L.q[i + 2] - e[i + 2];
-                    double x4 = L.q[i + 3] - e[i + 3];
+        // the leaf's points, 64 at a time; a leaf's block of the point array
+        // is dimension-major (point j, dimension i at i * len + j), so the
+        // lanes' loads coalesce
+        const int len = nd.end - nd.begin;
+        const double* blk = a.points + (size_t)nd.begin * d;
+        for (int c0 = 0; c0 < len; c0 += 64) {
+            const int j = c0 + lane;
+            const double kth0 = L.kth[0];  // never below the k-th distance at admission
+            bool alive = j < len;
+            double s = 0.0;
+            int i = 0;
+            while (i < d - 3) {
+                if (alive) {
+                    double x1 = L.q[i] - blk[(size_t)i * len + j];
+                    double x2 = L.q[i + 1] - blk[(size_t)(i + 1) * len + j];
+                    double x3 = L.q[i + 2] - blk[(size_t)(i + 2) * len + j];
+                    double x4 = L.q[i + 3] - blk[(size_t)(i + 3) * len + j];
                     s += x1 * x1 + x2 * x2 + x3 * x3 + x4 * x4;
-                    i += 4;
+                    if (s >= kth0) alive = false;  // leaf-node.h:52-55: cannot be admitted
                 }
-                while (i < d) {
-                    double x = L.q[i] - e[i];
-                    s += x * x;
-                    i++;
-                }
-                L.cd[lane] = s;
+                i += 4;
+                if (!__any(alive)) break;
             }
+            if (alive) {
+                for (; i < d; i++) {
+                    double x = L.q[i] - blk[(size_t)i * len + j];
+                    s += x * x;
+                }
+                alive = s < kth0;
+            }
+            if (alive) L.cd[lane] = s;
+            unsigned long long cand = __ballot(alive);
             wave_sync();
-            if (lane == 0) {
-                int cnt = min(64, nd.end - base);
+            if (lane == 0 && cand) {
                 int hn = L.st[0], hits = L.st[1];
                 double kth = L.kth[0];
-                for (int j = 0; j < cnt; j++) {
-                    double dist = L.cd[j];
+                while (cand) {
+                    int jj = __builtin_ctzll(cand);  // ascending point order
+                    cand &= cand - 1;
+                    double dist = L.cd[jj];
                     if (dist >= kth) continue;
-                    int pos = base + j;
+                    int pos = nd.begin + c0 + jj;
                     hits += a.hits[pos];
                     heap_push(L.hd, L.hp, hn, dist, pos);
                     int top_hits = a.hits[L.hp[0]];
@@ -222,6 +233,10 @@ __global__ void __launch_bounds__(64 * kWaves) knn_kernel(const KnnArgs a) {
             oi[j] = a.perm[L.hp[0]];
             j++;
             heap_pop(L.hd, L.hp, hn);
+        }
+        for (; j < k; j++) {  // unused slots
+            od[j] = 0.0;
+            oi[j] = -1;
         }
     }
 }

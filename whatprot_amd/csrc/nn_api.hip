@@ -276,12 +276,17 @@ int wp_nn_classifier_create(unsigned int num_timesteps, unsigned int num_channel
             free_nn(c);
             return WP_ERR_UNSUPPORTED;
         }
+        // points in the permuted order, each leaf's block dimension-major
         std::vector<double> P((size_t)n * d);
         std::vector<int> H(n);
-        for (int p = 0; p < n; p++) {
-            std::memcpy(&P[(size_t)p * d], &X[(size_t)perm[p] * d], sizeof(double) * d);
-            H[p] = hits[perm[p]];
+        for (const wp::KdNode& nd : nodes) {
+            if (nd.left >= 0) continue;
+            const int len = nd.end - nd.begin;
+            double* blk = &P[(size_t)nd.begin * d];
+            for (int j = 0; j < len; j++)
+                for (int i = 0; i < d; i++) blk[(size_t)i * len + j] = X[(size_t)perm[nd.begin + j] * d + i];
         }
+        for (int p = 0; p < n; p++) H[p] = hits[perm[p]];
         if (hipMalloc(&c->d_nodes, sizeof(wp::KdNode) * nodes.size()) != hipSuccess ||
             hipMalloc(&c->d_points, sizeof(double) * P.size()) != hipSuccess ||
             hipMalloc(&c->d_hits, sizeof(int) * n) != hipSuccess || hipMalloc(&c->d_perm, sizeof(int) * n) != hipSuccess) {
