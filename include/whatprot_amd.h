@@ -120,6 +120,9 @@ void wp_hmm_classifier_destroy(wp_hmm_classifier* clf);
 /* max_num_dyes over the classifier's dye-seqs (hmm-classifier.cc:47-65). */
 int wp_hmm_classifier_max_num_dyes(const wp_hmm_classifier* clf);
 
+/* Number of dye-seqs the classifier scores (no reference counterpart). */
+int wp_hmm_classifier_num_dye_seqs(const wp_hmm_classifier* clf);
+
 /* Classifies num_reads radiometries laid out [read][t][c] (t*C + c,
  * radiometry.cc:39-44) against every dye-seq. Replaces
  * HMMClassifier::classify(const vector<Radiometry>&)
@@ -258,6 +261,50 @@ void wp_hybrid_classifier_destroy(wp_hybrid_classifier* clf);
  * share. out_adjusted is the written score (score / total). */
 int wp_hybrid_classify(wp_hybrid_classifier* clf, int num_reads, const double* radiometries,
                        int* out_id, double* out_adjusted);
+
+/* ---------------------------------------------------------------------------
+ * Learned shortlist (no reference counterpart: it proposes HMM candidates in
+ * place of, or beside, the kNN stage of HybridClassifier). A ReLU MLP
+ * in_dim -> hidden (x num_hidden) -> out_dim with BF16 weights and
+ * activations, FP32 accumulation on the matrix cores, FP32 master weights
+ * trained with Adam (beta 0.9 / 0.999, eps 1e-8) on the mean softmax
+ * cross-entropy. Layer l's weights are [rows][cols] row-major with
+ * rows = in_dim (l = 0) or hidden, cols = hidden or out_dim (last layer).
+ * Inputs are radiometries divided by mu, [read][t][c].
+ * ------------------------------------------------------------------------- */
+typedef struct wp_mlp wp_mlp;
+
+/* Random initialisation from seed (He-uniform for the ReLU layers,
+ * LeCun-uniform for the logits, zero biases). */
+int wp_mlp_create(int in_dim, int hidden, int num_hidden, int out_dim, uint64_t seed, wp_mlp** out);
+void wp_mlp_destroy(wp_mlp* mlp);
+int wp_mlp_layer_shape(const wp_mlp* mlp, int layer, int* rows, int* cols);
+/* FP32 master weights and biases of one layer (host buffers). */
+int wp_mlp_get_layer(wp_mlp* mlp, int layer, float* W, float* b);
+int wp_mlp_set_layer(wp_mlp* mlp, int layer, const float* W, const float* b);
+/* FP32 logits [B][out_dim] of B inputs [B][in_dim] (host buffers). */
+int wp_mlp_logits(wp_mlp* mlp, int B, const float* x, float* out_logits);
+/* Mean cross-entropy of B labelled inputs and its gradients: gW[l], gb[l]
+ * host buffers shaped like the layers. No update. */
+int wp_mlp_gradients(wp_mlp* mlp, int B, const float* x, const int* labels, double* out_loss, float* const* gW,
+                     float* const* gb);
+/* Adam training: `epochs` passes over N labelled inputs in seeded shuffled
+ * order, batches of `batch` (the last partial batch of an epoch is dropped);
+ * out_epoch_loss[e] (may be NULL) is the mean batch loss of epoch e. */
+int wp_mlp_train(wp_mlp* mlp, int N, const float* x, const int* labels, int batch, int epochs, float lr,
+                 uint64_t seed, float* out_epoch_loss);
+/* The H highest-scoring outputs per read, in increasing index order
+ * (out_index[r * H + j], -1 past the end), and the largest softmax
+ * probability per read (may be NULL). */
+int wp_mlp_shortlist(wp_mlp* mlp, int num_reads, const double* radiometries, int H, int* out_index,
+                     float* out_max_posterior);
+/* Classification with the learned shortlist: reads whose largest MLP
+ * posterior is >= min_posterior are scored by the HMM against their H
+ * shortlisted dye-seqs (output index i = dye-seq i of clf), the others
+ * against every dye-seq, so every output is an HMM score (outputs as
+ * wp_hmm_classify's out_id / out_adjusted). */
+int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* mlp, int num_reads, const double* radiometries, int H,
+                          double min_posterior, int* out_id, double* out_adjusted, int* out_num_fallback);
 
 /* ---------------------------------------------------------------------------
  * Baum-Welch fit. Replaces HMMFitter (fitters/hmm-fitter.h:30-151) as run by

@@ -1,0 +1,98 @@
+"""GPU tests of the learned shortlist: the BF16 MFMA forward and backward
+kernels against the NumPy oracle (which rounds to bf16 where the GPU stores
+bf16), training, the shortlist, and what the trained network gains over an
+untrained one and over the full HMM."""
+import numpy as np
+import pytest
+
+import whatprot_amd as w
+from oracle import mlp as om
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("dims,B", [((36, 64, 3, 100), 64), ((20, 200, 2, 300), 150), ((36, 512, 3, 1000), 256)])
+def test_forward_matches_oracle(gpu, dims, B):
+    din, hid, nh, dout = dims
+    print(f"mlp forward {dims} B={B}", flush=True)
+    net = w.ShortlistMLP(din, hid, nh, dout, seed=3)
+    rng = np.random.default_rng(1)
+    layers = net.get_layers()
+    x = rng.normal(0, 1, (B, din)).astype(np.float32)
+    got = net.logits(x)
+    _, want = om.forward([(W.astype(np.float64), b.astype(np.float64)) for W, b in layers], x)
+    assert _rel(got, want) < 2e-3
+
+
+@pytest.mark.parametrize("dims,B", [((36, 64, 3, 100), 64), ((20, 200, 2, 300), 150)])
+def test_gradients_match_oracle(gpu, dims, B):
+    din, hid, nh, dout = dims
+    print(f"mlp gradients {dims} B={B}", flush=True)
+    net = w.ShortlistMLP(din, hid, nh, dout, seed=5)
+    rng = np.random.default_rng(2)
+    layers = [(W.astype(np.float64), b.astype(np.float64) + rng.normal(0, 0.05, b.shape)) for W, b in net.get_layers()]
+    net.set_layers(layers)
+    layers = [(W.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64)) for W, b in layers]
+    x = rng.normal(0, 1, (B, din)).astype(np.float32)
+    y = rng.integers(0, dout, B)
+    loss, grads = net.gradients(x, y)
+    want_loss, want = om.loss_and_gradients(layers, x, y)
+    assert loss == pytest.approx(want_loss, rel=2e-3)
+    for (gW, gb), (wW, wb) in zip(grads, want):
+        assert _rel(gW, wW) < 1e-2 and _rel(gb, wb) < 1e-2
+
+
+@pytest.fixture(scope="module")
+def small_task():
+    C, T, S = 3, 12, 40
+    model = w.SequencingModel.uniform(C)
+    strings = w.generate_dye_seqs(C, S, 6, 20, 0, 3, 81)
+    train_x, train_y = w.simulate_radiometries(model, T, strings, 60000, 82)
+    test_x, test_y = w.simulate_radiometries(model, T, strings, 2000, 83)
+    return C, T, S, model, strings, train_x, train_y, test_x, test_y
+
+
+def test_training_and_shortlist_gain(gpu, small_task):
+    """Training lowers the loss, and the trained network's top-H shortlist
+    holds the HMM's FP64 top-1 far more often than the untrained one's."""
+    C, T, S, model, strings, train_x, train_y, test_x, test_y = small_task
+    net = w.ShortlistMLP(T * C, 128, 3, S, seed=11)
+    H = 8
+    clf = w.HMMClassifier(T, C, model, w.SequencingSettings(5.0), [w.DyeSeqEntry(s, 1, i) for i, s in
+                                                                   enumerate(strings)], w.WP_PRECISION_FP64)
+    best = clf.classify_arrays(test_x)[0]
+
+    def recall(n):
+        idx, _ = n.shortlist(test_x, H)
+        return float(np.mean([best[i] in idx[i] for i in range(len(best))]))
+
+    before = recall(net)
+    losses = net.train(train_x.reshape(len(train_x), -1), train_y, batch=512, epochs=3, lr=2e-3)
+    after = recall(net)
+    print(f"losses {losses}, recall@{H} {before:.3f} -> {after:.3f}")
+    assert losses[-1] < losses[0] < np.log(S) + 0.5
+    assert after > 0.95 and after > before + 0.3
+
+
+def test_shortlist_classify(gpu, small_task):
+    C, T, S, model, strings, train_x, train_y, test_x, test_y = small_task
+    net = w.ShortlistMLP(T * C, 128, 3, S, seed=11)
+    net.train(train_x.reshape(len(train_x), -1), train_y, batch=512, epochs=2, lr=2e-3)
+    clf = w.HMMClassifier(T, C, model, w.SequencingSettings(5.0), [w.DyeSeqEntry(s, 1, i) for i, s in
+                                                                   enumerate(strings)], w.WP_PRECISION_FP64)
+    full_ids, full_adj, _, _ = clf.classify_arrays(test_x)
+    # every read falls back: identical to the full HMM
+    ids, adj, nf = net.classify(clf, test_x, 8, min_posterior=2.0)
+    assert nf == len(test_x)
+    np.testing.assert_array_equal(ids, full_ids)
+    np.testing.assert_array_equal(adj, full_adj)
+    # no fallback: each id is the HMM best among the shortlist
+    ids, adj, nf = net.classify(clf, test_x, 8, min_posterior=0.0)
+    idx, _ = net.shortlist(test_x, 8)
+    assert nf == 0
+    assert all(ids[i] in idx[i] for i in range(len(ids)))
+    assert np.mean(ids == full_ids) > 0.95

@@ -34,7 +34,7 @@ __all__ = [
     "ChannelModel", "SequencingModel", "SequencingSettings", "DyeSeqEntry", "ScoredClassification",
     "HMMClassifier", "HMMFitter", "FitSettings", "fit_mstep", "offset_exponential_fit", "fit_stats_len",
     "DyeTracks", "read_dye_tracks", "write_dye_tracks", "simulate_dye_tracks", "kdtree_order", "NNClassifier",
-    "HybridClassifier", "read_dye_seqs", "write_dye_seqs", "read_radiometries", "write_radiometries",
+    "HybridClassifier", "ShortlistMLP", "read_dye_seqs", "write_dye_seqs", "read_radiometries", "write_radiometries",
     "write_scored_classifications", "generate_dye_seqs", "simulate_radiometries",
     "WP_PRECISION_FP32", "WP_PRECISION_FP64", "WhatprotError", "DBL_MAX", "lib",
 ]
@@ -715,3 +715,100 @@ class HybridClassifier:
     def classify(self, radiometries) -> List[ScoredClassification]:
         ids, adj = self.classify_arrays(radiometries)
         return [ScoredClassification(int(i), float(a), math.nan, math.nan) for i, a in zip(ids, adj)]
+
+
+class ShortlistMLP:
+    """The learned shortlist's MLP (BF16 on the matrix cores, FP32 master
+    weights): in_dim -> hidden x num_hidden -> out_dim, one output per
+    dye-seq of the HMM classifier it serves."""
+
+    def __init__(self, in_dim, hidden, num_hidden, out_dim, seed=1):
+        h = ctypes.c_void_p()
+        check(lib().wp_mlp_create(int(in_dim), int(hidden), int(num_hidden), int(out_dim), int(seed),
+                                  ctypes.byref(h)))
+        self._h = h
+        self.in_dim, self.hidden, self.num_hidden, self.out_dim = in_dim, hidden, num_hidden, out_dim
+        self.num_layers = num_hidden + 1
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().wp_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def layer_shape(self, l):
+        r, c = ctypes.c_int(), ctypes.c_int()
+        check(lib().wp_mlp_layer_shape(self._h, l, ctypes.byref(r), ctypes.byref(c)))
+        return r.value, c.value
+
+    def get_layers(self):
+        out = []
+        for l in range(self.num_layers):
+            r, c = self.layer_shape(l)
+            W = np.zeros((r, c), dtype=np.float32)
+            b = np.zeros(c, dtype=np.float32)
+            check(lib().wp_mlp_get_layer(self._h, l, W.ctypes.data, b.ctypes.data))
+            out.append((W, b))
+        return out
+
+    def set_layers(self, layers):
+        for l, (W, b) in enumerate(layers):
+            W = np.ascontiguousarray(W, dtype=np.float32)
+            b = np.ascontiguousarray(b, dtype=np.float32)
+            if W.shape != self.layer_shape(l) or b.shape != (W.shape[1],):
+                raise WhatprotError(f"layer {l} must be {self.layer_shape(l)}")
+            check(lib().wp_mlp_set_layer(self._h, l, W.ctypes.data, b.ctypes.data))
+
+    def logits(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.in_dim)
+        out = np.zeros((x.shape[0], self.out_dim), dtype=np.float32)
+        check(lib().wp_mlp_logits(self._h, x.shape[0], x.ctypes.data, out.ctypes.data))
+        return out
+
+    def gradients(self, x, labels):
+        """(mean loss, [(dW, db)...]) without an update."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.in_dim)
+        y = np.ascontiguousarray(labels, dtype=np.int32)
+        gW, gb = [], []
+        for l in range(self.num_layers):
+            r, c = self.layer_shape(l)
+            gW.append(np.zeros((r, c), dtype=np.float32))
+            gb.append(np.zeros(c, dtype=np.float32))
+        pw = (ctypes.c_void_p * self.num_layers)(*[a.ctypes.data for a in gW])
+        pb = (ctypes.c_void_p * self.num_layers)(*[a.ctypes.data for a in gb])
+        loss = ctypes.c_double()
+        check(lib().wp_mlp_gradients(self._h, x.shape[0], x.ctypes.data, y.ctypes.data, ctypes.byref(loss), pw, pb))
+        return loss.value, list(zip(gW, gb))
+
+    def train(self, x, labels, batch=4096, epochs=1, lr=1e-3, seed=7):
+        """Adam over `epochs` shuffled passes; returns the mean loss per epoch."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.in_dim)
+        y = np.ascontiguousarray(labels, dtype=np.int32)
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        check(lib().wp_mlp_train(self._h, x.shape[0], x.ctypes.data, y.ctypes.data, int(batch), int(epochs),
+                                 float(lr), int(seed), losses.ctypes.data))
+        return losses[:epochs]
+
+    def shortlist(self, radiometries, H):
+        """(indices [N, H] in increasing order, max posterior [N])."""
+        r = np.ascontiguousarray(radiometries, dtype=np.float64).reshape(-1, self.in_dim)
+        idx = np.zeros((r.shape[0], H), dtype=np.int32)
+        post = np.zeros(r.shape[0], dtype=np.float32)
+        check(lib().wp_mlp_shortlist(self._h, r.shape[0], r.ctypes.data, int(H), idx.ctypes.data, post.ctypes.data))
+        return idx, post
+
+    def classify(self, classifier: "HMMClassifier", radiometries, H, min_posterior=0.5):
+        """wp_shortlist_classify: (ids, adjusted, number of full-HMM fallbacks)."""
+        r = np.ascontiguousarray(radiometries, dtype=np.float64)
+        N = r.shape[0]
+        ids = np.zeros(N, dtype=np.int32)
+        adj = np.zeros(N)
+        nf = ctypes.c_int()
+        check(lib().wp_shortlist_classify(classifier._h, self._h, N, r.ctypes.data, int(H), float(min_posterior),
+                                          ids.ctypes.data, adj.ctypes.data, ctypes.byref(nf)))
+        return ids, adj, nf.value

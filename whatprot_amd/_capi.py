@@ -153,6 +153,30 @@ def lib():
     L.wp_simulate_dye_tracks.argtypes = [P(ModelC), c_uint, c_uint, c_int, P(ctypes.c_char_p), c_void_p, c_void_p,
                                          c_int, ctypes.c_uint64, c_int, P(c_int), c_void_p, c_void_p, c_void_p,
                                          c_void_p, c_void_p]
+    L.wp_hmm_classifier_num_dye_seqs.restype = c_int
+    L.wp_hmm_classifier_num_dye_seqs.argtypes = [c_void_p]
+    L.wp_mlp_create.restype = c_int
+    L.wp_mlp_create.argtypes = [c_int, c_int, c_int, c_int, ctypes.c_uint64, P(c_void_p)]
+    L.wp_mlp_destroy.restype = None
+    L.wp_mlp_destroy.argtypes = [c_void_p]
+    L.wp_mlp_layer_shape.restype = c_int
+    L.wp_mlp_layer_shape.argtypes = [c_void_p, c_int, P(c_int), P(c_int)]
+    L.wp_mlp_get_layer.restype = c_int
+    L.wp_mlp_get_layer.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
+    L.wp_mlp_set_layer.restype = c_int
+    L.wp_mlp_set_layer.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
+    L.wp_mlp_logits.restype = c_int
+    L.wp_mlp_logits.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
+    L.wp_mlp_gradients.restype = c_int
+    L.wp_mlp_gradients.argtypes = [c_void_p, c_int, c_void_p, c_void_p, P(c_double), c_void_p, c_void_p]
+    L.wp_mlp_train.restype = c_int
+    L.wp_mlp_train.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, ctypes.c_float, ctypes.c_uint64,
+                               c_void_p]
+    L.wp_mlp_shortlist.restype = c_int
+    L.wp_mlp_shortlist.argtypes = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]
+    L.wp_shortlist_classify.restype = c_int
+    L.wp_shortlist_classify.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_int, c_double, c_void_p, c_void_p,
+                                        P(c_int)]
     L.wp_fit_offset_exponential.restype = c_int
     L.wp_fit_offset_exponential.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_void_p]
     _lib = L

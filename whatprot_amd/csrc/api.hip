@@ -369,6 +369,8 @@ void wp_hmm_classifier_destroy(wp_hmm_classifier* c) {
 
 int wp_hmm_classifier_max_num_dyes(const wp_hmm_classifier* c) { return c ? c->M : -1; }
 
+int wp_hmm_classifier_num_dye_seqs(const wp_hmm_classifier* c) { return c ? c->S : -1; }
+
 int wp_hmm_classify_device(wp_hmm_classifier* c, int N, const double* d_rads, int* d_id, double* d_adj,
                            double* d_ls, double* d_lt, void* stream) {
     if (!c || (N > 0 && (!d_rads || !d_id))) {

@@ -1,0 +1,502 @@
+// C ABI of the learned shortlist: a ReLU MLP with BF16 weights and
+// activations, FP32 accumulation and FP32 master weights (Adam), trained on
+// the GPU with the kernels of mlp_kernels.hip, and the shortlist classifier
+// that rescores its top-H candidates with the exact HMM (no reference
+// counterpart; the reference's candidate stage is the kNN of
+// classifiers/hybrid-classifier.cc). Declarations are in
+// include/whatprot_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "mlp_kernels.h"
+#include "model_tables.h"
+#include "wp_internal.h"
+
+struct wp_mlp {
+    int in_dim = 0, in_pad = 0, hidden = 0, nh = 0, out_dim = 0, L = 0;
+    std::vector<int> din, dout;  // per weight layer (din[0] = in_pad)
+    std::vector<float*> W, b, gW, gb, mW, vW, mb, vb;
+    std::vector<__bf16*> Wb;
+    long step = 0;
+    // batch work buffers
+    int cap = 0;
+    __bf16* X = nullptr;
+    std::vector<__bf16*> H, dH;
+    float* Z = nullptr;
+    __bf16* dZ = nullptr;
+    int* y = nullptr;
+    float* loss = nullptr;
+    float* post = nullptr;
+};
+
+namespace {
+
+void free_mlp(wp_mlp* m) {
+    for (auto* v : {&m->W, &m->b, &m->gW, &m->gb, &m->mW, &m->vW, &m->mb, &m->vb})
+        for (float* p : *v) hipFree(p);
+    for (__bf16* p : m->Wb) hipFree(p);
+    for (__bf16* p : m->H) hipFree(p);
+    for (__bf16* p : m->dH) hipFree(p);
+    hipFree(m->X);
+    hipFree(m->Z);
+    hipFree(m->dZ);
+    hipFree(m->y);
+    hipFree(m->loss);
+    hipFree(m->post);
+    delete m;
+}
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+    WP_HIP(hipMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1)));
+    WP_HIP(hipMemset(*p, 0, sizeof(T) * std::max<size_t>(n, 1)));
+    return WP_OK;
+}
+
+int ensure_batch(wp_mlp* m, int B) {
+    if (B <= m->cap) return WP_OK;
+    hipFree(m->X);
+    for (__bf16* p : m->H) hipFree(p);
+    for (__bf16* p : m->dH) hipFree(p);
+    hipFree(m->Z);
+    hipFree(m->dZ);
+    hipFree(m->y);
+    hipFree(m->loss);
+    hipFree(m->post);
+    m->H.assign(m->nh, nullptr);
+    m->dH.assign(m->nh, nullptr);
+    m->cap = 0;
+    int rc = dalloc(&m->X, (size_t)B * m->in_pad);
+    for (int l = 0; l < m->nh && !rc; l++) rc = dalloc(&m->H[l], (size_t)B * m->hidden);
+    for (int l = 0; l < m->nh && !rc; l++) rc = dalloc(&m->dH[l], (size_t)B * m->hidden);
+    if (!rc) rc = dalloc(&m->Z, (size_t)B * m->out_dim);
+    if (!rc) rc = dalloc(&m->dZ, (size_t)B * m->out_dim);
+    if (!rc) rc = dalloc(&m->y, (size_t)B);
+    if (!rc) rc = dalloc(&m->loss, (size_t)B);
+    if (!rc) rc = dalloc(&m->post, (size_t)B);
+    if (!rc) m->cap = B;
+    return rc;
+}
+
+// Z = MLP(X) for the first B rows of m->X.
+int forward(wp_mlp* m, int B) {
+    const __bf16* in = m->X;
+    for (int l = 0; l < m->L; l++) {
+        wp::GemmArgs g;
+        std::memset(&g, 0, sizeof(g));
+        g.M = B;
+        g.N = m->dout[l];
+        g.K = m->din[l];
+        g.A = in;
+        g.sam = m->din[l];
+        g.sak = 1;
+        g.B = m->Wb[l];
+        g.sbk = m->dout[l];
+        g.sbn = 1;
+        g.bias = m->b[l];
+        g.ldc = m->dout[l];
+        if (l < m->nh) {
+            g.Cb = m->H[l];
+            g.relu = 1;
+        } else {
+            g.Cf = m->Z;
+        }
+        WP_HIP(wp::gemm(g, nullptr));
+        if (l < m->nh) in = m->H[l];
+    }
+    return WP_OK;
+}
+
+// Gradients of the mean cross-entropy over the first B rows (labels in
+// m->y) into gW, gb; per-row losses in m->loss.
+int backward(wp_mlp* m, int B) {
+    WP_HIP(wp::softmax_ce(B, m->out_dim, m->Z, m->out_dim, m->y, 1.0f / (float)B, m->dZ, m->loss, m->post, nullptr));
+    const __bf16* dY = m->dZ;
+    for (int l = m->L - 1; l >= 0; l--) {
+        const __bf16* Ain = l == 0 ? m->X : m->H[l - 1];
+        // gW[l] = Ain^T dY
+        wp::GemmArgs g;
+        std::memset(&g, 0, sizeof(g));
+        g.M = m->din[l];
+        g.N = m->dout[l];
+        g.K = B;
+        g.A = Ain;
+        g.sam = 1;
+        g.sak = m->din[l];
+        g.B = dY;
+        g.sbk = m->dout[l];
+        g.sbn = 1;
+        g.Cf = m->gW[l];
+        g.ldc = m->dout[l];
+        WP_HIP(wp::gemm(g, nullptr));
+        WP_HIP(wp::colsum(B, m->dout[l], dY, m->dout[l], m->gb[l], nullptr));
+        if (l == 0) break;
+        // dH[l-1] = (dY W[l]^T) masked by ReLU'(H[l-1])
+        std::memset(&g, 0, sizeof(g));
+        g.M = B;
+        g.N = m->din[l];
+        g.K = m->dout[l];
+        g.A = dY;
+        g.sam = m->dout[l];
+        g.sak = 1;
+        g.B = m->Wb[l];
+        g.sbk = 1;
+        g.sbn = m->dout[l];
+        g.Cb = m->dH[l - 1];
+        g.ldc = m->din[l];
+        g.mask = m->H[l - 1];
+        g.ldm = m->din[l];
+        WP_HIP(wp::gemm(g, nullptr));
+        dY = m->dH[l - 1];
+    }
+    return WP_OK;
+}
+
+int adam_step(wp_mlp* m, float lr) {
+    m->step++;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.0f / (1.0f - std::pow(b1, (float)m->step));
+    const float c2 = 1.0f / (1.0f - std::pow(b2, (float)m->step));
+    for (int l = 0; l < m->L; l++) {
+        long nw = (long)m->din[l] * m->dout[l];
+        WP_HIP(wp::adam(nw, m->W[l], m->gW[l], m->mW[l], m->vW[l], lr, b1, b2, eps, c1, c2, m->Wb[l], nullptr));
+        WP_HIP(wp::adam(m->dout[l], m->b[l], m->gb[l], m->mb[l], m->vb[l], lr, b1, b2, eps, c1, c2, nullptr,
+                        nullptr));
+    }
+    return WP_OK;
+}
+
+uint64_t splitmix(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+double unit(uint64_t* s) { return (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+int upload_host_batch(wp_mlp* m, int B, const float* x) {
+    float* dx = nullptr;
+    WP_HIP(hipMalloc(&dx, sizeof(float) * (size_t)std::max(B, 1) * m->in_dim));
+    hipError_t e = hipMemcpy(dx, x, sizeof(float) * (size_t)B * m->in_dim, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = wp::pack_input(B, m->in_dim, m->in_pad, dx, nullptr, m->X, nullptr);
+    hipFree(dx);
+    WP_HIP(e);
+    return WP_OK;
+}
+
+bool bad_layer(const wp_mlp* m, int l) { return !m || l < 0 || l >= m->L; }
+
+}  // namespace
+
+extern "C" {
+
+int wp_mlp_create(int in_dim, int hidden, int num_hidden, int out_dim, uint64_t seed, wp_mlp** out) {
+    if (!out || in_dim < 1 || hidden < 1 || num_hidden < 1 || out_dim < 1) {
+        wp::set_error("bad MLP shape");
+        return WP_ERR_ARG;
+    }
+    *out = nullptr;
+    int rc = wp::require_device();
+    if (rc) return rc;
+    wp_mlp* m = new wp_mlp();
+    m->in_dim = in_dim;
+    m->in_pad = (in_dim + 31) / 32 * 32;
+    m->hidden = hidden;
+    m->nh = num_hidden;
+    m->out_dim = out_dim;
+    m->L = num_hidden + 1;
+    for (int l = 0; l < m->L; l++) {
+        m->din.push_back(l == 0 ? m->in_pad : hidden);
+        m->dout.push_back(l == m->L - 1 ? out_dim : hidden);
+    }
+    uint64_t st = seed;
+    for (int l = 0; l < m->L && !rc; l++) {
+        const int di = m->din[l], dout = m->dout[l], real_in = l == 0 ? in_dim : di;
+        // He-uniform for the ReLU layers, LeCun-uniform for the logits
+        const double lim = std::sqrt((l == m->L - 1 ? 3.0 : 6.0) / real_in);
+        std::vector<float> w((size_t)di * dout, 0.0f);
+        for (int i = 0; i < real_in; i++)
+            for (int o = 0; o < dout; o++) w[(size_t)i * dout + o] = (float)((2.0 * unit(&st) - 1.0) * lim);
+        float *W = nullptr, *bb = nullptr, *gW = nullptr, *gb = nullptr, *mW = nullptr, *vW = nullptr, *mb = nullptr,
+              *vb = nullptr;
+        __bf16* Wb = nullptr;
+        rc = dalloc(&W, w.size());
+        if (!rc) rc = dalloc(&bb, dout);
+        if (!rc) rc = dalloc(&gW, w.size());
+        if (!rc) rc = dalloc(&gb, dout);
+        if (!rc) rc = dalloc(&mW, w.size());
+        if (!rc) rc = dalloc(&vW, w.size());
+        if (!rc) rc = dalloc(&mb, dout);
+        if (!rc) rc = dalloc(&vb, dout);
+        if (!rc) rc = dalloc(&Wb, w.size());
+        m->W.push_back(W);
+        m->b.push_back(bb);
+        m->gW.push_back(gW);
+        m->gb.push_back(gb);
+        m->mW.push_back(mW);
+        m->vW.push_back(vW);
+        m->mb.push_back(mb);
+        m->vb.push_back(vb);
+        m->Wb.push_back(Wb);
+        if (!rc && hipMemcpy(W, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            wp::set_error("cannot upload the MLP weights");
+            rc = WP_ERR_HIP;
+        }
+        if (!rc && wp::to_bf16((long)w.size(), W, Wb, nullptr) != hipSuccess) rc = WP_ERR_HIP;
+    }
+    if (rc) {
+        free_mlp(m);
+        return rc;
+    }
+    *out = m;
+    return WP_OK;
+}
+
+void wp_mlp_destroy(wp_mlp* m) {
+    if (m) free_mlp(m);
+}
+
+int wp_mlp_layer_shape(const wp_mlp* m, int layer, int* rows, int* cols) {
+    if (bad_layer(m, layer) || !rows || !cols) {
+        wp::set_error("bad layer");
+        return WP_ERR_ARG;
+    }
+    *rows = layer == 0 ? m->in_dim : m->din[layer];
+    *cols = m->dout[layer];
+    return WP_OK;
+}
+
+int wp_mlp_get_layer(wp_mlp* m, int layer, float* W, float* b) {
+    if (bad_layer(m, layer) || !W || !b) {
+        wp::set_error("bad layer");
+        return WP_ERR_ARG;
+    }
+    int rows = layer == 0 ? m->in_dim : m->din[layer];
+    WP_HIP(hipMemcpy(W, m->W[layer], sizeof(float) * (size_t)rows * m->dout[layer], hipMemcpyDeviceToHost));
+    WP_HIP(hipMemcpy(b, m->b[layer], sizeof(float) * m->dout[layer], hipMemcpyDeviceToHost));
+    return WP_OK;
+}
+
+int wp_mlp_set_layer(wp_mlp* m, int layer, const float* W, const float* b) {
+    if (bad_layer(m, layer) || !W || !b) {
+        wp::set_error("bad layer");
+        return WP_ERR_ARG;
+    }
+    int rows = layer == 0 ? m->in_dim : m->din[layer];
+    WP_HIP(hipMemcpy(m->W[layer], W, sizeof(float) * (size_t)rows * m->dout[layer], hipMemcpyHostToDevice));
+    WP_HIP(hipMemcpy(m->b[layer], b, sizeof(float) * m->dout[layer], hipMemcpyHostToDevice));
+    WP_HIP(wp::to_bf16((long)m->din[layer] * m->dout[layer], m->W[layer], m->Wb[layer], nullptr));
+    return WP_OK;
+}
+
+int wp_mlp_logits(wp_mlp* m, int B, const float* x, float* out_logits) {
+    if (!m || B < 0 || (B > 0 && (!x || !out_logits))) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    if (B == 0) return WP_OK;
+    int rc = ensure_batch(m, B);
+    if (!rc) rc = upload_host_batch(m, B, x);
+    if (!rc) rc = forward(m, B);
+    if (rc) return rc;
+    WP_HIP(hipMemcpy(out_logits, m->Z, sizeof(float) * (size_t)B * m->out_dim, hipMemcpyDeviceToHost));
+    return WP_OK;
+}
+
+int wp_mlp_gradients(wp_mlp* m, int B, const float* x, const int* labels, double* out_loss, float* const* gW,
+                     float* const* gb) {
+    if (!m || B < 1 || !x || !labels || !gW || !gb) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    int rc = ensure_batch(m, B);
+    if (!rc) rc = upload_host_batch(m, B, x);
+    if (rc) return rc;
+    WP_HIP(hipMemcpy(m->y, labels, sizeof(int) * B, hipMemcpyHostToDevice));
+    rc = forward(m, B);
+    if (!rc) rc = backward(m, B);
+    if (rc) return rc;
+    std::vector<float> loss(B);
+    WP_HIP(hipMemcpy(loss.data(), m->loss, sizeof(float) * B, hipMemcpyDeviceToHost));
+    if (out_loss) *out_loss = std::accumulate(loss.begin(), loss.end(), 0.0) / B;
+    for (int l = 0; l < m->L; l++) {
+        int rows = l == 0 ? m->in_dim : m->din[l];
+        WP_HIP(hipMemcpy(gW[l], m->gW[l], sizeof(float) * (size_t)rows * m->dout[l], hipMemcpyDeviceToHost));
+        WP_HIP(hipMemcpy(gb[l], m->gb[l], sizeof(float) * m->dout[l], hipMemcpyDeviceToHost));
+    }
+    return WP_OK;
+}
+
+int wp_mlp_train(wp_mlp* m, int N, const float* x, const int* labels, int batch, int epochs, float lr,
+                 uint64_t seed, float* out_epoch_loss) {
+    if (!m || N < 1 || !x || !labels || batch < 1 || epochs < 0) {
+        wp::set_error("bad training arguments");
+        return WP_ERR_ARG;
+    }
+    for (int i = 0; i < N; i++)
+        if (labels[i] < 0 || labels[i] >= m->out_dim) {
+            wp::set_error("label outside [0, out_dim)");
+            return WP_ERR_ARG;
+        }
+    const int B = std::min(batch, N);
+    int rc = ensure_batch(m, B);
+    if (rc) return rc;
+    float* dx = nullptr;
+    int *dy = nullptr, *didx = nullptr;
+    float* dl = nullptr;
+    const int steps = N / B;
+    rc = dalloc(&dx, (size_t)N * m->in_dim);
+    if (!rc) rc = dalloc(&dy, (size_t)N);
+    if (!rc) rc = dalloc(&didx, (size_t)N);
+    if (!rc) rc = dalloc(&dl, (size_t)std::max(steps, 1));
+    if (!rc && (hipMemcpy(dx, x, sizeof(float) * (size_t)N * m->in_dim, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dy, labels, sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess)) {
+        wp::set_error("cannot upload the training set");
+        rc = WP_ERR_HIP;
+    }
+    std::vector<int> perm(N);
+    std::vector<float> sl(std::max(steps, 1));
+    uint64_t st = seed;
+    for (int ep = 0; ep < epochs && !rc; ep++) {
+        std::iota(perm.begin(), perm.end(), 0);
+        for (int i = N - 1; i > 0; i--) std::swap(perm[i], perm[(int)(splitmix(&st) % (uint64_t)(i + 1))]);
+        if (hipMemcpy(didx, perm.data(), sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess) {
+            wp::set_error("cannot upload the epoch order");
+            rc = WP_ERR_HIP;
+            break;
+        }
+        for (int s = 0; s < steps && !rc; s++) {
+            const int* idx = didx + (size_t)s * B;
+            if (wp::pack_input(B, m->in_dim, m->in_pad, dx, idx, m->X, nullptr) != hipSuccess ||
+                wp::gather_int(B, dy, idx, m->y, nullptr) != hipSuccess) {
+                wp::set_error("batch gather failed");
+                rc = WP_ERR_HIP;
+                break;
+            }
+            rc = forward(m, B);
+            if (!rc) rc = backward(m, B);
+            if (!rc) rc = adam_step(m, lr);
+            if (!rc && wp::mean(B, m->loss, dl + s, nullptr) != hipSuccess) rc = WP_ERR_HIP;
+        }
+        if (!rc && steps > 0) {
+            if (hipMemcpy(sl.data(), dl, sizeof(float) * steps, hipMemcpyDeviceToHost) != hipSuccess) {
+                wp::set_error("cannot read the losses");
+                rc = WP_ERR_HIP;
+            } else if (out_epoch_loss) {
+                out_epoch_loss[ep] = (float)(std::accumulate(sl.begin(), sl.begin() + steps, 0.0) / steps);
+            }
+        }
+    }
+    hipFree(dx);
+    hipFree(dy);
+    hipFree(didx);
+    hipFree(dl);
+    return rc;
+}
+
+int wp_mlp_shortlist(wp_mlp* m, int N, const double* radiometries, int H, int* out_index, float* out_max_posterior) {
+    if (!m || N < 0 || H < 1 || H > m->out_dim || (N > 0 && (!radiometries || !out_index))) {
+        wp::set_error("bad shortlist arguments");
+        return WP_ERR_ARG;
+    }
+    const int chunk = std::max(1, std::min(N, (int)std::max<long>(1, (512l << 20) / (4l * m->out_dim))));
+    int rc = ensure_batch(m, chunk);
+    if (rc) return rc;
+    double* dr = nullptr;
+    int* didx = nullptr;
+    rc = dalloc(&dr, (size_t)chunk * m->in_dim);
+    if (!rc) rc = dalloc(&didx, (size_t)chunk * H);
+    for (int r0 = 0; r0 < N && !rc; r0 += chunk) {
+        const int B = std::min(chunk, N - r0);
+        if (hipMemcpy(dr, radiometries + (size_t)r0 * m->in_dim, sizeof(double) * (size_t)B * m->in_dim,
+                      hipMemcpyHostToDevice) != hipSuccess ||
+            wp::pack_input(B, m->in_dim, m->in_pad, dr, nullptr, m->X, nullptr) != hipSuccess) {
+            wp::set_error("cannot stage the radiometries");
+            rc = WP_ERR_HIP;
+            break;
+        }
+        rc = forward(m, B);
+        if (rc) break;
+        if (wp::softmax_ce(B, m->out_dim, m->Z, m->out_dim, nullptr, 1.0f, nullptr, nullptr, m->post, nullptr) !=
+                hipSuccess ||
+            wp::topk(B, m->out_dim, H, m->Z, m->out_dim, didx, nullptr) != hipSuccess ||
+            hipMemcpy(out_index + (size_t)r0 * H, didx, sizeof(int) * (size_t)B * H, hipMemcpyDeviceToHost) !=
+                hipSuccess ||
+            (out_max_posterior &&
+             hipMemcpy(out_max_posterior + r0, m->post, sizeof(float) * B, hipMemcpyDeviceToHost) != hipSuccess)) {
+            wp::set_error("shortlist kernels failed");
+            rc = WP_ERR_HIP;
+        }
+    }
+    hipFree(dr);
+    hipFree(didx);
+    return rc;
+}
+
+int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, const double* radiometries, int H,
+                          double min_posterior, int* out_id, double* out_adjusted, int* out_num_fallback) {
+    if (!clf || !m || num_reads < 0 || (num_reads > 0 && (!radiometries || !out_id || !out_adjusted))) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    const int N = num_reads;
+    if (m->out_dim != wp_hmm_classifier_num_dye_seqs(clf)) {
+        wp::set_error("the MLP's outputs must be the classifier's dye-seqs");
+        return WP_ERR_ARG;
+    }
+    std::vector<int> idx((size_t)N * H);
+    std::vector<float> post(N);
+    int rc = wp_mlp_shortlist(m, N, radiometries, H, idx.data(), post.data());
+    if (rc) return rc;
+    const int d = m->in_dim;
+    std::vector<int> sure, unsure;
+    for (int r = 0; r < N; r++) (post[r] >= min_posterior ? sure : unsure).push_back(r);
+    auto gather = [&](const std::vector<int>& rows, std::vector<double>* x) {
+        x->resize(std::max<size_t>(rows.size(), 1) * d);
+        for (size_t i = 0; i < rows.size(); i++)
+            std::memcpy(&(*x)[i * d], radiometries + (size_t)rows[i] * d, sizeof(double) * d);
+    };
+    std::vector<double> x;
+    if (!sure.empty()) {  // HMM rescoring of the shortlist
+        gather(sure, &x);
+        std::vector<int> off(sure.size() + 1, 0), list;
+        for (size_t i = 0; i < sure.size(); i++) {
+            for (int j = 0; j < H; j++) {
+                int c = idx[(size_t)sure[i] * H + j];
+                if (c >= 0) list.push_back(c);
+            }
+            off[i + 1] = (int)list.size();
+        }
+        if (list.empty()) list.push_back(0);
+        std::vector<int> id(sure.size());
+        std::vector<double> adj(sure.size());
+        rc = wp_hmm_classify_candidates(clf, (int)sure.size(), x.data(), off.data(), list.data(), id.data(),
+                                        adj.data(), nullptr, nullptr);
+        if (rc) return rc;
+        for (size_t i = 0; i < sure.size(); i++) {
+            out_id[sure[i]] = id[i];
+            out_adjusted[sure[i]] = adj[i];
+        }
+    }
+    if (!unsure.empty()) {  // full HMM fallback
+        gather(unsure, &x);
+        std::vector<int> id(unsure.size());
+        std::vector<double> adj(unsure.size());
+        rc = wp_hmm_classify(clf, (int)unsure.size(), x.data(), id.data(), adj.data(), nullptr, nullptr);
+        if (rc) return rc;
+        for (size_t i = 0; i < unsure.size(); i++) {
+            out_id[unsure[i]] = id[i];
+            out_adjusted[unsure[i]] = adj[i];
+        }
+    }
+    if (out_num_fallback) *out_num_fallback = (int)unsure.size();
+    return WP_OK;
+}
+
+}  // extern "C"

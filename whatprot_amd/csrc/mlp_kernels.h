@@ -1,0 +1,39 @@
+// Launch interface of the learned-shortlist MLP kernels (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace wp {
+
+// C(m, n) = epilogue(sum_k A(m, k) B(k, n)); A(m, k) = A[m * sam + k * sak],
+// B(k, n) = B[k * sbk + n * sbn] (bf16). Epilogue: + bias[n], ReLU, zero where
+// mask(m, n) <= 0, then FP32 (Cf) and/or bf16 (Cb) stores with row stride ldc.
+struct GemmArgs {
+    int M, N, K;
+    const __bf16* A;
+    long sam, sak;
+    const __bf16* B;
+    long sbk, sbn;
+    float* Cf;
+    __bf16* Cb;
+    long ldc;
+    const float* bias;
+    int relu;
+    const __bf16* mask;
+    long ldm;
+};
+
+hipError_t gemm(const GemmArgs& g, hipStream_t s);
+hipError_t softmax_ce(int rows, int N, const float* Z, long ldz, const int* labels, float scale, __bf16* dZ,
+                      float* loss, float* max_post, hipStream_t s);
+hipError_t colsum(int M, int N, const __bf16* dY, long ld, float* db, hipStream_t s);
+hipError_t adam(long n, float* p, const float* g, float* m1, float* m2, float lr, float b1, float b2, float eps,
+                float c1, float c2, __bf16* pb, hipStream_t s);
+hipError_t to_bf16(long n, const float* x, __bf16* y, hipStream_t s);
+hipError_t pack_input(int rows, int din, int dpad, const float* x, const int* idx, __bf16* y, hipStream_t s);
+hipError_t pack_input(int rows, int din, int dpad, const double* x, const int* idx, __bf16* y, hipStream_t s);
+hipError_t gather_int(int n, const int* src, const int* idx, int* dst, hipStream_t s);
+hipError_t mean(int n, const float* x, float* out, hipStream_t s);
+hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, hipStream_t s);
+
+}  // namespace wp

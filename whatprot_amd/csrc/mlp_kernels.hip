@@ -1,0 +1,343 @@
+// Kernels of the learned shortlist: a BF16 MLP on the matrix cores.
+//
+// GEMM: C(m, n) = sum_k A(m, k) B(k, n) with bf16 operands read through
+// arbitrary (row, column) strides -- the forward product X W, the backward
+// products dY W^T and X^T dY all use it -- FP32 accumulation on
+// v_mfma_f32_16x16x32_bf16, and a fused epilogue (bias, ReLU, ReLU-derivative
+// mask, FP32 and/or BF16 stores). A 256-thread workgroup owns a 128 x 128
+// tile of C; its four wavefronts own 64 x 64 quarters as 4 x 4 MFMA tiles.
+// K advances 32 at a time through LDS tiles stored k-contiguous for both
+// operands, so every MFMA operand is one 16-byte LDS read.
+//
+// MFMA 16x16x32 lane layout (lane l, g = l / 16, r = l % 16): A operand row r,
+// k = 8g .. 8g+7; B operand column r, k = 8g .. 8g+7; accumulator rows
+// 4g .. 4g+3 of column r.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "mlp_kernels.h"
+
+namespace wp {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128, BN = 128, BK = 32, PAD = 8, LDK = BK + PAD;
+
+__device__ inline __bf16 to_bf16(float x) { return (__bf16)x; }
+__device__ inline float from_bf16(__bf16 x) { return (float)x; }
+
+__global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
+    __shared__ __align__(16) __bf16 As[BM * LDK];
+    __shared__ __align__(16) __bf16 Bs[BN * LDK];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+    const int lr = lane & 15, lg = lane >> 4;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bool a_k_contig = g.sak == 1;
+    const bool b_n_contig = g.sbn == 1;
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        // A tile [BM][BK] -> As[m][k]
+#pragma unroll 4
+        for (int it = 0; it < BM * BK / 256; it++) {
+            int e = tid + it * 256;
+            int m, k;
+            if (a_k_contig) {
+                m = e / BK;
+                k = e % BK;
+            } else {
+                m = e % BM;
+                k = e / BM;
+            }
+            int gm = m0 + m, gk = k0 + k;
+            __bf16 v = (__bf16)0.0f;
+            if (gm < g.M && gk < g.K) v = g.A[(long)gm * g.sam + (long)gk * g.sak];
+            As[m * LDK + k] = v;
+        }
+        // B tile [BK][BN] -> Bs[n][k]
+#pragma unroll 4
+        for (int it = 0; it < BN * BK / 256; it++) {
+            int e = tid + it * 256;
+            int n, k;
+            if (b_n_contig) {
+                n = e % BN;
+                k = e / BN;
+            } else {
+                n = e / BK;
+                k = e % BK;
+            }
+            int gn = n0 + n, gk = k0 + k;
+            __bf16 v = (__bf16)0.0f;
+            if (gn < g.N && gk < g.K) v = g.B[(long)gk * g.sbk + (long)gn * g.sbn];
+            Bs[n * LDK + k] = v;
+        }
+        __syncthreads();
+        bf16x8 af[4], bfr[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) af[i] = *(const bf16x8*)&As[(wm + 16 * i + lr) * LDK + 8 * lg];
+#pragma unroll
+        for (int j = 0; j < 4; j++) bfr[j] = *(const bf16x8*)&Bs[(wn + 16 * j + lr) * LDK + 8 * lg];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+    // epilogue
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int n = n0 + wn + 16 * j + lr;
+            if (n >= g.N) continue;
+            const float bias = g.bias ? g.bias[n] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + 16 * i + 4 * lg + r;
+                if (m >= g.M) continue;
+                float v = acc[i][j][r] + bias;
+                if (g.relu) v = fmaxf(v, 0.0f);
+                if (g.mask && !(from_bf16(g.mask[(long)m * g.ldm + n]) > 0.0f)) v = 0.0f;
+                if (g.Cf) g.Cf[(long)m * g.ldc + n] = v;
+                if (g.Cb) g.Cb[(long)m * g.ldc + n] = to_bf16(v);
+            }
+        }
+    }
+}
+
+// Row-wise softmax cross-entropy over N logits (one workgroup per row):
+// loss_r = logsumexp(z_r) - z_r[y_r]; dZ = (softmax - onehot) * scale as
+// bf16; max posterior = exp(max - logsumexp).
+__global__ void __launch_bounds__(256) softmax_ce_kernel(int N, const float* __restrict__ Z, long ldz,
+                                                          const int* __restrict__ labels, float scale,
+                                                          __bf16* __restrict__ dZ, float* __restrict__ loss,
+                                                          float* __restrict__ max_post) {
+    __shared__ float red[8];
+    const int r = blockIdx.x, tid = threadIdx.x;
+    const float* z = Z + (long)r * ldz;
+    float mx = -FLT_MAX;
+    for (int n = tid; n < N; n += 256) mx = fmaxf(mx, z[n]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    if ((tid & 63) == 0) red[tid >> 6] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    __syncthreads();
+    float s = 0.0f;
+    for (int n = tid; n < N; n += 256) s += expf(z[n] - mx);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    s = red[0] + red[1] + red[2] + red[3];
+    const float lse = mx + logf(s);
+    const int y = labels ? labels[r] : -1;
+    if (dZ) {
+        for (int n = tid; n < N; n += 256) {
+            float p = expf(z[n] - lse);
+            dZ[(long)r * ldz + n] = to_bf16((p - (n == y ? 1.0f : 0.0f)) * scale);
+        }
+    }
+    if (tid == 0) {
+        if (loss) loss[r] = (y >= 0 && y < N) ? lse - z[y] : 0.0f;
+        if (max_post) max_post[r] = expf(mx - lse);
+    }
+}
+
+// db[n] = sum_m dY(m, n) (bf16 input, FP32 sum), one thread per column.
+__global__ void colsum_kernel(int M, int N, const __bf16* __restrict__ dY, long ld, float* __restrict__ db) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.0f;
+    for (int m = 0; m < M; m++) s += from_bf16(dY[(long)m * ld + n]);
+    db[n] = s;
+}
+
+// Adam on FP32 master weights; refreshes the bf16 copy when given.
+__global__ void adam_kernel(long n, float* __restrict__ p, const float* __restrict__ grad, float* __restrict__ m1,
+                            float* __restrict__ m2, float lr, float b1, float b2, float eps, float c1, float c2,
+                            __bf16* __restrict__ pb) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gr = grad[i];
+    float a = b1 * m1[i] + (1.0f - b1) * gr;
+    float b = b2 * m2[i] + (1.0f - b2) * gr * gr;
+    m1[i] = a;
+    m2[i] = b;
+    float v = p[i] - lr * (a * c1) / (sqrtf(b * c2) + eps);
+    p[i] = v;
+    if (pb) pb[i] = to_bf16(v);
+}
+
+__global__ void to_bf16_kernel(long n, const float* __restrict__ x, __bf16* __restrict__ y) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = to_bf16(x[i]);
+}
+
+// Rows of x [rows][din] (FP64 or FP32) to bf16 [rows][dpad], zero padded.
+template <typename T>
+__global__ void pack_input_kernel(int rows, int din, int dpad, const T* __restrict__ x, const int* __restrict__ idx,
+                                  __bf16* __restrict__ y) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)rows * dpad) return;
+    int r = (int)(i / dpad), c = (int)(i % dpad);
+    int src = idx ? idx[r] : r;
+    y[i] = c < din ? to_bf16((float)x[(long)src * din + c]) : to_bf16(0.0f);
+}
+
+// Top-H logits of each row (one workgroup per row): radix selection of the
+// H-th largest key over the order-preserving unsigned image of the floats,
+// then every index above it and the lowest-indexed ties at it, in
+// increasing index order.
+__device__ inline unsigned int fkey(float f) {
+    unsigned int u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ void __launch_bounds__(256) topk_kernel(int N, int H, const float* __restrict__ Z, long ldz,
+                                                    int* __restrict__ out) {
+    __shared__ unsigned int hist[256];
+    __shared__ unsigned int sel[2];
+    __shared__ int cnt_hi, cnt_eq;
+    const int r = blockIdx.x, tid = threadIdx.x;
+    const float* z = Z + (long)r * ldz;
+    unsigned int prefix = 0, mask = 0;
+    int need = H;  // how many of the remaining top keys are still to place
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hist[tid] = 0;
+        __syncthreads();
+        for (int n = tid; n < N; n += 256) {
+            unsigned int k = fkey(z[n]);
+            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int acc = 0, b = 255;
+            for (; b > 0; b--) {
+                if (acc + (int)hist[b] >= need) break;
+                acc += (int)hist[b];
+            }
+            sel[0] = (unsigned int)b;
+            sel[1] = (unsigned int)(need - acc);
+        }
+        __syncthreads();
+        prefix |= sel[0] << shift;
+        mask |= 255u << shift;
+        need = (int)sel[1];
+        __syncthreads();
+    }
+    // prefix is the H-th largest key; `need` copies of it are taken
+    if (tid == 0) {
+        cnt_hi = 0;
+        cnt_eq = 0;
+    }
+    __syncthreads();
+    // one thread walks the row so the output is in index order
+    if (tid == 0) {
+        int o = 0, eq = 0;
+        int* dst = out + (long)r * H;
+        for (int n = 0; n < N && o < H; n++) {
+            unsigned int k = fkey(z[n]);
+            if (k > prefix || (k == prefix && eq < need)) {
+                if (k == prefix) eq++;
+                dst[o++] = n;
+            }
+        }
+        for (; o < H; o++) dst[o] = -1;
+    }
+}
+
+__global__ void gather_int_kernel(int n, const int* __restrict__ src, const int* __restrict__ idx,
+                                  int* __restrict__ dst) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+// out[slot] = mean of x[0..n) (one workgroup).
+__global__ void __launch_bounds__(256) mean_kernel(int n, const float* __restrict__ x, float* __restrict__ out) {
+    __shared__ float red[4];
+    float s = 0.0f;
+    for (int i = threadIdx.x; i < n; i += 256) s += x[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (red[0] + red[1] + red[2] + red[3]) / (float)n;
+}
+
+}  // namespace
+
+hipError_t gather_int(int n, const int* src, const int* idx, int* dst, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, src, idx, dst);
+    return hipGetLastError();
+}
+
+hipError_t mean(int n, const float* x, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(mean_kernel, dim3(1), dim3(256), 0, s, n, x, out);
+    return hipGetLastError();
+}
+
+hipError_t gemm(const GemmArgs& g, hipStream_t s) {
+    if (g.M <= 0 || g.N <= 0) return hipSuccess;
+    dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM);
+    hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+hipError_t softmax_ce(int rows, int N, const float* Z, long ldz, const int* labels, float scale, __bf16* dZ,
+                      float* loss, float* max_post, hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(softmax_ce_kernel, dim3(rows), dim3(256), 0, s, N, Z, ldz, labels, scale, dZ, loss, max_post);
+    return hipGetLastError();
+}
+
+hipError_t colsum(int M, int N, const __bf16* dY, long ld, float* db, hipStream_t s) {
+    hipLaunchKernelGGL(colsum_kernel, dim3((N + 255) / 256), dim3(256), 0, s, M, N, dY, ld, db);
+    return hipGetLastError();
+}
+
+hipError_t adam(long n, float* p, const float* g, float* m1, float* m2, float lr, float b1, float b2, float eps,
+                float c1, float c2, __bf16* pb, hipStream_t s) {
+    hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, p, g, m1, m2, lr, b1, b2,
+                       eps, c1, c2, pb);
+    return hipGetLastError();
+}
+
+hipError_t to_bf16(long n, const float* x, __bf16* y, hipStream_t s) {
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, x, y);
+    return hipGetLastError();
+}
+
+hipError_t pack_input(int rows, int din, int dpad, const float* x, const int* idx, __bf16* y, hipStream_t s) {
+    long n = (long)rows * dpad;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pack_input_kernel<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rows, din, dpad,
+                       x, idx, y);
+    return hipGetLastError();
+}
+
+hipError_t pack_input(int rows, int din, int dpad, const double* x, const int* idx, __bf16* y, hipStream_t s) {
+    long n = (long)rows * dpad;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pack_input_kernel<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rows, din,
+                       dpad, x, idx, y);
+    return hipGetLastError();
+}
+
+hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(topk_kernel, dim3(rows), dim3(256), 0, s, N, H, Z, ldz, out);
+    return hipGetLastError();
+}
+
+}  // namespace wp
