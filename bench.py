@@ -19,6 +19,12 @@ default); one step is one full classify call from host arrays (kNN search on
 the GPU, vote on the host, HMM rescoring of the 1,000 candidates on the GPU).
 CPU baseline: the reference's HybridClassifier on a bounded sample.
 
+--workload shortlist: the learned shortlist (MLP 36 -> 512 x 3 -> 20,000,
+BF16 on the matrix cores) trained on 10^7 simulated reads of the 20,000
+dye-seqs above, then 10^5 reads classified by FP32 HMM rescoring of its
+top-256 candidates (full-HMM fallback below posterior 0.5), against the plain
+FP64 GPU HMM over all 20,000 dye-seqs (-p 5 both).
+
 --workload fit: Baum-Welch fitting of one dye-seq ("..0..1..2..0..1..2..",
 2 dyes in each of 3 channels, T = 30) to 10^6 simulated reads; one step is one
 EM iteration (GPU E-step over every read + host M-step). CPU baseline: the
@@ -74,7 +80,9 @@ def main():
     ap.add_argument("--cpu-sample-reads", type=int, default=3000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
-    ap.add_argument("--workload", choices=["classify", "fit", "hybrid"], default="classify")
+    ap.add_argument("--workload", choices=["classify", "fit", "hybrid", "shortlist"], default="classify")
+    ap.add_argument("--train-reads", type=int, default=10000000)
+    ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--tracks-per-dye-seq", type=int, default=100)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (counter calibration)")
@@ -88,6 +96,8 @@ def main():
         return bench_fit(args)
     if args.workload == "hybrid":
         return bench_hybrid(args)
+    if args.workload == "shortlist":
+        return bench_shortlist(args)
 
     import numpy as np
     import torch
@@ -296,6 +306,102 @@ def bench_hybrid(args):
                                           w.WP_PRECISION_FP64)
                 ids64, _ = hc64.classify_arrays(rads[:n])
                 result["cpu_agreement_ids_fp64"] = float((rid == ids64).mean())
+    print(json.dumps(result), flush=True)
+
+
+def bench_shortlist(args):
+    import numpy as np
+    import torch
+
+    import whatprot_amd as w
+
+    T, C, S, N, H, cutoff = 12, 3, 20000, args.reads, 256, 5.0
+    model = w.SequencingModel.uniform(C)
+    strings = w.generate_dye_seqs(C, S, 6, 30, 0, 5, 1002)
+    entries = [w.DyeSeqEntry(s, 1, i) for i, s in enumerate(strings)]
+    log(f"simulating {args.train_reads} training reads and {N} query reads")
+    t0 = time.perf_counter()
+    tx, ty = w.simulate_radiometries(model, T, strings, args.train_reads, 4003)
+    tx = tx.reshape(len(tx), -1).astype(np.float32)
+    t_sim = time.perf_counter() - t0
+    qx, qy = w.simulate_radiometries(model, T, strings, N, 3002)
+    net = w.ShortlistMLP(T * C, 512, 3, S, seed=1)
+    log(f"training {args.epochs} epochs ({t_sim:.0f} s simulating)")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = net.train(tx, ty, batch=4096, epochs=args.epochs, lr=1e-3, seed=9)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del tx, ty
+    flops_step = 2 * 4096 * (64 * 512 + 2 * 512 * 512 + 512 * S) * 3
+    train_tflops = flops_step * (args.train_reads // 4096) * args.epochs / t_train / 1e12
+    clf64 = w.HMMClassifier(T, C, model, w.SequencingSettings(cutoff), entries, w.WP_PRECISION_FP64)
+    clf32 = w.HMMClassifier(T, C, model, w.SequencingSettings(cutoff), entries, w.WP_PRECISION_FP32)
+    log("plain FP64 GPU HMM over every dye-seq")
+    clf64.classify_arrays(qx[:1000])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ids64 = clf64.classify_arrays(qx)[0]
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    log("shortlist")
+    idx, post = net.shortlist(qx, H)
+    recall = float(np.mean([ids64[i] in idx[i] for i in range(N)]))
+    net.classify(clf32, qx[:1000], H, 0.5)
+    times = {}
+    out = {}
+    for key, (thr, crit) in {"max": (0.5, "max"), "mass": (0.5, "mass"), "none": (0.0, "max")}.items():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ids, adj, nf = net.classify(clf32, qx, H, thr, crit)
+        torch.cuda.synchronize()
+        times[key] = (time.perf_counter() - t0) / args.steps
+        out[key] = (ids, nf)
+    ids, nf = out["max"]
+    agree = float(np.mean(ids == ids64))
+    # where the picks differ, compare them in FP64 ln P (1e-4 relative = FP32 bound)
+    diff = np.nonzero(ids != ids64)[0][:2000]
+    ties = 0
+    if diff.size:
+        lp = clf64.pair_log_probability(qx[diff])
+        a = lp[np.arange(diff.size), ids[diff]]
+        b = lp[np.arange(diff.size), ids64[diff]]
+        ties = int(np.sum(a >= b - 1e-4 * np.maximum(1.0, np.abs(b))))
+    n_diff = int(np.sum(ids != ids64))
+    ms = times["max"] * 1e3
+    result = {
+        "metric": "learned-shortlist classify reads/s on 1 MI355X (MLP top-256 + FP32 HMM rescoring, -p 5)",
+        "value": N / times["max"],
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 (MLP) + fp32 (HMM)",
+        "data": "synthetic (seeded simulator, seqparams_atto647n_x3 priors with mu = 1)",
+        "config": {"workload": f"shortlist: MLP 36->512x3->{S} trained on {args.train_reads} simulated reads "
+                               f"({args.epochs} epochs), {N} query reads, H={H}, fallback below posterior 0.5",
+                   "reads": N, "dye_seqs": S, "train_reads": args.train_reads, "epochs": args.epochs},
+        "train_s": t_train,
+        "train_tflops_bf16": train_tflops,
+        "epoch_loss": [float(x) for x in losses],
+        "recall_fp64_top1_at_256": recall,
+        "fallback_fraction_at_0.5": nf / N,
+        "top1_vs_fp64": agree,
+        "top1_vs_fp64_tie_aware": float((N - n_diff + ties * n_diff / max(diff.size, 1)) / N),
+        "plain_fp64_hmm_s": t64,
+        "speedup_vs_plain_fp64": t64 / times["max"],
+        "mass_criterion": {"reads_per_s": N / times["mass"], "speedup_vs_plain_fp64": t64 / times["mass"],
+                           "fallback_fraction": out["mass"][1] / N,
+                           "top1_vs_fp64": float(np.mean(out["mass"][0] == ids64))},
+        "no_fallback": {"reads_per_s": N / times["none"], "speedup_vs_plain_fp64": t64 / times["none"],
+                        "top1_vs_fp64": float(np.mean(out["none"][0] == ids64))},
+        "top1_vs_truth": float(np.mean(ids == qy)),
+    }
     print(json.dumps(result), flush=True)
 
 

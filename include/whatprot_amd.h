@@ -168,6 +168,20 @@ int wp_hmm_classify_device(wp_hmm_classifier* clf,
                            double* d_out_log_total,
                            void* stream);
 
+/* Device-resident form of wp_hmm_classify_candidates: cand_offsets (N + 1)
+ * and cand_indices in device memory; an index outside [0, num_dye_seqs)
+ * scores as an empty pair (P = 0, id -1 if it is the pick). */
+int wp_hmm_classify_candidates_device(wp_hmm_classifier* clf,
+                                      int num_reads,
+                                      const double* d_radiometries,
+                                      const int* d_cand_offsets,
+                                      const int* d_cand_indices,
+                                      int* d_out_id,
+                                      double* d_out_adjusted,
+                                      double* d_out_log_score,
+                                      double* d_out_log_total,
+                                      void* stream);
+
 /* ln PeptideHMM::probability() for every (read, dye-seq) pair, row-major
  * [read][dye_seq] (peptide-hmm.cc:94-100); -inf where P = 0. Validation and
  * accuracy reporting. */
@@ -294,17 +308,23 @@ int wp_mlp_gradients(wp_mlp* mlp, int B, const float* x, const int* labels, doub
 int wp_mlp_train(wp_mlp* mlp, int N, const float* x, const int* labels, int batch, int epochs, float lr,
                  uint64_t seed, float* out_epoch_loss);
 /* The H highest-scoring outputs per read, in increasing index order
- * (out_index[r * H + j], -1 past the end), and the largest softmax
- * probability per read (may be NULL). */
+ * (out_index[r * H + j], -1 past the end), the largest softmax probability
+ * per read and the softmax mass of the H (either may be NULL). */
 int wp_mlp_shortlist(wp_mlp* mlp, int num_reads, const double* radiometries, int H, int* out_index,
-                     float* out_max_posterior);
-/* Classification with the learned shortlist: reads whose largest MLP
- * posterior is >= min_posterior are scored by the HMM against their H
+                     float* out_max_posterior, float* out_mass);
+
+/* Confidence a shortlist must reach to be trusted: the largest MLP
+ * posterior, or the posterior mass of the H shortlisted dye-seqs. */
+enum { WP_SHORTLIST_MAX_POSTERIOR = 0, WP_SHORTLIST_MASS = 1 };
+
+/* Classification with the learned shortlist: reads whose confidence
+ * (criterion) is >= min_posterior are scored by the HMM against their H
  * shortlisted dye-seqs (output index i = dye-seq i of clf), the others
  * against every dye-seq, so every output is an HMM score (outputs as
  * wp_hmm_classify's out_id / out_adjusted). */
 int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* mlp, int num_reads, const double* radiometries, int H,
-                          double min_posterior, int* out_id, double* out_adjusted, int* out_num_fallback);
+                          double min_posterior, int criterion, int* out_id, double* out_adjusted,
+                          int* out_num_fallback);
 
 /* ---------------------------------------------------------------------------
  * Baum-Welch fit. Replaces HMMFitter (fitters/hmm-fitter.h:30-151) as run by

@@ -794,21 +794,26 @@ class ShortlistMLP:
                                  float(lr), int(seed), losses.ctypes.data))
         return losses[:epochs]
 
-    def shortlist(self, radiometries, H):
-        """(indices [N, H] in increasing order, max posterior [N])."""
+    def shortlist(self, radiometries, H, with_mass=False):
+        """(indices [N, H] in increasing order, max posterior [N][, shortlist mass [N]])."""
         r = np.ascontiguousarray(radiometries, dtype=np.float64).reshape(-1, self.in_dim)
         idx = np.zeros((r.shape[0], H), dtype=np.int32)
         post = np.zeros(r.shape[0], dtype=np.float32)
-        check(lib().wp_mlp_shortlist(self._h, r.shape[0], r.ctypes.data, int(H), idx.ctypes.data, post.ctypes.data))
-        return idx, post
+        mass = np.zeros(r.shape[0], dtype=np.float32)
+        check(lib().wp_mlp_shortlist(self._h, r.shape[0], r.ctypes.data, int(H), idx.ctypes.data, post.ctypes.data,
+                                     mass.ctypes.data))
+        return (idx, post, mass) if with_mass else (idx, post)
 
-    def classify(self, classifier: "HMMClassifier", radiometries, H, min_posterior=0.5):
-        """wp_shortlist_classify: (ids, adjusted, number of full-HMM fallbacks)."""
+    def classify(self, classifier: "HMMClassifier", radiometries, H, min_posterior=0.5, criterion="max"):
+        """wp_shortlist_classify: (ids, adjusted, number of full-HMM fallbacks).
+        criterion "max" trusts a shortlist when the largest posterior reaches
+        min_posterior, "mass" when the shortlist's posterior mass does."""
         r = np.ascontiguousarray(radiometries, dtype=np.float64)
         N = r.shape[0]
         ids = np.zeros(N, dtype=np.int32)
         adj = np.zeros(N)
         nf = ctypes.c_int()
+        crit = {"max": 0, "mass": 1}[criterion]
         check(lib().wp_shortlist_classify(classifier._h, self._h, N, r.ctypes.data, int(H), float(min_posterior),
-                                          ids.ctypes.data, adj.ctypes.data, ctypes.byref(nf)))
+                                          crit, ids.ctypes.data, adj.ctypes.data, ctypes.byref(nf)))
         return ids, adj, nf.value

@@ -173,10 +173,10 @@ def lib():
     L.wp_mlp_train.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, ctypes.c_float, ctypes.c_uint64,
                                c_void_p]
     L.wp_mlp_shortlist.restype = c_int
-    L.wp_mlp_shortlist.argtypes = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]
+    L.wp_mlp_shortlist.argtypes = [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]
     L.wp_shortlist_classify.restype = c_int
-    L.wp_shortlist_classify.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_int, c_double, c_void_p, c_void_p,
-                                        P(c_int)]
+    L.wp_shortlist_classify.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_int, c_double, c_int, c_void_p,
+                                        c_void_p, P(c_int)]
     L.wp_fit_offset_exponential.restype = c_int
     L.wp_fit_offset_exponential.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_void_p]
     _lib = L

@@ -371,6 +371,16 @@ int wp_hmm_classifier_max_num_dyes(const wp_hmm_classifier* c) { return c ? c->M
 
 int wp_hmm_classifier_num_dye_seqs(const wp_hmm_classifier* c) { return c ? c->S : -1; }
 
+int wp_hmm_classify_candidates_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_cand_off,
+                                      const int* d_cand_idx, int* d_id, double* d_adj, double* d_ls, double* d_lt,
+                                      void* stream) {
+    if (!c || (N > 0 && (!d_rads || !d_id || !d_cand_off || !d_cand_idx))) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    return run_device(c, N, d_rads, d_cand_off, d_cand_idx, d_id, d_adj, d_ls, d_lt, nullptr, (hipStream_t)stream);
+}
+
 int wp_hmm_classify_device(wp_hmm_classifier* c, int N, const double* d_rads, int* d_id, double* d_adj,
                            double* d_ls, double* d_lt, void* stream) {
     if (!c || (N > 0 && (!d_rads || !d_id))) {

@@ -100,7 +100,7 @@ __device__ int compact_chunk(const ClassifyArgs& a, int r, int ncand, const int*
         int kk = kk0 + q;
         if (kk >= ncand) break;
         int i = cidx ? cidx[kk] : kk;
-        bool ok = !read_empty;
+        bool ok = !read_empty && i >= 0 && i < a.S;  // an index outside the table scores as empty
         if (ok) {
 #pragma unroll
             for (int c = 0; c < C; c++) ok = ok && (int)a.dsD[(size_t)i * C + c] >= sL[c];
@@ -190,7 +190,10 @@ __device__ void merge_and_store(const ClassifyArgs& a, int r, int ncand, const i
         log_total = 0.0;
     }
     int id = -1;
-    if (ncand > 0) id = a.dsId[cidx ? cidx[bk] : bk];
+    if (ncand > 0) {
+        int i = cidx ? cidx[bk] : bk;
+        id = (i >= 0 && i < a.S) ? a.dsId[i] : -1;
+    }
     a.out_id[r] = id;
     if (a.out_adjusted) a.out_adjusted[r] = adjusted;
     if (a.out_log_score) a.out_log_score[r] = log_score;

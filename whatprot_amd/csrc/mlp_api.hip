@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "mlp_kernels.h"
@@ -21,7 +22,7 @@ struct wp_mlp {
     int in_dim = 0, in_pad = 0, hidden = 0, nh = 0, out_dim = 0, L = 0;
     std::vector<int> din, dout;  // per weight layer (din[0] = in_pad)
     std::vector<float*> W, b, gW, gb, mW, vW, mb, vb;
-    std::vector<__bf16*> Wb;
+    std::vector<__bf16*> Wb, Wt;  // bf16 copies [rows][cols] and transposed [cols][rows]
     long step = 0;
     // batch work buffers
     int cap = 0;
@@ -32,6 +33,8 @@ struct wp_mlp {
     int* y = nullptr;
     float* loss = nullptr;
     float* post = nullptr;
+    float* lse = nullptr;
+    float* mass = nullptr;
 };
 
 namespace {
@@ -40,6 +43,7 @@ void free_mlp(wp_mlp* m) {
     for (auto* v : {&m->W, &m->b, &m->gW, &m->gb, &m->mW, &m->vW, &m->mb, &m->vb})
         for (float* p : *v) hipFree(p);
     for (__bf16* p : m->Wb) hipFree(p);
+    for (__bf16* p : m->Wt) hipFree(p);
     for (__bf16* p : m->H) hipFree(p);
     for (__bf16* p : m->dH) hipFree(p);
     hipFree(m->X);
@@ -48,6 +52,8 @@ void free_mlp(wp_mlp* m) {
     hipFree(m->y);
     hipFree(m->loss);
     hipFree(m->post);
+    hipFree(m->lse);
+    hipFree(m->mass);
     delete m;
 }
 
@@ -68,6 +74,8 @@ int ensure_batch(wp_mlp* m, int B) {
     hipFree(m->y);
     hipFree(m->loss);
     hipFree(m->post);
+    hipFree(m->lse);
+    hipFree(m->mass);
     m->H.assign(m->nh, nullptr);
     m->dH.assign(m->nh, nullptr);
     m->cap = 0;
@@ -79,6 +87,8 @@ int ensure_batch(wp_mlp* m, int B) {
     if (!rc) rc = dalloc(&m->y, (size_t)B);
     if (!rc) rc = dalloc(&m->loss, (size_t)B);
     if (!rc) rc = dalloc(&m->post, (size_t)B);
+    if (!rc) rc = dalloc(&m->lse, (size_t)B);
+    if (!rc) rc = dalloc(&m->mass, (size_t)B);
     if (!rc) m->cap = B;
     return rc;
 }
@@ -95,9 +105,9 @@ int forward(wp_mlp* m, int B) {
         g.A = in;
         g.sam = m->din[l];
         g.sak = 1;
-        g.B = m->Wb[l];
-        g.sbk = m->dout[l];
-        g.sbn = 1;
+        g.B = m->Wt[l];  // B(k, n) = Wt[n][k]: k-contiguous
+        g.sbk = 1;
+        g.sbn = m->din[l];
         g.bias = m->b[l];
         g.ldc = m->dout[l];
         if (l < m->nh) {
@@ -115,7 +125,8 @@ int forward(wp_mlp* m, int B) {
 // Gradients of the mean cross-entropy over the first B rows (labels in
 // m->y) into gW, gb; per-row losses in m->loss.
 int backward(wp_mlp* m, int B) {
-    WP_HIP(wp::softmax_ce(B, m->out_dim, m->Z, m->out_dim, m->y, 1.0f / (float)B, m->dZ, m->loss, m->post, nullptr));
+    WP_HIP(wp::softmax_ce(B, m->out_dim, m->Z, m->out_dim, m->y, 1.0f / (float)B, m->dZ, m->loss, m->post, nullptr,
+                          nullptr));
     const __bf16* dY = m->dZ;
     for (int l = m->L - 1; l >= 0; l--) {
         const __bf16* Ain = l == 0 ? m->X : m->H[l - 1];
@@ -165,6 +176,7 @@ int adam_step(wp_mlp* m, float lr) {
     for (int l = 0; l < m->L; l++) {
         long nw = (long)m->din[l] * m->dout[l];
         WP_HIP(wp::adam(nw, m->W[l], m->gW[l], m->mW[l], m->vW[l], lr, b1, b2, eps, c1, c2, m->Wb[l], nullptr));
+        WP_HIP(wp::transpose_bf16(m->din[l], m->dout[l], m->W[l], m->Wt[l], nullptr));
         WP_HIP(wp::adam(m->dout[l], m->b[l], m->gb[l], m->mb[l], m->vb[l], lr, b1, b2, eps, c1, c2, nullptr,
                         nullptr));
     }
@@ -225,7 +237,7 @@ int wp_mlp_create(int in_dim, int hidden, int num_hidden, int out_dim, uint64_t 
             for (int o = 0; o < dout; o++) w[(size_t)i * dout + o] = (float)((2.0 * unit(&st) - 1.0) * lim);
         float *W = nullptr, *bb = nullptr, *gW = nullptr, *gb = nullptr, *mW = nullptr, *vW = nullptr, *mb = nullptr,
               *vb = nullptr;
-        __bf16* Wb = nullptr;
+        __bf16 *Wb = nullptr, *Wt = nullptr;
         rc = dalloc(&W, w.size());
         if (!rc) rc = dalloc(&bb, dout);
         if (!rc) rc = dalloc(&gW, w.size());
@@ -235,6 +247,7 @@ int wp_mlp_create(int in_dim, int hidden, int num_hidden, int out_dim, uint64_t 
         if (!rc) rc = dalloc(&mb, dout);
         if (!rc) rc = dalloc(&vb, dout);
         if (!rc) rc = dalloc(&Wb, w.size());
+        if (!rc) rc = dalloc(&Wt, w.size());
         m->W.push_back(W);
         m->b.push_back(bb);
         m->gW.push_back(gW);
@@ -244,11 +257,14 @@ int wp_mlp_create(int in_dim, int hidden, int num_hidden, int out_dim, uint64_t 
         m->mb.push_back(mb);
         m->vb.push_back(vb);
         m->Wb.push_back(Wb);
+        m->Wt.push_back(Wt);
         if (!rc && hipMemcpy(W, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice) != hipSuccess) {
             wp::set_error("cannot upload the MLP weights");
             rc = WP_ERR_HIP;
         }
-        if (!rc && wp::to_bf16((long)w.size(), W, Wb, nullptr) != hipSuccess) rc = WP_ERR_HIP;
+        if (!rc && (wp::to_bf16((long)w.size(), W, Wb, nullptr) != hipSuccess ||
+                    wp::transpose_bf16(di, dout, W, Wt, nullptr) != hipSuccess))
+            rc = WP_ERR_HIP;
     }
     if (rc) {
         free_mlp(m);
@@ -292,6 +308,7 @@ int wp_mlp_set_layer(wp_mlp* m, int layer, const float* W, const float* b) {
     WP_HIP(hipMemcpy(m->W[layer], W, sizeof(float) * (size_t)rows * m->dout[layer], hipMemcpyHostToDevice));
     WP_HIP(hipMemcpy(m->b[layer], b, sizeof(float) * m->dout[layer], hipMemcpyHostToDevice));
     WP_HIP(wp::to_bf16((long)m->din[layer] * m->dout[layer], m->W[layer], m->Wb[layer], nullptr));
+    WP_HIP(wp::transpose_bf16(m->din[layer], m->dout[layer], m->W[layer], m->Wt[layer], nullptr));
     return WP_OK;
 }
 
@@ -400,7 +417,8 @@ int wp_mlp_train(wp_mlp* m, int N, const float* x, const int* labels, int batch,
     return rc;
 }
 
-int wp_mlp_shortlist(wp_mlp* m, int N, const double* radiometries, int H, int* out_index, float* out_max_posterior) {
+int wp_mlp_shortlist(wp_mlp* m, int N, const double* radiometries, int H, int* out_index, float* out_max_posterior,
+                     float* out_mass) {
     if (!m || N < 0 || H < 1 || H > m->out_dim || (N > 0 && (!radiometries || !out_index))) {
         wp::set_error("bad shortlist arguments");
         return WP_ERR_ARG;
@@ -423,13 +441,14 @@ int wp_mlp_shortlist(wp_mlp* m, int N, const double* radiometries, int H, int* o
         }
         rc = forward(m, B);
         if (rc) break;
-        if (wp::softmax_ce(B, m->out_dim, m->Z, m->out_dim, nullptr, 1.0f, nullptr, nullptr, m->post, nullptr) !=
-                hipSuccess ||
-            wp::topk(B, m->out_dim, H, m->Z, m->out_dim, didx, nullptr) != hipSuccess ||
+        if (wp::softmax_ce(B, m->out_dim, m->Z, m->out_dim, nullptr, 1.0f, nullptr, nullptr, m->post, m->lse,
+                           nullptr) != hipSuccess ||
+            wp::topk(B, m->out_dim, H, m->Z, m->out_dim, didx, m->lse, m->mass, nullptr) != hipSuccess ||
             hipMemcpy(out_index + (size_t)r0 * H, didx, sizeof(int) * (size_t)B * H, hipMemcpyDeviceToHost) !=
                 hipSuccess ||
             (out_max_posterior &&
-             hipMemcpy(out_max_posterior + r0, m->post, sizeof(float) * B, hipMemcpyDeviceToHost) != hipSuccess)) {
+             hipMemcpy(out_max_posterior + r0, m->post, sizeof(float) * B, hipMemcpyDeviceToHost) != hipSuccess) ||
+            (out_mass && hipMemcpy(out_mass + r0, m->mass, sizeof(float) * B, hipMemcpyDeviceToHost) != hipSuccess)) {
             wp::set_error("shortlist kernels failed");
             rc = WP_ERR_HIP;
         }
@@ -440,52 +459,104 @@ int wp_mlp_shortlist(wp_mlp* m, int N, const double* radiometries, int H, int* o
 }
 
 int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, const double* radiometries, int H,
-                          double min_posterior, int* out_id, double* out_adjusted, int* out_num_fallback) {
-    if (!clf || !m || num_reads < 0 || (num_reads > 0 && (!radiometries || !out_id || !out_adjusted))) {
+                          double min_posterior, int criterion, int* out_id, double* out_adjusted,
+                          int* out_num_fallback) {
+    if (!clf || !m || num_reads < 0 || H < 1 || (num_reads > 0 && (!radiometries || !out_id || !out_adjusted))) {
         wp::set_error("null argument");
         return WP_ERR_ARG;
     }
-    const int N = num_reads;
-    if (m->out_dim != wp_hmm_classifier_num_dye_seqs(clf)) {
+    if (criterion != WP_SHORTLIST_MAX_POSTERIOR && criterion != WP_SHORTLIST_MASS) {
+        wp::set_error("unknown shortlist criterion");
+        return WP_ERR_ARG;
+    }
+    const int N = num_reads, S = wp_hmm_classifier_num_dye_seqs(clf), d = m->in_dim;
+    if (m->out_dim != S) {
         wp::set_error("the MLP's outputs must be the classifier's dye-seqs");
         return WP_ERR_ARG;
     }
-    std::vector<int> idx((size_t)N * H);
-    std::vector<float> post(N);
-    int rc = wp_mlp_shortlist(m, N, radiometries, H, idx.data(), post.data());
+    if (out_num_fallback) *out_num_fallback = 0;
+    if (N == 0) return WP_OK;
+    H = std::min(H, S);
+    const int chunk = std::max(1, std::min(N, (int)std::max<long>(1, (512l << 20) / (4l * m->out_dim))));
+    int rc = ensure_batch(m, chunk);
     if (rc) return rc;
-    const int d = m->in_dim;
-    std::vector<int> sure, unsure;
-    for (int r = 0; r < N; r++) (post[r] >= min_posterior ? sure : unsure).push_back(r);
-    auto gather = [&](const std::vector<int>& rows, std::vector<double>* x) {
-        x->resize(std::max<size_t>(rows.size(), 1) * d);
-        for (size_t i = 0; i < rows.size(); i++)
-            std::memcpy(&(*x)[i * d], radiometries + (size_t)rows[i] * d, sizeof(double) * d);
-    };
-    std::vector<double> x;
-    if (!sure.empty()) {  // HMM rescoring of the shortlist
-        gather(sure, &x);
-        std::vector<int> off(sure.size() + 1, 0), list;
-        for (size_t i = 0; i < sure.size(); i++) {
-            for (int j = 0; j < H; j++) {
-                int c = idx[(size_t)sure[i] * H + j];
-                if (c >= 0) list.push_back(c);
-            }
-            off[i + 1] = (int)list.size();
+    // everything stays on the device: radiometries, shortlists, confidences
+    double* d_rads = nullptr;
+    int *d_cand = nullptr, *d_off = nullptr, *d_id = nullptr, *d_sel = nullptr, *d_cand2 = nullptr;
+    float* d_conf = nullptr;
+    double *d_adj = nullptr, *d_rads2 = nullptr;
+    std::vector<int> off(N + 1);
+    for (int r = 0; r <= N; r++) off[r] = r * H;
+    rc = dalloc(&d_rads, (size_t)N * d);
+    if (!rc) rc = dalloc(&d_cand, (size_t)N * H);
+    if (!rc) rc = dalloc(&d_off, (size_t)N + 1);
+    if (!rc) rc = dalloc(&d_conf, (size_t)N);
+    if (!rc) rc = dalloc(&d_id, (size_t)N);
+    if (!rc) rc = dalloc(&d_adj, (size_t)N);
+    auto hip_ok = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && !rc) {
+            wp::set_error(std::string(what) + ": " + hipGetErrorString(e));
+            rc = WP_ERR_HIP;
         }
-        if (list.empty()) list.push_back(0);
-        std::vector<int> id(sure.size());
-        std::vector<double> adj(sure.size());
-        rc = wp_hmm_classify_candidates(clf, (int)sure.size(), x.data(), off.data(), list.data(), id.data(),
-                                        adj.data(), nullptr, nullptr);
-        if (rc) return rc;
-        for (size_t i = 0; i < sure.size(); i++) {
+        return rc == WP_OK;
+    };
+    if (!rc) hip_ok(hipMemcpy(d_rads, radiometries, sizeof(double) * (size_t)N * d, hipMemcpyHostToDevice), "upload");
+    if (!rc) hip_ok(hipMemcpy(d_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice), "upload");
+    for (int r0 = 0; r0 < N && !rc; r0 += chunk) {
+        const int B = std::min(chunk, N - r0);
+        if (!hip_ok(wp::pack_input(B, d, m->in_pad, d_rads + (size_t)r0 * d, nullptr, m->X, nullptr), "pack"))
+            break;
+        rc = forward(m, B);
+        if (rc) break;
+        hip_ok(wp::softmax_ce(B, m->out_dim, m->Z, m->out_dim, nullptr, 1.0f, nullptr, nullptr, m->post, m->lse,
+                              nullptr), "softmax");
+        hip_ok(wp::topk(B, m->out_dim, H, m->Z, m->out_dim, d_cand + (size_t)r0 * H, m->lse, m->mass, nullptr),
+               "top-k");
+        hip_ok(hipMemcpy(d_conf + r0, criterion == WP_SHORTLIST_MASS ? m->mass : m->post, sizeof(float) * B,
+                         hipMemcpyDeviceToDevice), "copy");
+    }
+    std::vector<float> conf(N);
+    if (!rc) hip_ok(hipMemcpy(conf.data(), d_conf, sizeof(float) * N, hipMemcpyDeviceToHost), "download");
+    std::vector<int> sure, unsure;
+    for (int r = 0; r < N && !rc; r++) (conf[r] >= min_posterior ? sure : unsure).push_back(r);
+    if (!rc && !sure.empty()) {  // HMM rescoring of the shortlists, on the device
+        const int ns = (int)sure.size();
+        const double* rs = d_rads;
+        const int* cs = d_cand;
+        if (ns < N) {
+            rc = dalloc(&d_sel, (size_t)ns);
+            if (!rc) rc = dalloc(&d_rads2, (size_t)ns * d);
+            if (!rc) rc = dalloc(&d_cand2, (size_t)ns * H);
+            if (!rc) hip_ok(hipMemcpy(d_sel, sure.data(), sizeof(int) * ns, hipMemcpyHostToDevice), "upload");
+            if (!rc) hip_ok(wp::gather_rows(ns, d, d_rads, d_sel, d_rads2, nullptr), "gather");
+            if (!rc) hip_ok(wp::gather_rows(ns, H, d_cand, d_sel, d_cand2, nullptr), "gather");
+            rs = d_rads2;
+            cs = d_cand2;
+        }
+        if (!rc) rc = wp_hmm_classify_candidates_device(clf, ns, rs, d_off, cs, d_id, d_adj, nullptr, nullptr, nullptr);
+        std::vector<int> id(ns);
+        std::vector<double> adj(ns);
+        if (!rc) hip_ok(hipMemcpy(id.data(), d_id, sizeof(int) * ns, hipMemcpyDeviceToHost), "download");
+        if (!rc) hip_ok(hipMemcpy(adj.data(), d_adj, sizeof(double) * ns, hipMemcpyDeviceToHost), "download");
+        for (int i = 0; i < ns && !rc; i++) {
             out_id[sure[i]] = id[i];
             out_adjusted[sure[i]] = adj[i];
         }
     }
+    hipFree(d_rads);
+    hipFree(d_cand);
+    hipFree(d_off);
+    hipFree(d_conf);
+    hipFree(d_id);
+    hipFree(d_adj);
+    hipFree(d_sel);
+    hipFree(d_rads2);
+    hipFree(d_cand2);
+    if (rc) return rc;
     if (!unsure.empty()) {  // full HMM fallback
-        gather(unsure, &x);
+        std::vector<double> x(unsure.size() * (size_t)d);
+        for (size_t i = 0; i < unsure.size(); i++)
+            std::memcpy(&x[i * d], radiometries + (size_t)unsure[i] * d, sizeof(double) * d);
         std::vector<int> id(unsure.size());
         std::vector<double> adj(unsure.size());
         rc = wp_hmm_classify(clf, (int)unsure.size(), x.data(), id.data(), adj.data(), nullptr, nullptr);

@@ -25,15 +25,19 @@ struct GemmArgs {
 
 hipError_t gemm(const GemmArgs& g, hipStream_t s);
 hipError_t softmax_ce(int rows, int N, const float* Z, long ldz, const int* labels, float scale, __bf16* dZ,
-                      float* loss, float* max_post, hipStream_t s);
+                      float* loss, float* max_post, float* lse, hipStream_t s);
 hipError_t colsum(int M, int N, const __bf16* dY, long ld, float* db, hipStream_t s);
 hipError_t adam(long n, float* p, const float* g, float* m1, float* m2, float lr, float b1, float b2, float eps,
                 float c1, float c2, __bf16* pb, hipStream_t s);
 hipError_t to_bf16(long n, const float* x, __bf16* y, hipStream_t s);
 hipError_t pack_input(int rows, int din, int dpad, const float* x, const int* idx, __bf16* y, hipStream_t s);
 hipError_t pack_input(int rows, int din, int dpad, const double* x, const int* idx, __bf16* y, hipStream_t s);
+hipError_t gather_rows(int rows, int w, const double* src, const int* idx, double* dst, hipStream_t s);
+hipError_t gather_rows(int rows, int w, const int* src, const int* idx, int* dst, hipStream_t s);
 hipError_t gather_int(int n, const int* src, const int* idx, int* dst, hipStream_t s);
 hipError_t mean(int n, const float* x, float* out, hipStream_t s);
-hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, hipStream_t s);
+hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, const float* lse, float* mass,
+                hipStream_t s);
+hipError_t transpose_bf16(int rows, int cols, const float* W, __bf16* Wt, hipStream_t s);
 
 }  // namespace wp

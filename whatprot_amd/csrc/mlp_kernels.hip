@@ -42,42 +42,97 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const bool a_k_contig = g.sak == 1;
-    const bool b_n_contig = g.sbn == 1;
+    // 16-byte loads: along k when k is contiguous (stored as is), else along
+    // the row index (m or n) when that is contiguous (scattered into LDS);
+    // a vector that would cross the matrix edge is loaded element-wise
+    const int a_mode = (g.sak == 1 && g.sam % 8 == 0) ? 1 : (g.sam == 1 && g.sak % 8 == 0) ? 2 : 0;
+    const int b_mode = (g.sbk == 1 && g.sbn % 8 == 0) ? 1 : (g.sbn == 1 && g.sbk % 8 == 0) ? 2 : 0;
     for (int k0 = 0; k0 < g.K; k0 += BK) {
-        // A tile [BM][BK] -> As[m][k]
-#pragma unroll 4
-        for (int it = 0; it < BM * BK / 256; it++) {
-            int e = tid + it * 256;
-            int m, k;
-            if (a_k_contig) {
-                m = e / BK;
-                k = e % BK;
-            } else {
-                m = e % BM;
-                k = e / BM;
+        if (a_mode == 1) {
+#pragma unroll
+            for (int it = 0; it < 2; it++) {
+                int e = tid + it * 256;
+                int m = e >> 2, kv = (e & 3) * 8;
+                int gm = m0 + m, gk = k0 + kv;
+                bf16x8 v;
+                if (gm < g.M && gk + 8 <= g.K) {
+                    v = *(const bf16x8*)&g.A[(long)gm * g.sam + gk];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        v[q] = (gm < g.M && gk + q < g.K) ? g.A[(long)gm * g.sam + gk + q] : (__bf16)0.0f;
+                }
+                *(bf16x8*)&As[m * LDK + kv] = v;
             }
-            int gm = m0 + m, gk = k0 + k;
-            __bf16 v = (__bf16)0.0f;
-            if (gm < g.M && gk < g.K) v = g.A[(long)gm * g.sam + (long)gk * g.sak];
-            As[m * LDK + k] = v;
+        } else if (a_mode == 2) {
+#pragma unroll
+            for (int it = 0; it < 2; it++) {
+                int e = tid + it * 256;
+                int k = e >> 4, mv = (e & 15) * 8;
+                int gm = m0 + mv, gk = k0 + k;
+                bf16x8 v;
+                if (gk < g.K && gm + 8 <= g.M) {
+                    v = *(const bf16x8*)&g.A[(long)gk * g.sak + gm];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        v[q] = (gk < g.K && gm + q < g.M) ? g.A[(long)gk * g.sak + gm + q] : (__bf16)0.0f;
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) As[(mv + q) * LDK + k] = v[q];
+            }
+        } else {
+            for (int it = 0; it < BM * BK / 256; it++) {
+                int e = tid + it * 256;
+                int m = e / BK, k = e % BK;
+                int gm = m0 + m, gk = k0 + k;
+                __bf16 v = (__bf16)0.0f;
+                if (gm < g.M && gk < g.K) v = g.A[(long)gm * g.sam + (long)gk * g.sak];
+                As[m * LDK + k] = v;
+            }
         }
-        // B tile [BK][BN] -> Bs[n][k]
-#pragma unroll 4
-        for (int it = 0; it < BN * BK / 256; it++) {
-            int e = tid + it * 256;
-            int n, k;
-            if (b_n_contig) {
-                n = e % BN;
-                k = e / BN;
-            } else {
-                n = e / BK;
-                k = e % BK;
+        if (b_mode == 1) {
+#pragma unroll
+            for (int it = 0; it < 2; it++) {
+                int e = tid + it * 256;
+                int n = e >> 2, kv = (e & 3) * 8;
+                int gn = n0 + n, gk = k0 + kv;
+                bf16x8 v;
+                if (gn < g.N && gk + 8 <= g.K) {
+                    v = *(const bf16x8*)&g.B[(long)gn * g.sbn + gk];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        v[q] = (gn < g.N && gk + q < g.K) ? g.B[(long)gn * g.sbn + gk + q] : (__bf16)0.0f;
+                }
+                *(bf16x8*)&Bs[n * LDK + kv] = v;
             }
-            int gn = n0 + n, gk = k0 + k;
-            __bf16 v = (__bf16)0.0f;
-            if (gn < g.N && gk < g.K) v = g.B[(long)gk * g.sbk + (long)gn * g.sbn];
-            Bs[n * LDK + k] = v;
+        } else if (b_mode == 2) {
+#pragma unroll
+            for (int it = 0; it < 2; it++) {
+                int e = tid + it * 256;
+                int k = e >> 4, nv = (e & 15) * 8;
+                int gn = n0 + nv, gk = k0 + k;
+                bf16x8 v;
+                if (gk < g.K && gn + 8 <= g.N) {
+                    v = *(const bf16x8*)&g.B[(long)gk * g.sbk + gn];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        v[q] = (gk < g.K && gn + q < g.N) ? g.B[(long)gk * g.sbk + gn + q] : (__bf16)0.0f;
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) Bs[(nv + q) * LDK + k] = v[q];
+            }
+        } else {
+            for (int it = 0; it < BN * BK / 256; it++) {
+                int e = tid + it * 256;
+                int n = e / BK, k = e % BK;
+                int gn = n0 + n, gk = k0 + k;
+                __bf16 v = (__bf16)0.0f;
+                if (gn < g.N && gk < g.K) v = g.B[(long)gk * g.sbk + (long)gn * g.sbn];
+                Bs[n * LDK + k] = v;
+            }
         }
         __syncthreads();
         bf16x8 af[4], bfr[4];
@@ -120,7 +175,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
 __global__ void __launch_bounds__(256) softmax_ce_kernel(int N, const float* __restrict__ Z, long ldz,
                                                           const int* __restrict__ labels, float scale,
                                                           __bf16* __restrict__ dZ, float* __restrict__ loss,
-                                                          float* __restrict__ max_post) {
+                                                          float* __restrict__ max_post, float* __restrict__ lse_out) {
     __shared__ float red[8];
     const int r = blockIdx.x, tid = threadIdx.x;
     const float* z = Z + (long)r * ldz;
@@ -150,6 +205,7 @@ __global__ void __launch_bounds__(256) softmax_ce_kernel(int N, const float* __r
     if (tid == 0) {
         if (loss) loss[r] = (y >= 0 && y < N) ? lse - z[y] : 0.0f;
         if (max_post) max_post[r] = expf(mx - lse);
+        if (lse_out) lse_out[r] = lse;
     }
 }
 
@@ -204,12 +260,17 @@ __device__ inline unsigned int fkey(float f) {
 }
 
 __global__ void __launch_bounds__(256) topk_kernel(int N, int H, const float* __restrict__ Z, long ldz,
-                                                    int* __restrict__ out) {
+                                                    int* __restrict__ out, const float* __restrict__ lse,
+                                                    float* __restrict__ mass) {
     __shared__ unsigned int hist[256];
     __shared__ unsigned int sel[2];
-    __shared__ int cnt_hi, cnt_eq;
     const int r = blockIdx.x, tid = threadIdx.x;
     const float* z = Z + (long)r * ldz;
+    if (N <= H) {  // every index, then padding
+        for (int o = tid; o < H; o += 256) out[(long)r * H + o] = o < N ? o : -1;
+        if (mass && tid == 0) mass[r] = 1.0f;
+        return;
+    }
     unsigned int prefix = 0, mask = 0;
     int need = H;  // how many of the remaining top keys are still to place
     for (int shift = 24; shift >= 0; shift -= 8) {
@@ -235,25 +296,64 @@ __global__ void __launch_bounds__(256) topk_kernel(int N, int H, const float* __
         need = (int)sel[1];
         __syncthreads();
     }
-    // prefix is the H-th largest key; `need` copies of it are taken
+    // prefix is the H-th largest key; the first `need` keys equal to it (in
+    // index order) are taken with every larger key. Each thread owns a
+    // contiguous slice of the row; slice counts are scanned so the output
+    // stays in index order.
+    __shared__ int scan_hi[256], scan_eq[256];
+    const int per = (N + 255) / 256;
+    const int b = tid * per, e = min(N, b + per);
+    int hi = 0, eq = 0;
+    for (int n = b; n < e; n++) {
+        unsigned int k = fkey(z[n]);
+        hi += k > prefix;
+        eq += k == prefix;
+    }
+    scan_hi[tid] = hi;
+    scan_eq[tid] = eq;
+    __syncthreads();
     if (tid == 0) {
-        cnt_hi = 0;
-        cnt_eq = 0;
+        int ah = 0, ae = 0;
+        for (int t = 0; t < 256; t++) {
+            int h = scan_hi[t], q = scan_eq[t];
+            scan_hi[t] = ah;
+            scan_eq[t] = ae;
+            ah += h;
+            ae += q;
+        }
     }
     __syncthreads();
-    // one thread walks the row so the output is in index order
-    if (tid == 0) {
-        int o = 0, eq = 0;
-        int* dst = out + (long)r * H;
-        for (int n = 0; n < N && o < H; n++) {
-            unsigned int k = fkey(z[n]);
-            if (k > prefix || (k == prefix && eq < need)) {
-                if (k == prefix) eq++;
-                dst[o++] = n;
-            }
+    int eq_before = scan_eq[tid];
+    int took_eq_before = min(eq_before, need);
+    int o = scan_hi[tid] + took_eq_before;
+    int* dst = out + (long)r * H;
+    int eq_seen = eq_before;
+    const float l = lse ? lse[r] : 0.0f;
+    float ms = 0.0f;
+    for (int n = b; n < e && o < H; n++) {
+        unsigned int k = fkey(z[n]);
+        if (k > prefix || (k == prefix && eq_seen < need)) {
+            dst[o++] = n;
+            ms += expf(z[n] - l);
         }
-        for (; o < H; o++) dst[o] = -1;
+        if (k == prefix) eq_seen++;
     }
+    if (mass) {  // softmax mass of the shortlist
+        __shared__ float red[4];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) ms += __shfl_xor(ms, off, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = ms;
+        __syncthreads();
+        if (tid == 0) mass[r] = red[0] + red[1] + red[2] + red[3];
+    }
+}
+
+// Wt[n][k] = bf16(W[k][n]) for W [rows][cols] FP32.
+__global__ void transpose_bf16_kernel(int rows, int cols, const float* __restrict__ W, __bf16* __restrict__ Wt) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)rows * cols) return;
+    int n = (int)(i / rows), k = (int)(i % rows);
+    Wt[i] = to_bf16(W[(long)k * cols + n]);
 }
 
 __global__ void gather_int_kernel(int n, const int* __restrict__ src, const int* __restrict__ idx,
@@ -262,7 +362,7 @@ __global__ void gather_int_kernel(int n, const int* __restrict__ src, const int*
     if (i < n) dst[i] = src[idx[i]];
 }
 
-// out[slot] = mean of x[0..n) (one workgroup).
+// *out = mean of x[0..n) (one workgroup).
 __global__ void __launch_bounds__(256) mean_kernel(int n, const float* __restrict__ x, float* __restrict__ out) {
     __shared__ float red[4];
     float s = 0.0f;
@@ -274,7 +374,33 @@ __global__ void __launch_bounds__(256) mean_kernel(int n, const float* __restric
     if (threadIdx.x == 0) *out = (red[0] + red[1] + red[2] + red[3]) / (float)n;
 }
 
+// dst[i][0..w) = src[idx[i]][0..w)
+template <typename T>
+__global__ void gather_rows_kernel(int rows, int w, const T* __restrict__ src, const int* __restrict__ idx,
+                                   T* __restrict__ dst) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)rows * w) return;
+    int r = (int)(i / w), c = (int)(i % w);
+    dst[i] = src[(long)idx[r] * w + c];
+}
+
 }  // namespace
+
+hipError_t gather_rows(int rows, int w, const double* src, const int* idx, double* dst, hipStream_t s) {
+    long n = (long)rows * w;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_rows_kernel<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rows, w, src,
+                       idx, dst);
+    return hipGetLastError();
+}
+
+hipError_t gather_rows(int rows, int w, const int* src, const int* idx, int* dst, hipStream_t s) {
+    long n = (long)rows * w;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_rows_kernel<int>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rows, w, src, idx,
+                       dst);
+    return hipGetLastError();
+}
 
 hipError_t gather_int(int n, const int* src, const int* idx, int* dst, hipStream_t s) {
     if (n <= 0) return hipSuccess;
@@ -295,9 +421,10 @@ hipError_t gemm(const GemmArgs& g, hipStream_t s) {
 }
 
 hipError_t softmax_ce(int rows, int N, const float* Z, long ldz, const int* labels, float scale, __bf16* dZ,
-                      float* loss, float* max_post, hipStream_t s) {
+                      float* loss, float* max_post, float* lse, hipStream_t s) {
     if (rows <= 0) return hipSuccess;
-    hipLaunchKernelGGL(softmax_ce_kernel, dim3(rows), dim3(256), 0, s, N, Z, ldz, labels, scale, dZ, loss, max_post);
+    hipLaunchKernelGGL(softmax_ce_kernel, dim3(rows), dim3(256), 0, s, N, Z, ldz, labels, scale, dZ, loss, max_post,
+                       lse);
     return hipGetLastError();
 }
 
@@ -334,9 +461,17 @@ hipError_t pack_input(int rows, int din, int dpad, const double* x, const int* i
     return hipGetLastError();
 }
 
-hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, hipStream_t s) {
+hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, const float* lse, float* mass,
+                hipStream_t s) {
     if (rows <= 0) return hipSuccess;
-    hipLaunchKernelGGL(topk_kernel, dim3(rows), dim3(256), 0, s, N, H, Z, ldz, out);
+    hipLaunchKernelGGL(topk_kernel, dim3(rows), dim3(256), 0, s, N, H, Z, ldz, out, lse, mass);
+    return hipGetLastError();
+}
+
+hipError_t transpose_bf16(int rows, int cols, const float* W, __bf16* Wt, hipStream_t s) {
+    long n = (long)rows * cols;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(transpose_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rows, cols, W, Wt);
     return hipGetLastError();
 }
 
