@@ -96,3 +96,25 @@ def test_shortlist_classify(gpu, small_task):
     assert nf == 0
     assert all(ids[i] in idx[i] for i in range(len(ids)))
     assert np.mean(ids == full_ids) > 0.95
+
+
+@pytest.mark.parametrize("S", [40, 700, 5000, 20000, 40000])
+def test_shortlist_topk_matches_logits(gpu, S):
+    """The shortlist is the top H of the network's own logits (ties to the
+    lower index, output in increasing index order) and its mass is their
+    softmax mass; covers every register-resident row size and the generic
+    kernel above 32768 outputs."""
+    rng = np.random.default_rng(S)
+    net = w.ShortlistMLP(12, 64, 2, S, seed=5)
+    x = rng.normal(0.0, 3.0, size=(300, 12))
+    z = net.logits(x.astype(np.float32)).astype(np.float64)
+    keys = z.astype(np.float32).view(np.int32).astype(np.int64)
+    keys = np.where(keys < 0, -(keys & 0x7FFFFFFF), keys)  # order-preserving integer image
+    for H in sorted({1, 37, min(256, S - 1)}):
+        idx, post, mass = net.shortlist(x, H, with_mass=True)
+        ref = np.sort(np.argsort(-keys, axis=1, kind="stable")[:, :H], axis=1)
+        np.testing.assert_array_equal(idx, ref)
+        p = np.exp(z - z.max(axis=1, keepdims=True))
+        p /= p.sum(axis=1, keepdims=True)
+        np.testing.assert_allclose(post, p.max(axis=1), rtol=1e-4)
+        np.testing.assert_allclose(mass, np.take_along_axis(p, ref, axis=1).sum(axis=1), rtol=1e-4, atol=1e-6)

@@ -348,6 +348,98 @@ __global__ void __launch_bounds__(256) topk_kernel(int N, int H, const float* __
     }
 }
 
+// Top-H for rows of at most 256 * PER logits, held in registers (element
+// n = tid + 256 j in key[j]): the H-th largest key K is found bit by bit,
+// MSB first, as the largest K with #(key >= K) >= H, each count a sum of
+// wave ballots (no atomics). Output positions in increasing index order come
+// from per-(j, wave) ballot counts of keys > K and == K, scanned in (j, wave)
+// order, which is index order.
+template <int PER>
+__global__ void __launch_bounds__(256) topk_reg_kernel(int N, int H, const float* __restrict__ Z, long ldz,
+                                                        int* __restrict__ out, const float* __restrict__ lse,
+                                                        float* __restrict__ mass) {
+    __shared__ int red[2][4];
+    __shared__ int cnt[2][PER][4];
+    __shared__ int tot[2];
+    __shared__ float fred[4];
+    const int r = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const float* z = Z + (long)r * ldz;
+    unsigned int key[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int n = tid + 256 * j;
+        key[j] = n < N ? fkey(z[n]) : 0u;  // padding: below every finite logit
+    }
+    unsigned int K = 0;
+    for (int bit = 31; bit >= 0; bit--) {
+        const unsigned int c = K | (1u << bit);
+        int cw = 0;
+#pragma unroll
+        for (int j = 0; j < PER; j++) cw += __popcll(__ballot(key[j] >= c));
+        if (lane == 0) red[bit & 1][w] = cw;
+        __syncthreads();  // double-buffered: one barrier per bit
+        const int total = red[bit & 1][0] + red[bit & 1][1] + red[bit & 1][2] + red[bit & 1][3];
+        if (total >= H) K = c;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int n = tid + 256 * j;
+        const unsigned long long bg = __ballot(key[j] > K), be = __ballot(key[j] == K && n < N);
+        if (lane == 0) {
+            cnt[0][j][w] = __popcll(bg);
+            cnt[1][j][w] = __popcll(be);
+        }
+    }
+    __syncthreads();
+    if (w < 2) {  // wave 0 scans the > K counts, wave 1 the == K counts
+        int* a = &cnt[w][0][0];
+        constexpr int M = 4 * PER, PL = (M + 63) / 64;
+        const int b = lane * PL, e = min(M, b + PL);
+        int sum = 0;
+        for (int i = b; i < e; i++) sum += a[i];
+        int incl = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        int run = incl - sum;
+        for (int i = b; i < e; i++) {
+            const int v = a[i];
+            a[i] = run;
+            run += v;
+        }
+        if (lane == 63) tot[w] = incl;
+    }
+    __syncthreads();
+    const int need = H - tot[0];  // ties at K taken, lowest index first
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int* dst = out + (long)r * H;
+    const float l = lse ? lse[r] : 0.0f;
+    float ms = 0.0f;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int n = tid + 256 * j;
+        const bool g = key[j] > K, e = key[j] == K && n < N;
+        const unsigned long long bg = __ballot(g), be = __ballot(e);
+        const int gpos = cnt[0][j][w] + __popcll(bg & below);
+        const int epos = cnt[1][j][w] + __popcll(be & below);
+        const int o = gpos + min(epos, need);
+        if ((g || (e && epos < need)) && o < H) {
+            dst[o] = n;
+            const unsigned int k = key[j];
+            ms += expf(__uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k) - l);
+        }
+    }
+    if (mass) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) ms += __shfl_xor(ms, off, 64);
+        if (lane == 0) fred[w] = ms;
+        __syncthreads();
+        if (tid == 0) mass[r] = fred[0] + fred[1] + fred[2] + fred[3];
+    }
+}
+
 // Wt[n][k] = bf16(W[k][n]) for W [rows][cols] FP32.
 __global__ void transpose_bf16_kernel(int rows, int cols, const float* __restrict__ W, __bf16* __restrict__ Wt) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -464,6 +556,21 @@ hipError_t pack_input(int rows, int din, int dpad, const double* x, const int* i
 hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, const float* lse, float* mass,
                 hipStream_t s) {
     if (rows <= 0) return hipSuccess;
+    if (N > H) {  // register-resident rows up to 32768 logits
+#define WP_TOPK_REG(P)                                                                                          \
+    if (N <= 256 * (P)) {                                                                                       \
+        hipLaunchKernelGGL(topk_reg_kernel<P>, dim3(rows), dim3(256), 0, s, N, H, Z, ldz, out, lse, mass);       \
+        return hipGetLastError();                                                                               \
+    }
+        WP_TOPK_REG(4)
+        WP_TOPK_REG(8)
+        WP_TOPK_REG(16)
+        WP_TOPK_REG(32)
+        WP_TOPK_REG(64)
+        WP_TOPK_REG(80)
+        WP_TOPK_REG(128)
+#undef WP_TOPK_REG
+    }
     hipLaunchKernelGGL(topk_kernel, dim3(rows), dim3(256), 0, s, N, H, Z, ldz, out, lse, mass);
     return hipGetLastError();
 }
