@@ -96,6 +96,7 @@ struct wp_hmm_classifier {
     uint8_t* wL = nullptr;
     uint8_t* wH = nullptr;
     int* wNb = nullptr;
+    uint8_t* wWide = nullptr;  // per read: box densities too wide for the quad-rows kernel
     int wcap = 0;
     int maxbox = 1;  // prod_c (M+1), bound on any read's box
     // optional kernel timing: events around prep and classify launches
@@ -106,10 +107,14 @@ struct wp_hmm_classifier {
 
 namespace {
 
+enum BinKernel { kGather = 0, kRowPerLane = 3, kQuadRows = 5 };
+
 struct Bin {
     int lo, hi, ngroups;
-    bool rows;  // row-per-lane kernel (small boxes) instead of the gather kernel
+    int kernel;  // BinKernel: quad-rows (FP32, boxes <= 16), row-per-lane (FP64, <= 8) or gather
 };
+
+size_t bin_lds_bytes(const wp_hmm_classifier* c, const Bin& b);
 
 constexpr size_t kLdsBudget = 48 * 1024;
 
@@ -140,13 +145,26 @@ std::vector<Bin> make_bins(const wp_hmm_classifier* c) {
         size_t per = wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, hi, 1, c->G) - fixed;
         int ng = (int)((kLdsBudget > fixed ? kLdsBudget - fixed : 0) / (per ? per : 1));
         ng = std::max(1, std::min(ng, wp::kBlockThreads / c->G));
-        bool rows = wp::v3_supports(c->precision, c->T, hi) &&
-                    wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, hi) <= lds_hard_limit();
-        bins.push_back({lo, hi, ng, rows});
+        int kernel = kGather;
+        if (wp::v5_supports(c->precision, c->T, hi) &&
+            wp::v5_lds_bytes(c->T, c->C, c->tri, hi) <= lds_hard_limit())
+            kernel = kQuadRows;
+        else if (wp::v3_supports(c->precision, c->T, hi) &&
+                 wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, hi) <= lds_hard_limit())
+            kernel = kRowPerLane;
+        bins.push_back({lo, hi, ng, kernel});
         lo = hi;
         if (lo >= c->maxbox) break;
     }
     return bins;
+}
+
+size_t bin_lds_bytes(const wp_hmm_classifier* c, const Bin& b) {
+    switch (b.kernel) {
+    case kRowPerLane: return wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi);
+    case kQuadRows: return wp::v5_lds_bytes(c->T, c->C, c->tri, b.hi);
+    default: return wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi, b.ngroups, c->G);
+    }
 }
 
 int ensure_workspace(wp_hmm_classifier* c, int N) {
@@ -154,13 +172,16 @@ int ensure_workspace(wp_hmm_classifier* c, int N) {
     hipFree(c->wL);
     hipFree(c->wH);
     hipFree(c->wNb);
+    hipFree(c->wWide);
     c->wL = c->wH = nullptr;
     c->wNb = nullptr;
+    c->wWide = nullptr;
     c->wcap = 0;
     size_t tc = (size_t)N * c->T * c->C;
     WP_HIP(hipMalloc(&c->wL, tc));
     WP_HIP(hipMalloc(&c->wH, tc));
     WP_HIP(hipMalloc(&c->wNb, sizeof(int) * (size_t)N));
+    WP_HIP(hipMalloc(&c->wWide, (size_t)N));
     c->wcap = N;
     return WP_OK;
 }
@@ -187,6 +208,8 @@ int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_c
     a.Lr = c->wL;
     a.Hr = c->wH;
     a.nbmax = c->wNb;
+    a.wide = c->wWide;
+    a.wide_pass = 0;
     a.dsD = c->dD;
     a.dsSeq = c->dSeq;
     a.dsTrack = c->dTrack;
@@ -206,8 +229,21 @@ int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_c
         a.nb_hi = b.hi;
         a.NB = b.hi;
         a.ngroups = b.ngroups;
-        if (b.rows) WP_HIP(wp::launch_classify_v3(a, c->precision, stream));
-        else WP_HIP(wp::launch_classify(a, c->precision, c->G, stream));
+        if (b.kernel == kRowPerLane) WP_HIP(wp::launch_classify_v3(a, c->precision, stream));
+        else if (b.kernel == kQuadRows) {
+            // reads whose densities span too wide a range are flagged by the
+            // quad-rows kernel and scored by the per-entry exponent kernels
+            WP_HIP(wp::launch_classify_v5(a, stream));
+            a.wide_pass = 1;
+            if (wp::v3_supports(c->precision, c->T, b.hi) &&
+                wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi) <= lds_hard_limit())
+                WP_HIP(wp::launch_classify_v3(a, c->precision, stream));
+            else
+                WP_HIP(wp::launch_classify(a, c->precision, c->G, stream));
+            a.wide_pass = 0;
+        } else {
+            WP_HIP(wp::launch_classify(a, c->precision, c->G, stream));
+        }
     }
     if (c->timing) {
         WP_HIP(hipEventRecord(c->ev[2], stream));
@@ -328,9 +364,7 @@ int wp_hmm_classifier_create(unsigned int num_timesteps, unsigned int num_channe
     wp::binomial_tables(*seq_model, C, M, binom.data());
     // every bin must fit at least one pair group
     for (const Bin& b : make_bins(c)) {
-        size_t need = b.rows ? wp::v3_lds_bytes(precision, T, C, c->tri, b.hi)
-                             : wp::classify_lds_bytes(precision, T, C, c->tri, b.hi, b.ngroups, c->G);
-        if (need > lds_hard_limit()) {
+        if (bin_lds_bytes(c, b) > lds_hard_limit()) {
             wp::set_error("state box too large for LDS at this precision (max_num_dyes/channels)");
             delete c;
             return WP_ERR_UNSUPPORTED;
@@ -364,6 +398,7 @@ void wp_hmm_classifier_destroy(wp_hmm_classifier* c) {
     hipFree(c->wL);
     hipFree(c->wH);
     hipFree(c->wNb);
+    hipFree(c->wWide);
     delete c;
 }
 

@@ -18,6 +18,8 @@ struct ClassifyArgs {
     const uint8_t* Hr;        // [N][T][C] upper box bound (prefix min of range highs)
     const int* nbmax;         // [N] max_t prod_c (Hr - Lr); 0 = every pair empty
     int nb_lo, nb_hi;         // this launch takes reads with nb_lo < nbmax <= nb_hi
+    uint8_t* wide;            // [N] set by the quad-rows kernel: densities too wide for it
+    int wide_pass;            // 1: score only the reads flagged in wide[]
     int NB;                   // per-group state capacity in count-box entries
     int ngroups;              // pair groups per workgroup
     const uint8_t* dsD;       // [S][C] dyes per channel
@@ -49,6 +51,12 @@ hipError_t launch_classify(const ClassifyArgs& a, int precision, int group_lanes
 bool v3_supports(int precision, int T, int NB);
 size_t v3_lds_bytes(int precision, int T, int C, int tri, int NB);
 hipError_t launch_classify_v3(const ClassifyArgs& a, int precision, hipStream_t stream);
+
+// Quad-rows classify (FP32) for reads whose count box is NB in {4, 8, 16}
+// states; T <= 32 (NB <= 8) or T <= 16 (NB = 16).
+bool v5_supports(int precision, int T, int NB);
+size_t v5_lds_bytes(int T, int C, int tri, int NB);
+hipError_t launch_classify_v5(const ClassifyArgs& a, hipStream_t stream);
 
 // Per-read algorithmic work (non-empty pairs, state-timesteps, FLOPs) -> [N][3].
 void launch_work(const DevModel& m, int N, int S, const uint8_t* d_Lr, const uint8_t* d_Hr,

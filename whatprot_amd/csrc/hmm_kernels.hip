@@ -456,6 +456,7 @@ classify_kernel(const ClassifyArgs a) {
     const int nbr = a.nbmax[r];
     // Empty reads go with the first bin (nb_lo == 0 there).
     if (!((nbr > a.nb_lo && nbr <= a.nb_hi) || (nbr == 0 && a.nb_lo == 0))) return;
+    if (a.wide_pass && !a.wide[r]) return;
 
     const int T = a.m.T;
     const int tid = threadIdx.x;

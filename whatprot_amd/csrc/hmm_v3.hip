@@ -92,6 +92,7 @@ __global__ void __launch_bounds__(kBlockThreads) classify_v3_kernel(const Classi
     if (r >= a.N) return;
     const int nbr = a.nbmax[r];
     if (!((nbr > a.nb_lo && nbr <= a.nb_hi) || (nbr == 0 && a.nb_lo == 0))) return;
+    if (a.wide_pass && !a.wide[r]) return;
 
     const int T = a.m.T;
     const int tri = a.tri;

@@ -1,0 +1,450 @@
+// Classify kernel (FP32 rescaled) for reads whose count box holds at most 16
+// states: a quad of lanes per candidate, several Edman rows per lane.
+//
+// Same recursion as the row-per-lane kernel (read-level boxes, composite
+// Bleach matrices Wt and Edman-shifted W1[c] shared by every candidate of
+// the read, Edman success handed to the next row), laid out so that no lane
+// works on rows that cannot hold mass yet: lane p of a candidate's quad keeps
+// rows e = p + 4k for k < R in registers (row-slot k). Row e is zero before
+// t = e, so slot k is skipped, uniformly over the wave, while t < 4k. The
+// Edman success of row e - 1 reaches row e through one quad DPP rotation:
+// lanes 1..3 take slot k of the lane below, lane 0 takes slot k - 1 of lane 3.
+//
+// Scaling. Emission densities are stored per timestep relative to the
+// largest density of that timestep's box (E = Ehat 2^X_t), and every row
+// carries one power-of-two exponent, so the per-entry work of a step is the
+// Edman coefficients, the two updates, the emission product and a rescale;
+// row exponents meet only where the Edman success crosses rows and in the
+// final sum of a pair.
+#include <float.h>
+#include <math.h>
+
+#include "classify_common.h"
+#include "hmm_kernels.h"
+
+namespace wp {
+namespace {
+
+using namespace cc;
+
+constexpr int kV5Threads = 256;
+constexpr int kV5Lanes = 4;  // lanes per candidate: one DPP quad
+constexpr int kV5MaxSpan = 90;  // largest binary-exponent spread of one box's densities
+
+struct V5Layout {
+    size_t rad, L, H, nbt, bstr, coord, kcf, ev, etab, xt, W, binom, list, scan, gres, total;
+};
+
+// W block per timestep: (1 + C) matrices (Wt, then W1[c]) of NB x NB with a
+// stride of NB * NB + 4 entries, so the shifted matrices of different
+// channels start in different LDS banks.
+__host__ __device__ inline V5Layout v5_layout(int T, int C, int tri, int NB) {
+    V5Layout l;
+    const size_t ms = (size_t)NB * NB + 4;
+    const int ngroups = kV5Threads / kV5Lanes;
+    size_t o = 0;
+    l.rad = o;   o = align16(o + sizeof(double) * T * C);
+    l.L = o;     o = align16(o + sizeof(int) * T * C);
+    l.H = o;     o = align16(o + sizeof(int) * T * C);
+    l.nbt = o;   o = align16(o + sizeof(int) * T);
+    l.bstr = o;  o = align16(o + sizeof(int) * T * C);
+    l.coord = o; o = align16(o + sizeof(int) * (size_t)T * NB * C);
+    l.kcf = o;   o = align16(o + sizeof(float) * (size_t)T * NB * C);
+    l.ev = o;    o = align16(o + sizeof(double) * (size_t)T * NB);
+    l.etab = o;  o = align16(o + sizeof(float) * (size_t)T * NB);
+    l.xt = o;    o = align16(o + sizeof(int) * T);
+    l.W = o;     o = align16(o + sizeof(float) * (size_t)T * (1 + C) * ms);
+    l.binom = o; o = align16(o + sizeof(double) * 2 * C * tri);
+    l.list = o;  o = align16(o + sizeof(int) * kV5Threads * 4);
+    l.scan = o;  o = align16(o + sizeof(int) * (kV5Threads / 64 + 2));
+    l.gres = o;  o = align16(o + sizeof(double) * 4 * ngroups);
+    l.total = o;
+    return l;
+}
+
+// Lane p of each quad receives the value of lane (p + 3) % 4.
+__device__ inline int quad_rot_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 3 | (0 << 2) | (1 << 4) | (2 << 6), 0xf, 0xf, false);
+}
+__device__ inline float quad_rot(float v) { return __int_as_float(quad_rot_i(__float_as_int(v))); }
+
+template <int C, int NB, int R>
+__global__ void __launch_bounds__(kV5Threads) classify_v5_kernel(const ClassifyArgs a) {
+    constexpr int G = kV5Lanes;
+    constexpr int MS = NB * NB + 4;
+    constexpr int NGR = kV5Threads / G;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int r = blockIdx.x;
+    if (r >= a.N) return;
+    const int nbr = a.nbmax[r];
+    if (!((nbr > a.nb_lo && nbr <= a.nb_hi) || (nbr == 0 && a.nb_lo == 0))) return;
+
+    const int T = a.m.T;
+    const int tri = a.tri;
+    const int tid = threadIdx.x;
+    const int group = tid / G;
+    const int p = tid % G;
+    const V5Layout lay = v5_layout(T, C, tri, NB);
+    double* s_rad = (double*)(smem + lay.rad);
+    int* sL = (int*)(smem + lay.L);
+    int* sH = (int*)(smem + lay.H);
+    int* nbt = (int*)(smem + lay.nbt);
+    int* bstr = (int*)(smem + lay.bstr);
+    int* coord = (int*)(smem + lay.coord);
+    float* kcf = (float*)(smem + lay.kcf);
+    double* ev = (double*)(smem + lay.ev);
+    float* etab = (float*)(smem + lay.etab);
+    int* xt = (int*)(smem + lay.xt);
+    float* Wall = (float*)(smem + lay.W);
+    double* bin = (double*)(smem + lay.binom);
+    int* list = (int*)(smem + lay.list);
+    int* scan = (int*)(smem + lay.scan);
+    double* gres = (double*)(smem + lay.gres);
+    const int ncand = a.cand_off ? a.cand_off[r + 1] - a.cand_off[r] : a.S;
+    const int* cidx = a.cand_off ? a.cand_idx + a.cand_off[r] : nullptr;
+    const bool read_empty = nbr == 0;
+
+    // ---- prologue: read-level boxes, coordinates, emission densities
+    for (int k = tid; k < T * C; k += kV5Threads) {
+        s_rad[k] = a.rads[(size_t)r * T * C + k];
+        sL[k] = a.Lr[(size_t)r * T * C + k];
+        sH[k] = a.Hr[(size_t)r * T * C + k];
+    }
+    for (int k = tid; k < 2 * C * tri; k += kV5Threads) bin[k] = a.binom[k];
+    if (tid == 0) scan[0] = 0;
+    __syncthreads();
+    if (tid < T) {
+        const int t = tid;
+        int s = 1;
+        for (int c = C - 1; c >= 0; c--) {
+            bstr[t * C + c] = s;
+            const int w = sH[t * C + c] - sL[t * C + c];
+            s *= w > 0 ? w : 0;
+        }
+        nbt[t] = read_empty ? 0 : s;
+    }
+    __syncthreads();
+    // peptide-emission.cc:165-180
+    for (int k = tid; k < T * NB; k += kV5Threads) {
+        const int t = k / NB, q = k % NB;
+        unsigned int n[WP_MAX_CHANNELS];
+        double v = 0.0;
+        if (q < nbt[t]) {
+            int rem = q;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const int d = rem / bstr[t * C + c];
+                rem -= d * bstr[t * C + c];
+                n[c] = (unsigned int)(sL[t * C + c] + d);
+            }
+            v = 1.0;
+#pragma unroll
+            for (int c = 0; c < C; c++) v *= dev_pdf<C>(a.m, c, s_rad[t * C + c], n);
+        } else {
+#pragma unroll
+            for (int c = 0; c < C; c++) n[c] = 0;
+        }
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            coord[k * C + c] = (int)n[c];
+            kcf[k * C + c] = (float)n[c];
+        }
+        ev[k] = v;
+    }
+    __syncthreads();
+    if (tid < T) {  // X_t: the largest binary exponent of the box's densities
+        int x = kNoExp, lo = -kNoExp;
+        for (int q = 0; q < NB; q++) {
+            const double v = ev[tid * NB + q];
+            if (v > 0.0) {
+                int ex;
+                frexp(v, &ex);
+                x = max(x, ex);
+                lo = min(lo, ex);
+            }
+        }
+        xt[tid] = x == kNoExp ? 0 : x;
+        if (x != kNoExp && x - lo > kV5MaxSpan) scan[0] = 1;  // cleared before the barrier above
+    }
+    __syncthreads();
+    // Densities spanning more than 2^kV5MaxSpan in one box could fall below
+    // FP32's range relative to the box maximum: such a read is flagged and
+    // left to the kernels that keep one exponent per density.
+    const bool wide = scan[0] != 0;
+    if (tid == 0) a.wide[r] = wide ? 1 : 0;
+    if (wide) return;
+    for (int k = tid; k < T * NB; k += kV5Threads) etab[k] = (float)ldexp(ev[k], -xt[k / NB]);
+    // Composite Bleach between the boxes of t-1 and t and its Edman-shifted
+    // forms: W1[c][q][q2] maps onto n'(q2) + 1_c and is zero at or above
+    // U = min(Hr[t] + 1, Hr[t-1]) (edman-transition.cc:28-60).
+    const double* ble = bin + C * tri;
+    for (int k = tid; k < T * (1 + C) * NB * NB; k += kV5Threads) {
+        const int t = k / ((1 + C) * NB * NB);
+        int rem = k - t * (1 + C) * NB * NB;
+        const int mat = rem / (NB * NB);
+        rem -= mat * NB * NB;
+        const int q = rem / NB, q2 = rem % NB;
+        double v = 0.0;
+        if (t >= 1 && q < nbt[t - 1] && q2 < nbt[t]) {
+            v = 1.0;
+            const int sc = mat - 1;  // shifted channel, -1 for Wt
+            for (int c = 0; c < C; c++) {
+                const int from = coord[((t - 1) * NB + q) * C + c];
+                const int to = coord[(t * NB + q2) * C + c] + (c == sc ? 1 : 0);
+                if (c == sc && to >= min(sH[t * C + c] + 1, sH[(t - 1) * C + c])) v = 0.0;
+                if (from < to) v = 0.0;
+                else v *= ble[c * tri + from * (from + 1) / 2 + to];
+            }
+        }
+        Wall[((size_t)t * (1 + C) + mat) * MS + rem] = (float)v;
+    }
+    __syncthreads();
+
+    CandAcc<true> acc;
+    const float pf = (float)a.m.pf;
+    const float opf = (float)(1.0 - a.m.pf);
+    const float pcb = (float)a.m.p_cb;
+    const double* dud = bin;
+
+    for (int base = 0; base < ncand; base += kV5Threads * 4) {
+        const int nlive = compact_chunk<C>(a, r, ncand, cidx, base, read_empty, sL, list, scan);
+        for (int q0 = group; q0 < nlive; q0 += NGR) {
+            const int kk = list[q0];
+            const int i = cidx ? cidx[kk] : kk;
+            int D[C];
+#pragma unroll
+            for (int c = 0; c < C; c++) D[c] = a.dsD[(size_t)i * C + c];
+            // Edman coefficients of each row-slot: keep = aK max(K - kc, 0) + cst,
+            // success = aK (kc + 1), with aK = (1 - pf) / K for a dye of channel
+            // c_e at position e, and aK = 0, cst = 1 - pf for no dye there.
+            int cix[R];
+            float aK[R], cst[R], Kf[R];
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const int e = p + G * k;
+                const int ce = e < T ? (int)a.dsSeq[(size_t)i * T + e] : -1;
+                const int K = ce >= 0 ? (int)a.dsTrack[((size_t)i * T + e) * C + ce] : 1;
+                cix[k] = ce < 0 ? 0 : ce;
+                aK[k] = ce >= 0 ? opf / (float)K : 0.0f;
+                cst[k] = ce >= 0 ? 0.0f : opf;
+                Kf[k] = (float)K;
+            }
+            float A[R][NB], B[R][NB];
+            int kx[R];     // slot k holds values x 2^-kx[k]
+            float dl[R];   // detached share of slot k's row
+            float Sr[R];   // sum of slot k's row after the last emission
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+#pragma unroll
+                for (int q = 0; q < NB; q++) {
+                    A[k][q] = 0.0f;
+                    B[k][q] = 0.0f;
+                }
+                kx[k] = kNoExp;
+                dl[k] = 0.0f;
+                Sr[k] = 0.0f;
+            }
+            // t = 0: InitialBlock, Dud_c, Emit(0): row 0 (lane 0, slot 0)
+            if (p == 0) {
+                const float pib = (float)a.m.p_ib;
+                float m = 0.0f;
+#pragma unroll
+                for (int q = 0; q < NB; q++) {
+                    bool ok = true;
+                    float xa = 1.0f - pib, xb = pib;
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        const int n = coord[q * C + c];
+                        ok = ok && n <= D[c];
+                        if (ok) {
+                            const float pd = (float)dud[c * tri + D[c] * (D[c] + 1) / 2 + n];
+                            xa = pd * xa;
+                            xb = pd * xb;
+                        }
+                    }
+                    const float em = ok ? etab[q] : 0.0f;
+                    A[0][q] = xa * em;
+                    B[0][q] = xb * em;
+                    m = fmaxf(m, fmaxf(A[0][q], B[0][q]));
+                }
+                const int km = exp_of(m);
+                const int kf = km == kNoExp ? 0 : km;
+                float S = 0.0f;
+#pragma unroll
+                for (int q = 0; q < NB; q++) {
+                    A[0][q] = ldexpf(A[0][q], -kf);
+                    B[0][q] = ldexpf(B[0][q], -kf);
+                    S += A[0][q] + B[0][q];
+                }
+                Sr[0] = S;
+                kx[0] = km == kNoExp ? kNoExp : xt[0] + kf;
+            }
+
+            for (int t = 1; t < T; t++) {
+                const float* Wt = Wall + (size_t)t * (1 + C) * MS;
+                const int nbp = nbt[t - 1];
+                const float pd = (float)a.m.p_detach[t - 1];
+                const float cAA = (1.0f - pd) * (1.0f - pcb);
+                const float cB = 1.0f - pd;
+                const float* Et = etab + (size_t)t * NB;
+                const float E0 = Et[0];
+                const int Xt = xt[t];
+                bool allow = true;
+#pragma unroll
+                for (int c = 0; c < C; c++) allow = allow && sL[t * C + c] == 0;
+                const float* kct = kcf + (size_t)t * NB * C;
+                float succ_prev[NB];  // this lane's slot k - 1 Edman success (old scale)
+                int kprev = kNoExp;   // and slot k - 1's exponent before this step
+#pragma unroll
+                for (int q = 0; q < NB; q++) succ_prev[q] = 0.0f;
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    if (t < G * k) continue;  // every row of slot k starts after t (uniform)
+                    const float* W1 = Wt + (1 + cix[k]) * MS;
+                    float BA[NB], BB[NB], BA1[NB];
+#pragma unroll
+                    for (int q2 = 0; q2 < NB; q2++) {
+                        BA[q2] = 0.0f;
+                        BB[q2] = 0.0f;
+                        BA1[q2] = 0.0f;
+                    }
+#pragma unroll
+                    for (int q = 0; q < NB; q++) {
+                        if (q >= nbp) continue;  // uniform: the previous box's size
+                        const float av = A[k][q], bv = B[k][q];
+#pragma unroll
+                        for (int q2 = 0; q2 < NB; q2++) {
+                            const float w = Wt[q * NB + q2];
+                            BA[q2] += w * av;
+                            BB[q2] += w * bv;
+                            BA1[q2] += W1[q * NB + q2] * av;
+                        }
+                    }
+                    // exponents of this row and of the row below it (e - 1)
+                    const int ko = kx[k];
+                    const int kfrom = quad_rot_i(p == G - 1 ? kprev : ko);
+                    int kn = max(ko, kfrom);
+                    if (kn == kNoExp) kn = 0;
+                    const float sown = ko == kNoExp ? 0.0f : ldexpf(1.0f, ko - kn);
+                    const float sfrom = kfrom == kNoExp ? 0.0f : ldexpf(1.0f, kfrom - kn);
+                    const float fAo = cAA * pf * sown, fAf = cAA * sfrom;
+                    const float fB = cB * sown, fBA = cB * pcb * sown;
+                    // Detach(t-1): pd times the row's previous sum.
+                    float dlp = (dl[k] + pd * Sr[k]) * sown;
+                    float m = 0.0f;
+#pragma unroll
+                    for (int q2 = 0; q2 < NB; q2++) {
+                        const float kc = kct[q2 * C + cix[k]];
+                        const float cba = fmaf(fmaxf(Kf[k] - kc, 0.0f), aK[k], cst[k]);
+                        const float cba1 = fmaf(kc, aK[k], aK[k]);
+                        const float succ = fmaf(cba1, BA1[q2], cba * BA[q2]);
+                        const float from = quad_rot(p == G - 1 ? succ_prev[q2] : succ);
+                        succ_prev[q2] = succ;
+                        // Emit(t) (peptide-emission.cc:212-243)
+                        const float em = Et[q2];
+                        const float na = fmaf(fAf, from, fAo * BA[q2]) * em;
+                        const float nb = fmaf(fBA, BA[q2], fB * BB[q2]) * em;
+                        m = fmaxf(m, fmaxf(na, nb));
+                        A[k][q2] = na;
+                        B[k][q2] = nb;
+                    }
+                    // the detached share keeps density E_t(0) only if the box includes zero
+                    dlp = allow ? dlp * E0 : 0.0f;
+                    m = fmaxf(m, dlp);
+                    kprev = ko;
+                    const int km = exp_of(m);
+                    const int kf = km == kNoExp ? 0 : km;
+                    float S = 0.0f;
+#pragma unroll
+                    for (int q2 = 0; q2 < NB; q2++) {
+                        A[k][q2] = ldexpf(A[k][q2], -kf);
+                        B[k][q2] = ldexpf(B[k][q2], -kf);
+                        S += A[k][q2] + B[k][q2];
+                    }
+                    Sr[k] = S;
+                    dl[k] = ldexpf(dlp, -kf);
+                    kx[k] = km == kNoExp ? kNoExp : kn + Xt + kf;
+                }
+            }
+
+            // P = sum over rows of (row sum + detached share) (peptide-state-vector.cc:52-54)
+            int kl = kNoExp;
+#pragma unroll
+            for (int k = 0; k < R; k++)
+                if (Sr[k] + dl[k] > 0.0f) kl = max(kl, kx[k]);
+            float part = 0.0f;
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const float own = Sr[k] + dl[k];
+                if (own > 0.0f && kx[k] != kNoExp) part += ldexpf(own, kx[k] - kl);
+            }
+            const int K = group_max_int<G>(kl);
+            part = (K == kNoExp || kl == kNoExp) ? 0.0f : ldexpf(part, kl - K);
+            const float s = group_sum<float, G>(part);
+            const double logp = s > 0.0f ? log((double)s) + (double)K * 0.69314718055994530942 : -INFINITY;
+            if (a.out_pair_logp && !cidx && p == 0) a.out_pair_logp[(size_t)r * a.S + i] = logp;
+            acc.add(logp, exp(logp), kk, a.dsCount[i]);
+        }
+        __syncthreads();
+    }
+    merge_and_store<true>(a, r, ncand, cidx, NGR, group, p, acc, gres);
+}
+
+template <int C, int NB, int R>
+hipError_t launch_v5_t(const ClassifyArgs& a, hipStream_t stream) {
+    const V5Layout l = v5_layout(a.m.T, C, a.tri, NB);
+    if (l.total > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)classify_v5_kernel<C, NB, R>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((classify_v5_kernel<C, NB, R>), dim3(a.N), dim3(kV5Threads), l.total, stream, a);
+    return hipGetLastError();
+}
+
+// Row-slots per lane: R = ceil(T / 4) rounded up to an instantiated count.
+template <int C, int NB>
+hipError_t launch_v5_nb(const ClassifyArgs& a, hipStream_t stream) {
+    const int T = a.m.T;
+    if (T <= 4) return launch_v5_t<C, NB, 1>(a, stream);
+    if (T <= 8) return launch_v5_t<C, NB, 2>(a, stream);
+    if (T <= 12) return launch_v5_t<C, NB, 3>(a, stream);
+    if (T <= 16) return launch_v5_t<C, NB, 4>(a, stream);
+    if constexpr (NB <= 8) {
+        if (T <= 32) return launch_v5_t<C, NB, 8>(a, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int C>
+hipError_t launch_v5_c(const ClassifyArgs& a, hipStream_t stream) {
+    switch (a.NB) {
+    case 4: return launch_v5_nb<C, 4>(a, stream);
+    case 8: return launch_v5_nb<C, 8>(a, stream);
+    case 16: return launch_v5_nb<C, 16>(a, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+bool v5_supports(int precision, int T, int NB) {
+    if (precision != WP_PRECISION_FP32) return false;
+    if (NB == 4 || NB == 8) return T <= 32;
+    return NB == 16 && T <= 16;
+}
+
+size_t v5_lds_bytes(int T, int C, int tri, int NB) { return v5_layout(T, C, tri, NB).total; }
+
+hipError_t launch_classify_v5(const ClassifyArgs& a, hipStream_t stream) {
+    if (a.N <= 0) return hipSuccess;
+    switch (a.m.C) {
+    case 1: return launch_v5_c<1>(a, stream);
+    case 2: return launch_v5_c<2>(a, stream);
+    case 3: return launch_v5_c<3>(a, stream);
+    case 4: return launch_v5_c<4>(a, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace wp
