@@ -35,6 +35,7 @@ struct wp_mlp {
     float* post = nullptr;
     float* lse = nullptr;
     float* mass = nullptr;
+    float* csum = nullptr;  // row-chunk partial sums of the bias gradients
 };
 
 namespace {
@@ -54,6 +55,7 @@ void free_mlp(wp_mlp* m) {
     hipFree(m->post);
     hipFree(m->lse);
     hipFree(m->mass);
+    hipFree(m->csum);
     delete m;
 }
 
@@ -76,6 +78,7 @@ int ensure_batch(wp_mlp* m, int B) {
     hipFree(m->post);
     hipFree(m->lse);
     hipFree(m->mass);
+    hipFree(m->csum);
     m->H.assign(m->nh, nullptr);
     m->dH.assign(m->nh, nullptr);
     m->cap = 0;
@@ -89,6 +92,7 @@ int ensure_batch(wp_mlp* m, int B) {
     if (!rc) rc = dalloc(&m->post, (size_t)B);
     if (!rc) rc = dalloc(&m->lse, (size_t)B);
     if (!rc) rc = dalloc(&m->mass, (size_t)B);
+    if (!rc) rc = dalloc(&m->csum, wp::colsum_work_floats(B, std::max(m->hidden, m->out_dim)));
     if (!rc) m->cap = B;
     return rc;
 }
@@ -145,7 +149,7 @@ int backward(wp_mlp* m, int B) {
         g.Cf = m->gW[l];
         g.ldc = m->dout[l];
         WP_HIP(wp::gemm(g, nullptr));
-        WP_HIP(wp::colsum(B, m->dout[l], dY, m->dout[l], m->gb[l], nullptr));
+        WP_HIP(wp::colsum(B, m->dout[l], dY, m->dout[l], m->gb[l], m->csum, nullptr));
         if (l == 0) break;
         // dH[l-1] = (dY W[l]^T) masked by ReLU'(H[l-1])
         std::memset(&g, 0, sizeof(g));

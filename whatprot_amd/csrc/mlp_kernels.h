@@ -26,7 +26,9 @@ struct GemmArgs {
 hipError_t gemm(const GemmArgs& g, hipStream_t s);
 hipError_t softmax_ce(int rows, int N, const float* Z, long ldz, const int* labels, float scale, __bf16* dZ,
                       float* loss, float* max_post, float* lse, hipStream_t s);
-hipError_t colsum(int M, int N, const __bf16* dY, long ld, float* db, hipStream_t s);
+// db[n] = sum over M rows of dY(m, n); work holds colsum_work_floats(M, N) floats.
+size_t colsum_work_floats(int M, int N);
+hipError_t colsum(int M, int N, const __bf16* dY, long ld, float* db, float* work, hipStream_t s);
 hipError_t adam(long n, float* p, const float* g, float* m1, float* m2, float lr, float b1, float b2, float eps,
                 float c1, float c2, __bf16* pb, hipStream_t s);
 hipError_t to_bf16(long n, const float* x, __bf16* y, hipStream_t s);

@@ -209,12 +209,30 @@ __global__ void __launch_bounds__(256) softmax_ce_kernel(int N, const float* __r
     }
 }
 
-// db[n] = sum_m dY(m, n) (bf16 input, FP32 sum), one thread per column.
-__global__ void colsum_kernel(int M, int N, const __bf16* __restrict__ dY, long ld, float* __restrict__ db) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
+// db[n] = sum_m dY(m, n) (bf16 input, FP32 sums) in two deterministic
+// passes: workgroup (x, y) sums rows [y * kColsumRows, (y + 1) * kColsumRows)
+// of 64 columns (4 row lanes of 64 coalesced columns, then an LDS sum) into
+// part[y][n]; the second pass adds the row chunks in order.
+constexpr int kColsumRows = 128;
+__global__ void __launch_bounds__(256) colsum_part_kernel(int M, int N, const __bf16* __restrict__ dY, long ld,
+                                                          float* __restrict__ part) {
+    __shared__ float red[4][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + tx;
+    const int m0 = blockIdx.y * kColsumRows, m1 = min(M, m0 + kColsumRows);
+    float s = 0.0f;
+    if (n < N)
+        for (int m = m0 + ty; m < m1; m += 4) s += from_bf16(dY[(long)m * ld + n]);
+    red[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && n < N) part[(long)blockIdx.y * N + n] = (red[0][tx] + red[1][tx]) + (red[2][tx] + red[3][tx]);
+}
+
+__global__ void colsum_final_kernel(int chunks, int N, const float* __restrict__ part, float* __restrict__ db) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     float s = 0.0f;
-    for (int m = 0; m < M; m++) s += from_bf16(dY[(long)m * ld + n]);
+    for (int c = 0; c < chunks; c++) s += part[(long)c * N + n];
     db[n] = s;
 }
 
@@ -520,8 +538,17 @@ hipError_t softmax_ce(int rows, int N, const float* Z, long ldz, const int* labe
     return hipGetLastError();
 }
 
-hipError_t colsum(int M, int N, const __bf16* dY, long ld, float* db, hipStream_t s) {
-    hipLaunchKernelGGL(colsum_kernel, dim3((N + 255) / 256), dim3(256), 0, s, M, N, dY, ld, db);
+size_t colsum_work_floats(int M, int N) { return (size_t)((M + kColsumRows - 1) / kColsumRows) * N; }
+
+hipError_t colsum(int M, int N, const __bf16* dY, long ld, float* db, float* work, hipStream_t s) {
+    if (N <= 0) return hipSuccess;
+    const int chunks = (M + kColsumRows - 1) / kColsumRows;
+    if (chunks <= 0) {
+        hipLaunchKernelGGL(colsum_final_kernel, dim3((N + 255) / 256), dim3(256), 0, s, 0, N, work, db);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(colsum_part_kernel, dim3((N + 63) / 64, chunks), dim3(256), 0, s, M, N, dY, ld, work);
+    hipLaunchKernelGGL(colsum_final_kernel, dim3((N + 255) / 256), dim3(256), 0, s, chunks, N, work, db);
     return hipGetLastError();
 }
 

@@ -426,27 +426,28 @@ int wp_hybrid_classify(wp_hybrid_classifier* c, int num_reads, const double* rad
     if (rc) return rc;
     // candidates per read (hybrid-classifier.cc:55-66), in the nn order
     std::vector<std::vector<Scored>> cands(N);
+    std::vector<std::vector<int>> cidx(N);  // candidates as dye-seq table indices
     std::vector<double> subfraction(N, 0.0);
     parallel_for(N, [&](int b, int e) {
         for (int r = b; r < e; r++) {
             cands[r] = top_of(nn, cnt[r], &idx[(size_t)r * nn->k], &dist[(size_t)r * nn->k], (unsigned)c->h);
             double s = 0.0;
+            cidx[r].reserve(cands[r].size());
             for (const Scored& x : cands[r]) {
                 auto it = c->id_count.find(x.id);
                 s += x.adjusted() * (double)(it == c->id_count.end() ? 0 : it->second);
+                auto jt = c->id_index.find(x.id);
+                cidx[r].push_back(jt == c->id_index.end() ? 0 : jt->second);
             }
             subfraction[r] = s;
         }
     });
-    std::vector<int> off(N + 1, 0), list;
-    for (int r = 0; r < N; r++) {
-        for (const Scored& x : cands[r]) {
-            auto it = c->id_index.find(x.id);
-            list.push_back(it == c->id_index.end() ? 0 : it->second);
-        }
-        off[r + 1] = (int)list.size();
-    }
-    if (list.empty()) list.push_back(0);
+    std::vector<int> off(N + 1, 0);
+    for (int r = 0; r < N; r++) off[r + 1] = off[r] + (int)cidx[r].size();
+    std::vector<int> list(std::max(off[N], 1), 0);
+    parallel_for(N, [&](int b, int e) {
+        for (int r = b; r < e; r++) std::copy(cidx[r].begin(), cidx[r].end(), list.begin() + off[r]);
+    });
     std::vector<int> hid(N);
     std::vector<double> hadj(N);
     rc = wp_hmm_classify_candidates(c->hmm, N, radiometries, off.data(), list.data(), hid.data(), hadj.data(),
