@@ -26,6 +26,11 @@ top-256 candidates (full-HMM fallback when the shortlist holds under half the
 posterior mass), against the plain
 FP64 GPU HMM over all 20,000 dye-seqs (-p 5 both).
 
+--workload simulate: the GPU simulator (simulate dt): 20,000 dye-seqs x 1,000
+peptides each (2x10^7 simulated dye tracks), deduplicated on the device by a
+radix sort of the packed tracks; one step is one call. CPU baseline: the host
+simulator (same streams) on 2x10^5 peptides.
+
 --workload fit: Baum-Welch fitting of one dye-seq ("..0..1..2..0..1..2..",
 2 dyes in each of 3 channels, T = 30) to 10^6 simulated reads; one step is one
 EM iteration (GPU E-step over every read + host M-step). CPU baseline: the
@@ -81,7 +86,8 @@ def main():
     ap.add_argument("--cpu-sample-reads", type=int, default=3000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
-    ap.add_argument("--workload", choices=["classify", "fit", "hybrid", "shortlist"], default="classify")
+    ap.add_argument("--workload", choices=["classify", "fit", "hybrid", "shortlist", "simulate"], default="classify")
+    ap.add_argument("--peptides-per-dye-seq", type=int, default=1000)
     ap.add_argument("--train-reads", type=int, default=10000000)
     ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--tracks-per-dye-seq", type=int, default=100)
@@ -99,6 +105,8 @@ def main():
         return bench_hybrid(args)
     if args.workload == "shortlist":
         return bench_shortlist(args)
+    if args.workload == "simulate":
+        return bench_simulate(args)
 
     import numpy as np
     import torch
@@ -109,7 +117,7 @@ def main():
     model = w.SequencingModel.uniform(C)  # seqparams_atto647n_x3 priors with mu = 1
     log(f"generating {S} dye-seqs and {N} reads")
     strings = w.generate_dye_seqs(C, S, 6, 30, 0, 5, 1001)
-    rads, true_idx = w.simulate_radiometries(model, T, strings, N, 2001)
+    rads, true_idx = w.simulate_radiometries(model, T, strings, N, 2001, gpu=True)
     entries = [w.DyeSeqEntry(s, 1, i) for i, s in enumerate(strings)]
     prec = w.WP_PRECISION_FP32 if args.precision == "fp32" else w.WP_PRECISION_FP64
     clf = w.HMMClassifier(T, C, model, w.SequencingSettings(cutoff), entries, prec)
@@ -236,9 +244,9 @@ def bench_hybrid(args):
     strings = w.generate_dye_seqs(C, S, 6, 30, 0, 5, 1002)
     seqs = [w.DyeSeqEntry(s, 1, i) for i, s in enumerate(strings)]
     t0 = time.perf_counter()
-    tracks = w.simulate_dye_tracks(model, T, seqs, args.tracks_per_dye_seq, 2002)
+    tracks = w.simulate_dye_tracks(model, T, seqs, args.tracks_per_dye_seq, 2002, gpu=True)
     t_sim = time.perf_counter() - t0
-    rads, true_idx = w.simulate_radiometries(model, T, strings, N, 3002)
+    rads, true_idx = w.simulate_radiometries(model, T, strings, N, 3002, gpu=True)
     log(f"{len(tracks)} distinct dye-tracks ({t_sim:.1f} s); building classifiers")
     prec = w.WP_PRECISION_FP32 if args.precision == "fp32" else w.WP_PRECISION_FP64
     t0 = time.perf_counter()
@@ -322,10 +330,10 @@ def bench_shortlist(args):
     entries = [w.DyeSeqEntry(s, 1, i) for i, s in enumerate(strings)]
     log(f"simulating {args.train_reads} training reads and {N} query reads")
     t0 = time.perf_counter()
-    tx, ty = w.simulate_radiometries(model, T, strings, args.train_reads, 4003)
+    tx, ty = w.simulate_radiometries(model, T, strings, args.train_reads, 4003, gpu=True)
     tx = tx.reshape(len(tx), -1).astype(np.float32)
     t_sim = time.perf_counter() - t0
-    qx, qy = w.simulate_radiometries(model, T, strings, N, 3002)
+    qx, qy = w.simulate_radiometries(model, T, strings, N, 3002, gpu=True)
     net = w.ShortlistMLP(T * C, 512, 3, S, seed=1)
     log(f"training {args.epochs} epochs ({t_sim:.0f} s simulating)")
     torch.cuda.synchronize()
@@ -407,6 +415,67 @@ def bench_shortlist(args):
     print(json.dumps(result), flush=True)
 
 
+def bench_simulate(args):
+    """GPU dye-track simulation with deduplication (the input stage of
+    classify nn / hybrid): 20,000 dye-seqs x --peptides-per-dye-seq peptides,
+    C = 3, T = 12. One step = one wp_simulate_dye_tracks_gpu call (simulate,
+    sort-based dedup, copy of the distinct tracks to the host)."""
+    import numpy as np
+    import torch
+
+    import whatprot_amd as w
+
+    T, C, S, per = 12, 3, 20000, args.peptides_per_dye_seq
+    model = w.SequencingModel.uniform(C)
+    strings = w.generate_dye_seqs(C, S, 6, 30, 0, 5, 1002)
+    seqs = [w.DyeSeqEntry(s, 1, i) for i, s in enumerate(strings)]
+    peptides = S * per
+    cap = min(peptides, 12_000_000)
+    log(f"warmup ({peptides} peptides per step)")
+    for _ in range(args.warmup):
+        tr = w.simulate_dye_tracks(model, T, seqs, per, 7, gpu=True, max_tracks=cap)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        tr = w.simulate_dye_tracks(model, T, seqs, per, 100 + k, gpu=True, max_tracks=cap)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    ntr = len(tr)
+    log(f"{ntr} distinct tracks, {dt * 1e3:.1f} ms/step")
+    # algorithmic bytes: the T*C one-byte counts of every peptide written once
+    # and read once by the key packing, plus the distinct tracks (T*C shorts)
+    bytes_step = peptides * T * C * 2 + ntr * T * C * 2
+    achieved = bytes_step / dt / 1e9
+    result = {
+        "metric": "simulated dye-track peptides/s with dedup on 1 MI355X",
+        "value": peptides / dt,
+        "unit": "peptides/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8 counts + fp64 rng",
+        "data": "synthetic (seeded simulator, seqparams_atto647n_x3 priors)",
+        "config": {"workload": f"simulate dt: {S} dye-seqs x {per} peptides, C=3, T=12, dedup",
+                   "dye_seqs": S, "peptides": peptides, "distinct_tracks": ntr},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": None,
+                     "kernel": "whole call (simulate + radix-sort dedup + host copy)"},
+    }
+    if not args.no_cpu_baseline:
+        n = 10
+        t0 = time.perf_counter()
+        w.simulate_dye_tracks(model, T, seqs, n, 100)
+        d1 = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": S * n / d1, "unit": "peptides/s", "cores": 1, "kind": "port",
+                                  "sample": f"{S * n} peptides of the same workload through the host simulator "
+                                            f"(same streams, std::map dedup), {d1:.1f} s wall"}
+    print(json.dumps(result), flush=True)
+
+
 def fit_flops_per_read(T, D):
     """Algorithmic FLOPs of one read's E-step: the forward pass by the
     per-state counts of the classify roofline over full (uncut) boxes, the
@@ -434,7 +503,7 @@ def bench_fit(args):
     truth = w.SequencingModel.uniform(C, p_edman_failure=0.07, p_bleach=0.04, p_dud=0.08, p_detach=0.04)
     start = w.SequencingModel.uniform(C)
     log(f"simulating {N} reads of {ds}")
-    rads, _ = w.simulate_radiometries(truth, T, [ds], N, 3004)
+    rads, _ = w.simulate_radiometries(truth, T, [ds], N, 3004, gpu=True)
     prec = w.WP_PRECISION_FP32 if args.precision == "fp32" else w.WP_PRECISION_FP64
     f = w.HMMFitter(T, C, ds, rads, prec)
     log("warmup")

@@ -411,9 +411,12 @@ int wp_hmm_fit(wp_hmm_fitter* fitter, const wp_model* seq_model, const wp_fit_se
 int wp_fit_offset_exponential(int n, const double* y, const double* w, const int* hold, double* p);
 
 /* ---------------------------------------------------------------------------
- * Synthetic inputs (host). Seeded and portable; the streams differ from the
- * reference's clock-seeded std::default_random_engine, the distributions
- * follow its simulator.
+ * Synthetic inputs (host, and the same streams on the GPU below). Seeded and
+ * portable; the streams differ from the reference's clock-seeded
+ * std::default_random_engine, the distributions follow its simulator. Each
+ * simulated peptide (dye tracks, in dye-seq, copy, repeat order) or read
+ * attempt (radiometries) g draws from its own xoshiro256** stream seeded by
+ * splitmix64 from seed + (g + 1) * 0xD1B54A32D192ED03.
  * ------------------------------------------------------------------------- */
 
 /* Random dye-seq strings, num_dye_seqs rows of `stride` bytes (NUL padded):
@@ -442,16 +445,42 @@ int wp_simulate_radiometries(const wp_model* seq_model, unsigned int num_timeste
  * of dye_seq_counts[i] peptides of every dye-seq, all-zero tracks dropped,
  * identical tracks merged with per source dye-seq id its count and hits.
  * Output in the layout wp_nn_classifier_create takes, tracks ordered by their
- * counts, sources by id; each source array needs room for max_tracks *
- * num_dye_seqs entries at most (one per distinct track and dye-seq). Fails
- * with WP_ERR_ARG (and *out_num_tracks set) when more than max_tracks
- * distinct tracks result. */
+ * counts, sources by id. out_track_counts holds max_tracks tracks and
+ * out_source_offsets max_tracks + 1 entries; the three source arrays hold
+ * max_sources entries (one per distinct (track, dye-seq id); at most the
+ * number of simulated peptides). With more tracks or sources than that it
+ * writes only *out_num_tracks and *out_num_sources and fails with
+ * WP_ERR_ARG. */
 int wp_simulate_dye_tracks(const wp_model* seq_model, unsigned int num_timesteps,
                            unsigned int num_channels, int num_dye_seqs,
                            const char* const* dye_strings, const int* dye_seq_counts,
                            const int* dye_seq_ids, int per_peptide, uint64_t seed, int max_tracks,
-                           int* out_num_tracks, short* out_track_counts, int* out_source_offsets,
+                           int max_sources, int* out_num_tracks, int* out_num_sources,
+                           short* out_track_counts, int* out_source_offsets,
                            int* out_source_ids, int* out_source_counts, int* out_source_hits);
+
+/* The two simulators above on the GPU (no reference counterpart for the
+ * device itself; the survey's first "next" component, replacing the
+ * reference's MR-MPI dedup-dye-tracks run): one thread per peptide or read
+ * attempt, each drawing from its own stream exactly as the host functions
+ * do, so the dye tracks (and the deduplicated output) are identical to
+ * wp_simulate_dye_tracks / the tracks behind wp_simulate_radiometries; the
+ * deduplication is a device sort of the packed tracks. Radiometries can
+ * differ from the host's in the last bit of a float32 where the device's
+ * log or pow rounds differently. Dye-seqs of at most 64 positions;
+ * wp_simulate_radiometries_gpu takes at most 4 channels. */
+int wp_simulate_radiometries_gpu(const wp_model* seq_model, unsigned int num_timesteps,
+                                 unsigned int num_channels, int num_dye_seqs,
+                                 const char* const* dye_strings, const int* weights, int num_reads,
+                                 uint64_t seed, double* out_radiometries, int* out_dye_seq_index);
+
+int wp_simulate_dye_tracks_gpu(const wp_model* seq_model, unsigned int num_timesteps,
+                               unsigned int num_channels, int num_dye_seqs,
+                               const char* const* dye_strings, const int* dye_seq_counts,
+                               const int* dye_seq_ids, int per_peptide, uint64_t seed, int max_tracks,
+                               int max_sources, int* out_num_tracks, int* out_num_sources,
+                               short* out_track_counts, int* out_source_offsets,
+                               int* out_source_ids, int* out_source_counts, int* out_source_hits);
 
 #ifdef __cplusplus
 }

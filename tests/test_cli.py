@@ -1,4 +1,4 @@
-"""Command line (whatprot-amd): usage rules and the host-only simulate modes."""
+"""Command line (whatprot-amd): usage rules (no GPU needed)."""
 import os
 import subprocess
 
@@ -32,24 +32,3 @@ def test_incorrect_usage(args):
 def test_bootstrap_refused():
     r = run("fit", "-P", "p", "-x", "..0", "-L", "0.01", "-R", "r", "-b", "10", "-c", "0.9")
     assert r.returncode == 1 and "bootstrap" in r.stderr
-
-
-def test_simulate_rad_and_dt(tmp_path):
-    sp = tmp_path / "sp.json"
-    write_seqparams(sp, 2)
-    ds = tmp_path / "ds.tsv"
-    w.write_dye_seqs(str(ds), 2, [w.DyeSeqEntry("..0.1..0", 2, 7), w.DyeSeqEntry(".1.1", 1, 9)])
-    rad, ys, dt = tmp_path / "r.tsv", tmp_path / "y.tsv", tmp_path / "dt.tsv"
-    r = run("simulate", "rad", "-P", str(sp), "-t", "6", "-g", "50", "-S", str(ds), "-R", str(rad), "-Y", str(ys),
-            "--seed", "3")
-    assert r.returncode == 0, r.stderr
-    model = w.SequencingModel.from_json(str(sp))
-    _, _, rads = w.read_radiometries(str(rad), model)
-    assert rads.shape == (50, 6, 2)
-    ids = [int(x) for x in ys.read_text().split()]
-    assert ids[0] == 50 and set(ids[1:]) <= {7, 9}
-    r = run("simulate", "dt", "-P", str(sp), "-t", "6", "-g", "20", "-S", str(ds), "-T", str(dt), "--seed", "3")
-    assert r.returncode == 0, r.stderr
-    tr = w.read_dye_tracks(str(dt))
-    assert tr.num_timesteps == 6 and tr.num_channels == 2 and len(tr) > 3
-    assert set(tr.ids.tolist()) <= {7, 9}

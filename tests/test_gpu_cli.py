@@ -89,3 +89,25 @@ def test_fit(gpu, ref, files):
     _, _, want = ref.fit(model, ds, back, 1e-3)
     assert log_l == pytest.approx(want, rel=1e-8)
     assert "Edman failure rate: " in outp and "Channel 1 info: (Bleach rate: " in outp
+
+
+def test_simulate_rad_and_dt(gpu, tmp_path):
+    sp = tmp_path / "sp.json"
+    write_seqparams(sp, 2)
+    ds = tmp_path / "ds.tsv"
+    w.write_dye_seqs(str(ds), 2, [w.DyeSeqEntry("..0.1..0", 2, 7), w.DyeSeqEntry(".1.1", 1, 9)])
+    rad, ys, dt = tmp_path / "r.tsv", tmp_path / "y.tsv", tmp_path / "dt.tsv"
+    run("simulate", "rad", "-P", str(sp), "-t", "6", "-g", "50", "-S", str(ds), "-R", str(rad), "-Y", str(ys),
+        "--seed", "3")
+    model = w.SequencingModel.from_json(str(sp))
+    _, _, rads = w.read_radiometries(str(rad), model)
+    assert rads.shape == (50, 6, 2)
+    ids = [int(x) for x in ys.read_text().split()]
+    assert ids[0] == 50 and set(ids[1:]) <= {7, 9}
+    run("simulate", "dt", "-P", str(sp), "-t", "6", "-g", "20", "-S", str(ds), "-T", str(dt), "--seed", "3")
+    tr = w.read_dye_tracks(str(dt))
+    assert tr.num_timesteps == 6 and tr.num_channels == 2 and len(tr) > 3
+    assert set(tr.ids.tolist()) <= {7, 9}
+    # the GPU simulator behind `simulate dt` reproduces the host's streams
+    ref = w.simulate_dye_tracks(model, 6, [w.DyeSeqEntry("..0.1..0", 2, 7), w.DyeSeqEntry(".1.1", 1, 9)], 20, 3)
+    assert (tr.counts == ref.counts).all() and (tr.hits == ref.hits).all() and (tr.ids == ref.ids).all()

@@ -255,8 +255,11 @@ def _cstrings(strings):
     return arr, enc
 
 
-def simulate_radiometries(seq_model: SequencingModel, T, dye_strings, n, seed, weights=None):
-    """Returns (rads[n,T,C] in the model's units, dye-seq index per read)."""
+def simulate_radiometries(seq_model: SequencingModel, T, dye_strings, n, seed, weights=None, gpu=False):
+    """Returns (rads[n,T,C] in the model's units, dye-seq index per read).
+    gpu=True runs wp_simulate_radiometries_gpu (the same streams on the
+    device: identical dye tracks, intensities equal up to the last float32
+    bit)."""
     C = seq_model.num_channels
     m = seq_model.to_c()
     arr, keep = _cstrings(dye_strings)
@@ -265,9 +268,9 @@ def simulate_radiometries(seq_model: SequencingModel, T, dye_strings, n, seed, w
     w = None
     if weights is not None:
         w = np.ascontiguousarray(weights, dtype=np.int32)
-    check(lib().wp_simulate_radiometries(ctypes.byref(m), T, C, len(dye_strings), arr,
-                                         w.ctypes.data if w is not None else None, n, seed,
-                                         out.ctypes.data, idx.ctypes.data))
+    fn = lib().wp_simulate_radiometries_gpu if gpu else lib().wp_simulate_radiometries
+    check(fn(ctypes.byref(m), T, C, len(dye_strings), arr, w.ctypes.data if w is not None else None, n, seed,
+             out.ctypes.data, idx.ctypes.data))
     return out, idx
 
 
@@ -562,25 +565,29 @@ def write_dye_tracks(path, tracks: DyeTracks):
             f.write("\t".join(parts) + "\t" + str(e - b) + "\t" + "\t".join(src) + "\n")
 
 
-def simulate_dye_tracks(seq_model: SequencingModel, T, dye_seqs: Sequence[DyeSeqEntry], per_peptide, seed) -> DyeTracks:
+def simulate_dye_tracks(seq_model: SequencingModel, T, dye_seqs: Sequence[DyeSeqEntry], per_peptide, seed,
+                        gpu=False, max_tracks=None) -> DyeTracks:
     """generate_dye_tracks + dedup (generate-dye-tracks.cc:28-63,
-    dedup-dye-tracks.cc:35-148) with this library's seeded generator."""
+    dedup-dye-tracks.cc:35-148) with this library's seeded generator;
+    gpu=True simulates and deduplicates on the device (identical output)."""
     m = seq_model.to_c()
     C = seq_model.num_channels
     arr, keep = _cstrings([d.dye_string for d in dye_seqs])
     n = len(dye_seqs)
     cnt = np.ascontiguousarray([d.count for d in dye_seqs], dtype=np.int32)
     ids = np.ascontiguousarray([d.id for d in dye_seqs], dtype=np.int32)
-    cap = int(max(1, cnt.sum() * per_peptide))
+    peptides = int(max(1, cnt.clip(min=0).sum() * per_peptide))
+    cap = peptides if max_tracks is None else int(max_tracks)
     counts = np.zeros((cap, T, C), dtype=np.int16)
     off = np.zeros(cap + 1, dtype=np.int32)
-    sid = np.zeros(cap, dtype=np.int32)
-    scnt = np.zeros(cap, dtype=np.int32)
-    shits = np.zeros(cap, dtype=np.int32)
-    nt = ctypes.c_int()
-    check(lib().wp_simulate_dye_tracks(ctypes.byref(m), T, C, n, arr, cnt.ctypes.data, ids.ctypes.data,
-                                       int(per_peptide), int(seed), cap, ctypes.byref(nt), counts.ctypes.data,
-                                       off.ctypes.data, sid.ctypes.data, scnt.ctypes.data, shits.ctypes.data))
+    sid = np.zeros(peptides, dtype=np.int32)  # one source per (track, id): at most one per peptide
+    scnt = np.zeros(peptides, dtype=np.int32)
+    shits = np.zeros(peptides, dtype=np.int32)
+    nt, ns_ = ctypes.c_int(), ctypes.c_int()
+    fn = lib().wp_simulate_dye_tracks_gpu if gpu else lib().wp_simulate_dye_tracks
+    check(fn(ctypes.byref(m), T, C, n, arr, cnt.ctypes.data, ids.ctypes.data, int(per_peptide), int(seed), cap,
+             peptides, ctypes.byref(nt), ctypes.byref(ns_), counts.ctypes.data, off.ctypes.data, sid.ctypes.data,
+             scnt.ctypes.data, shits.ctypes.data))
     k = nt.value
     ns = int(off[k])
     return DyeTracks(T, C, counts[:k].copy(), off[:k + 1].copy(), sid[:ns].copy(), scnt[:ns].copy(),

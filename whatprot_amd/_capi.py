@@ -151,8 +151,15 @@ def lib():
     L.wp_hybrid_classify.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p]
     L.wp_simulate_dye_tracks.restype = c_int
     L.wp_simulate_dye_tracks.argtypes = [P(ModelC), c_uint, c_uint, c_int, P(ctypes.c_char_p), c_void_p, c_void_p,
-                                         c_int, ctypes.c_uint64, c_int, P(c_int), c_void_p, c_void_p, c_void_p,
-                                         c_void_p, c_void_p]
+                                         c_int, ctypes.c_uint64, c_int, c_int, P(c_int), P(c_int), c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p]
+    L.wp_simulate_radiometries_gpu.restype = c_int
+    L.wp_simulate_radiometries_gpu.argtypes = [P(ModelC), c_uint, c_uint, c_int, P(ctypes.c_char_p),
+                                           c_void_p, c_int, ctypes.c_uint64, c_void_p, c_void_p]
+    L.wp_simulate_dye_tracks_gpu.restype = c_int
+    L.wp_simulate_dye_tracks_gpu.argtypes = [P(ModelC), c_uint, c_uint, c_int, P(ctypes.c_char_p), c_void_p, c_void_p,
+                                         c_int, ctypes.c_uint64, c_int, c_int, P(c_int), P(c_int), c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p]
     L.wp_hmm_classifier_num_dye_seqs.restype = c_int
     L.wp_hmm_classifier_num_dye_seqs.argtypes = [c_void_p]
     L.wp_mlp_create.restype = c_int

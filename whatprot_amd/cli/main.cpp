@@ -560,7 +560,7 @@ int simulate(const Args& a, bool rad) {
     if (rad) {  // generate_radiometries with the true model (run-simulate-rad.cc)
         std::vector<double> v((size_t)g * T * C);
         std::vector<int> idx(g);
-        check(wp_simulate_radiometries(&truth, T, C, (int)ds.strings.size(), strs.data(), ds.counts.data(), g, a.seed,
+        check(wp_simulate_radiometries_gpu(&truth, T, C, (int)ds.strings.size(), strs.data(), ds.counts.data(), g, a.seed,
                                        v.data(), idx.data()));
         std::ofstream f(a.get('R'));
         f << T << "\n" << C << "\n" << g << "\n";
@@ -579,11 +579,12 @@ int simulate(const Args& a, bool rad) {
     long long total = 0;
     for (int c : ds.counts) total += c;
     int cap = (int)std::min<long long>(INT_MAX / 2, std::max<long long>(1, total * g));
-    int n = 0;
+    int n = 0, ns = 0;
     std::vector<short> counts((size_t)cap * T * C);
     std::vector<int> off(cap + 1), ids(cap), cnts(cap), hits(cap);
-    check(wp_simulate_dye_tracks(&truth, T, C, (int)ds.strings.size(), strs.data(), ds.counts.data(), ds.ids.data(), g,
-                                 a.seed, cap, &n, counts.data(), off.data(), ids.data(), cnts.data(), hits.data()));
+    check(wp_simulate_dye_tracks_gpu(&truth, T, C, (int)ds.strings.size(), strs.data(), ds.counts.data(), ds.ids.data(), g,
+                                     a.seed, cap, cap, &n, &ns, counts.data(), off.data(), ids.data(), cnts.data(),
+                                     hits.data()));
     std::ofstream f(a.get('T'));  // write_dye_tracks (dye-tracks-io.cc:75-103)
     f << T << "\n" << C << "\n" << n << "\n";
     for (int i = 0; i < n; i++) {

@@ -630,13 +630,65 @@ int wp_simulate_radiometries(const wp_model* seq_model, unsigned int num_timeste
     return WP_OK;
 }
 
+int wp_simulate_radiometries_gpu(const wp_model* seq_model, unsigned int num_timesteps,
+                                 unsigned int num_channels, int num_dye_seqs,
+                                 const char* const* dye_strings, const int* weights, int num_reads,
+                                 uint64_t seed, double* out_radiometries, int* out_dye_seq_index) {
+    if (!seq_model || num_dye_seqs < 1 || !dye_strings || num_reads < 0 || !out_radiometries ||
+        num_timesteps < 1 || num_timesteps > WP_MAX_TIMESTEPS || num_channels < 1 || num_channels > 4) {
+        wp::set_error("bad simulator arguments");
+        return WP_ERR_ARG;
+    }
+    int rc = wp::require_device();
+    if (rc) return rc;
+    int cap = 0;
+    for (int i = 0; i < num_dye_seqs; i++) cap = std::max(cap, (int)std::strlen(dye_strings[i]));
+    cap = std::max(cap, 1);
+    std::vector<short> seqs((size_t)num_dye_seqs * cap, -1);
+    std::vector<int> lens(num_dye_seqs);
+    for (int i = 0; i < num_dye_seqs; i++) lens[i] = wp::parse_dye_seq(dye_strings[i], &seqs[(size_t)i * cap], cap);
+    wp::DevModel dm;
+    wp::fill_dev_model(*seq_model, (int)num_timesteps, (int)num_channels, 1, DBL_MAX, &dm);
+    return wp::gpu_simulate_radiometries(*seq_model, dm, (int)num_timesteps, (int)num_channels, num_dye_seqs,
+                                         seqs.data(), cap, lens.data(), weights, num_reads, seed, out_radiometries,
+                                         out_dye_seq_index);
+}
+
+int wp_simulate_dye_tracks_gpu(const wp_model* seq_model, unsigned int num_timesteps, unsigned int num_channels,
+                               int num_dye_seqs, const char* const* dye_strings, const int* dye_seq_counts,
+                               const int* dye_seq_ids, int per_peptide, uint64_t seed, int max_tracks,
+                               int max_sources, int* out_num_tracks, int* out_num_sources,
+                               short* out_track_counts, int* out_source_offsets,
+                               int* out_source_ids, int* out_source_counts, int* out_source_hits) {
+    if (!seq_model || num_dye_seqs < 1 || !dye_strings || !dye_seq_counts || !dye_seq_ids || per_peptide < 0 ||
+        !out_num_tracks || !out_num_sources || num_timesteps < 1 || num_timesteps > WP_MAX_TIMESTEPS || num_channels < 1 ||
+        num_channels > WP_MAX_CHANNELS) {
+        wp::set_error("bad simulator arguments");
+        return WP_ERR_ARG;
+    }
+    int rc = wp::require_device();
+    if (rc) return rc;
+    int cap = 0;
+    for (int i = 0; i < num_dye_seqs; i++) cap = std::max(cap, (int)std::strlen(dye_strings[i]));
+    cap = std::max(cap, 1);
+    std::vector<short> seqs((size_t)num_dye_seqs * cap, -1);
+    std::vector<int> lens(num_dye_seqs);
+    for (int i = 0; i < num_dye_seqs; i++) lens[i] = wp::parse_dye_seq(dye_strings[i], &seqs[(size_t)i * cap], cap);
+    return wp::gpu_simulate_dye_tracks(*seq_model, (int)num_timesteps, (int)num_channels, num_dye_seqs, seqs.data(),
+                                       cap, lens.data(), dye_seq_counts, dye_seq_ids, per_peptide, seed, max_tracks,
+                                       max_sources, out_num_tracks, out_num_sources, out_track_counts,
+                                       out_source_offsets, out_source_ids,
+                                       out_source_counts, out_source_hits);
+}
+
 int wp_simulate_dye_tracks(const wp_model* seq_model, unsigned int num_timesteps, unsigned int num_channels,
                            int num_dye_seqs, const char* const* dye_strings, const int* dye_seq_counts,
                            const int* dye_seq_ids, int per_peptide, uint64_t seed, int max_tracks,
-                           int* out_num_tracks, short* out_track_counts, int* out_source_offsets,
+                           int max_sources, int* out_num_tracks, int* out_num_sources,
+                           short* out_track_counts, int* out_source_offsets,
                            int* out_source_ids, int* out_source_counts, int* out_source_hits) {
     if (!seq_model || num_dye_seqs < 1 || !dye_strings || !dye_seq_counts || !dye_seq_ids || per_peptide < 0 ||
-        !out_num_tracks || num_timesteps < 1 || num_timesteps > WP_MAX_TIMESTEPS || num_channels < 1 ||
+        !out_num_tracks || !out_num_sources || num_timesteps < 1 || num_timesteps > WP_MAX_TIMESTEPS || num_channels < 1 ||
         num_channels > WP_MAX_CHANNELS) {
         wp::set_error("bad simulator arguments");
         return WP_ERR_ARG;
@@ -653,8 +705,9 @@ int wp_simulate_dye_tracks(const wp_model* seq_model, unsigned int num_timesteps
                                     cap, lens.data(), dye_seq_counts, dye_seq_ids, per_peptide, seed, &tc, &off, &id,
                                     &cnt, &hits);
     *out_num_tracks = n;
-    if (n > max_tracks) {
-        wp::set_error("more distinct dye-tracks than max_tracks");
+    *out_num_sources = (int)id.size();
+    if (n > max_tracks || (int)id.size() > max_sources) {
+        wp::set_error("more distinct dye-tracks or sources than max_tracks / max_sources");
         return WP_ERR_ARG;
     }
     if (!out_track_counts || !out_source_offsets || !out_source_ids || !out_source_counts || !out_source_hits) {

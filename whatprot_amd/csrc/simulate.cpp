@@ -1,11 +1,16 @@
-// Seeded synthetic-input generator: random dye-seqs and radiometries drawn
-// from the sequencing model's forward process. Restates the generative model
-// of whatprot's simulator (simulation/generate-dye-track.cc:29-110,
-// generate-radiometry.cc:28-62, generate-radiometries.cc:30-64) with a
-// portable counter-free RNG (xoshiro256** seeded by splitmix64) instead of the
-// reference's clock-seeded std::default_random_engine, so runs are
-// reproducible across machines. Streams therefore differ from the reference;
-// the distribution is the same.
+// Seeded synthetic-input generator, host side: random dye-seqs, and dye tracks
+// and radiometries drawn from the sequencing model's forward process.
+// Restates the generative model of whatprot's simulator
+// (simulation/generate-dye-track.cc:29-110, generate-radiometry.cc:28-62,
+// generate-radiometries.cc:30-64) with a portable RNG (xoshiro256** seeded by
+// splitmix64) instead of the reference's clock-seeded
+// std::default_random_engine, so runs are reproducible across machines.
+// Streams therefore differ from the reference; the distribution is the same.
+//
+// Every simulated peptide (dye tracks) or read attempt (radiometries) draws
+// from its own stream g, seeded from seed + (g + 1) * 0xD1B54A32D192ED03, and
+// the detach probability of each timestep comes from one table, so the GPU
+// simulator (simulate_gpu.hip) produces the same tracks bit for bit.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -13,50 +18,21 @@
 #include <map>
 #include <vector>
 
+#include "sim_rng.h"
 #include "wp_internal.h"
 
 namespace wp {
 
 namespace {
 
-struct Rng {
-    uint64_t s[4];
-    explicit Rng(uint64_t seed) {
-        uint64_t x = seed;
-        for (int i = 0; i < 4; i++) {
-            x += 0x9e3779b97f4a7c15ull;
-            uint64_t z = x;
-            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-            s[i] = z ^ (z >> 31);
-        }
-    }
-    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
-    uint64_t next() {
-        uint64_t r = rotl(s[1] * 5, 7) * 9;
-        uint64_t t = s[1] << 17;
-        s[2] ^= s[0];
-        s[3] ^= s[1];
-        s[1] ^= s[2];
-        s[0] ^= s[3];
-        s[2] ^= t;
-        s[3] = rotl(s[3], 45);
-        return r;
-    }
-    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
-    bool bernoulli(double p) { return uniform() < p; }
-    // uniform integer in [lo, hi]
-    int range(int lo, int hi) { return lo + (int)(next() % (uint64_t)(hi - lo + 1)); }
-    double normal() {  // Marsaglia polar method
-        for (;;) {
-            double u = 2.0 * uniform() - 1.0, v = 2.0 * uniform() - 1.0;
-            double s2 = u * u + v * v;
-            if (s2 > 0.0 && s2 < 1.0) return u * std::sqrt(-2.0 * std::log(s2) / s2);
-        }
-    }
-};
+
 
 }  // namespace
+
+// p_detach of each timestep: DecayingRateModel (decaying-rate-model.cc:47-50).
+void detach_table(const wp_model& m, int T, double* pd) {
+    for (int t = 0; t < T; t++) pd[t] = m.p_detach + m.p_initial_detach * std::exp(-(double)t * m.p_initial_detach_decay);
+}
 
 // Dye-seq generator: length ~ U{len_min..len_max}; per-channel dye counts
 // k_c ~ U{k_min..k_max} capped so the total fits the length and redrawn while
@@ -95,7 +71,7 @@ void generate_dye_seqs(int C, int n, int len_min, int len_max, int k_min, int k_
 
 // One dye track following generate-dye-track.cc:29-110 (duds; initial block;
 // per cycle: record, cyclic block, detach, Edman unless blocked, bleach).
-static void dye_track(const wp_model& m, const short* seq0, int len, int T, int C, Rng& rng,
+static void dye_track(const wp_model& m, const double* pdt, const short* seq0, int len, int T, int C, Rng& rng,
                       unsigned int* track) {
     std::vector<short> ds(seq0, seq0 + len);
     for (int i = 0; i < len; i++)
@@ -109,8 +85,7 @@ static void dye_track(const wp_model& m, const short* seq0, int len, int T, int 
         if (e < len) {
             for (int c = 0; c < C; c++) track[t * C + c] = (unsigned int)counts[c];
             if (rng.bernoulli(m.p_cyclic_block)) block = true;
-            double pd = m.p_detach + m.p_initial_detach * std::exp(-(double)t * m.p_initial_detach_decay);
-            if (rng.bernoulli(pd)) {
+            if (rng.bernoulli(pdt[t])) {
                 e = len;
                 continue;
             }
@@ -138,7 +113,9 @@ static void dye_track(const wp_model& m, const short* seq0, int len, int T, int 
 void simulate_radiometries(const wp_model& m, int T, int C, int nds, const short* seqs, int cap,
                            const int* lens, const int* weights, int n, uint64_t seed,
                            double* out, int* out_idx, unsigned int* out_tracks) {
-    Rng rng(seed);
+#pragma clang fp contract(off)
+    std::vector<double> pdt(T);
+    detach_table(m, T, pdt.data());
     std::vector<double> cum(nds);
     double acc = 0.0;
     for (int i = 0; i < nds; i++) {
@@ -147,11 +124,12 @@ void simulate_radiometries(const wp_model& m, int T, int C, int nds, const short
     }
     std::vector<unsigned int> track((size_t)T * C);
     int made = 0;
-    while (made < n) {
+    for (uint64_t attempt = 0; made < n; attempt++) {
+        Rng rng = stream_rng(seed, attempt);
         double u = rng.uniform() * acc;
         int i = (int)(std::upper_bound(cum.begin(), cum.end(), u) - cum.begin());
         if (i >= nds) i = nds - 1;
-        dye_track(m, seqs + (size_t)i * cap, lens[i], T, C, rng, track.data());
+        dye_track(m, pdt.data(), seqs + (size_t)i * cap, lens[i], T, C, rng, track.data());
         bool trivial = true;
         for (int c = 0; c < C; c++)
             if (track[c] != 0) trivial = false;
@@ -181,14 +159,17 @@ int simulate_dye_tracks(const wp_model& m, int T, int C, int nds, const short* s
                         const int* counts, const int* ids, int per_peptide, uint64_t seed,
                         std::vector<short>* track_counts, std::vector<int>* src_off, std::vector<int>* src_id,
                         std::vector<int>* src_count, std::vector<int>* src_hits) {
-    Rng rng(seed);
+    std::vector<double> pdt(T);
+    detach_table(m, T, pdt.data());
     std::map<std::vector<short>, std::map<int, std::pair<int, int>>> table;
     std::vector<unsigned int> track((size_t)T * C);
     std::vector<short> key((size_t)T * C);
+    uint64_t g = 0;  // peptide index: dye-seq, then copy, then repeat
     for (int i = 0; i < nds; i++) {
         for (int copy = 0; copy < counts[i]; copy++) {
-            for (int j = 0; j < per_peptide; j++) {
-                dye_track(m, seqs + (size_t)i * cap, lens[i], T, C, rng, track.data());
+            for (int j = 0; j < per_peptide; j++, g++) {
+                Rng rng = stream_rng(seed, g);
+                dye_track(m, pdt.data(), seqs + (size_t)i * cap, lens[i], T, C, rng, track.data());
                 bool trivial = true;
                 for (int c = 0; c < C; c++)
                     if (track[c] != 0) trivial = false;

@@ -48,6 +48,19 @@ void simulate_radiometries(const wp_model& m, int T, int C, int nds, const short
                            const int* lens, const int* weights, int n, uint64_t seed,
                            double* out, int* out_idx, unsigned int* out_tracks);
 
+// p_detach of each timestep t < T (decaying-rate-model.cc:47-50).
+void detach_table(const wp_model& m, int T, double* pd);
+
+struct DevModel;
+// GPU simulator (simulate_gpu.hip): the host simulator's streams, on the device.
+int gpu_simulate_dye_tracks(const wp_model& m, int T, int C, int nds, const short* seqs, int cap, const int* lens,
+                            const int* counts, const int* ids, int per_peptide, uint64_t seed, int max_tracks,
+                            int max_sources, int* out_num, int* out_num_sources, short* out_tracks, int* out_off,
+                            int* out_id, int* out_count, int* out_hits);
+int gpu_simulate_radiometries(const wp_model& m, const DevModel& dm, int T, int C, int nds, const short* seqs,
+                              int cap, const int* lens, const int* weights, int n, uint64_t seed, double* out,
+                              int* out_idx);
+
 int simulate_dye_tracks(const wp_model& m, int T, int C, int nds, const short* seqs, int cap, const int* lens,
                         const int* counts, const int* ids, int per_peptide, uint64_t seed,
                         std::vector<short>* track_counts, std::vector<int>* src_off, std::vector<int>* src_id,
