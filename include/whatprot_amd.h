@@ -405,6 +405,29 @@ int wp_hmm_fit(wp_hmm_fitter* fitter, const wp_model* seq_model, const wp_fit_se
                double stopping_threshold, double max_runtime, int max_iterations,
                wp_model* out_model, double* out_step_size, double* out_log_l, int* out_iterations);
 
+/* Bootstrap fit (fitters/bootstrap-fit.cc:37-173, as run by run-fit.cc:
+ * 107-130): the fit of wp_hmm_fit on every read (out_best_model,
+ * out_step_size, out_log_l), then num_rounds fits from seq_model on
+ * resamples of the reads drawn with replacement (out_models[i],
+ * out_log_ls[i]: the model of round i and the log-likelihood of its
+ * resample), the largest final step size of a round, and per parameter the
+ * order statistics at (1 - ci) / 2 and 1 - (1 - ci) / 2 of the rounds as the
+ * lower and upper bounds (the other fields as out_best_model; index clamped
+ * to the last round). A resample is evaluated on the device as per-read
+ * multiplicities (wp_fit_bootstrap_counts), so the reads are not copied
+ * again; round i draws from its own seeded stream in place of the
+ * reference's clock-seeded engine. */
+int wp_hmm_fit_bootstrap(wp_hmm_fitter* fitter, const wp_model* seq_model, const wp_fit_settings* settings,
+                         double stopping_threshold, double max_runtime, int num_rounds,
+                         double confidence_interval, uint64_t seed, wp_model* out_models, double* out_log_ls,
+                         wp_model* out_best_model, double* out_step_size, double* out_log_l,
+                         wp_model* out_ci_min, wp_model* out_ci_max, double* out_worst_step);
+
+/* Multiplicity of each of num_reads reads in bootstrap round `round`:
+ * num_reads draws of a uniform index from stream `round` of the seed (the
+ * resampling of bootstrap-fit.cc:76-85). Host only. */
+int wp_fit_bootstrap_counts(int num_reads, uint64_t seed, int round, int* out_counts);
+
 /* least_squares_fit_of_offset_exponential (util/levmar-helper.cc:44-85):
  * weighted fit of y_t ~ p[0] + p[1] exp(-p[2] t) for t = 0..n-1 within
  * [0, 1]^3, hold[i] != 0 pinning p[i]; p is updated in place. Host only. */

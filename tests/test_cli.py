@@ -29,6 +29,8 @@ def test_incorrect_usage(args):
     assert r.returncode == 1 and "INCORRECT USAGE" in r.stdout
 
 
-def test_bootstrap_refused():
-    r = run("fit", "-P", "p", "-x", "..0", "-L", "0.01", "-R", "r", "-b", "10", "-c", "0.9")
-    assert r.returncode == 1 and "bootstrap" in r.stderr
+@pytest.mark.parametrize("extra", [["-b", "10"], ["-c", "0.9"], ["-Y", "out.csv"]])
+def test_bootstrap_flags_need_each_other(extra):
+    """-b and -c come together, and -Y only with them (main.cc:413-446)."""
+    r = run("fit", "-P", "p", "-x", "..0", "-L", "0.01", "-R", "r", *extra)
+    assert r.returncode == 1 and "INCORRECT USAGE" in r.stdout

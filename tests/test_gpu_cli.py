@@ -91,6 +91,25 @@ def test_fit(gpu, ref, files):
     assert "Edman failure rate: " in outp and "Channel 1 info: (Bleach rate: " in outp
 
 
+def test_fit_bootstrap(gpu, files):
+    """fit -b -c -Y: the full fit as without -b, the bounds printed and one
+    write_params row per round (params-io.cc:37-54)."""
+    d, C, T = files["d"], files["C"], 10
+    ds = ".0.1.1.0"
+    rads, _ = w.simulate_radiometries(files["model"], T, [ds], 80, 75)
+    rp, yp = d / "boot_r.tsv", d / "boot.csv"
+    write_radiometries_scaled(rp, rads, MU)
+    plain = run("fit", "-P", files["sp"], "-x", ds, "-L", 1e-3, "-R", rp)
+    outp = run("fit", "-P", files["sp"], "-x", ds, "-L", 1e-3, "-R", rp, "-b", 5, "-c", 0.8, "-Y", yp, "--seed", 4)
+    assert re.search(r"log\(L\): (\S+)", outp).group(1) == re.search(r"log\(L\): (\S+)", plain).group(1)
+    assert "lower bounds: Edman failure rate: " in outp and "upper bounds: " in outp
+    assert "worst bootstrap step size: " in outp
+    lines = yp.read_text().splitlines()
+    assert lines[0] == ("p_edman_failure,p_detach,p_initial_detach,p_initial_detach_decay,p_initial_block,"
+                        "p_cyclic_block,ch0:p_bleach,ch0:p_dud,ch1:p_bleach,ch1:p_dud,log(L)")
+    assert len(lines) == 6 and all(len(x.split(",")) == 11 for x in lines[1:])
+
+
 def test_simulate_rad_and_dt(gpu, tmp_path):
     sp = tmp_path / "sp.json"
     write_seqparams(sp, 2)

@@ -125,3 +125,43 @@ def test_fit_holds_and_iteration_cap(gpu):
         st, _ = f.estep(m)
         m, _ = w.fit_mstep(T, C, ds, m, len(rads), st, fs)
     assert m.channel_models[1].p_bleach == pytest.approx(got.channel_models[1].p_bleach, abs=1e-15)
+
+
+def test_bootstrap_rounds_are_fits_of_resamples(gpu):
+    """Each bootstrap round is wp_hmm_fit on the resample its multiplicities
+    describe (the reads repeated, a fitter of their own); the bounds are the
+    rounds' order statistics (bootstrap-fit.cc:98-167). A multiplicity adds
+    w times a read's statistics where the repeated reads add them one by one,
+    so the sums round differently and the 10-iteration detach-rate fit of
+    every M-step passes that on: run to a tight threshold, the parameters
+    agree to 1e-5 (a resample read wrongly moves them by ~1e-2)."""
+    C, T, ds = 2, 8, ".0.1.0"
+    truth = w.SequencingModel.uniform(C, p_edman_failure=0.08, p_bleach=0.04, p_dud=0.1, p_detach=0.03)
+    rads, _ = w.simulate_radiometries(truth, T, [ds], 120, 31)
+    rads = rads / np.array([cm.mu for cm in truth.channel_models])
+    start = w.SequencingModel.uniform(C)
+    f = w.HMMFitter(T, C, ds, rads, w.WP_PRECISION_FP64)
+    R, ci, seed = 6, 0.8, 17
+    thr = 1e-8  # both EM runs go to the fixed point, so the stop is not decided by the last bits
+    res = f.bootstrap_fit(start, thr, R, ci, seed=seed)
+    full, step, log_l, _ = f.fit(start, thr)
+    assert log_l == pytest.approx(res["log_l"], rel=1e-12)
+    for i in range(R):
+        cnt = w.bootstrap_counts(len(rads), seed, i)
+        assert cnt.sum() == len(rads)
+        sub = np.repeat(rads, cnt, axis=0)
+        got, _, ll, _ = w.HMMFitter(T, C, ds, sub, w.WP_PRECISION_FP64).fit(start, thr)
+        want = res["models"][i]
+        for k in ("p_edman_failure", "p_detach", "p_initial_block", "p_cyclic_block"):
+            assert getattr(got, k) == pytest.approx(getattr(want, k), abs=1e-5), (i, k)
+        for c in range(C):
+            assert got.channel_models[c].p_bleach == pytest.approx(want.channel_models[c].p_bleach, abs=1e-5)
+            assert got.channel_models[c].p_dud == pytest.approx(want.channel_models[c].p_dud, abs=1e-5)
+        assert ll == pytest.approx(res["log_ls"][i], rel=1e-7)
+    lo_i, hi_i = int((1 - ci) / 2 * R), min(int((1 - (1 - ci) / 2) * R), R - 1)
+    for k in ("p_edman_failure", "p_detach", "p_initial_block"):
+        v = sorted(getattr(m, k) for m in res["models"])
+        assert getattr(res["ci_min"], k) == v[lo_i] and getattr(res["ci_max"], k) == v[hi_i]
+    v = sorted(m.channel_models[1].p_bleach for m in res["models"])
+    assert res["ci_min"].channel_models[1].p_bleach == v[lo_i]
+    assert res["ci_max"].channel_models[1].p_bleach == v[hi_i]

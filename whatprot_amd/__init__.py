@@ -500,6 +500,53 @@ class HMMFitter:
                                ctypes.byref(out), ctypes.byref(step), ctypes.byref(log_l), ctypes.byref(it)))
         return SequencingModel.from_c(out), step.value, log_l.value, it.value
 
+    def bootstrap_fit(self, seq_model: SequencingModel, stopping_threshold, num_rounds, confidence_interval,
+                      seed=1, max_runtime=float("inf"), settings: Optional[FitSettings] = None):
+        """bootstrap_fit (fitters/bootstrap-fit.cc:37-173): returns a dict with
+        the full fit ('model', 'step', 'log_l'), the rounds ('models',
+        'log_ls'), 'ci_min', 'ci_max' and 'worst_step'."""
+        m = seq_model.to_c()
+        R = int(num_rounds)
+        models = (ModelC * max(R, 1))()
+        log_ls = np.zeros(max(R, 1))
+        best, lo, hi = ModelC(), ModelC(), ModelC()
+        step, log_l, worst = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        fs = settings.to_c() if settings is not None else None
+        check(lib().wp_hmm_fit_bootstrap(self._h, ctypes.byref(m), ctypes.byref(fs) if fs is not None else None,
+                                         float(stopping_threshold), float(min(max_runtime, 1e300)), R,
+                                         float(confidence_interval), int(seed), ctypes.cast(models, ctypes.c_void_p),
+                                         log_ls.ctypes.data, ctypes.byref(best), ctypes.byref(step),
+                                         ctypes.byref(log_l), ctypes.byref(lo), ctypes.byref(hi),
+                                         ctypes.byref(worst)))
+        return {"model": SequencingModel.from_c(best), "step": step.value, "log_l": log_l.value,
+                "models": [SequencingModel.from_c(models[i]) for i in range(R)], "log_ls": log_ls[:R].copy(),
+                "ci_min": SequencingModel.from_c(lo), "ci_max": SequencingModel.from_c(hi),
+                "worst_step": worst.value}
+
+
+def bootstrap_counts(num_reads, seed, round_index):
+    """wp_fit_bootstrap_counts: multiplicity of each read in a bootstrap round."""
+    out = np.zeros(max(int(num_reads), 1), dtype=np.int32)
+    check(lib().wp_fit_bootstrap_counts(int(num_reads), int(seed), int(round_index), out.ctypes.data))
+    return out[:int(num_reads)]
+
+
+def write_params(path, num_channels, models: Sequence[SequencingModel], log_ls):
+    """write_params (io/params-io.cc:37-54): one CSV row per bootstrap round,
+    17 significant digits."""
+    with open(path, "w") as f:
+        f.write("p_edman_failure,p_detach,p_initial_detach,p_initial_detach_decay,p_initial_block,p_cyclic_block")
+        for c in range(num_channels):
+            f.write(f",ch{c}:p_bleach,ch{c}:p_dud")
+        f.write(",log(L)\n")
+        for m, ll in zip(models, log_ls):
+            vals = [m.p_edman_failure, m.p_detach, m.p_initial_detach, m.p_initial_detach_decay,
+                    m.p_initial_block, m.p_cyclic_block]
+            for c in range(num_channels):
+                vals += [m.channel_models[c].p_bleach, m.channel_models[c].p_dud]
+            vals.append(ll)
+            f.write(",".join(f"{v:.17g}" for v in vals) + "\n")
+
 
 @dataclass
 class DyeTracks:

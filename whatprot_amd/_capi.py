@@ -125,6 +125,12 @@ def lib():
     L.wp_fit_mstep.restype = c_int
     L.wp_fit_mstep.argtypes = [c_uint, c_uint, ctypes.c_char_p, P(ModelC), c_int, c_void_p,
                                P(FitSettingsC), P(ModelC), P(c_double)]
+    L.wp_hmm_fit_bootstrap.restype = c_int
+    L.wp_hmm_fit_bootstrap.argtypes = [c_void_p, P(ModelC), P(FitSettingsC), c_double, c_double, c_int, c_double,
+                                       ctypes.c_uint64, c_void_p, c_void_p, P(ModelC), P(c_double), P(c_double),
+                                       P(ModelC), P(ModelC), P(c_double)]
+    L.wp_fit_bootstrap_counts.restype = c_int
+    L.wp_fit_bootstrap_counts.argtypes = [c_int, ctypes.c_uint64, c_int, c_void_p]
     L.wp_hmm_fit.restype = c_int
     L.wp_hmm_fit.argtypes = [c_void_p, P(ModelC), P(FitSettingsC), c_double, c_double, c_int, P(ModelC),
                              P(c_double), P(c_double), P(c_int)]

@@ -5,14 +5,15 @@
 //   whatprot-amd classify nn     -P seqparams.json -k K -s SIG -T dyetracks.tsv -R rads.tsv -Y out.csv
 //   whatprot-amd classify hybrid -P ... -k K -s SIG -H H -S ... -T ... -R ... -Y ... [-p cutoff]
 //   whatprot-amd fit -P seqparams.json -x DYESEQ -L THRESHOLD -R rads.tsv [-F fit.json] [-M minutes]
+//                    [-b ROUNDS -c CI [-Y bootstrap.csv]]
 //   whatprot-amd simulate rad -P ... -t T -g N -S dyeseqs.tsv -R rads.tsv -Y true_ids.tsv
 //   whatprot-amd simulate dt  -P ... -t T -g N -S dyeseqs.tsv -T dyetracks.tsv
 //
 // Flags keep the reference's short and long names and required sets. Files
 // are read and written in the reference's formats (io/*.cc). Additions:
 // --precision fp64|fp32 for the HMM paths (default fp64, the reference's
-// arithmetic) and --seed for simulation (the reference seeds from the clock).
-// Bootstrap fitting (-b, -c) is not part of this path and is refused.
+// arithmetic) and --seed for simulation and bootstrap resampling (the
+// reference seeds from the clock).
 #include <chrono>
 #include <cctype>
 #include <cfloat>
@@ -353,6 +354,7 @@ const char* kUsage =
     "                  -R radiometries -Y results [-p hmmprune]\n"
     "  fit             -P seqparams -x dyeseqstring -L stoppingthreshold -R radiometries\n"
     "                  [-F fitsettings] [-M maxruntime (minutes)]\n"
+    "                  [-b numbootstrap -c confidenceinterval [-Y results]]\n"
     "  simulate rad    -P seqparams -t timesteps -g numgenerate -S dyeseqs -R radiometries -Y results\n"
     "  simulate dt     -P seqparams -t timesteps -g numgenerate -S dyeseqs -T dyetracks\n"
     "  extras: --precision fp64|fp32 (HMM arithmetic), --seed N (simulation)\n";
@@ -539,8 +541,36 @@ int fit(const Args& a) {
     double step = 0.0, log_l = 0.0;
     int iters = 0;
     double t1 = now();
-    check(wp_hmm_fit(f, &model, &fs, std::stod(a.get('L')), minutes * 60.0, 0, &out, &step, &log_l, &iters));
-    std::cout << "Finished parameter fitting (" << now() - t1 << " seconds, " << iters << " iterations).\n";
+    const double ci = a.has('c') ? std::stod(a.get('c')) : 0.0;
+    if (ci == 0.0) {  // run-fit.cc:88-99
+        check(wp_hmm_fit(f, &model, &fs, std::stod(a.get('L')), minutes * 60.0, 0, &out, &step, &log_l, &iters));
+        std::cout << "Finished parameter fitting (" << now() - t1 << " seconds, " << iters << " iterations).\n";
+    } else {  // run-fit.cc:100-130 and bootstrap-fit.cc:37-173
+        const int b = std::stoi(a.get('b'));
+        std::vector<wp_model> models(std::max(b, 1));
+        std::vector<double> log_ls(std::max(b, 1));
+        wp_model lo, hi;
+        double worst = 0.0;
+        check(wp_hmm_fit_bootstrap(f, &model, &fs, std::stod(a.get('L')), minutes * 60.0, b, ci, a.seed,
+                                   models.data(), log_ls.data(), &out, &step, &log_l, &lo, &hi, &worst));
+        std::cout << "lower bounds: " << debug_string(lo) << "\n";
+        std::cout << "upper bounds: " << debug_string(hi) << "\n";
+        std::cout << "worst bootstrap step size: " << worst << "\n";
+        std::cout << "Finished parameter fitting (" << now() - t1 << " seconds).\n";
+        if (a.has('Y')) {  // write_params (params-io.cc:37-54)
+            std::ofstream y(a.get('Y'));
+            y << "p_edman_failure,p_detach,p_initial_detach,p_initial_detach_decay,p_initial_block,p_cyclic_block";
+            for (int c = 0; c < C; c++) y << ",ch" << c << ":p_bleach,ch" << c << ":p_dud";
+            y << ",log(L)\n" << std::setprecision(17);
+            for (int i = 0; i < b; i++) {
+                const wp_model& m = models[i];
+                y << m.p_edman_failure << "," << m.p_detach << "," << m.p_initial_detach << ","
+                  << m.p_initial_detach_decay << "," << m.p_initial_block << "," << m.p_cyclic_block;
+                for (int c = 0; c < C; c++) y << "," << m.channels[c].p_bleach << "," << m.channels[c].p_dud;
+                y << "," << log_ls[i] << "\n";
+            }
+        }
+    }
     wp_hmm_fitter_destroy(f);
     // print_parameter_results and print_final_step_size (cmd-line-out.cc)
     std::cout << "Parameters:\n" << debug_string(out) << "\n\n";
@@ -618,12 +648,9 @@ int main(int argc, char** argv) {
             if (v == "hybrid" && needs(a, "PksHSTRY", "p")) return classify_hybrid(a);
             return usage();
         }
-        if (mode == "fit" && a.pos.size() == 1) {
-            if (a.has('b') || a.has('c')) {
-                std::cerr << "whatprot-amd: bootstrap fitting (-b, -c) is not supported\n";
-                return 1;
-            }
-            if (needs(a, "PLxR", "FM")) return fit(a);
+        if (mode == "fit" && a.pos.size() == 1) {  // main.cc:413-446: -b and -c both or neither
+            if (a.has('b') != a.has('c')) return usage();
+            if (a.has('b') ? needs(a, "PLxRbc", "FMY") : needs(a, "PLxR", "FM")) return fit(a);
             return usage();
         }
         if (mode == "simulate" && a.pos.size() == 2) {

@@ -13,6 +13,7 @@
 
 #include "fit_kernels.h"
 #include "model_tables.h"
+#include "sim_rng.h"
 #include "wp_internal.h"
 
 struct wp_hmm_fitter {
@@ -30,6 +31,7 @@ struct wp_hmm_fitter {
     size_t hist_stride = 0;
     double* d_partial = nullptr;
     double* d_out = nullptr;
+    int* d_weight = nullptr;  // bootstrap multiplicities [N]
 };
 
 namespace {
@@ -44,6 +46,7 @@ void free_fitter(wp_hmm_fitter* f) {
     hipFree(f->d_hist);
     hipFree(f->d_partial);
     hipFree(f->d_out);
+    hipFree(f->d_weight);
     delete f;
 }
 
@@ -98,7 +101,7 @@ double model_distance(const wp_model& a, const wp_model& b, int C) {
     return d;
 }
 
-int estep(wp_hmm_fitter* f, const wp_model* model, double* stats, double* sum_log_p) {
+int estep(wp_hmm_fitter* f, const wp_model* model, double* stats, double* sum_log_p, const int* d_weight = nullptr) {
     if (!model || model->num_channels != f->C) {
         wp::set_error("model channels != fitter channels");
         return WP_ERR_ARG;
@@ -124,6 +127,7 @@ int estep(wp_hmm_fitter* f, const wp_model* model, double* stats, double* sum_lo
     a.hist_stride = f->hist_stride;
     a.partial = f->d_partial;
     a.L = f->L;
+    a.weight = d_weight;
     std::vector<double> out((size_t)f->L + 1, 0.0);
     if (f->N > 0) {
         WP_HIP(wp::launch_fit_estep(a, f->precision, f->grid, f->d_out, nullptr));
@@ -330,13 +334,15 @@ int wp_hmm_fitter_estep(wp_hmm_fitter* f, const wp_model* model, double* out_sta
     return estep(f, model, out_stats, out_sum_log_p);
 }
 
-int wp_hmm_fit(wp_hmm_fitter* f, const wp_model* model, const wp_fit_settings* settings,
-               double stopping_threshold, double max_runtime, int max_iterations, wp_model* out_model,
-               double* out_step_size, double* out_log_l, int* out_iterations) {
-    if (!f || !model || !out_model) {
-        wp::set_error("null argument");
-        return WP_ERR_ARG;
-    }
+}  // extern "C"
+
+namespace {
+
+// The EM loop of HMMFitter::fit (hmm-fitter.h:55-113) over the fitter's reads,
+// each counted d_weight[r] times (null: once).
+int run_em(wp_hmm_fitter* f, const wp_model* model, const wp_fit_settings* settings, double stopping_threshold,
+           double max_runtime, int max_iterations, const int* d_weight, wp_model* out_model, double* out_step_size,
+           double* out_log_l, int* out_iterations) {
     std::string ds;
     for (int i = 0; i < f->len; i++) ds += f->seq[i] < 0 ? '.' : (char)('0' + f->seq[i]);
     std::vector<double> st((size_t)f->L);
@@ -344,8 +350,8 @@ int wp_hmm_fit(wp_hmm_fitter* f, const wp_model* model, const wp_fit_settings* s
     double step = 0.0;
     int it = 0;
     auto start = std::chrono::steady_clock::now();
-    while (true) {  // hmm-fitter.h:55-113
-        int rc = estep(f, &sm, st.data(), nullptr);
+    while (true) {
+        int rc = estep(f, &sm, st.data(), nullptr, d_weight);
         if (rc) return rc;
         rc = wp_fit_mstep(f->T, f->C, ds.c_str(), &sm, f->N, st.data(), settings, &next, &step);
         if (rc) return rc;
@@ -359,9 +365,100 @@ int wp_hmm_fit(wp_hmm_fitter* f, const wp_model* model, const wp_fit_settings* s
     if (out_step_size) *out_step_size = step;
     if (out_iterations) *out_iterations = it;
     double log_l = 0.0;  // HMMFitter::log_likelihood under the fitted model
-    int rc = estep(f, &next, nullptr, &log_l);
+    int rc = estep(f, &next, nullptr, &log_l, d_weight);
     if (rc) return rc;
     if (out_log_l) *out_log_l = log_l;
+    return WP_OK;
+}
+
+// Order statistic i of one parameter over the rounds' models.
+double order_stat(std::vector<double> v, size_t i) {
+    std::sort(v.begin(), v.end());
+    return v[std::min(i, v.size() - 1)];
+}
+
+}  // namespace
+
+extern "C" {
+
+int wp_hmm_fit(wp_hmm_fitter* f, const wp_model* model, const wp_fit_settings* settings,
+               double stopping_threshold, double max_runtime, int max_iterations, wp_model* out_model,
+               double* out_step_size, double* out_log_l, int* out_iterations) {
+    if (!f || !model || !out_model) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    return run_em(f, model, settings, stopping_threshold, max_runtime, max_iterations, nullptr, out_model,
+                  out_step_size, out_log_l, out_iterations);
+}
+
+int wp_fit_bootstrap_counts(int num_reads, uint64_t seed, int round, int* out_counts) {
+    if (num_reads < 0 || round < 0 || (num_reads > 0 && !out_counts)) {
+        wp::set_error("bad bootstrap arguments");
+        return WP_ERR_ARG;
+    }
+    std::fill(out_counts, out_counts + num_reads, 0);
+    wp::Rng rng = wp::stream_rng(seed, (uint64_t)round);
+    for (int j = 0; j < num_reads; j++) out_counts[rng.next() % (uint64_t)num_reads]++;
+    return WP_OK;
+}
+
+int wp_hmm_fit_bootstrap(wp_hmm_fitter* f, const wp_model* model, const wp_fit_settings* settings,
+                         double stopping_threshold, double max_runtime, int num_rounds,
+                         double confidence_interval, uint64_t seed, wp_model* out_models, double* out_log_ls,
+                         wp_model* out_best_model, double* out_step_size, double* out_log_l, wp_model* out_ci_min,
+                         wp_model* out_ci_max, double* out_worst_step) {
+    if (!f || !model || !out_best_model || num_rounds < 0 || (num_rounds > 0 && (!out_models || !out_log_ls)) ||
+        !(confidence_interval >= 0.0 && confidence_interval <= 1.0)) {
+        wp::set_error("bad bootstrap arguments");
+        return WP_ERR_ARG;
+    }
+    // the fit on every read (bootstrap-fit.cc:72-75)
+    int rc = run_em(f, model, settings, stopping_threshold, max_runtime, 0, nullptr, out_best_model, out_step_size,
+                    out_log_l, nullptr);
+    if (rc) return rc;
+    if (num_rounds == 0) return WP_OK;
+    if (!f->d_weight) {
+        WP_HIP(hipMalloc(&f->d_weight, sizeof(int) * std::max(f->N, 1)));
+    }
+    std::vector<int> w(std::max(f->N, 1));
+    double worst = 0.0;
+    for (int i = 0; i < num_rounds; i++) {  // bootstrap-fit.cc:76-93: a resample of N reads with replacement
+        wp_fit_bootstrap_counts(f->N, seed, i, w.data());
+        WP_HIP(hipMemcpy(f->d_weight, w.data(), sizeof(int) * f->N, hipMemcpyHostToDevice));
+        double step = 0.0;
+        rc = run_em(f, model, settings, stopping_threshold, max_runtime, 0, f->d_weight, &out_models[i], &step,
+                    &out_log_ls[i], nullptr);
+        if (rc) return rc;
+        worst = std::max(worst, step);
+    }
+    if (out_worst_step) *out_worst_step = worst;
+    // confidence bounds: per parameter, the order statistics at (1 - ci) / 2
+    // and 1 - (1 - ci) / 2 of the rounds (bootstrap-fit.cc:98-167)
+    const size_t lo = (size_t)((1.0 - confidence_interval) / 2.0 * (double)num_rounds);
+    const size_t hi = (size_t)((1.0 - (1.0 - confidence_interval) / 2.0) * (double)num_rounds);
+    wp_model mn = *out_best_model, mx = *out_best_model;
+    auto bound = [&](auto get, auto set) {
+        std::vector<double> v(num_rounds);
+        for (int i = 0; i < num_rounds; i++) v[i] = get(out_models[i]);
+        set(mn, order_stat(v, lo));
+        set(mx, order_stat(v, hi));
+    };
+    bound([](const wp_model& m) { return m.p_edman_failure; }, [](wp_model& m, double x) { m.p_edman_failure = x; });
+    bound([](const wp_model& m) { return m.p_detach; }, [](wp_model& m, double x) { m.p_detach = x; });
+    bound([](const wp_model& m) { return m.p_initial_detach; }, [](wp_model& m, double x) { m.p_initial_detach = x; });
+    bound([](const wp_model& m) { return m.p_initial_detach_decay; },
+          [](wp_model& m, double x) { m.p_initial_detach_decay = x; });
+    bound([](const wp_model& m) { return m.p_initial_block; }, [](wp_model& m, double x) { m.p_initial_block = x; });
+    bound([](const wp_model& m) { return m.p_cyclic_block; }, [](wp_model& m, double x) { m.p_cyclic_block = x; });
+    for (int c = 0; c < f->C; c++) {
+        bound([c](const wp_model& m) { return m.channels[c].p_bleach; },
+              [c](wp_model& m, double x) { m.channels[c].p_bleach = x; });
+        bound([c](const wp_model& m) { return m.channels[c].p_dud; },
+              [c](wp_model& m, double x) { m.channels[c].p_dud = x; });
+    }
+    if (out_ci_min) *out_ci_min = mn;
+    if (out_ci_max) *out_ci_max = mx;
     return WP_OK;
 }
 

@@ -26,6 +26,7 @@ struct FitArgs {
     size_t hist_stride;           // elements per workgroup
     double* partial;              // [grid][L + 1] per-workgroup totals
     int L;                        // WP_FIT_STATS_LEN(T, C)
+    const int* weight;            // [N] multiplicity of each read (bootstrap resamples), or null = 1
 };
 
 // Dynamic LDS of one workgroup ((size_t)-1 for an unsupported C).

@@ -287,6 +287,10 @@ __global__ void __launch_bounds__(kThreads) fit_estep_kernel(const FitArgs a) {
     };
 
     for (int r = blockIdx.x; r < a.N; r += gridDim.x) {
+        // A read drawn w times adds w times its statistics and ln P; reads
+        // not drawn are skipped (uniform over the workgroup).
+        const int wr = a.weight ? a.weight[r] : 1;
+        if (wr == 0) continue;
         // ---- emission table of this read (peptide-emission.cc:165-181)
         const double* rad = a.rads + (size_t)r * T * C;
         for (int i = tid; i < (int)S; i += kThreads) {
@@ -397,7 +401,7 @@ __global__ void __launch_bounds__(kThreads) fit_estep_kernel(const FitArgs a) {
             }
         }
         if (zero) {  // P == 0: no statistics (generic-hmm.h:77-87)
-            if (tid == 0) sum_logp += -INFINITY;
+            if (tid == 0) sum_logp += -INFINITY;  // any positive multiplicity
             __syncthreads();
             continue;
         }
@@ -406,7 +410,7 @@ __global__ void __launch_bounds__(kThreads) fit_estep_kernel(const FitArgs a) {
         block_sum2(tot, unused, L.red);
         tot += (double)dl;
         const double lp2 = (double)K + log2(tot);
-        if (tid == 0) sum_logp += lp2 * 0.69314718055994530942;
+        if (tid == 0) sum_logp += (double)wr * (lp2 * 0.69314718055994530942);
 
         // ---- backward pass with the statistics
         for (int i = tid; i < (int)S; i += kThreads) {
@@ -420,7 +424,7 @@ __global__ void __launch_bounds__(kThreads) fit_estep_kernel(const FitArgs a) {
             int k = emit(t, t + 1, bA, bB, bd);
             J += k;
             if (t >= 1) {
-                const double F = exp2((double)(L.Kt[t - 1] + J) - lp2);
+                const double F = (double)wr * exp2((double)(L.Kt[t - 1] + J) - lp2);
                 const Real pd = (Real)m.p_detach[t - 1];
                 const Real* hA = hist_at(t - 1, 0);
                 const Real* hB = hist_at(t - 1, 1);
@@ -498,7 +502,7 @@ __global__ void __launch_bounds__(kThreads) fit_estep_kernel(const FitArgs a) {
                     __syncthreads();
                 }
             } else {
-                const double F = exp2((double)J - lp2);
+                const double F = (double)wr * exp2((double)J - lp2);
                 for (int i = tid; i < NB; i += kThreads) {  // Y0 = initial block(start)
                     YA(0)[i] = i == NB - 1 ? (1 - pib) : (Real)0;
                     YB(0)[i] = i == NB - 1 ? pib : (Real)0;
