@@ -1,5 +1,9 @@
 """Builds libwhatprot_amd.so for gfx950 with hipcc (in tree, no JIT cache) and
-the whatprot-amd command line over its C ABI."""
+the whatprot-amd command line over its C ABI.
+
+Every source compiles to its own object in parallel (each .hip file carries its
+own gfx950 code object), then one hipcc line links the shared library."""
+import concurrent.futures
 import glob
 import os
 import subprocess
@@ -7,6 +11,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libwhatprot_amd.so")
 CLI_OUT = os.path.join(HERE, "whatprot-amd")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
@@ -17,7 +22,6 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # packed FP32 instructions stay off (their forms are unreliable on this pool)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
@@ -26,9 +30,33 @@ FLAGS = [
 ]
 
 
+def _jobs():
+    try:
+        n = int(os.environ.get("MAX_JOBS", "0"))
+    except ValueError:
+        n = 0
+    return max(1, min(16, n or os.cpu_count() or 4))
+
+
+def _compile(src, verbose):
+    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    cmd = [HIPCC] + FLAGS + ["-c", "-o", obj, src]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.stderr:
+        print(r.stderr, file=sys.stderr, end="", flush=True)
+    if r.returncode != 0:
+        raise subprocess.CalledProcessError(r.returncode, cmd)
+    return obj
+
+
 def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
-    cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp"] + srcs
+    with concurrent.futures.ThreadPoolExecutor(_jobs()) as ex:
+        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp"] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
