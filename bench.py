@@ -174,6 +174,7 @@ def main():
     classify_ms = float(np.mean(cls_ms))
     log(f"ms/step {ms_per_step:.3f}  prep {np.mean(prep_ms):.3f} ms  classify {classify_ms:.3f} ms")
 
+    variants, tune_ms = clf.variants()
     pairs_nonempty, states, flops = clf.work(rads)
     achieved = flops / (classify_ms * 1e-3) / 1e12
 
@@ -195,6 +196,9 @@ def main():
                    "precision": args.precision},
         "reads_per_s": N / (ms_per_step * 1e-3),
         "wall_s": wall,
+        "launch": {"graph_replay": True, "per_bin_read_lists": True,
+                   "quad_rows_variant_per_bin": [int(v) for v in variants],
+                   "autotune_ms_per_bin": [[round(float(a), 3), round(float(b), 3)] for a, b in tune_ms]},
         "nonempty_pairs": pairs_nonempty,
         "state_timesteps": states,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

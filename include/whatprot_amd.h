@@ -197,6 +197,25 @@ int wp_hmm_pair_log_probability(wp_hmm_classifier* clf,
 int wp_hmm_set_timing(wp_hmm_classifier* clf, int enable);
 int wp_hmm_last_timing(wp_hmm_classifier* clf, double* prep_ms, double* classify_ms);
 
+/* Kernel selection (no reference counterpart). FP32 reads with count boxes of
+ * at most 16 states run the quad-rows kernel in one of two variants with
+ * identical outputs: 0 computes every timestep at the bin's box capacity,
+ * 1 at each timestep's own box class. mode -1 (the default) times both on the
+ * first call's reads and keeps the faster per bin; 0 or 1 forces one. Setting
+ * the mode discards an earlier choice. */
+int wp_hmm_set_variant(wp_hmm_classifier* clf, int mode);
+
+/* The per-bin choice (0/1, or -1 before the first call) and, when the first
+ * call timed them, each variant's best time in ms ([bin][2], -1 if untimed).
+ * Writes up to cap bins; returns the number of bins of this classifier. */
+int wp_hmm_get_variants(const wp_hmm_classifier* clf, int* out_variant, double* out_ms, int cap);
+
+/* Graph replay (no reference counterpart). The *_device entry points capture
+ * their launch sequence as a HIP graph and replay it while the number of
+ * reads, the device pointers and the kernel choice repeat (default on);
+ * enable 0 launches every kernel eagerly. */
+int wp_hmm_set_graph(wp_hmm_classifier* clf, int enable);
+
 /* Algorithmic work of classifying these reads against every dye-seq (no
  * reference counterpart; used for roofline accounting): out3[0] = non-empty
  * pairs, out3[1] = pruned state-timesteps sum_t (t+1) prod_c (H-L),

@@ -3,6 +3,14 @@ import sys
 
 import pytest
 
+try:
+    # PyTorch-ROCm bundles its own HIP runtime. When both are used in one
+    # process, torch's must be loaded before libwhatprot_amd.so initialises
+    # the system one, or torch finds no GPU afterwards.
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

@@ -360,6 +360,25 @@ class HMMClassifier:
         check(lib().wp_hmm_last_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def set_variant(self, mode):
+        """Quad-rows kernel variant: -1 autotune on the next call (default),
+        0 bin-wide boxes, 1 per-timestep box classes (identical outputs)."""
+        check(lib().wp_hmm_set_variant(self._h, int(mode)))
+
+    def variants(self):
+        """(per-bin variant chosen, or -1 before the first call; per-bin
+        [variant 0 ms, variant 1 ms] from the autotuner, -1 where untimed)."""
+        n = lib().wp_hmm_get_variants(self._h, None, None, 0)
+        check(min(n, 0))
+        v = np.zeros(max(n, 1), dtype=np.int32)
+        ms = np.zeros((max(n, 1), 2), dtype=np.float64)
+        check(min(lib().wp_hmm_get_variants(self._h, v.ctypes.data, ms.ctypes.data, n), 0))
+        return v[:n], ms[:n]
+
+    def set_graph(self, enable=True):
+        """Graph replay of the device entry points' launch sequence."""
+        check(lib().wp_hmm_set_graph(self._h, 1 if enable else 0))
+
     def work(self, radiometries):
         """(non-empty pairs, state-timesteps, algorithmic FLOPs) of classifying these reads."""
         r = self._rads(radiometries)

@@ -108,6 +108,12 @@ def lib():
     L.wp_hmm_set_timing.argtypes = [c_void_p, c_int]
     L.wp_hmm_last_timing.restype = c_int
     L.wp_hmm_last_timing.argtypes = [c_void_p, P(c_double), P(c_double)]
+    L.wp_hmm_set_variant.restype = c_int
+    L.wp_hmm_set_variant.argtypes = [c_void_p, c_int]
+    L.wp_hmm_get_variants.restype = c_int
+    L.wp_hmm_get_variants.argtypes = [c_void_p, c_void_p, c_void_p, c_int]
+    L.wp_hmm_set_graph.restype = c_int
+    L.wp_hmm_set_graph.argtypes = [c_void_p, c_int]
     L.wp_hmm_work.restype = c_int
     L.wp_hmm_work.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
     L.wp_generate_dye_seqs.restype = c_int

@@ -97,8 +97,24 @@ struct wp_hmm_classifier {
     uint8_t* wH = nullptr;
     int* wNb = nullptr;
     uint8_t* wWide = nullptr;  // per read: box densities too wide for the quad-rows kernel
+    int* wLists = nullptr;      // [kMaxBins][N] reads of each bin
+    int* wWideLists = nullptr;  // [kMaxBins][N] reads each quad-rows bin hands to its wide pass
+    int* wCtr = nullptr;        // counters: bin sizes, wide-list sizes, one queue per launch
     int wcap = 0;
     int maxbox = 1;  // prod_c (M+1), bound on any read's box
+    // quad-rows kernel variant per bin: chosen by timing both on the first
+    // call (variant_mode -1) or forced (0 bin-wide boxes, 1 per-timestep classes)
+    int variant_mode = -1;
+    int tuned = 0;
+    int variant[wp::kMaxBins] = {0};
+    float tune_ms[wp::kMaxBins][2] = {{-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f},
+                                       {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}};
+    // the device entry points replay a captured graph of the launch sequence
+    // while the call's shape and pointers repeat
+    int use_graph = 1;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<const void*> gkey;
     // optional kernel timing: events around prep and classify launches
     int timing = 0;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -167,49 +183,69 @@ size_t bin_lds_bytes(const wp_hmm_classifier* c, const Bin& b) {
     }
 }
 
+// Counter block of one call: bin sizes, then wide-list sizes.
+constexpr int kCtrCount = 0;
+constexpr int kCtrWide = wp::kMaxBins;
+constexpr int kCtrLen = 2 * wp::kMaxBins;
+
+void drop_graph(wp_hmm_classifier* c) {
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gkey.clear();
+}
+
 int ensure_workspace(wp_hmm_classifier* c, int N) {
     if (N <= c->wcap) return WP_OK;
+    drop_graph(c);  // the captured launches hold the old workspace
     hipFree(c->wL);
     hipFree(c->wH);
     hipFree(c->wNb);
     hipFree(c->wWide);
+    hipFree(c->wLists);
+    hipFree(c->wWideLists);
+    hipFree(c->wCtr);
     c->wL = c->wH = nullptr;
     c->wNb = nullptr;
     c->wWide = nullptr;
+    c->wLists = c->wWideLists = c->wCtr = nullptr;
     c->wcap = 0;
     size_t tc = (size_t)N * c->T * c->C;
     WP_HIP(hipMalloc(&c->wL, tc));
     WP_HIP(hipMalloc(&c->wH, tc));
     WP_HIP(hipMalloc(&c->wNb, sizeof(int) * (size_t)N));
     WP_HIP(hipMalloc(&c->wWide, (size_t)N));
+    WP_HIP(hipMalloc(&c->wLists, sizeof(int) * (size_t)wp::kMaxBins * N));
+    WP_HIP(hipMalloc(&c->wWideLists, sizeof(int) * (size_t)wp::kMaxBins * N));
+    WP_HIP(hipMalloc(&c->wCtr, sizeof(int) * kCtrLen));
     c->wcap = N;
     return WP_OK;
 }
 
-int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_cand_off,
-               const int* d_cand_idx, int* d_id, double* d_adj, double* d_ls, double* d_lt,
-               double* d_pair, hipStream_t stream) {
-    if (N < 0) {
-        wp::set_error("num_reads < 0");
-        return WP_ERR_ARG;
-    }
-    if (N == 0) return WP_OK;
-    int rc = ensure_workspace(c, N);
-    if (rc) return rc;
-    if (c->timing) WP_HIP(hipEventRecord(c->ev[0], stream));
-    wp::launch_read_prep(c->dm, N, d_rads, c->wL, c->wH, c->wNb, stream);
-    WP_HIP(hipGetLastError());
-    if (c->timing) WP_HIP(hipEventRecord(c->ev[1], stream));
+struct CallArgs {
+    int N;
+    const double* d_rads;
+    const int* d_cand_off;
+    const int* d_cand_idx;
+    int* d_id;
+    double *d_adj, *d_ls, *d_lt, *d_pair;
+};
+
+wp::ClassifyArgs base_args(const wp_hmm_classifier* c, const CallArgs& x) {
     wp::ClassifyArgs a;
     a.m = c->dm;
-    a.N = N;
+    a.N = x.N;
     a.S = c->S;
-    a.rads = d_rads;
+    a.rads = x.d_rads;
     a.Lr = c->wL;
     a.Hr = c->wH;
     a.nbmax = c->wNb;
     a.wide = c->wWide;
     a.wide_pass = 0;
+    a.list = nullptr;
+    a.list_count = nullptr;
+    a.wide_list = nullptr;
+    a.wide_count = nullptr;
+    a.variant = 0;
     a.dsD = c->dD;
     a.dsSeq = c->dSeq;
     a.dsTrack = c->dTrack;
@@ -217,38 +253,176 @@ int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_c
     a.dsId = c->dId;
     a.binom = c->dBinom;
     a.tri = c->tri;
-    a.cand_off = d_cand_off;
-    a.cand_idx = d_cand_idx;
-    a.out_id = d_id;
-    a.out_adjusted = d_adj;
-    a.out_log_score = d_ls;
-    a.out_log_total = d_lt;
-    a.out_pair_logp = d_pair;
-    for (const Bin& b : make_bins(c)) {
-        a.nb_lo = b.lo;
-        a.nb_hi = b.hi;
-        a.NB = b.hi;
-        a.ngroups = b.ngroups;
-        if (b.kernel == kRowPerLane) WP_HIP(wp::launch_classify_v3(a, c->precision, stream));
-        else if (b.kernel == kQuadRows) {
-            // reads whose densities span too wide a range are flagged by the
-            // quad-rows kernel and scored by the per-entry exponent kernels
-            WP_HIP(wp::launch_classify_v5(a, stream));
-            a.wide_pass = 1;
-            if (wp::v3_supports(c->precision, c->T, b.hi) &&
-                wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi) <= lds_hard_limit())
-                WP_HIP(wp::launch_classify_v3(a, c->precision, stream));
-            else
-                WP_HIP(wp::launch_classify(a, c->precision, c->G, stream));
-            a.wide_pass = 0;
-        } else {
-            WP_HIP(wp::launch_classify(a, c->precision, c->G, stream));
-        }
+    a.cand_off = x.d_cand_off;
+    a.cand_idx = x.d_cand_idx;
+    a.out_id = x.d_id;
+    a.out_adjusted = x.d_adj;
+    a.out_log_score = x.d_ls;
+    a.out_log_total = x.d_lt;
+    a.out_pair_logp = x.d_pair;
+    return a;
+}
+
+// Counters cleared, emission ranges and boxes, reads appended to their bins.
+void enqueue_prep(wp_hmm_classifier* c, const std::vector<Bin>& bins, const CallArgs& x, hipStream_t stream) {
+    wp::launch_zero_ints(c->wCtr, kCtrLen, stream);
+    wp::launch_read_prep(c->dm, x.N, x.d_rads, c->wL, c->wH, c->wNb, stream);
+    wp::BinEdges e;
+    e.nbins = (int)bins.size();
+    e.edge[0] = 0;
+    for (size_t b = 0; b < bins.size(); b++) e.edge[b + 1] = bins[b].hi;
+    wp::launch_bin_reads(x.N, c->wNb, e, c->wLists, c->wCtr + kCtrCount, stream);
+}
+
+// The main launch of bin b (for a quad-rows bin, the given variant).
+hipError_t launch_bin_main(wp_hmm_classifier* c, const std::vector<Bin>& bins, int b, const CallArgs& x,
+                           int variant, hipStream_t stream) {
+    const Bin& bin = bins[b];
+    wp::ClassifyArgs a = base_args(c, x);
+    a.nb_lo = bin.lo;
+    a.nb_hi = bin.hi;
+    a.NB = bin.hi;
+    a.ngroups = bin.ngroups;
+    a.list = c->wLists + (size_t)b * x.N;
+    a.list_count = c->wCtr + kCtrCount + b;
+    a.wide_list = c->wWideLists + (size_t)b * x.N;
+    a.wide_count = c->wCtr + kCtrWide + b;
+    a.variant = variant;
+    if (bin.kernel == kRowPerLane) return wp::launch_classify_v3(a, c->precision, stream);
+    if (bin.kernel == kQuadRows) return wp::launch_classify_v5(a, stream);
+    return wp::launch_classify(a, c->precision, c->G, stream);
+}
+
+// The wide pass of a quad-rows bin: the reads its main launch flagged.
+hipError_t launch_bin_wide(wp_hmm_classifier* c, const std::vector<Bin>& bins, int b, const CallArgs& x,
+                           hipStream_t stream) {
+    const Bin& bin = bins[b];
+    wp::ClassifyArgs a = base_args(c, x);
+    a.nb_lo = bin.lo;
+    a.nb_hi = bin.hi;
+    a.NB = bin.hi;
+    a.ngroups = bin.ngroups;
+    a.wide_pass = 1;
+    a.list = c->wWideLists + (size_t)b * x.N;
+    a.list_count = c->wCtr + kCtrWide + b;
+    if (wp::v3_supports(c->precision, c->T, bin.hi) &&
+        wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, bin.hi) <= lds_hard_limit())
+        return wp::launch_classify_v3(a, c->precision, stream);
+    return wp::launch_classify(a, c->precision, c->G, stream);
+}
+
+// The whole call: prep, binning, each bin's launch (and wide pass).
+int enqueue_call(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream, bool events) {
+    const std::vector<Bin> bins = make_bins(c);
+    if (events) WP_HIP(hipEventRecord(c->ev[0], stream));
+    enqueue_prep(c, bins, x, stream);
+    WP_HIP(hipGetLastError());
+    if (events) WP_HIP(hipEventRecord(c->ev[1], stream));
+    for (int b = 0; b < (int)bins.size(); b++) {
+        WP_HIP(launch_bin_main(c, bins, b, x, c->variant[b], stream));
+        if (bins[b].kernel == kQuadRows) WP_HIP(launch_bin_wide(c, bins, b, x, stream));
     }
-    if (c->timing) {
+    if (events) {
         WP_HIP(hipEventRecord(c->ev[2], stream));
         c->ev_recorded = 1;
     }
+    return WP_OK;
+}
+
+// Picks each quad-rows bin's variant by timing both on this call's reads
+// (two runs each, the faster second run wins; both give identical outputs).
+int autotune(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream) {
+    const std::vector<Bin> bins = make_bins(c);
+    bool any = false;
+    for (const Bin& b : bins) any = any || b.kernel == kQuadRows;
+    if (!any || c->variant_mode >= 0) {
+        for (int b = 0; b < wp::kMaxBins; b++) c->variant[b] = c->variant_mode > 0 ? 1 : 0;
+        c->tuned = 1;
+        return WP_OK;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    WP_HIP(hipEventCreate(&e0));
+    WP_HIP(hipEventCreate(&e1));
+    int rc = WP_OK;
+    auto fail = [&](hipError_t e) {
+        wp::set_error(std::string("autotune: ") + hipGetErrorString(e));
+        rc = WP_ERR_HIP;
+    };
+    enqueue_prep(c, bins, x, stream);
+    for (int b = 0; b < (int)bins.size() && !rc; b++) {
+        c->variant[b] = 0;
+        if (bins[b].kernel != kQuadRows) continue;
+        float best[2] = {1e30f, 1e30f};
+        for (int trial = 0; trial < 4 && !rc; trial++) {
+            const int v = trial & 1;
+            // a fresh wide list for each trial
+            wp::launch_zero_ints(c->wCtr + kCtrWide + b, 1, stream);
+            hipError_t e = hipEventRecord(e0, stream);
+            if (e == hipSuccess) e = launch_bin_main(c, bins, b, x, v, stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) {
+                fail(e);
+                break;
+            }
+            if (trial >= 2) best[v] = std::min(best[v], ms);
+        }
+        c->tune_ms[b][0] = best[0];
+        c->tune_ms[b][1] = best[1];
+        c->variant[b] = best[1] < best[0] ? 1 : 0;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (!rc) c->tuned = 1;
+    return rc;
+}
+
+int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_cand_off,
+               const int* d_cand_idx, int* d_id, double* d_adj, double* d_ls, double* d_lt,
+               double* d_pair, hipStream_t stream, bool graph_ok = false) {
+    if (N < 0) {
+        wp::set_error("num_reads < 0");
+        return WP_ERR_ARG;
+    }
+    if (N == 0) return WP_OK;
+    int rc = ensure_workspace(c, N);
+    if (rc) return rc;
+    const CallArgs x{N, d_rads, d_cand_off, d_cand_idx, d_id, d_adj, d_ls, d_lt, d_pair};
+    if (!c->tuned) {
+        rc = autotune(c, x, stream);
+        if (rc) return rc;
+    }
+    if (!graph_ok || !c->use_graph || c->timing) return enqueue_call(c, x, stream, c->timing != 0);
+    std::vector<const void*> key = {(const void*)(intptr_t)N, d_rads, d_cand_off, d_cand_idx, d_id, d_adj,
+                                    d_ls, d_lt, d_pair};
+    for (int b = 0; b < wp::kMaxBins; b++) key.push_back((const void*)(intptr_t)c->variant[b]);
+    if (!c->gexec || key != c->gkey) {
+        drop_graph(c);
+        if (!c->cap_stream) WP_HIP(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+        WP_HIP(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeRelaxed));
+        rc = enqueue_call(c, x, c->cap_stream, false);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) {
+            wp::set_error(std::string("graph capture: ") + hipGetErrorString(e));
+            return WP_ERR_HIP;
+        }
+        e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->gexec = nullptr;
+            wp::set_error(std::string("graph instantiate: ") + hipGetErrorString(e));
+            return WP_ERR_HIP;
+        }
+        c->gkey = key;
+    }
+    WP_HIP(hipGraphLaunch(c->gexec, stream));
     return WP_OK;
 }
 
@@ -389,6 +563,11 @@ void wp_hmm_classifier_destroy(wp_hmm_classifier* c) {
     if (!c) return;
     for (int k = 0; k < 3; k++)
         if (c->ev[k]) hipEventDestroy(c->ev[k]);
+    drop_graph(c);
+    if (c->cap_stream) hipStreamDestroy(c->cap_stream);
+    hipFree(c->wLists);
+    hipFree(c->wWideLists);
+    hipFree(c->wCtr);
     hipFree(c->dD);
     hipFree(c->dSeq);
     hipFree(c->dTrack);
@@ -413,7 +592,8 @@ int wp_hmm_classify_candidates_device(wp_hmm_classifier* c, int N, const double*
         wp::set_error("null argument");
         return WP_ERR_ARG;
     }
-    return run_device(c, N, d_rads, d_cand_off, d_cand_idx, d_id, d_adj, d_ls, d_lt, nullptr, (hipStream_t)stream);
+    return run_device(c, N, d_rads, d_cand_off, d_cand_idx, d_id, d_adj, d_ls, d_lt, nullptr, (hipStream_t)stream,
+                      true);
 }
 
 int wp_hmm_classify_device(wp_hmm_classifier* c, int N, const double* d_rads, int* d_id, double* d_adj,
@@ -423,7 +603,7 @@ int wp_hmm_classify_device(wp_hmm_classifier* c, int N, const double* d_rads, in
         return WP_ERR_ARG;
     }
     return run_device(c, N, d_rads, nullptr, nullptr, d_id, d_adj, d_ls, d_lt, nullptr,
-                      (hipStream_t)stream);
+                      (hipStream_t)stream, true);
 }
 
 static int classify_host(wp_hmm_classifier* c, int N, const double* rads, const int* cand_off,
@@ -534,6 +714,45 @@ int wp_hmm_pair_log_probability(wp_hmm_classifier* c, int N, const double* rads,
     }
     std::vector<int> ids(std::max(N, 1));
     return classify_host(c, N, rads, nullptr, nullptr, ids.data(), nullptr, nullptr, nullptr, out_log_p);
+}
+
+int wp_hmm_set_variant(wp_hmm_classifier* c, int mode) {
+    if (!c || mode < -1 || mode > 1) {
+        wp::set_error("wp_hmm_set_variant: mode must be -1, 0 or 1");
+        return WP_ERR_ARG;
+    }
+    c->variant_mode = mode;
+    c->tuned = 0;
+    for (int b = 0; b < wp::kMaxBins; b++) c->variant[b] = 0;
+    for (int b = 0; b < wp::kMaxBins; b++) c->tune_ms[b][0] = c->tune_ms[b][1] = -1.f;
+    drop_graph(c);
+    return WP_OK;
+}
+
+int wp_hmm_get_variants(const wp_hmm_classifier* c, int* out_variant, double* out_ms, int cap) {
+    if (!c) {
+        wp::set_error("null classifier");
+        return WP_ERR_ARG;
+    }
+    const int nb = (int)make_bins(c).size();
+    for (int b = 0; b < std::min(nb, cap); b++) {
+        if (out_variant) out_variant[b] = c->tuned ? c->variant[b] : -1;
+        if (out_ms) {
+            out_ms[2 * b] = c->tune_ms[b][0];
+            out_ms[2 * b + 1] = c->tune_ms[b][1];
+        }
+    }
+    return nb;
+}
+
+int wp_hmm_set_graph(wp_hmm_classifier* c, int enable) {
+    if (!c) {
+        wp::set_error("null classifier");
+        return WP_ERR_ARG;
+    }
+    c->use_graph = enable ? 1 : 0;
+    drop_graph(c);
+    return WP_OK;
 }
 
 int wp_hmm_set_timing(wp_hmm_classifier* c, int enable) {

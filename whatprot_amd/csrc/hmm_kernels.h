@@ -20,6 +20,13 @@ struct ClassifyArgs {
     int nb_lo, nb_hi;         // this launch takes reads with nb_lo < nbmax <= nb_hi
     uint8_t* wide;            // [N] set by the quad-rows kernel: densities too wide for it
     int wide_pass;            // 1: score only the reads flagged in wide[]
+    // Reads of this launch: workgroup k scores read list[k] for k < *list_count
+    // (the list is built on the device; later workgroups exit at once).
+    const int* list;
+    const int* list_count;
+    int* wide_list;           // quad-rows: reads handed to the wide pass are appended here
+    int* wide_count;
+    int variant;              // quad-rows: 1 = per-timestep box classes, 0 = bin-wide boxes
     int NB;                   // per-group state capacity in count-box entries
     int ngroups;              // pair groups per workgroup
     const uint8_t* dsD;       // [S][C] dyes per channel
@@ -37,6 +44,23 @@ struct ClassifyArgs {
     double* out_log_total;
     double* out_pair_logp;    // [N][S] or null
 };
+
+// Largest number of read bins (box-size classes) of one classify call.
+constexpr int kMaxBins = 16;
+
+// Bin edges of one call: bin b takes reads with edge[b] < nbmax <= edge[b + 1]
+// (reads with nbmax == 0, every pair empty, go to bin 0).
+struct BinEdges {
+    int nbins;
+    int edge[kMaxBins + 1];
+};
+
+// Clears n ints (a fill kernel, so the sequence stays graph-capturable).
+void launch_zero_ints(int* d, int n, hipStream_t stream);
+
+// Appends every read to its bin's list: lists[b * N + k], counts[b].
+void launch_bin_reads(int N, const int* d_nbmax, const BinEdges& e, int* d_lists, int* d_counts,
+                      hipStream_t stream);
 
 // Per-timestep emission ranges then pruned boxes for every read.
 void launch_read_prep(const DevModel& m, int N, const double* d_rads, uint8_t* d_Lr, uint8_t* d_Hr,

@@ -49,6 +49,8 @@
 #include <float.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "classify_common.h"
 #include "hmm_kernels.h"
 
@@ -446,12 +448,10 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
     return res;
 }
 
+// One read of the gather kernel. Block-uniform returns only.
 template <typename Real, int C, int G>
-__global__ void __launch_bounds__(kBlockThreads) WP_CLASSIFY_OCCUPANCY
-classify_kernel(const ClassifyArgs a) {
+__device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, unsigned char* smem) {
     constexpr bool kRescale = sizeof(Real) == 4;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int r = blockIdx.x;
     if (r >= a.N) return;
     const int nbr = a.nbmax[r];
     // Empty reads go with the first bin (nb_lo == 0 there).
@@ -556,15 +556,24 @@ classify_kernel(const ClassifyArgs a) {
     merge_and_store<kRescale>(a, r, ncand, cidx, ngroups, group, lane, acc, gres);
 }
 
+// One workgroup per read of the bin's compacted list.
+template <typename Real, int C, int G>
+__global__ void __launch_bounds__(kBlockThreads) WP_CLASSIFY_OCCUPANCY
+classify_kernel(const ClassifyArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    if ((int)blockIdx.x >= *a.list_count) return;
+    gather_read<Real, C, G>(a, a.list[blockIdx.x], smem);
+}
+
 template <typename Real, int C, int G>
 hipError_t launch_t(const ClassifyArgs& a, hipStream_t stream) {
     int nthreads = a.ngroups * G;
     if (nthreads < 64) nthreads = 64;
     nthreads = (nthreads + 63) / 64 * 64;
     Layout l = make_layout<Real>(a.m.T, C, a.tri, a.NB, a.ngroups, G, nthreads);
+    const void* fn = (const void*)classify_kernel<Real, C, G>;
     if (l.total > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)classify_kernel<Real, C, G>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((classify_kernel<Real, C, G>), dim3(a.N), dim3(nthreads), l.total, stream, a);
@@ -629,7 +638,40 @@ __global__ void work_kernel(DevModel m, int N, int S, const uint8_t* __restrict_
     out[(size_t)r * 3 + 2] = flops;
 }
 
+__global__ void zero_ints_kernel(int* d, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = 0;
+}
+
+// Appends read r to its bin's list. The order inside a list follows the
+// atomics; every read's outputs are computed independently of that order.
+__global__ void bin_reads_kernel(int N, const int* __restrict__ nbmax, BinEdges e, int* __restrict__ lists,
+                                 int* __restrict__ counts) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const int nb = nbmax[r];
+    int b = 0;
+    if (nb > 0) {
+        b = -1;
+        for (int k = 0; k < e.nbins; k++)
+            if (nb > e.edge[k] && nb <= e.edge[k + 1]) b = k;
+    }
+    if (b < 0) return;  // outside every bin: no launch takes it
+    lists[(size_t)b * N + atomicAdd(&counts[b], 1)] = r;
+}
+
 }  // namespace
+
+void launch_zero_ints(int* d, int n, hipStream_t stream) {
+    if (n > 0) hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, d, n);
+}
+
+void launch_bin_reads(int N, const int* d_nbmax, const BinEdges& e, int* d_lists, int* d_counts,
+                      hipStream_t stream) {
+    if (N > 0)
+        hipLaunchKernelGGL(bin_reads_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, N, d_nbmax, e, d_lists,
+                           d_counts);
+}
 
 void launch_read_prep(const DevModel& m, int N, const double* d_rads, uint8_t* d_Lr, uint8_t* d_Hr,
                       int* d_nbmax, hipStream_t stream) {

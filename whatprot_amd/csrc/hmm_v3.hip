@@ -25,6 +25,8 @@
 #include <float.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "classify_common.h"
 #include "hmm_kernels.h"
 
@@ -84,11 +86,10 @@ __device__ inline double shift_up(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// One read of the row-per-lane kernel. Block-uniform returns only.
 template <typename Real, int C, int NB, int G>
-__global__ void __launch_bounds__(kBlockThreads) classify_v3_kernel(const ClassifyArgs a) {
+__device__ __forceinline__ void v3_read(const ClassifyArgs& a, const int r, unsigned char* smem) {
     constexpr bool kRescale = sizeof(Real) == 4;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int r = blockIdx.x;
     if (r >= a.N) return;
     const int nbr = a.nbmax[r];
     if (!((nbr > a.nb_lo && nbr <= a.nb_hi) || (nbr == 0 && a.nb_lo == 0))) return;
@@ -412,12 +413,20 @@ __global__ void __launch_bounds__(kBlockThreads) classify_v3_kernel(const Classi
     merge_and_store<kRescale>(a, r, ncand, cidx, ngroups, group, lane, acc, gres);
 }
 
+// One workgroup per read of the bin's compacted list.
+template <typename Real, int C, int NB, int G>
+__global__ void __launch_bounds__(kBlockThreads) classify_v3_kernel(const ClassifyArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    if ((int)blockIdx.x >= *a.list_count) return;
+    v3_read<Real, C, NB, G>(a, a.list[blockIdx.x], smem);
+}
+
 template <typename Real, int C, int NB, int G>
 hipError_t launch_v3_t(const ClassifyArgs& a, hipStream_t stream) {
     V3Layout<Real> l = v3_layout<Real>(a.m.T, C, a.tri, NB, kBlockThreads / G, kBlockThreads);
+    const void* fn = (const void*)classify_v3_kernel<Real, C, NB, G>;
     if (l.total > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)classify_v3_kernel<Real, C, NB, G>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((classify_v3_kernel<Real, C, NB, G>), dim3(a.N), dim3(kBlockThreads), l.total, stream, a);

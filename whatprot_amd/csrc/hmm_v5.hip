@@ -16,8 +16,16 @@
 // Edman coefficients, the two updates, the emission product and a rescale;
 // row exponents meet only where the Edman success crosses rows and in the
 // final sum of a pair.
+//
+// Per-timestep box classes. A read's box shrinks as its dyes leave (most
+// timesteps of most reads have a one-state box), so each timestep t runs at
+// the class K_t = the power of two >= max(|box(t-1)|, |box(t)|), not at the
+// bin's capacity NB: the composite matrices of t are stored K_t x K_t and the
+// step body is instantiated per class, chosen uniformly per read.
 #include <float.h>
 #include <math.h>
+
+#include <algorithm>
 
 #include "classify_common.h"
 #include "hmm_kernels.h"
@@ -32,12 +40,12 @@ constexpr int kV5Lanes = 4;  // lanes per candidate: one DPP quad
 constexpr int kV5MaxSpan = 90;  // largest binary-exponent spread of one box's densities
 
 struct V5Layout {
-    size_t rad, L, H, nbt, bstr, coord, kcf, ev, etab, xt, W, binom, list, scan, gres, total;
+    size_t rad, L, H, nbt, kcl, woff, bstr, coord, kcf, ev, etab, xt, W, binom, list, scan, gres, total;
 };
 
-// W block per timestep: (1 + C) matrices (Wt, then W1[c]) of NB x NB with a
-// stride of NB * NB + 4 entries, so the shifted matrices of different
-// channels start in different LDS banks.
+// W block per timestep: (1 + C) matrices (Wt, then W1[c]) of K_t x K_t with a
+// stride of K_t * K_t + 4 entries, so the shifted matrices of different
+// channels start in different LDS banks; sized for K_t = NB at every t.
 __host__ __device__ inline V5Layout v5_layout(int T, int C, int tri, int NB) {
     V5Layout l;
     const size_t ms = (size_t)NB * NB + 4;
@@ -47,6 +55,8 @@ __host__ __device__ inline V5Layout v5_layout(int T, int C, int tri, int NB) {
     l.L = o;     o = align16(o + sizeof(int) * T * C);
     l.H = o;     o = align16(o + sizeof(int) * T * C);
     l.nbt = o;   o = align16(o + sizeof(int) * T);
+    l.kcl = o;   o = align16(o + sizeof(int) * T);
+    l.woff = o;  o = align16(o + sizeof(int) * T);
     l.bstr = o;  o = align16(o + sizeof(int) * T * C);
     l.coord = o; o = align16(o + sizeof(int) * (size_t)T * NB * C);
     l.kcf = o;   o = align16(o + sizeof(float) * (size_t)T * NB * C);
@@ -68,13 +78,111 @@ __device__ inline int quad_rot_i(int v) {
 }
 __device__ inline float quad_rot(float v) { return __int_as_float(quad_rot_i(__float_as_int(v))); }
 
-template <int C, int NB, int R>
-__global__ void __launch_bounds__(kV5Threads) classify_v5_kernel(const ClassifyArgs a) {
+// Per-timestep constants of one step t >= 1, shared by every candidate.
+struct V5Step {
+    const float* Wt;   // K x K composite Bleach, then the C Edman-shifted forms, K * K + 4 apart
+    const float* Et;   // densities of box(t) relative to 2^Xt
+    const float* kct;  // [NB][C] counts of box(t) as floats
+    int nbp, t, Xt, p;
+    float pd, cAA, cB, pf, pcb;
+    bool allow;        // box(t) contains the all-zero count (the detached share keeps E_t(0))
+};
+
+// One step t >= 1 of a candidate's recursion (CyclicBlock, Detach(t-1),
+// Bleach_c, Edman, Emit(t)) on a box of class K (both box(t-1) and box(t) fit
+// in K entries). Entries >= K of A and B are stale and never read: inputs stop
+// at |box(t-1)| and outputs at K.
+template <int C, int NB, int R, int K>
+__device__ __forceinline__ void v5_step(const V5Step& st, const int (&cix)[R], const float (&aK)[R],
+                                        const float (&cst)[R], const float (&Kf)[R], float (&A)[R][NB],
+                                        float (&B)[R][NB], int (&kx)[R], float (&dl)[R], float (&Sr)[R]) {
     constexpr int G = kV5Lanes;
-    constexpr int MS = NB * NB + 4;
+    constexpr int MS = K * K + 4;
+    const float* Wt = st.Wt;
+    const int p = st.p;
+    const float E0 = st.Et[0];
+    float succ_prev[K];  // this lane's slot k - 1 Edman success (old scale)
+    int kprev = kNoExp;  // and slot k - 1's exponent before this step
+#pragma unroll
+    for (int q = 0; q < K; q++) succ_prev[q] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        if (st.t < G * k) continue;  // every row of slot k starts after t (uniform)
+        const float* W1 = Wt + (1 + cix[k]) * MS;
+        float BA[K], BB[K], BA1[K];
+#pragma unroll
+        for (int q2 = 0; q2 < K; q2++) {
+            BA[q2] = 0.0f;
+            BB[q2] = 0.0f;
+            BA1[q2] = 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            if (q >= st.nbp) continue;  // uniform: the previous box's size
+            const float av = A[k][q], bv = B[k][q];
+#pragma unroll
+            for (int q2 = 0; q2 < K; q2++) {
+                const float w = Wt[q * K + q2];
+                BA[q2] += w * av;
+                BB[q2] += w * bv;
+                BA1[q2] += W1[q * K + q2] * av;
+            }
+        }
+        // exponents of this row and of the row below it (e - 1)
+        const int ko = kx[k];
+        const int kfrom = quad_rot_i(p == G - 1 ? kprev : ko);
+        int kn = max(ko, kfrom);
+        if (kn == kNoExp) kn = 0;
+        const float sown = ko == kNoExp ? 0.0f : ldexpf(1.0f, ko - kn);
+        const float sfrom = kfrom == kNoExp ? 0.0f : ldexpf(1.0f, kfrom - kn);
+        const float fAo = st.cAA * st.pf * sown, fAf = st.cAA * sfrom;
+        const float fB = st.cB * sown, fBA = st.cB * st.pcb * sown;
+        // Detach(t-1): pd times the row's previous sum.
+        float dlp = (dl[k] + st.pd * Sr[k]) * sown;
+        float m = 0.0f;
+#pragma unroll
+        for (int q2 = 0; q2 < K; q2++) {
+            const float kc = st.kct[q2 * C + cix[k]];
+            const float cba = fmaf(fmaxf(Kf[k] - kc, 0.0f), aK[k], cst[k]);
+            const float cba1 = fmaf(kc, aK[k], aK[k]);
+            const float succ = fmaf(cba1, BA1[q2], cba * BA[q2]);
+            const float from = quad_rot(p == G - 1 ? succ_prev[q2] : succ);
+            succ_prev[q2] = succ;
+            // Emit(t) (peptide-emission.cc:212-243)
+            const float em = st.Et[q2];
+            const float na = fmaf(fAf, from, fAo * BA[q2]) * em;
+            const float nb = fmaf(fBA, BA[q2], fB * BB[q2]) * em;
+            m = fmaxf(m, fmaxf(na, nb));
+            A[k][q2] = na;
+            B[k][q2] = nb;
+        }
+        // the detached share keeps density E_t(0) only if the box includes zero
+        dlp = st.allow ? dlp * E0 : 0.0f;
+        m = fmaxf(m, dlp);
+        kprev = ko;
+        const int km = exp_of(m);
+        const int kf = km == kNoExp ? 0 : km;
+        float S = 0.0f;
+#pragma unroll
+        for (int q2 = 0; q2 < K; q2++) {
+            A[k][q2] = ldexpf(A[k][q2], -kf);
+            B[k][q2] = ldexpf(B[k][q2], -kf);
+            S += A[k][q2] + B[k][q2];
+        }
+        Sr[k] = S;
+        dl[k] = ldexpf(dlp, -kf);
+        kx[k] = km == kNoExp ? kNoExp : kn + st.Xt + kf;
+    }
+}
+
+// One read: prologue (boxes, densities, composite matrices) then its
+// candidates. kClass: each timestep at its own box class; otherwise every
+// timestep at the bin's capacity NB. Block-uniform returns only.
+template <int C, int NB, int R, bool kClass>
+__device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsigned char* smem) {
+    constexpr int G = kV5Lanes;
     constexpr int NGR = kV5Threads / G;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int r = blockIdx.x;
+    constexpr int MSNB = NB * NB + 4;
     if (r >= a.N) return;
     const int nbr = a.nbmax[r];
     if (!((nbr > a.nb_lo && nbr <= a.nb_hi) || (nbr == 0 && a.nb_lo == 0))) return;
@@ -89,6 +197,8 @@ __global__ void __launch_bounds__(kV5Threads) classify_v5_kernel(const ClassifyA
     int* sL = (int*)(smem + lay.L);
     int* sH = (int*)(smem + lay.H);
     int* nbt = (int*)(smem + lay.nbt);
+    int* kcl = (int*)(smem + lay.kcl);
+    int* woff = (int*)(smem + lay.woff);
     int* bstr = (int*)(smem + lay.bstr);
     int* coord = (int*)(smem + lay.coord);
     float* kcf = (float*)(smem + lay.kcf);
@@ -122,6 +232,18 @@ __global__ void __launch_bounds__(kV5Threads) classify_v5_kernel(const ClassifyA
             s *= w > 0 ? w : 0;
         }
         nbt[t] = read_empty ? 0 : s;
+    }
+    __syncthreads();
+    if (kClass && tid == 0) {  // per-timestep classes and the offsets of their W blocks
+        int o = 0;
+        for (int t = 0; t < T; t++) {
+            const int need = t == 0 ? nbt[0] : max(nbt[t - 1], nbt[t]);
+            int k = 1;
+            while (k < need) k *= 2;
+            kcl[t] = k;
+            woff[t] = o;
+            if (t >= 1) o += (1 + C) * (k * k + 4);
+        }
     }
     __syncthreads();
     // peptide-emission.cc:165-180
@@ -171,12 +293,17 @@ __global__ void __launch_bounds__(kV5Threads) classify_v5_kernel(const ClassifyA
     // FP32's range relative to the box maximum: such a read is flagged and
     // left to the kernels that keep one exponent per density.
     const bool wide = scan[0] != 0;
-    if (tid == 0) a.wide[r] = wide ? 1 : 0;
+    if (tid == 0) {
+        a.wide[r] = wide ? 1 : 0;
+        if (wide) a.wide_list[atomicAdd(a.wide_count, 1)] = r;
+    }
     if (wide) return;
     for (int k = tid; k < T * NB; k += kV5Threads) etab[k] = (float)ldexp(ev[k], -xt[k / NB]);
     // Composite Bleach between the boxes of t-1 and t and its Edman-shifted
     // forms: W1[c][q][q2] maps onto n'(q2) + 1_c and is zero at or above
-    // U = min(Hr[t] + 1, Hr[t-1]) (edman-transition.cc:28-60).
+    // U = min(Hr[t] + 1, Hr[t-1]) (edman-transition.cc:28-60). Timestep t's
+    // matrices are K_t x K_t, K_t + 4 entries apart (different LDS banks for
+    // the shifted matrices of different channels).
     const double* ble = bin + C * tri;
     for (int k = tid; k < T * (1 + C) * NB * NB; k += kV5Threads) {
         const int t = k / ((1 + C) * NB * NB);
@@ -184,6 +311,8 @@ __global__ void __launch_bounds__(kV5Threads) classify_v5_kernel(const ClassifyA
         const int mat = rem / (NB * NB);
         rem -= mat * NB * NB;
         const int q = rem / NB, q2 = rem % NB;
+        const int K = kClass ? kcl[t] : NB;
+        if (t == 0 || q >= K || q2 >= K) continue;
         double v = 0.0;
         if (t >= 1 && q < nbt[t - 1] && q2 < nbt[t]) {
             v = 1.0;
@@ -196,7 +325,7 @@ __global__ void __launch_bounds__(kV5Threads) classify_v5_kernel(const ClassifyA
                 else v *= ble[c * tri + from * (from + 1) / 2 + to];
             }
         }
-        Wall[((size_t)t * (1 + C) + mat) * MS + rem] = (float)v;
+        Wall[(kClass ? woff[t] : t * (1 + C) * MSNB) + mat * (K * K + 4) + q * K + q2] = (float)v;
     }
     __syncthreads();
 
@@ -281,90 +410,33 @@ __global__ void __launch_bounds__(kV5Threads) classify_v5_kernel(const ClassifyA
             }
 
             for (int t = 1; t < T; t++) {
-                const float* Wt = Wall + (size_t)t * (1 + C) * MS;
-                const int nbp = nbt[t - 1];
-                const float pd = (float)a.m.p_detach[t - 1];
-                const float cAA = (1.0f - pd) * (1.0f - pcb);
-                const float cB = 1.0f - pd;
-                const float* Et = etab + (size_t)t * NB;
-                const float E0 = Et[0];
-                const int Xt = xt[t];
+                V5Step st;
+                st.Wt = Wall + (kClass ? woff[t] : t * (1 + C) * MSNB);
+                st.nbp = nbt[t - 1];
+                st.t = t;
+                st.pd = (float)a.m.p_detach[t - 1];
+                st.cAA = (1.0f - st.pd) * (1.0f - pcb);
+                st.cB = 1.0f - st.pd;
+                st.pf = pf;
+                st.pcb = pcb;
+                st.Et = etab + (size_t)t * NB;
+                st.Xt = xt[t];
                 bool allow = true;
 #pragma unroll
                 for (int c = 0; c < C; c++) allow = allow && sL[t * C + c] == 0;
-                const float* kct = kcf + (size_t)t * NB * C;
-                float succ_prev[NB];  // this lane's slot k - 1 Edman success (old scale)
-                int kprev = kNoExp;   // and slot k - 1's exponent before this step
-#pragma unroll
-                for (int q = 0; q < NB; q++) succ_prev[q] = 0.0f;
-#pragma unroll
-                for (int k = 0; k < R; k++) {
-                    if (t < G * k) continue;  // every row of slot k starts after t (uniform)
-                    const float* W1 = Wt + (1 + cix[k]) * MS;
-                    float BA[NB], BB[NB], BA1[NB];
-#pragma unroll
-                    for (int q2 = 0; q2 < NB; q2++) {
-                        BA[q2] = 0.0f;
-                        BB[q2] = 0.0f;
-                        BA1[q2] = 0.0f;
-                    }
-#pragma unroll
-                    for (int q = 0; q < NB; q++) {
-                        if (q >= nbp) continue;  // uniform: the previous box's size
-                        const float av = A[k][q], bv = B[k][q];
-#pragma unroll
-                        for (int q2 = 0; q2 < NB; q2++) {
-                            const float w = Wt[q * NB + q2];
-                            BA[q2] += w * av;
-                            BB[q2] += w * bv;
-                            BA1[q2] += W1[q * NB + q2] * av;
-                        }
-                    }
-                    // exponents of this row and of the row below it (e - 1)
-                    const int ko = kx[k];
-                    const int kfrom = quad_rot_i(p == G - 1 ? kprev : ko);
-                    int kn = max(ko, kfrom);
-                    if (kn == kNoExp) kn = 0;
-                    const float sown = ko == kNoExp ? 0.0f : ldexpf(1.0f, ko - kn);
-                    const float sfrom = kfrom == kNoExp ? 0.0f : ldexpf(1.0f, kfrom - kn);
-                    const float fAo = cAA * pf * sown, fAf = cAA * sfrom;
-                    const float fB = cB * sown, fBA = cB * pcb * sown;
-                    // Detach(t-1): pd times the row's previous sum.
-                    float dlp = (dl[k] + pd * Sr[k]) * sown;
-                    float m = 0.0f;
-#pragma unroll
-                    for (int q2 = 0; q2 < NB; q2++) {
-                        const float kc = kct[q2 * C + cix[k]];
-                        const float cba = fmaf(fmaxf(Kf[k] - kc, 0.0f), aK[k], cst[k]);
-                        const float cba1 = fmaf(kc, aK[k], aK[k]);
-                        const float succ = fmaf(cba1, BA1[q2], cba * BA[q2]);
-                        const float from = quad_rot(p == G - 1 ? succ_prev[q2] : succ);
-                        succ_prev[q2] = succ;
-                        // Emit(t) (peptide-emission.cc:212-243)
-                        const float em = Et[q2];
-                        const float na = fmaf(fAf, from, fAo * BA[q2]) * em;
-                        const float nb = fmaf(fBA, BA[q2], fB * BB[q2]) * em;
-                        m = fmaxf(m, fmaxf(na, nb));
-                        A[k][q2] = na;
-                        B[k][q2] = nb;
-                    }
-                    // the detached share keeps density E_t(0) only if the box includes zero
-                    dlp = allow ? dlp * E0 : 0.0f;
-                    m = fmaxf(m, dlp);
-                    kprev = ko;
-                    const int km = exp_of(m);
-                    const int kf = km == kNoExp ? 0 : km;
-                    float S = 0.0f;
-#pragma unroll
-                    for (int q2 = 0; q2 < NB; q2++) {
-                        A[k][q2] = ldexpf(A[k][q2], -kf);
-                        B[k][q2] = ldexpf(B[k][q2], -kf);
-                        S += A[k][q2] + B[k][q2];
-                    }
-                    Sr[k] = S;
-                    dl[k] = ldexpf(dlp, -kf);
-                    kx[k] = km == kNoExp ? kNoExp : kn + Xt + kf;
+                st.allow = allow;
+                st.kct = kcf + (size_t)t * NB * C;
+                st.p = p;
+                if constexpr (!kClass) {
+                    v5_step<C, NB, R, NB>(st, cix, aK, cst, Kf, A, B, kx, dl, Sr);
+                    continue;
                 }
+                const int Kt = kcl[t];  // uniform over the workgroup
+                if (Kt <= 1) v5_step<C, NB, R, 1>(st, cix, aK, cst, Kf, A, B, kx, dl, Sr);
+                else if (Kt == 2) v5_step<C, NB, R, 2>(st, cix, aK, cst, Kf, A, B, kx, dl, Sr);
+                else if (Kt == 4 || NB == 4) v5_step<C, NB, R, (NB < 4 ? NB : 4)>(st, cix, aK, cst, Kf, A, B, kx, dl, Sr);
+                else if (Kt == 8 || NB == 8) v5_step<C, NB, R, (NB < 8 ? NB : 8)>(st, cix, aK, cst, Kf, A, B, kx, dl, Sr);
+                else v5_step<C, NB, R, NB>(st, cix, aK, cst, Kf, A, B, kx, dl, Sr);
             }
 
             // P = sum over rows of (row sum + detached share) (peptide-state-vector.cc:52-54)
@@ -390,16 +462,32 @@ __global__ void __launch_bounds__(kV5Threads) classify_v5_kernel(const ClassifyA
     merge_and_store<true>(a, r, ncand, cidx, NGR, group, p, acc, gres);
 }
 
-template <int C, int NB, int R>
-hipError_t launch_v5_t(const ClassifyArgs& a, hipStream_t stream) {
+// One workgroup per read of the bin's compacted list: a persistent loop over
+// a queue costs this kernel registers (the compiler keeps read-invariant
+// values live across reads), which lowers its occupancy.
+template <int C, int NB, int R, bool kClass>
+__global__ void __launch_bounds__(kV5Threads) classify_v5_kernel(const ClassifyArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    if ((int)blockIdx.x >= *a.list_count) return;
+    v5_read<C, NB, R, kClass>(a, a.list[blockIdx.x], smem);
+}
+
+template <int C, int NB, int R, bool kClass>
+hipError_t launch_v5_v(const ClassifyArgs& a, hipStream_t stream) {
     const V5Layout l = v5_layout(a.m.T, C, a.tri, NB);
     if (l.total > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)classify_v5_kernel<C, NB, R>,
+        hipError_t e = hipFuncSetAttribute((const void*)classify_v5_kernel<C, NB, R, kClass>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((classify_v5_kernel<C, NB, R>), dim3(a.N), dim3(kV5Threads), l.total, stream, a);
+    // one workgroup per read of the bin's list (workgroups past its length exit)
+    hipLaunchKernelGGL((classify_v5_kernel<C, NB, R, kClass>), dim3(a.N), dim3(kV5Threads), l.total, stream, a);
     return hipGetLastError();
+}
+
+template <int C, int NB, int R>
+hipError_t launch_v5_t(const ClassifyArgs& a, hipStream_t stream) {
+    return a.variant ? launch_v5_v<C, NB, R, true>(a, stream) : launch_v5_v<C, NB, R, false>(a, stream);
 }
 
 // Row-slots per lane: R = ceil(T / 4) rounded up to an instantiated count.
