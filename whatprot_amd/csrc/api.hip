@@ -128,7 +128,13 @@ enum BinKernel { kGather = 0, kRowPerLane = 3, kQuadRows = 5 };
 struct Bin {
     int lo, hi, ngroups;
     int kernel;  // BinKernel: quad-rows (FP32, boxes <= 16), row-per-lane (FP64, <= 8) or gather
+    int cap_sep, ngroups_sep;  // gather bins: state capacity and groups of the separable launch
 };
+
+// The separable gather launch sizes its state buffers at 1/kSepCapDiv of the
+// T x NB entries the bin could need, so more pairs fit in LDS; a read that
+// needs more goes to the composite wide pass.
+constexpr int kSepCapDiv = 2;
 
 size_t bin_lds_bytes(const wp_hmm_classifier* c, const Bin& b);
 
@@ -157,10 +163,15 @@ std::vector<Bin> make_bins(const wp_hmm_classifier* c) {
     for (int hi : edges) {
         if (hi <= lo) continue;
         if (hi > c->maxbox) hi = c->maxbox;
-        size_t fixed = wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, hi, 0, c->G);
-        size_t per = wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, hi, 1, c->G) - fixed;
-        int ng = (int)((kLdsBudget > fixed ? kLdsBudget - fixed : 0) / (per ? per : 1));
-        ng = std::max(1, std::min(ng, wp::kBlockThreads / c->G));
+        auto groups_for = [&](int cap) {
+            size_t fixed = wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, hi, cap, 0, c->G);
+            size_t per = wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, hi, cap, 1, c->G) - fixed;
+            int g = (int)((kLdsBudget > fixed ? kLdsBudget - fixed : 0) / (per ? per : 1));
+            return std::max(1, std::min(g, wp::kBlockThreads / c->G));
+        };
+        const int ng = groups_for(c->T * hi);
+        const int cap_sep = std::max(1, c->T * hi / kSepCapDiv);
+        const int ng_sep = groups_for(cap_sep);
         int kernel = kGather;
         if (wp::v5_supports(c->precision, c->T, hi) &&
             wp::v5_lds_bytes(c->T, c->C, c->tri, hi) <= lds_hard_limit())
@@ -168,7 +179,7 @@ std::vector<Bin> make_bins(const wp_hmm_classifier* c) {
         else if (wp::v3_supports(c->precision, c->T, hi) &&
                  wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, hi) <= lds_hard_limit())
             kernel = kRowPerLane;
-        bins.push_back({lo, hi, ng, kernel});
+        bins.push_back({lo, hi, ng, kernel, cap_sep, ng_sep});
         lo = hi;
         if (lo >= c->maxbox) break;
     }
@@ -179,7 +190,9 @@ size_t bin_lds_bytes(const wp_hmm_classifier* c, const Bin& b) {
     switch (b.kernel) {
     case kRowPerLane: return wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi);
     case kQuadRows: return wp::v5_lds_bytes(c->T, c->C, c->tri, b.hi);
-    default: return wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi, b.ngroups, c->G);
+    default:
+        return std::max(wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi, c->T * b.hi, b.ngroups, c->G),
+                        wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi, b.cap_sep, b.ngroups_sep, c->G));
     }
 }
 
@@ -246,6 +259,7 @@ wp::ClassifyArgs base_args(const wp_hmm_classifier* c, const CallArgs& x) {
     a.wide_list = nullptr;
     a.wide_count = nullptr;
     a.variant = 0;
+    a.cap = 0;
     a.dsD = c->dD;
     a.dsSeq = c->dSeq;
     a.dsTrack = c->dTrack;
@@ -282,6 +296,7 @@ hipError_t launch_bin_main(wp_hmm_classifier* c, const std::vector<Bin>& bins, i
     a.nb_lo = bin.lo;
     a.nb_hi = bin.hi;
     a.NB = bin.hi;
+    a.cap = c->T * bin.hi;
     a.ngroups = bin.ngroups;
     a.list = c->wLists + (size_t)b * x.N;
     a.list_count = c->wCtr + kCtrCount + b;
@@ -290,10 +305,15 @@ hipError_t launch_bin_main(wp_hmm_classifier* c, const std::vector<Bin>& bins, i
     a.variant = variant;
     if (bin.kernel == kRowPerLane) return wp::launch_classify_v3(a, c->precision, stream);
     if (bin.kernel == kQuadRows) return wp::launch_classify_v5(a, stream);
+    a.variant = 1;  // gather bins: separable Bleach passes; reads that outgrow them go to the wide pass
+    a.cap = bin.cap_sep;
+    a.ngroups = bin.ngroups_sep;
     return wp::launch_classify(a, c->precision, c->G, stream);
 }
 
-// The wide pass of a quad-rows bin: the reads its main launch flagged.
+// The wide pass of a bin: the reads its main launch handed over (quad-rows:
+// densities too wide for one exponent per timestep; gather: boxes too large
+// for the separable passes), scored by the per-entry exponent kernels.
 hipError_t launch_bin_wide(wp_hmm_classifier* c, const std::vector<Bin>& bins, int b, const CallArgs& x,
                            hipStream_t stream) {
     const Bin& bin = bins[b];
@@ -301,6 +321,7 @@ hipError_t launch_bin_wide(wp_hmm_classifier* c, const std::vector<Bin>& bins, i
     a.nb_lo = bin.lo;
     a.nb_hi = bin.hi;
     a.NB = bin.hi;
+    a.cap = c->T * bin.hi;
     a.ngroups = bin.ngroups;
     a.wide_pass = 1;
     a.list = c->wWideLists + (size_t)b * x.N;
@@ -320,7 +341,7 @@ int enqueue_call(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream, bo
     if (events) WP_HIP(hipEventRecord(c->ev[1], stream));
     for (int b = 0; b < (int)bins.size(); b++) {
         WP_HIP(launch_bin_main(c, bins, b, x, c->variant[b], stream));
-        if (bins[b].kernel == kQuadRows) WP_HIP(launch_bin_wide(c, bins, b, x, stream));
+        if (bins[b].kernel != kRowPerLane) WP_HIP(launch_bin_wide(c, bins, b, x, stream));
     }
     if (events) {
         WP_HIP(hipEventRecord(c->ev[2], stream));

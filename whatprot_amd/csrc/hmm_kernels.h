@@ -28,6 +28,7 @@ struct ClassifyArgs {
     int* wide_count;
     int variant;              // quad-rows: 1 = per-timestep box classes, 0 = bin-wide boxes
     int NB;                   // per-group state capacity in count-box entries
+    int cap;                  // gather kernel: entries of each of a group's four state buffers
     int ngroups;              // pair groups per workgroup
     const uint8_t* dsD;       // [S][C] dyes per channel
     const int8_t* dsSeq;      // [S][T] channel of position e (-1 = none)
@@ -86,8 +87,8 @@ hipError_t launch_classify_v5(const ClassifyArgs& a, hipStream_t stream);
 void launch_work(const DevModel& m, int N, int S, const uint8_t* d_Lr, const uint8_t* d_Hr,
                  const int* d_nbmax, const uint8_t* d_D, double* d_out, hipStream_t stream);
 
-// Bytes of dynamic LDS one classify workgroup needs.
-size_t classify_lds_bytes(int precision, int T, int C, int tri, int NB, int ngroups, int group_lanes);
+// Bytes of dynamic LDS one gather-kernel workgroup needs (state buffers of cap entries).
+size_t classify_lds_bytes(int precision, int T, int C, int tri, int NB, int cap, int ngroups, int group_lanes);
 
 // Chunk of candidates compacted per pass.
 constexpr int kCandChunk = 1024;

@@ -76,7 +76,7 @@ struct Layout {
 };
 
 template <typename Real>
-__host__ __device__ inline Layout make_layout(int T, int C, int tri, int NB, int ngroups, int G,
+__host__ __device__ inline Layout make_layout(int T, int C, int tri, int NB, int cap, int ngroups, int G,
                                               int nthreads) {
     Layout l;
     size_t o = 0;
@@ -92,7 +92,7 @@ __host__ __device__ inline Layout make_layout(int T, int C, int tri, int NB, int
     l.scan = o;  o = align16(o + sizeof(int) * (nthreads / 64 + 2));
     l.gres = o;  o = align16(o + sizeof(double) * 4 * ngroups);
     l.gseq = o;  o = align16(o + sizeof(int) * 3 * T * ngroups);
-    l.state = o; o = align16(o + sizeof(Real) * 4 * (size_t)T * NB * ngroups);
+    l.state = o; o = align16(o + sizeof(Real) * 4 * (size_t)cap * ngroups);
     l.total = o;
     return l;
 }
@@ -448,8 +448,359 @@ __device__ PairResult<Real> score_pair(const ClassifyArgs& a, int i, int lane, R
     return res;
 }
 
-// One read of the gather kernel. Block-uniform returns only.
+// One channel's Bleach (binomial-transition.cc:91-136) as a pass over a
+// group's state: out[.., j, ..] = sum_{i >= j} P_c(i, j) in[.., i, ..] along
+// axis c, every other axis unchanged, rows e < es. Boxes are absolute count
+// ranges; arrays are [box index][e].
+template <typename Real, int C, int G, int c>
+__device__ inline void bleach_pass(const Real* inA, const Real* inB, Real* outA, Real* outB, int es,
+                                   const int* ilo, const int* ihi, const int* istr, const int* olo,
+                                   const int* ohi, const Real* ble, int tri, int lane) {
+    int ow[C], ostr[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) ow[k] = ohi[k] - olo[k];
+    const int nbox = box_strides<C>(ow, ostr);
+    const int nout = nbox * es;
+    const Divider by_e(es);
+    Divider by_str[C];
+#pragma unroll
+    for (int k = 0; k < C; k++) by_str[k] = Divider(ostr[k]);
+    for (int q = lane; q < nout; q += G) {
+        const int ni = by_e.div(q);
+        const int e = q - ni * es;
+        int rem = ni, base = 0, j = 0;
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            const int m = by_str[k].div(rem);
+            rem -= m * ostr[k];
+            if (k == c) j = olo[k] + m;
+            else base += m * istr[k];  // same range on both sides
+        }
+        Real sa = 0, sb = 0;
+        const Real* row = ble + c * tri;
+        for (int n = max(j, ilo[c]); n < ihi[c]; n++) {
+            const Real pw = row[n * (n + 1) / 2 + j];
+            const int idx = (base + (n - ilo[c]) * istr[c]) * es + e;
+            sa += pw * inA[idx];
+            sb += pw * inB[idx];
+        }
+        outA[q] = sa;
+        outB[q] = sb;
+    }
+}
+
+template <typename Real, int C, int G, int c>
+struct BleachPasses {
+    // Applies channels c..C-1 in turn, ping-ponging between (pA, pB) and
+    // (qA, qB); the box of channels < c is already the output box. Returns
+    // true when the result sits in (qA, qB).
+    __device__ static inline bool run(Real* pA, Real* pB, Real* qA, Real* qB, int es, int* lo, int* hi,
+                                      const int* lo2, const int* U, const Real* ble, int tri, int lane) {
+        int w[C], str[C], nlo[C], nhi[C];
+#pragma unroll
+        for (int k = 0; k < C; k++) {
+            w[k] = hi[k] - lo[k];
+            nlo[k] = k == c ? lo2[k] : lo[k];
+            nhi[k] = k == c ? U[k] : hi[k];
+        }
+        box_strides<C>(w, str);
+        bleach_pass<Real, C, G, c>(pA, pB, qA, qB, es, lo, hi, str, nlo, nhi, ble, tri, lane);
+        wave_sync();
+        lo[c] = lo2[c];
+        hi[c] = U[c];
+        return !BleachPasses<Real, C, G, c + 1>::run(qA, qB, pA, pB, es, lo, hi, lo2, U, ble, tri, lane);
+    }
+};
 template <typename Real, int C, int G>
+struct BleachPasses<Real, C, G, C> {
+    __device__ static inline bool run(Real*, Real*, Real*, Real*, int, int*, int*, const int*, const int*,
+                                      const Real*, int, int) {
+        return false;
+    }
+};
+
+// score_pair with the composite Bleach applied as C separable passes: per
+// step, the state is bleached channel by channel onto the box
+// X = prod_c [lo2_c, U_c) (the next box plus the Edman source row n + 1_c),
+// then Edman and Emit read X. Same recursion and scaling as score_pair, with
+// per-state work proportional to the box widths instead of the box sizes.
+// Every array of a step must fit in cap entries.
+template <typename Real, int C, int G, bool kRescale>
+__device__ PairResult<Real> score_pair_sep(const ClassifyArgs& a, int i, int lane, Real* st, int cap,
+                                           const int* sL, const int* sH, const int* eoff,
+                                           const int* estr, const Real* etab, const short* eexp,
+                                           const Real* bin, int* gseq, int* gK) {
+    const int T = a.m.T;
+    const int tri = a.tri;
+    int D[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) D[c] = a.dsD[(size_t)i * C + c];
+    for (int e = lane; e < T; e += G) {
+        int ch = a.dsSeq[(size_t)i * T + e];
+        gseq[e] = ch;
+        int K = ch >= 0 ? (int)a.dsTrack[((size_t)i * T + e) * C + ch] : 1;
+        gK[e] = K;
+        ((float*)(gK + T))[e] = 1.0f / (float)K;
+    }
+    const float* ginvK = (const float*)(gK + T);
+    const Real pf = (Real)a.m.pf;
+    const Real one_m_pf = (Real)(1.0 - a.m.pf);
+    const Real* dud = bin;
+    const Real* ble = bin + C * tri;
+    Real* curA = st;
+    Real* curB = st + cap;
+    Real* altA = st + 2 * cap;
+    Real* altB = st + 3 * cap;
+
+    // --- t = 0: InitialBlock, Dud_c, Emit(0) over [0,1) x [L0, H0) ---------
+    int lo[C], hi[C], w[C], str[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        lo[c] = sL[c];
+        hi[c] = min(sH[c], D[c] + 1);
+        w[c] = hi[c] - lo[c];
+    }
+    int nb = box_strides<C>(w, str);
+    int logacc = 0;
+    Real S = 0;
+    {
+        const Real pib = (Real)a.m.p_ib;
+        int K0 = 0;
+        if constexpr (kRescale) {
+            int km = kNoExp;
+            for (int q = lane; q < nb; q += G) {
+                int rem = q, ei = eoff[0];
+                Real v = 1;
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    int n = lo[c] + rem / str[c];
+                    rem -= (n - lo[c]) * str[c];
+                    v = dud[c * tri + D[c] * (D[c] + 1) / 2 + n] * v;
+                    ei += (n - sL[c]) * estr[c];
+                }
+                km = max(km, exp_of(v) + (int)eexp[ei]);
+            }
+            km = group_max_int<G>(km);
+            K0 = km == kNoExp ? 0 : km;
+            logacc += K0;
+        }
+        for (int q = lane; q < nb; q += G) {
+            int rem = q, ei = eoff[0];
+            Real va = (Real)1 - pib, vb = pib;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                int n = lo[c] + rem / str[c];
+                rem -= (n - lo[c]) * str[c];
+                Real p = dud[c * tri + D[c] * (D[c] + 1) / 2 + n];
+                va = p * va;
+                vb = p * vb;
+                ei += (n - sL[c]) * estr[c];
+            }
+            Real em = emission<Real, kRescale>(etab, eexp, ei, K0);
+            va *= em;
+            vb *= em;
+            curA[q] = va;
+            curB[q] = vb;
+            S += va + vb;
+        }
+        S = group_sum<Real, G>(S);
+    }
+    Real delta = 0;
+    wave_sync();
+
+    for (int t = 1; t < T; t++) {
+        const int* Lt = sL + t * C;
+        const int* Ht = sH + t * C;
+        int lo2[C], hi2[C], U[C], w2[C], str2[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            lo2[c] = Lt[c];
+            hi2[c] = min(Ht[c], D[c] + 1);
+            U[c] = min(hi2[c] + 1, hi[c]);
+            w2[c] = hi2[c] - lo2[c];
+        }
+        // Bleach_c for every channel: (cur) -> X, rows e < t
+        int xlo[C], xhi[C], xw[C], xstr[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            xlo[c] = lo[c];
+            xhi[c] = hi[c];
+        }
+        const bool in_alt = BleachPasses<Real, C, G, 0>::run(curA, curB, altA, altB, t, xlo, xhi, lo2, U, ble,
+                                                             tri, lane);
+        const Real* XA = in_alt ? altA : curA;
+        const Real* XB = in_alt ? altB : curB;
+        Real* Aout = in_alt ? curA : altA;
+        Real* Bout = in_alt ? curB : altB;
+#pragma unroll
+        for (int c = 0; c < C; c++) xw[c] = xhi[c] - xlo[c];
+        box_strides<C>(xw, xstr);
+
+        const int nb2 = box_strides<C>(w2, str2);
+        const int nout = (t + 1) * nb2;
+        const Divider by_e(t + 1);
+        Divider by_str[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) by_str[c] = Divider(str2[c]);
+        const int eo = eoff[t];
+        const int* es = estr + t * C;
+        const Real pd = (Real)a.m.p_detach[t - 1];
+        const Real pcb = (Real)a.m.p_cb;
+        delta = delta + pd * S;
+        const Real cAA = ((Real)1 - pd) * ((Real)1 - pcb);
+        const Real cB = (Real)1 - pd;
+        int km = kNoExp;
+        Real Snew = 0;
+        // Edman (edman-transition.cc:62-162) and Emit(t), reading X
+        for (int q = lane; q < nout; q += G) {
+            const int ni = by_e.div(q);
+            const int e = q - ni * (t + 1);
+            int n[C];
+            int rem = ni, ei = eo, xi = 0;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const int qc = by_str[c].div(rem);
+                n[c] = lo2[c] + qc;
+                rem -= qc * str2[c];
+                ei += (n[c] - Lt[c]) * es[c];
+                xi += qc * xstr[c];
+            }
+            const Real blA = e < t ? XA[xi * t + e] : (Real)0;
+            const Real blB = e < t ? XB[xi * t + e] : (Real)0;
+            Real acc = 0;
+            if (e >= 1) {
+                const int ce = gseq[e - 1];
+                const Real p1 = XA[xi * t + e - 1];
+                if (ce < 0) {
+                    acc += one_m_pf * p1;
+                } else {
+                    const int Ke = gK[e - 1];
+                    int kc = 0, Uc = 0, sc = 0;
+#pragma unroll
+                    for (int c = 0; c < C; c++)
+                        if (c == ce) {
+                            kc = n[c];
+                            Uc = U[c];
+                            sc = xstr[c];
+                        }
+                    Real ratio, ratio2;
+                    if constexpr (kRescale) {
+                        const float inv = ginvK[e - 1];
+                        ratio = (Real)kc * inv;
+                        ratio2 = (Real)(kc + 1) * inv;
+                    } else {
+                        ratio = (Real)kc / (Real)Ke;
+                        ratio2 = (Real)(kc + 1) / (Real)Ke;
+                    }
+                    if (kc < Ke) acc += one_m_pf * ((Real)1 - ratio) * p1;
+                    if (kc + 1 < Uc) acc += one_m_pf * ratio2 * XA[(xi + sc) * t + e - 1];
+                }
+            }
+            acc += pf * blA;
+            Real ao = cAA * acc;
+            Real bo = e < t ? cB * (blB + pcb * blA) : (Real)0;
+            if constexpr (kRescale) {
+                Aout[q] = ao;
+                Bout[q] = bo;
+                km = max(km, max(exp_of(ao), exp_of(bo)) + (int)eexp[ei]);
+            } else {
+                Real em = etab[ei];
+                ao *= em;
+                bo *= em;
+                Aout[q] = ao;
+                Bout[q] = bo;
+                Snew += ao + bo;
+            }
+        }
+        bool allow = true;
+#pragma unroll
+        for (int c = 0; c < C; c++) allow = allow && lo2[c] == 0;
+        int ei0 = eo;
+#pragma unroll
+        for (int c = 0; c < C; c++) ei0 -= Lt[c] * es[c];
+        if constexpr (kRescale) {
+            km = group_max_int<G>(km);
+            if (allow) km = max(km, exp_of(delta) + (int)eexp[ei0]);
+            const int K = km == kNoExp ? 0 : km;
+            logacc += K;
+            for (int q = lane; q < nout; q += G) {
+                const int ni = by_e.div(q);
+                int rem = ni, ei = eo;
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    const int nc = by_str[c].div(rem);
+                    rem -= nc * str2[c];
+                    ei += (nc + lo2[c] - Lt[c]) * es[c];
+                }
+                Real em = emission<Real, kRescale>(etab, eexp, ei, K);
+                Real ao = Aout[q] * em, bo = Bout[q] * em;
+                Aout[q] = ao;
+                Bout[q] = bo;
+                Snew += ao + bo;
+            }
+            delta = allow ? delta * emission<Real, kRescale>(etab, eexp, ei0, K) : (Real)0;
+        } else {
+            delta = allow ? delta * etab[ei0] : (Real)0;
+        }
+        S = group_sum<Real, G>(Snew);
+        wave_sync();
+        // the new state is in (Aout, Bout); the other pair is free
+        Real* fa = in_alt ? altA : curA;
+        Real* fb = in_alt ? altB : curB;
+        curA = Aout;
+        curB = Bout;
+        altA = fa;
+        altB = fb;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            lo[c] = lo2[c];
+            hi[c] = hi2[c];
+            w[c] = w2[c];
+            str[c] = str2[c];
+        }
+        nb = nb2;
+    }
+
+    PairResult<Real> res;
+    if (kRescale) {
+        double s = (double)S + (double)delta;
+        res.logp = s > 0 ? log(s) + (double)logacc * 0.69314718055994530942 : -INFINITY;
+        res.p = exp(res.logp);
+    } else {
+        double s = (double)S + (double)delta;
+        res.p = s;
+        res.logp = log(s);
+    }
+    wave_sync();
+    return res;
+}
+
+// Entries the separable passes of read r need at most (read-level bounds,
+// an upper bound for every candidate): t * |box| for each intermediate box
+// of step t, (t + 1) * |box(t)| for the state.
+template <int C>
+__device__ inline int sep_need(int T, const int* sL, const int* sH) {
+    int need = 0;
+    for (int t = 0; t < T; t++) {
+        int s = t + 1;
+        for (int c = 0; c < C; c++) s *= max(sH[t * C + c] - sL[t * C + c], 0);
+        need = max(need, s);
+        if (t == 0) continue;
+        for (int k = 0; k < C; k++) {  // after the pass of channel k
+            int v = t;
+            for (int c = 0; c < C; c++) {
+                const int h1 = sH[(t - 1) * C + c], l1 = sL[(t - 1) * C + c];
+                const int u = min(sH[t * C + c] + 1, h1);
+                v *= c <= k ? max(u - sL[t * C + c], 0) : max(h1 - l1, 0);
+            }
+            need = max(need, v);
+        }
+    }
+    return need;
+}
+
+// One read of the gather kernel. Block-uniform returns only.
+template <typename Real, int C, int G, bool kSep>
 __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, unsigned char* smem) {
     constexpr bool kRescale = sizeof(Real) == 4;
     if (r >= a.N) return;
@@ -464,7 +815,7 @@ __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, 
     const int ngroups = a.ngroups;
     const int group = tid / G;
     const int lane = tid % G;
-    const Layout lay = make_layout<Real>(T, C, a.tri, a.NB, ngroups, G, nthreads);
+    const Layout lay = make_layout<Real>(T, C, a.tri, a.NB, a.cap, ngroups, G, nthreads);
     double* s_rad = (double*)(smem + lay.rad);
     int* sL = (int*)(smem + lay.L);
     int* sH = (int*)(smem + lay.H);
@@ -535,7 +886,22 @@ __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, 
     // The pair is non-empty iff D_c >= L[0][c] for every channel.
 
     CandAcc<kRescale> acc;
-    const int cap = T * a.NB;
+    const int cap = a.cap;
+    if constexpr (kSep) {
+        // a read whose separable passes could outgrow the state buffers is
+        // handed to the composite form (this bin's wide pass)
+        if (tid == 0) scan[0] = sep_need<C>(T, sL, sH) > cap ? 1 : 0;
+        __syncthreads();
+        const bool over = scan[0] != 0;
+        __syncthreads();
+        if (over) {
+            if (tid == 0) {
+                a.wide[r] = 1;
+                a.wide_list[atomicAdd(a.wide_count, 1)] = r;
+            }
+            return;
+        }
+    }
     Real* st = state + (size_t)group * 4 * cap;
     int* gseq = gseq_all + group * 3 * T;
     int* gK = gseq + T;
@@ -545,8 +911,13 @@ __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, 
             for (int q = group; q < nlive; q += ngroups) {
                 const int kk = list[q];
                 const int i = cidx ? cidx[kk] : kk;
-                PairResult<Real> pr = score_pair<Real, C, G, kRescale>(a, i, lane, st, cap, sL, sH, eoff,
-                                                                        estr, etab, eexp, bin, gseq, gK);
+                PairResult<Real> pr;
+                if constexpr (kSep)
+                    pr = score_pair_sep<Real, C, G, kRescale>(a, i, lane, st, cap, sL, sH, eoff, estr, etab, eexp,
+                                                              bin, gseq, gK);
+                else
+                    pr = score_pair<Real, C, G, kRescale>(a, i, lane, st, cap, sL, sH, eoff, estr, etab, eexp,
+                                                          bin, gseq, gK);
                 if (a.out_pair_logp && !cidx && lane == 0) a.out_pair_logp[(size_t)r * a.S + i] = pr.logp;
                 acc.add(pr.logp, pr.p, kk, a.dsCount[i]);
             }
@@ -557,33 +928,33 @@ __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, 
 }
 
 // One workgroup per read of the bin's compacted list.
-template <typename Real, int C, int G>
+template <typename Real, int C, int G, bool kSep>
 __global__ void __launch_bounds__(kBlockThreads) WP_CLASSIFY_OCCUPANCY
 classify_kernel(const ClassifyArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     if ((int)blockIdx.x >= *a.list_count) return;
-    gather_read<Real, C, G>(a, a.list[blockIdx.x], smem);
+    gather_read<Real, C, G, kSep>(a, a.list[blockIdx.x], smem);
 }
 
-template <typename Real, int C, int G>
+template <typename Real, int C, int G, bool kSep>
 hipError_t launch_t(const ClassifyArgs& a, hipStream_t stream) {
     int nthreads = a.ngroups * G;
     if (nthreads < 64) nthreads = 64;
     nthreads = (nthreads + 63) / 64 * 64;
-    Layout l = make_layout<Real>(a.m.T, C, a.tri, a.NB, a.ngroups, G, nthreads);
-    const void* fn = (const void*)classify_kernel<Real, C, G>;
+    Layout l = make_layout<Real>(a.m.T, C, a.tri, a.NB, a.cap, a.ngroups, G, nthreads);
+    const void* fn = (const void*)classify_kernel<Real, C, G, kSep>;
     if (l.total > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((classify_kernel<Real, C, G>), dim3(a.N), dim3(nthreads), l.total, stream, a);
+    hipLaunchKernelGGL((classify_kernel<Real, C, G, kSep>), dim3(a.N), dim3(nthreads), l.total, stream, a);
     return hipGetLastError();
 }
 
 template <typename Real, int C>
 hipError_t launch_c(const ClassifyArgs& a, int G, hipStream_t stream) {
     if (G != kGroupLanes) return hipErrorInvalidValue;
-    return launch_t<Real, C, kGroupLanes>(a, stream);
+    return a.variant ? launch_t<Real, C, kGroupLanes, true>(a, stream) : launch_t<Real, C, kGroupLanes, false>(a, stream);
 }
 
 template <typename Real>
@@ -703,12 +1074,12 @@ void launch_work(const DevModel& m, int N, int S, const uint8_t* d_Lr, const uin
                            d_nbmax, d_D, d_out);
 }
 
-size_t classify_lds_bytes(int precision, int T, int C, int tri, int NB, int ngroups, int G) {
+size_t classify_lds_bytes(int precision, int T, int C, int tri, int NB, int cap, int ngroups, int G) {
     int nthreads = ngroups * G;
     if (nthreads < 64) nthreads = 64;
     nthreads = (nthreads + 63) / 64 * 64;
-    if (precision == WP_PRECISION_FP32) return make_layout<float>(T, C, tri, NB, ngroups, G, nthreads).total;
-    return make_layout<double>(T, C, tri, NB, ngroups, G, nthreads).total;
+    if (precision == WP_PRECISION_FP32) return make_layout<float>(T, C, tri, NB, cap, ngroups, G, nthreads).total;
+    return make_layout<double>(T, C, tri, NB, cap, ngroups, G, nthreads).total;
 }
 
 }  // namespace wp
