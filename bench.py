@@ -90,7 +90,7 @@ def main():
     ap.add_argument("--peptides-per-dye-seq", type=int, default=1000)
     ap.add_argument("--train-reads", type=int, default=10000000)
     ap.add_argument("--epochs", type=int, default=2)
-    ap.add_argument("--tracks-per-dye-seq", type=int, default=100)
+    ap.add_argument("--tracks-per-dye-seq", type=int, default=1000)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -251,11 +251,12 @@ def bench_hybrid(args):
     tracks = w.simulate_dye_tracks(model, T, seqs, args.tracks_per_dye_seq, 2002, gpu=True)
     t_sim = time.perf_counter() - t0
     rads, true_idx = w.simulate_radiometries(model, T, strings, N, 3002, gpu=True)
-    log(f"{len(tracks)} distinct dye-tracks ({t_sim:.1f} s); building classifiers")
+    log(f"{len(tracks)} distinct dye-tracks ({t_sim:.1f} s); building the hybrid classifier (host kd-tree)")
     prec = w.WP_PRECISION_FP32 if args.precision == "fp32" else w.WP_PRECISION_FP64
     t0 = time.perf_counter()
     hc = w.HybridClassifier(T, C, model, w.SequencingSettings(cutoff), k, sig, tracks, H, seqs, prec)
     t_build = time.perf_counter() - t0
+    log(f"built in {t_build:.1f} s; building the kNN classifier for the search-only timing")
     nn = w.NNClassifier(T, C, model, k, sig, tracks)
     log("warmup")
     for _ in range(args.warmup):
@@ -300,6 +301,7 @@ def bench_hybrid(args):
         ref = Reference.load()
         if ref is not None:
             n = min(N, 200)
+            log(f"CPU baseline: the reference's HybridClassifier on {n} reads (its kd-tree build first)")
             # construction alone (kd-tree build, precomputations), then with n reads
             t1 = time.perf_counter()
             ref.hybrid_classify(model, tracks, k, sig, H, strings, [1] * S, list(range(S)), rads[:0], cutoff)
@@ -308,6 +310,7 @@ def bench_hybrid(args):
             rid, _, _ = ref.hybrid_classify(model, tracks, k, sig, H, strings, [1] * S, list(range(S)), rads[:n],
                                             cutoff)
             d1 = time.perf_counter() - t1 - d0
+            log(f"reference: construction {d0:.1f} s, {n} reads {d1:.1f} s")
             result["cpu_baseline"] = {"value": n / d1, "unit": "reads/s", "cores": int(Oracle().L.wpo_num_threads()),
                                       "kind": "reference",
                                       "sample": f"{n} reads of the same workload through HybridClassifier::classify "

@@ -98,7 +98,8 @@ struct wp_hmm_classifier {
     int* wNb = nullptr;
     uint8_t* wWide = nullptr;  // per read: box densities too wide for the quad-rows kernel
     int* wLists = nullptr;      // [kMaxBins][N] reads of each bin
-    int* wWideLists = nullptr;  // [kMaxBins][N] reads each quad-rows bin hands to its wide pass
+    int* wWideLists = nullptr;  // [kMaxBins][N] reads each bin's main launch hands to its wide pass
+    int* wWide2Lists = nullptr; // [kMaxBins][N] gather bins: reads the full-size separable pass hands on
     int* wCtr = nullptr;        // counters: bin sizes, wide-list sizes, one queue per launch
     int wcap = 0;
     int maxbox = 1;  // prod_c (M+1), bound on any read's box
@@ -196,10 +197,11 @@ size_t bin_lds_bytes(const wp_hmm_classifier* c, const Bin& b) {
     }
 }
 
-// Counter block of one call: bin sizes, then wide-list sizes.
+// Counter block of one call: bin sizes, then the sizes of the two wide lists.
 constexpr int kCtrCount = 0;
 constexpr int kCtrWide = wp::kMaxBins;
-constexpr int kCtrLen = 2 * wp::kMaxBins;
+constexpr int kCtrWide2 = 2 * wp::kMaxBins;
+constexpr int kCtrLen = 3 * wp::kMaxBins;
 
 void drop_graph(wp_hmm_classifier* c) {
     if (c->gexec) hipGraphExecDestroy(c->gexec);
@@ -216,11 +218,12 @@ int ensure_workspace(wp_hmm_classifier* c, int N) {
     hipFree(c->wWide);
     hipFree(c->wLists);
     hipFree(c->wWideLists);
+    hipFree(c->wWide2Lists);
     hipFree(c->wCtr);
     c->wL = c->wH = nullptr;
     c->wNb = nullptr;
     c->wWide = nullptr;
-    c->wLists = c->wWideLists = c->wCtr = nullptr;
+    c->wLists = c->wWideLists = c->wWide2Lists = c->wCtr = nullptr;
     c->wcap = 0;
     size_t tc = (size_t)N * c->T * c->C;
     WP_HIP(hipMalloc(&c->wL, tc));
@@ -229,6 +232,7 @@ int ensure_workspace(wp_hmm_classifier* c, int N) {
     WP_HIP(hipMalloc(&c->wWide, (size_t)N));
     WP_HIP(hipMalloc(&c->wLists, sizeof(int) * (size_t)wp::kMaxBins * N));
     WP_HIP(hipMalloc(&c->wWideLists, sizeof(int) * (size_t)wp::kMaxBins * N));
+    WP_HIP(hipMalloc(&c->wWide2Lists, sizeof(int) * (size_t)wp::kMaxBins * N));
     WP_HIP(hipMalloc(&c->wCtr, sizeof(int) * kCtrLen));
     c->wcap = N;
     return WP_OK;
@@ -311,11 +315,13 @@ hipError_t launch_bin_main(wp_hmm_classifier* c, const std::vector<Bin>& bins, i
     return wp::launch_classify(a, c->precision, c->G, stream);
 }
 
-// The wide pass of a bin: the reads its main launch handed over (quad-rows:
-// densities too wide for one exponent per timestep; gather: boxes too large
-// for the separable passes), scored by the per-entry exponent kernels.
+// The wide pass of a bin: the reads its main launch handed over. Quad-rows
+// bins: densities too wide for one exponent per timestep, scored by the
+// per-entry exponent kernels. Gather bins: boxes too large for the
+// half-size separable launch; stage 1 runs the separable passes at the
+// bin's full capacity, stage 2 the composite form on what still does not fit.
 hipError_t launch_bin_wide(wp_hmm_classifier* c, const std::vector<Bin>& bins, int b, const CallArgs& x,
-                           hipStream_t stream) {
+                           int stage, hipStream_t stream) {
     const Bin& bin = bins[b];
     wp::ClassifyArgs a = base_args(c, x);
     a.nb_lo = bin.lo;
@@ -324,8 +330,14 @@ hipError_t launch_bin_wide(wp_hmm_classifier* c, const std::vector<Bin>& bins, i
     a.cap = c->T * bin.hi;
     a.ngroups = bin.ngroups;
     a.wide_pass = 1;
-    a.list = c->wWideLists + (size_t)b * x.N;
-    a.list_count = c->wCtr + kCtrWide + b;
+    a.list = (stage == 2 ? c->wWide2Lists : c->wWideLists) + (size_t)b * x.N;
+    a.list_count = c->wCtr + (stage == 2 ? kCtrWide2 : kCtrWide) + b;
+    if (bin.kernel == kGather && stage == 1) {
+        a.variant = 1;
+        a.wide_list = c->wWide2Lists + (size_t)b * x.N;
+        a.wide_count = c->wCtr + kCtrWide2 + b;
+        return wp::launch_classify(a, c->precision, c->G, stream);
+    }
     if (wp::v3_supports(c->precision, c->T, bin.hi) &&
         wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, bin.hi) <= lds_hard_limit())
         return wp::launch_classify_v3(a, c->precision, stream);
@@ -341,7 +353,8 @@ int enqueue_call(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream, bo
     if (events) WP_HIP(hipEventRecord(c->ev[1], stream));
     for (int b = 0; b < (int)bins.size(); b++) {
         WP_HIP(launch_bin_main(c, bins, b, x, c->variant[b], stream));
-        if (bins[b].kernel != kRowPerLane) WP_HIP(launch_bin_wide(c, bins, b, x, stream));
+        if (bins[b].kernel != kRowPerLane) WP_HIP(launch_bin_wide(c, bins, b, x, 1, stream));
+        if (bins[b].kernel == kGather) WP_HIP(launch_bin_wide(c, bins, b, x, 2, stream));
     }
     if (events) {
         WP_HIP(hipEventRecord(c->ev[2], stream));
@@ -588,6 +601,7 @@ void wp_hmm_classifier_destroy(wp_hmm_classifier* c) {
     if (c->cap_stream) hipStreamDestroy(c->cap_stream);
     hipFree(c->wLists);
     hipFree(c->wWideLists);
+    hipFree(c->wWide2Lists);
     hipFree(c->wCtr);
     hipFree(c->dD);
     hipFree(c->dSeq);

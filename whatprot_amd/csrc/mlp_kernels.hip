@@ -367,51 +367,79 @@ __global__ void __launch_bounds__(256) topk_kernel(int N, int H, const float* __
 }
 
 // Top-H for rows of at most 256 * PER logits, held in registers (element
-// n = tid + 256 j in key[j]): the H-th largest key K is found bit by bit,
-// MSB first, as the largest K with #(key >= K) >= H, each count a sum of
-// wave ballots (no atomics). Output positions in increasing index order come
-// from per-(j, wave) ballot counts of keys > K and == K, scanned in (j, wave)
-// order, which is index order.
-template <int PER>
-__global__ void __launch_bounds__(256) topk_reg_kernel(int N, int H, const float* __restrict__ Z, long ldz,
-                                                        int* __restrict__ out, const float* __restrict__ lse,
-                                                        float* __restrict__ mass) {
-    __shared__ int red[2][4];
-    __shared__ int cnt[2][PER][4];
+// n = tid + 256 j in key[j]): the H-th largest key K is found two bits at a
+// time, MSB first, as the largest K with #(key >= K) >= H; each round counts
+// the keys at or above the three candidate extensions of K in registers and
+// reduces them over the wave and the block (no atomics, one barrier per
+// round). Output positions in increasing index order come from per-(j, wave)
+// ballot counts of keys > K and == K, scanned in (j, wave) order, which is
+// index order.
+template <int PER, int NT>
+__global__ void __launch_bounds__(NT) topk_reg_kernel(int N, int H, const float* __restrict__ Z, long ldz,
+                                                       int* __restrict__ out, const float* __restrict__ lse,
+                                                       float* __restrict__ mass) {
+    constexpr int NW = NT / 64;
+    __shared__ int red[2][3][NW];
+    __shared__ int cnt[2][PER][NW];
     __shared__ int tot[2];
-    __shared__ float fred[4];
+    __shared__ float fred[NW];
     const int r = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const float* z = Z + (long)r * ldz;
     unsigned int key[PER];
 #pragma unroll
     for (int j = 0; j < PER; j++) {
-        const int n = tid + 256 * j;
+        const int n = tid + NT * j;
         key[j] = n < N ? fkey(z[n]) : 0u;  // padding: below every finite logit
     }
     unsigned int K = 0;
-    for (int bit = 31; bit >= 0; bit--) {
-        const unsigned int c = K | (1u << bit);
-        int cw = 0;
+#pragma unroll 1
+    for (int round = 0; round < 16; round++) {
+        const int sh = 30 - 2 * round;  // bits sh+1, sh of K decided this round
+        int c1 = 0, c2 = 0, c3 = 0;
 #pragma unroll
-        for (int j = 0; j < PER; j++) cw += __popcll(__ballot(key[j] >= c));
-        if (lane == 0) red[bit & 1][w] = cw;
-        __syncthreads();  // double-buffered: one barrier per bit
-        const int total = red[bit & 1][0] + red[bit & 1][1] + red[bit & 1][2] + red[bit & 1][3];
-        if (total >= H) K = c;
+        for (int j = 0; j < PER; j++) {
+            const unsigned int d = key[j] >= K ? (key[j] - K) >> sh : 0u;  // the key's next two bits, above K
+            c1 += d >= 1u ? 1 : 0;
+            c2 += d >= 2u ? 1 : 0;
+            c3 += d >= 3u ? 1 : 0;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            c1 += __shfl_xor(c1, off, 64);
+            c2 += __shfl_xor(c2, off, 64);
+            c3 += __shfl_xor(c3, off, 64);
+        }
+        const int buf = round & 1;
+        if (lane == 0) {
+            red[buf][0][w] = c1;
+            red[buf][1][w] = c2;
+            red[buf][2][w] = c3;
+        }
+        __syncthreads();  // double-buffered: one barrier per round
+        int t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+        for (int v = 0; v < NW; v++) {
+            t1 += red[buf][0][v];
+            t2 += red[buf][1][v];
+            t3 += red[buf][2][v];
+        }
+        const unsigned int add = t3 >= H ? 3u : t2 >= H ? 2u : t1 >= H ? 1u : 0u;
+        K |= add << sh;
     }
 #pragma unroll
     for (int j = 0; j < PER; j++) {
-        const int n = tid + 256 * j;
+        const int n = tid + NT * j;
         const unsigned long long bg = __ballot(key[j] > K), be = __ballot(key[j] == K && n < N);
         if (lane == 0) {
             cnt[0][j][w] = __popcll(bg);
             cnt[1][j][w] = __popcll(be);
         }
+        __builtin_amdgcn_sched_barrier(0);  // keeps the ballot masks of one j live at a time
     }
     __syncthreads();
     if (w < 2) {  // wave 0 scans the > K counts, wave 1 the == K counts
         int* a = &cnt[w][0][0];
-        constexpr int M = 4 * PER, PL = (M + 63) / 64;
+        constexpr int M = NW * PER, PL = (M + 63) / 64;
         const int b = lane * PL, e = min(M, b + PL);
         int sum = 0;
         for (int i = b; i < e; i++) sum += a[i];
@@ -437,7 +465,7 @@ __global__ void __launch_bounds__(256) topk_reg_kernel(int N, int H, const float
     float ms = 0.0f;
 #pragma unroll
     for (int j = 0; j < PER; j++) {
-        const int n = tid + 256 * j;
+        const int n = tid + NT * j;
         const bool g = key[j] > K, e = key[j] == K && n < N;
         const unsigned long long bg = __ballot(g), be = __ballot(e);
         const int gpos = cnt[0][j][w] + __popcll(bg & below);
@@ -448,13 +476,19 @@ __global__ void __launch_bounds__(256) topk_reg_kernel(int N, int H, const float
             const unsigned int k = key[j];
             ms += expf(__uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k) - l);
         }
+        __builtin_amdgcn_sched_barrier(0);
     }
     if (mass) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) ms += __shfl_xor(ms, off, 64);
         if (lane == 0) fred[w] = ms;
         __syncthreads();
-        if (tid == 0) mass[r] = fred[0] + fred[1] + fred[2] + fred[3];
+        if (tid == 0) {
+            float m = 0.0f;
+#pragma unroll
+            for (int v = 0; v < NW; v++) m += fred[v];
+            mass[r] = m;
+        }
     }
 }
 
@@ -584,18 +618,20 @@ hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, cons
                 hipStream_t s) {
     if (rows <= 0) return hipSuccess;
     if (N > H) {  // register-resident rows up to 32768 logits
-#define WP_TOPK_REG(P)                                                                                          \
-    if (N <= 256 * (P)) {                                                                                       \
-        hipLaunchKernelGGL(topk_reg_kernel<P>, dim3(rows), dim3(256), 0, s, N, H, Z, ldz, out, lse, mass);       \
+        // at most 32 keys per thread: 256 threads per row up to 8192 logits,
+        // 1024 threads per row beyond
+#define WP_TOPK_REG(P, NT)                                                                                      \
+    if (N <= (NT) * (P)) {                                                                                      \
+        hipLaunchKernelGGL((topk_reg_kernel<P, NT>), dim3(rows), dim3(NT), 0, s, N, H, Z, ldz, out, lse, mass);  \
         return hipGetLastError();                                                                               \
     }
-        WP_TOPK_REG(4)
-        WP_TOPK_REG(8)
-        WP_TOPK_REG(16)
-        WP_TOPK_REG(32)
-        WP_TOPK_REG(64)
-        WP_TOPK_REG(80)
-        WP_TOPK_REG(128)
+        WP_TOPK_REG(4, 256)
+        WP_TOPK_REG(8, 256)
+        WP_TOPK_REG(16, 256)
+        WP_TOPK_REG(32, 256)
+        WP_TOPK_REG(12, 1024)
+        WP_TOPK_REG(20, 1024)
+        WP_TOPK_REG(32, 1024)
 #undef WP_TOPK_REG
     }
     hipLaunchKernelGGL(topk_kernel, dim3(rows), dim3(256), 0, s, N, H, Z, ldz, out, lse, mass);
