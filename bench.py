@@ -46,6 +46,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3
+# HBM bytes of one classify step of the default workload (range, bounds, binning and
+# classify kernels): FETCH_SIZE 63.6 MiB x 2.0 + WRITE_SIZE 30.5 MiB x 1.0, each scaled by a
+# 512 MiB device copy in the same rocprofv3 --pmc runs.
+CLASSIFY_HBM_BYTES = 162_000_000
 
 
 def log(msg):
@@ -202,9 +206,12 @@ def main():
         "nonempty_pairs": pairs_nonempty,
         "state_timesteps": states,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "classify_kernel (all bins)", "kernel_ms": classify_ms,
-                     "flops_per_launch": flops},
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": CLASSIFY_HBM_BYTES if N == 100000 and S == 1000
+                     else None,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per step "
+                                       "(profiles/classify_fp32_100k_x_1k_cut5.md)",
+                     "kernel": "the classify launches of one call (quad-rows, gather and wide passes of every bin)",
+                     "kernel_ms": classify_ms, "flops_per_launch": flops},
     }
     if not args.no_fp64_check and prec == w.WP_PRECISION_FP32:
         log("FP64 reference run for top-1 agreement")
