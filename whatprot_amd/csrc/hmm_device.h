@@ -31,16 +31,19 @@ __device__ inline double dev_adjusted_mu(const DevModel& m, int channel, const u
     for (int c = 0; c < C; c++)
         if (c == channel) own = counts[c];
     amu *= (double)own;
+    // A factor of exactly 1.0 (no interaction, the usual case) multiplies by
+    // pow(1.0, n) == 1.0: skipping it leaves amu bit for bit unchanged.
 #pragma unroll
     for (int c = 0; c < C; c++) {
+        const double fl = m.flat[channel][c], in = m.inter[channel][c];
         if (c == channel) {
             if (counts[c] > 1) {
-                amu *= m.flat[channel][c];
-                amu *= pow(m.inter[channel][c], (double)(counts[c] - 1));
+                if (fl != 1.0) amu *= fl;
+                if (in != 1.0) amu *= pow(in, (double)(counts[c] - 1));
             }
         } else if (counts[c] > 0) {
-            amu *= m.flat[channel][c];
-            amu *= pow(m.inter[channel][c], (double)counts[c]);
+            if (fl != 1.0) amu *= fl;
+            if (in != 1.0) amu *= pow(in, (double)counts[c]);
         }
     }
     return amu;

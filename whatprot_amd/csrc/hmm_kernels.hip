@@ -775,22 +775,27 @@ __device__ PairResult<Real> score_pair_sep(const ClassifyArgs& a, int i, int lan
     return res;
 }
 
-// Entries the separable passes of read r need at most (read-level bounds,
-// an upper bound for every candidate): t * |box| for each intermediate box
-// of step t, (t + 1) * |box(t)| for the state.
-template <int C>
-__device__ inline int sep_need(int T, const int* sL, const int* sH) {
+// Entries a pair's forward pass needs in each state buffer, with the box
+// bounds capped at D_c + 1 (cap[c]; pass a large cap for the read-level
+// bound of every candidate): (t + 1) * |box(t)| for the state, and with
+// kSepPasses also t * |box| for each intermediate box of the separable
+// passes of step t.
+template <int C, bool kSepPasses>
+__device__ inline int state_need(int T, const int* sL, const int* sH, const int* cap) {
     int need = 0;
     for (int t = 0; t < T; t++) {
         int s = t + 1;
-        for (int c = 0; c < C; c++) s *= max(sH[t * C + c] - sL[t * C + c], 0);
+#pragma unroll
+        for (int c = 0; c < C; c++) s *= max(min(sH[t * C + c], cap[c]) - sL[t * C + c], 0);
         need = max(need, s);
-        if (t == 0) continue;
+        if (!kSepPasses || t == 0) continue;
+#pragma unroll
         for (int k = 0; k < C; k++) {  // after the pass of channel k
             int v = t;
+#pragma unroll
             for (int c = 0; c < C; c++) {
-                const int h1 = sH[(t - 1) * C + c], l1 = sL[(t - 1) * C + c];
-                const int u = min(sH[t * C + c] + 1, h1);
+                const int h1 = min(sH[(t - 1) * C + c], cap[c]), l1 = sL[(t - 1) * C + c];
+                const int u = min(min(sH[t * C + c], cap[c]) + 1, h1);
                 v *= c <= k ? max(u - sL[t * C + c], 0) : max(h1 - l1, 0);
             }
             need = max(need, v);
@@ -888,9 +893,15 @@ __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, 
     CandAcc<kRescale> acc;
     const int cap = a.cap;
     if constexpr (kSep) {
-        // a read whose separable passes could outgrow the state buffers is
-        // handed to the composite form (this bin's wide pass)
-        if (tid == 0) scan[0] = sep_need<C>(T, sL, sH) > cap ? 1 : 0;
+        // a read one of whose candidates could outgrow the state buffers even
+        // in the composite form is handed to this bin's wide pass; each pair
+        // of the others runs separable if its own passes fit, composite if not
+        if (tid == 0) {
+            int nocap[C];
+#pragma unroll
+            for (int c = 0; c < C; c++) nocap[c] = 1 << 20;
+            scan[0] = state_need<C, false>(T, sL, sH, nocap) > cap ? 1 : 0;
+        }
         __syncthreads();
         const bool over = scan[0] != 0;
         __syncthreads();
@@ -912,7 +923,14 @@ __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, 
                 const int kk = list[q];
                 const int i = cidx ? cidx[kk] : kk;
                 PairResult<Real> pr;
-                if constexpr (kSep)
+                bool sep = kSep;
+                if constexpr (kSep) {
+                    int dcap[C];
+#pragma unroll
+                    for (int c = 0; c < C; c++) dcap[c] = (int)a.dsD[(size_t)i * C + c] + 1;
+                    sep = state_need<C, true>(T, sL, sH, dcap) <= cap;  // uniform over the group
+                }
+                if (sep)
                     pr = score_pair_sep<Real, C, G, kRescale>(a, i, lane, st, cap, sL, sH, eoff, estr, etab, eexp,
                                                               bin, gseq, gK);
                 else

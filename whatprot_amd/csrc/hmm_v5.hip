@@ -305,14 +305,24 @@ __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsi
     // matrices are K_t x K_t, K_t + 4 entries apart (different LDS banks for
     // the shifted matrices of different channels).
     const double* ble = bin + C * tri;
-    for (int k = tid; k < T * (1 + C) * NB * NB; k += kV5Threads) {
-        const int t = k / ((1 + C) * NB * NB);
-        int rem = k - t * (1 + C) * NB * NB;
-        const int mat = rem / (NB * NB);
-        rem -= mat * NB * NB;
-        const int q = rem / NB, q2 = rem % NB;
-        const int K = kClass ? kcl[t] : NB;
-        if (t == 0 || q >= K || q2 >= K) continue;
+    // entries of the W blocks of t >= 1: (1 + C) matrices of K_t x K_t each
+    const int wtotal = kClass ? woff[T - 1] + (T > 1 ? (1 + C) * (kcl[T - 1] * kcl[T - 1] + 4) : 0)
+                              : (T - 1) * (1 + C) * MSNB;
+    for (int k = tid; k < wtotal; k += kV5Threads) {
+        int t = 1, K = NB, rem;
+        if constexpr (kClass) {
+            while (t + 1 < T && woff[t + 1] <= k) t++;
+            K = kcl[t];
+            rem = k - woff[t];
+        } else {
+            t = 1 + k / ((1 + C) * MSNB);
+            rem = k - (t - 1) * (1 + C) * MSNB;
+        }
+        const int ms = K * K + 4;
+        const int mat = rem / ms;
+        rem -= mat * ms;
+        if (rem >= K * K) continue;  // the 4-entry gap between matrices
+        const int q = rem / K, q2 = rem % K;
         double v = 0.0;
         if (t >= 1 && q < nbt[t - 1] && q2 < nbt[t]) {
             v = 1.0;
@@ -325,7 +335,7 @@ __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsi
                 else v *= ble[c * tri + from * (from + 1) / 2 + to];
             }
         }
-        Wall[(kClass ? woff[t] : t * (1 + C) * MSNB) + mat * (K * K + 4) + q * K + q2] = (float)v;
+        Wall[(kClass ? woff[t] : t * (1 + C) * MSNB) + mat * ms + rem] = (float)v;
     }
     __syncthreads();
 
@@ -465,8 +475,13 @@ __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsi
 // One workgroup per read of the bin's compacted list: a persistent loop over
 // a queue costs this kernel registers (the compiler keeps read-invariant
 // values live across reads), which lowers its occupancy.
+// Boxes of up to 8 states keep at least 2 waves per SIMD (the register
+// allocator otherwise drifts to 1 for the per-timestep-class form).
+template <int NB>
+constexpr int v5_min_waves() { return NB <= 8 ? 2 : 1; }
+
 template <int C, int NB, int R, bool kClass>
-__global__ void __launch_bounds__(kV5Threads) classify_v5_kernel(const ClassifyArgs a) {
+__global__ void __launch_bounds__(kV5Threads, v5_min_waves<NB>()) classify_v5_kernel(const ClassifyArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     if ((int)blockIdx.x >= *a.list_count) return;
     v5_read<C, NB, R, kClass>(a, a.list[blockIdx.x], smem);
