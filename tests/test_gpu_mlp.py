@@ -118,3 +118,36 @@ def test_shortlist_topk_matches_logits(gpu, S):
         p /= p.sum(axis=1, keepdims=True)
         np.testing.assert_allclose(post, p.max(axis=1), rtol=1e-4)
         np.testing.assert_allclose(mass, np.take_along_axis(p, ref, axis=1).sum(axis=1), rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("pattern", ["ties", "ascending", "distinct"])
+@pytest.mark.parametrize("S", [12000, 20000, 30000])
+def test_topk_filter_and_fallback(gpu, S, pattern):
+    """Rows whose logits are the last layer's biases (weights zeroed): heavy
+    ties and ascending values put more than 4096 logits above the group-maxima
+    threshold (the full register search runs), distinct random values keep the
+    filtered path; both give the top H of the logits, ties to the lower index,
+    in increasing index order."""
+    rng = np.random.default_rng(S + len(pattern))
+    net = w.ShortlistMLP(12, 64, 2, S, seed=3)
+    layers = net.get_layers()
+    W, b = layers[-1]
+    W[...] = 0.0
+    if pattern == "ties":
+        b[...] = rng.integers(0, 4, size=b.shape).astype(b.dtype)
+    elif pattern == "ascending":
+        b[...] = (np.arange(b.size) * 1e-3).reshape(b.shape).astype(b.dtype)
+    else:
+        b[...] = rng.normal(0.0, 2.0, size=b.shape).astype(b.dtype)
+    net.set_layers(layers)
+    x = rng.normal(0.0, 3.0, size=(40, 12))
+    z = net.logits(x.astype(np.float32)).astype(np.float64)
+    keys = z.astype(np.float32).view(np.int32).astype(np.int64)
+    keys = np.where(keys < 0, -(keys & 0x7FFFFFFF), keys)
+    for H in (1, 256):
+        idx, post, mass = net.shortlist(x, H, with_mass=True)
+        ref = np.sort(np.argsort(-keys, axis=1, kind="stable")[:, :H], axis=1)
+        np.testing.assert_array_equal(idx, ref)
+        p = np.exp(z - z.max(axis=1, keepdims=True))
+        p /= p.sum(axis=1, keepdims=True)
+        np.testing.assert_allclose(mass, np.take_along_axis(p, ref, axis=1).sum(axis=1), rtol=1e-4, atol=1e-6)

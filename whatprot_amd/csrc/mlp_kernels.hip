@@ -375,39 +375,25 @@ __global__ void __launch_bounds__(256) topk_kernel(int N, int H, const float* __
 // ballot counts of keys > K and == K, scanned in (j, wave) order, which is
 // index order.
 template <int PER, int NT>
-__global__ void __launch_bounds__(NT) topk_reg_kernel(int N, int H, const float* __restrict__ Z, long ldz,
-                                                       int* __restrict__ out, const float* __restrict__ lse,
-                                                       float* __restrict__ mass) {
+__device__ void topk_reg_row(int r, int N, int H, const unsigned int (&key)[PER], int* __restrict__ out,
+                             const float* __restrict__ lse, float* __restrict__ mass) {
     constexpr int NW = NT / 64;
     __shared__ int red[2][3][NW];
     __shared__ int cnt[2][PER][NW];
     __shared__ int tot[2];
     __shared__ float fred[NW];
-    const int r = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const float* z = Z + (long)r * ldz;
-    unsigned int key[PER];
-#pragma unroll
-    for (int j = 0; j < PER; j++) {
-        const int n = tid + NT * j;
-        key[j] = n < N ? fkey(z[n]) : 0u;  // padding: below every finite logit
-    }
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     unsigned int K = 0;
 #pragma unroll 1
     for (int round = 0; round < 16; round++) {
         const int sh = 30 - 2 * round;  // bits sh+1, sh of K decided this round
-        int c1 = 0, c2 = 0, c3 = 0;
+        int c1 = 0, c2 = 0, c3 = 0;  // wave counts from ballots (scalar, no lane exchange)
 #pragma unroll
         for (int j = 0; j < PER; j++) {
             const unsigned int d = key[j] >= K ? (key[j] - K) >> sh : 0u;  // the key's next two bits, above K
-            c1 += d >= 1u ? 1 : 0;
-            c2 += d >= 2u ? 1 : 0;
-            c3 += d >= 3u ? 1 : 0;
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            c1 += __shfl_xor(c1, off, 64);
-            c2 += __shfl_xor(c2, off, 64);
-            c3 += __shfl_xor(c3, off, 64);
+            c1 += __popcll(__ballot(d >= 1u));
+            c2 += __popcll(__ballot(d >= 2u));
+            c3 += __popcll(__ballot(d >= 3u));
         }
         const int buf = round & 1;
         if (lane == 0) {
@@ -477,6 +463,221 @@ __global__ void __launch_bounds__(NT) topk_reg_kernel(int N, int H, const float*
             ms += expf(__uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k) - l);
         }
         __builtin_amdgcn_sched_barrier(0);
+    }
+    if (mass) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) ms += __shfl_xor(ms, off, 64);
+        if (lane == 0) fred[w] = ms;
+        __syncthreads();
+        if (tid == 0) {
+            float m = 0.0f;
+#pragma unroll
+            for (int v = 0; v < NW; v++) m += fred[v];
+            mass[r] = m;
+        }
+    }
+}
+
+template <int PER, int NT>
+__global__ void __launch_bounds__(NT) topk_reg_kernel(int N, int H, const float* __restrict__ Z, long ldz,
+                                                       int* __restrict__ out, const float* __restrict__ lse,
+                                                       float* __restrict__ mass) {
+    const float* z = Z + (long)blockIdx.x * ldz;
+    unsigned int key[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int n = threadIdx.x + NT * j;
+        key[j] = n < N ? fkey(z[n]) : 0u;  // padding: below every finite logit
+    }
+    topk_reg_row<PER, NT>(blockIdx.x, N, H, key, out, lse, mass);
+}
+
+// H-th largest of the keys the block holds (P per thread, 0 = absent), two
+// bits per round as in topk_reg_row; every thread returns the same key.
+template <int P, int NT>
+__device__ unsigned int block_kth(const unsigned int (&key)[P], int H) {
+    constexpr int NW = NT / 64;
+    __shared__ int red[2][3][NW];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned int K = 0;
+#pragma unroll 1
+    for (int round = 0; round < 16; round++) {
+        const int sh = 30 - 2 * round;
+        int c1 = 0, c2 = 0, c3 = 0;  // wave counts from ballots (scalar, no lane exchange)
+#pragma unroll
+        for (int j = 0; j < P; j++) {
+            const unsigned int d = key[j] >= K ? (key[j] - K) >> sh : 0u;
+            c1 += __popcll(__ballot(d >= 1u));
+            c2 += __popcll(__ballot(d >= 2u));
+            c3 += __popcll(__ballot(d >= 3u));
+        }
+        const int buf = round & 1;
+        if (lane == 0) {
+            red[buf][0][w] = c1;
+            red[buf][1][w] = c2;
+            red[buf][2][w] = c3;
+        }
+        __syncthreads();
+        int t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+        for (int v = 0; v < NW; v++) {
+            t1 += red[buf][0][v];
+            t2 += red[buf][1][v];
+            t3 += red[buf][2][v];
+        }
+        K |= (t3 >= H ? 3u : t2 >= H ? 2u : t1 >= H ? 1u : 0u) << sh;
+    }
+    __syncthreads();
+    return K;
+}
+
+// Top-H by filter then select. The maxima of the row's groups of 64
+// consecutive logits give tau = the H-th largest group maximum; at least H
+// logits are >= tau, so the H-th largest logit is too. The logits >= tau are
+// compacted in index order into LDS (at most CAP of them) and the exact
+// selection of topk_reg_row runs on that list alone (a threshold <= tau
+// counts every candidate, >= H, exactly as over the whole row). A row with
+// more than CAP candidates takes topk_reg_row over all its keys. Same output
+// as topk_reg_kernel.
+template <int PER, int NT, int CAP>
+__global__ void __launch_bounds__(NT) topk_filter_kernel(int N, int H, const float* __restrict__ Z, long ldz,
+                                                          int* __restrict__ out, const float* __restrict__ lse,
+                                                          float* __restrict__ mass) {
+    constexpr int NW = NT / 64, G = PER * NW, GP = (G + NT - 1) / NT, CPT = CAP / NT;
+    __shared__ unsigned int gm[G];
+    __shared__ int cnt[PER][NW];
+    __shared__ unsigned int ckey[CAP];
+    __shared__ int cidx[CAP];
+    __shared__ int wsum[2][NW];
+    __shared__ int total;
+    __shared__ float fred[NW];
+    const int r = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const float* z = Z + (long)r * ldz;
+    unsigned int key[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int n = tid + NT * j;
+        key[j] = n < N ? fkey(z[n]) : 0u;
+    }
+    // group (j, w): the 64 keys of wave w at step j
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        unsigned int m = key[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, off, 64));
+        if (lane == 0) gm[j * NW + w] = m;
+    }
+    __syncthreads();
+    unsigned int gk[GP];
+#pragma unroll
+    for (int i = 0; i < GP; i++) {
+        const int g = tid + NT * i;
+        gk[i] = g < G ? gm[g] : 0u;
+    }
+    const unsigned int tau = G >= H ? block_kth<GP, NT>(gk, H) : 0u;
+    // candidates in index order: per-(j, wave) counts, scanned in (j, wave) order
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const bool f = tid + NT * j < N && key[j] >= tau;
+        const unsigned long long b = __ballot(f);
+        if (lane == 0) cnt[j][w] = __popcll(b);
+    }
+    __syncthreads();
+    if (w == 0) {
+        int* a = &cnt[0][0];
+        constexpr int M = PER * NW, PL = (M + 63) / 64;
+        const int b = lane * PL, e = min(M, b + PL);
+        int sum = 0;
+        for (int i = b; i < e; i++) sum += a[i];
+        int incl = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        int run = incl - sum;
+        for (int i = b; i < e; i++) {
+            const int v = a[i];
+            a[i] = run;
+            run += v;
+        }
+        if (lane == 63) total = incl;
+    }
+    __syncthreads();
+    const int M = total;
+    if (M > CAP) {  // block-uniform
+        topk_reg_row<PER, NT>(r, N, H, key, out, lse, mass);
+        return;
+    }
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int n = tid + NT * j;
+        const bool f = n < N && key[j] >= tau;
+        const unsigned long long b = __ballot(f);
+        if (f) {
+            const int pos = cnt[j][w] + __popcll(b & below);
+            ckey[pos] = key[j];
+            cidx[pos] = n;
+        }
+    }
+    __syncthreads();
+    // thread t holds the contiguous candidates [t CPT, t CPT + CPT)
+    unsigned int ck[CPT];
+#pragma unroll
+    for (int i = 0; i < CPT; i++) {
+        const int c = tid * CPT + i;
+        ck[i] = c < M ? ckey[c] : 0u;
+    }
+    const unsigned int K = block_kth<CPT, NT>(ck, H);
+    int gt = 0, eq = 0;
+#pragma unroll
+    for (int i = 0; i < CPT; i++) {
+        const bool v = tid * CPT + i < M;
+        gt += v && ck[i] > K ? 1 : 0;
+        eq += v && ck[i] == K ? 1 : 0;
+    }
+    // exclusive block scan of (gt, eq) in thread order (= index order)
+    int ig = gt, ie = eq;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int a = __shfl_up(ig, off, 64), b = __shfl_up(ie, off, 64);
+        if (lane >= off) {
+            ig += a;
+            ie += b;
+        }
+    }
+    if (lane == 63) {
+        wsum[0][w] = ig;
+        wsum[1][w] = ie;
+    }
+    __syncthreads();
+    int bg = ig - gt, be = ie - eq, tg = 0;
+#pragma unroll
+    for (int v = 0; v < NW; v++) {
+        if (v < w) {
+            bg += wsum[0][v];
+            be += wsum[1][v];
+        }
+        tg += wsum[0][v];
+    }
+    const int need = H - tg;  // ties at K taken, lowest index first
+    int* dst = out + (long)r * H;
+    const float l = lse ? lse[r] : 0.0f;
+    float ms = 0.0f;
+#pragma unroll
+    for (int i = 0; i < CPT; i++) {
+        const int c = tid * CPT + i;
+        if (c >= M) break;
+        const unsigned int k = ck[i];
+        const bool g = k > K, e = k == K;
+        const int o = bg + min(be, need);
+        if ((g || (e && be < need)) && o < H) {
+            dst[o] = cidx[c];
+            ms += expf(__uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k) - l);
+        }
+        bg += g ? 1 : 0;
+        be += e ? 1 : 0;
     }
     if (mass) {
 #pragma unroll
@@ -629,10 +830,18 @@ hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, cons
         WP_TOPK_REG(8, 256)
         WP_TOPK_REG(16, 256)
         WP_TOPK_REG(32, 256)
-        WP_TOPK_REG(12, 1024)
-        WP_TOPK_REG(20, 1024)
-        WP_TOPK_REG(32, 1024)
 #undef WP_TOPK_REG
+        // filter then select, the full register search as its fallback
+#define WP_TOPK_FILTER(P)                                                                                       \
+    if (N <= 1024 * (P) && H <= 1024) {                                                                         \
+        hipLaunchKernelGGL((topk_filter_kernel<P, 1024, 4096>), dim3(rows), dim3(1024), 0, s, N, H, Z, ldz, out,  \
+                           lse, mass);                                                                          \
+        return hipGetLastError();                                                                               \
+    }
+        WP_TOPK_FILTER(12)
+        WP_TOPK_FILTER(20)
+        WP_TOPK_FILTER(32)
+#undef WP_TOPK_FILTER
     }
     hipLaunchKernelGGL(topk_kernel, dim3(rows), dim3(256), 0, s, N, H, Z, ldz, out, lse, mass);
     return hipGetLastError();
