@@ -130,6 +130,7 @@ struct Bin {
     int lo, hi, ngroups;
     int kernel;  // BinKernel: quad-rows (FP32, boxes <= 16), row-per-lane (FP64, <= 8) or gather
     int cap_sep, ngroups_sep;  // gather bins: state capacity and groups of the separable launch
+    int ngroups_wide;          // groups of the wide passes (few reads: as many groups as LDS allows)
 };
 
 // The separable gather launch sizes its state buffers at 1/kSepCapDiv of the
@@ -164,15 +165,19 @@ std::vector<Bin> make_bins(const wp_hmm_classifier* c) {
     for (int hi : edges) {
         if (hi <= lo) continue;
         if (hi > c->maxbox) hi = c->maxbox;
-        auto groups_for = [&](int cap) {
+        auto groups_for = [&](int cap, size_t budget) {
             size_t fixed = wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, hi, cap, 0, c->G);
             size_t per = wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, hi, cap, 1, c->G) - fixed;
-            int g = (int)((kLdsBudget > fixed ? kLdsBudget - fixed : 0) / (per ? per : 1));
-            return std::max(1, std::min(g, wp::kBlockThreads / c->G));
+            int g = (int)((budget > fixed ? budget - fixed : 0) / (per ? per : 1));
+            g = std::max(1, std::min(g, wp::kBlockThreads / c->G));
+            // the layout also grows with the thread count: check the exact size
+            while (g > 1 && wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, hi, cap, g, c->G) > budget) g--;
+            return g;
         };
-        const int ng = groups_for(c->T * hi);
+        const int ng = groups_for(c->T * hi, kLdsBudget);
         const int cap_sep = std::max(1, c->T * hi / kSepCapDiv);
-        const int ng_sep = groups_for(cap_sep);
+        const int ng_sep = groups_for(cap_sep, kLdsBudget);
+        const int ng_wide = groups_for(c->T * hi, lds_hard_limit() - 1024);
         int kernel = kGather;
         if (wp::v5_supports(c->precision, c->T, hi) &&
             wp::v5_lds_bytes(c->T, c->C, c->tri, hi) <= lds_hard_limit())
@@ -180,7 +185,7 @@ std::vector<Bin> make_bins(const wp_hmm_classifier* c) {
         else if (wp::v3_supports(c->precision, c->T, hi) &&
                  wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, hi) <= lds_hard_limit())
             kernel = kRowPerLane;
-        bins.push_back({lo, hi, ng, kernel, cap_sep, ng_sep});
+        bins.push_back({lo, hi, ng, kernel, cap_sep, ng_sep, ng_wide});
         lo = hi;
         if (lo >= c->maxbox) break;
     }
@@ -192,8 +197,9 @@ size_t bin_lds_bytes(const wp_hmm_classifier* c, const Bin& b) {
     case kRowPerLane: return wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi);
     case kQuadRows: return wp::v5_lds_bytes(c->T, c->C, c->tri, b.hi);
     default:
-        return std::max(wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi, c->T * b.hi, b.ngroups, c->G),
-                        wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi, b.cap_sep, b.ngroups_sep, c->G));
+        return std::max(
+            wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi, c->T * b.hi, b.ngroups_wide, c->G),
+            wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi, b.cap_sep, b.ngroups_sep, c->G));
     }
 }
 
@@ -330,6 +336,7 @@ hipError_t launch_bin_wide(wp_hmm_classifier* c, const std::vector<Bin>& bins, i
     a.cap = c->T * bin.hi;
     a.ngroups = bin.ngroups;
     a.wide_pass = 1;
+    a.ngroups = bin.ngroups_wide;  // few reads: every group the LDS holds works on each
     a.list = (stage == 2 ? c->wWide2Lists : c->wWideLists) + (size_t)b * x.N;
     a.list_count = c->wCtr + (stage == 2 ? kCtrWide2 : kCtrWide) + b;
     if (bin.kernel == kGather && stage == 1) {

@@ -71,6 +71,16 @@ namespace {
 using namespace cc;
 
 // LDS carve-up shared by host sizing and the kernel.
+// Entries between the state buffers of consecutive pair groups: 4 * cap
+// rounded so that group g starts 16 4-byte words (a quarter of the 64 banks)
+// after group g - 1, and lanes of different groups reading the same offset of
+// their buffers do not collide on one bank.
+template <typename Real>
+__host__ __device__ inline int group_stride(int cap) {
+    constexpr int per = 64 * 4 / (int)sizeof(Real);  // entries per 64 banks
+    return (4 * cap + per - 1) / per * per + per / 4;
+}
+
 struct Layout {
     size_t rad, L, H, eoff, estr, etab, eexp, binom, list, scan, gres, gseq, state, total;
 };
@@ -92,7 +102,7 @@ __host__ __device__ inline Layout make_layout(int T, int C, int tri, int NB, int
     l.scan = o;  o = align16(o + sizeof(int) * (nthreads / 64 + 2));
     l.gres = o;  o = align16(o + sizeof(double) * 4 * ngroups);
     l.gseq = o;  o = align16(o + sizeof(int) * 3 * T * ngroups);
-    l.state = o; o = align16(o + sizeof(Real) * 4 * (size_t)cap * ngroups);
+    l.state = o; o = align16(o + sizeof(Real) * (size_t)group_stride<Real>(cap) * ngroups);
     l.total = o;
     return l;
 }
@@ -913,7 +923,7 @@ __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, 
             return;
         }
     }
-    Real* st = state + (size_t)group * 4 * cap;
+    Real* st = state + (size_t)group * group_stride<Real>(cap);
     int* gseq = gseq_all + group * 3 * T;
     int* gK = gseq + T;
     for (int base = 0; base < ncand; base += nthreads * 4) {
