@@ -445,9 +445,8 @@ int wp_mlp_shortlist(wp_mlp* m, int N, const double* radiometries, int H, int* o
         }
         rc = forward(m, B);
         if (rc) break;
-        if (wp::softmax_ce(B, m->out_dim, m->Z, m->out_dim, nullptr, 1.0f, nullptr, nullptr, m->post, m->lse,
-                           nullptr) != hipSuccess ||
-            wp::topk(B, m->out_dim, H, m->Z, m->out_dim, didx, m->lse, m->mass, nullptr) != hipSuccess ||
+        if (wp::topk_softmax(B, m->out_dim, H, m->Z, m->out_dim, didx, m->lse, m->post, m->mass, nullptr) !=
+                hipSuccess ||
             hipMemcpy(out_index + (size_t)r0 * H, didx, sizeof(int) * (size_t)B * H, hipMemcpyDeviceToHost) !=
                 hipSuccess ||
             (out_max_posterior &&
@@ -512,9 +511,8 @@ int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, cons
             break;
         rc = forward(m, B);
         if (rc) break;
-        hip_ok(wp::softmax_ce(B, m->out_dim, m->Z, m->out_dim, nullptr, 1.0f, nullptr, nullptr, m->post, m->lse,
-                              nullptr), "softmax");
-        hip_ok(wp::topk(B, m->out_dim, H, m->Z, m->out_dim, d_cand + (size_t)r0 * H, m->lse, m->mass, nullptr),
+        hip_ok(wp::topk_softmax(B, m->out_dim, H, m->Z, m->out_dim, d_cand + (size_t)r0 * H, m->lse, m->post,
+                                m->mass, nullptr),
                "top-k");
         hip_ok(hipMemcpy(d_conf + r0, criterion == WP_SHORTLIST_MASS ? m->mass : m->post, sizeof(float) * B,
                          hipMemcpyDeviceToDevice), "copy");

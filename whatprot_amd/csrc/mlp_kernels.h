@@ -40,6 +40,10 @@ hipError_t gather_int(int n, const int* src, const int* idx, int* dst, hipStream
 hipError_t mean(int n, const float* x, float* out, hipStream_t s);
 hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, const float* lse, float* mass,
                 hipStream_t s);
+// topk and the row statistics of softmax_ce (log-sum-exp, largest posterior)
+// in one pass over the logits where the row fits the filtered top-k.
+hipError_t topk_softmax(int rows, int N, int H, const float* Z, long ldz, int* out, float* lse, float* max_post,
+                        float* mass, hipStream_t s);
 hipError_t transpose_bf16(int rows, int cols, const float* W, __bf16* Wt, hipStream_t s);
 
 }  // namespace wp

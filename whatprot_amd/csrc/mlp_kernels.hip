@@ -272,6 +272,11 @@ __global__ void pack_input_kernel(int rows, int din, int dpad, const T* __restri
 // H-th largest key over the order-preserving unsigned image of the floats,
 // then every index above it and the lowest-indexed ties at it, in
 // increasing index order.
+// The logit an order-preserving key stands for (the inverse of fkey).
+__device__ inline float key_value(unsigned int k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
 __device__ inline unsigned int fkey(float f) {
     unsigned int u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
@@ -375,8 +380,8 @@ __global__ void __launch_bounds__(256) topk_kernel(int N, int H, const float* __
 // ballot counts of keys > K and == K, scanned in (j, wave) order, which is
 // index order.
 template <int PER, int NT>
-__device__ void topk_reg_row(int r, int N, int H, const unsigned int (&key)[PER], int* __restrict__ out,
-                             const float* __restrict__ lse, float* __restrict__ mass) {
+__device__ void topk_reg_row(int r, int N, int H, const unsigned int (&key)[PER], int* __restrict__ out, float l,
+                             float* __restrict__ mass) {
     constexpr int NW = NT / 64;
     __shared__ int red[2][3][NW];
     __shared__ int cnt[2][PER][NW];
@@ -447,7 +452,6 @@ __device__ void topk_reg_row(int r, int N, int H, const unsigned int (&key)[PER]
     const int need = H - tot[0];  // ties at K taken, lowest index first
     const unsigned long long below = (1ull << lane) - 1ull;
     int* dst = out + (long)r * H;
-    const float l = lse ? lse[r] : 0.0f;
     float ms = 0.0f;
 #pragma unroll
     for (int j = 0; j < PER; j++) {
@@ -489,7 +493,7 @@ __global__ void __launch_bounds__(NT) topk_reg_kernel(int N, int H, const float*
         const int n = threadIdx.x + NT * j;
         key[j] = n < N ? fkey(z[n]) : 0u;  // padding: below every finite logit
     }
-    topk_reg_row<PER, NT>(blockIdx.x, N, H, key, out, lse, mass);
+    topk_reg_row<PER, NT>(blockIdx.x, N, H, key, out, lse ? lse[blockIdx.x] : 0.0f, mass);
 }
 
 // H-th largest of the keys the block holds (P per thread, 0 = absent), two
@@ -542,7 +546,8 @@ __device__ unsigned int block_kth(const unsigned int (&key)[P], int H) {
 template <int PER, int NT, int CAP>
 __global__ void __launch_bounds__(NT) topk_filter_kernel(int N, int H, const float* __restrict__ Z, long ldz,
                                                           int* __restrict__ out, const float* __restrict__ lse,
-                                                          float* __restrict__ mass) {
+                                                          float* __restrict__ mass, float* __restrict__ lse_out,
+                                                          float* __restrict__ post_out) {
     constexpr int NW = NT / 64, G = PER * NW, GP = (G + NT - 1) / NT, CPT = CAP / NT;
     __shared__ unsigned int gm[G];
     __shared__ int cnt[PER][NW];
@@ -573,6 +578,38 @@ __global__ void __launch_bounds__(NT) topk_filter_kernel(int N, int H, const flo
     for (int i = 0; i < GP; i++) {
         const int g = tid + NT * i;
         gk[i] = g < G ? gm[g] : 0u;
+    }
+    float l = lse ? lse[r] : 0.0f;
+    if (lse_out) {  // the row's log-sum-exp and largest posterior (softmax_ce_kernel's sums)
+        __shared__ float sred[NW];
+        unsigned int mk = 0u;
+#pragma unroll
+        for (int i = 0; i < GP; i++) mk = max(mk, gk[i]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mk = max(mk, (unsigned int)__shfl_xor((int)mk, off, 64));
+        if (lane == 0) sred[w] = __uint_as_float(mk);
+        __syncthreads();
+        unsigned int rk = 0u;
+#pragma unroll
+        for (int v = 0; v < NW; v++) rk = max(rk, __float_as_uint(sred[v]));
+        const float mx = key_value(rk);
+        __syncthreads();
+        float sum = 0.0f;
+#pragma unroll
+        for (int j = 0; j < PER; j++)
+            if (tid + NT * j < N) sum += expf(key_value(key[j]) - mx);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+        if (lane == 0) sred[w] = sum;
+        __syncthreads();
+        float tot_s = 0.0f;
+#pragma unroll
+        for (int v = 0; v < NW; v++) tot_s += sred[v];
+        l = mx + logf(tot_s);
+        if (tid == 0) {
+            lse_out[r] = l;
+            if (post_out) post_out[r] = expf(mx - l);
+        }
     }
     const unsigned int tau = G >= H ? block_kth<GP, NT>(gk, H) : 0u;
     // candidates in index order: per-(j, wave) counts, scanned in (j, wave) order
@@ -606,7 +643,7 @@ __global__ void __launch_bounds__(NT) topk_filter_kernel(int N, int H, const flo
     __syncthreads();
     const int M = total;
     if (M > CAP) {  // block-uniform
-        topk_reg_row<PER, NT>(r, N, H, key, out, lse, mass);
+        topk_reg_row<PER, NT>(r, N, H, key, out, l, mass);
         return;
     }
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -663,7 +700,6 @@ __global__ void __launch_bounds__(NT) topk_filter_kernel(int N, int H, const flo
     }
     const int need = H - tg;  // ties at K taken, lowest index first
     int* dst = out + (long)r * H;
-    const float l = lse ? lse[r] : 0.0f;
     float ms = 0.0f;
 #pragma unroll
     for (int i = 0; i < CPT; i++) {
@@ -835,7 +871,7 @@ hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, cons
 #define WP_TOPK_FILTER(P)                                                                                       \
     if (N <= 1024 * (P) && H <= 1024) {                                                                         \
         hipLaunchKernelGGL((topk_filter_kernel<P, 1024, 4096>), dim3(rows), dim3(1024), 0, s, N, H, Z, ldz, out,  \
-                           lse, mass);                                                                          \
+                           lse, mass, nullptr, nullptr);                                                        \
         return hipGetLastError();                                                                               \
     }
         WP_TOPK_FILTER(12)
@@ -845,6 +881,26 @@ hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, cons
     }
     hipLaunchKernelGGL(topk_kernel, dim3(rows), dim3(256), 0, s, N, H, Z, ldz, out, lse, mass);
     return hipGetLastError();
+}
+
+hipError_t topk_softmax(int rows, int N, int H, const float* Z, long ldz, int* out, float* lse, float* max_post,
+                        float* mass, hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
+    if (N > 8192 && N > H && H <= 1024) {  // one pass over the logits
+#define WP_TOPK_FUSED(P)                                                                                        \
+    if (N <= 1024 * (P)) {                                                                                      \
+        hipLaunchKernelGGL((topk_filter_kernel<P, 1024, 4096>), dim3(rows), dim3(1024), 0, s, N, H, Z, ldz, out,  \
+                           nullptr, mass, lse, max_post);                                                       \
+        return hipGetLastError();                                                                               \
+    }
+        WP_TOPK_FUSED(12)
+        WP_TOPK_FUSED(20)
+        WP_TOPK_FUSED(32)
+#undef WP_TOPK_FUSED
+    }
+    hipError_t e = softmax_ce(rows, N, Z, ldz, nullptr, 1.0f, nullptr, nullptr, max_post, lse, s);
+    if (e != hipSuccess) return e;
+    return topk(rows, N, H, Z, ldz, out, lse, mass, s);
 }
 
 hipError_t transpose_bf16(int rows, int cols, const float* W, __bf16* Wt, hipStream_t s) {
