@@ -496,239 +496,6 @@ __global__ void __launch_bounds__(NT) topk_reg_kernel(int N, int H, const float*
     topk_reg_row<PER, NT>(blockIdx.x, N, H, key, out, lse ? lse[blockIdx.x] : 0.0f, mass);
 }
 
-// H-th largest of the keys the block holds (P per thread, 0 = absent), two
-// bits per round as in topk_reg_row; every thread returns the same key.
-template <int P, int NT>
-__device__ unsigned int block_kth(const unsigned int (&key)[P], int H) {
-    constexpr int NW = NT / 64;
-    __shared__ int red[2][3][NW];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned int K = 0;
-#pragma unroll 1
-    for (int round = 0; round < 16; round++) {
-        const int sh = 30 - 2 * round;
-        int c1 = 0, c2 = 0, c3 = 0;  // wave counts from ballots (scalar, no lane exchange)
-#pragma unroll
-        for (int j = 0; j < P; j++) {
-            const unsigned int d = key[j] >= K ? (key[j] - K) >> sh : 0u;
-            c1 += __popcll(__ballot(d >= 1u));
-            c2 += __popcll(__ballot(d >= 2u));
-            c3 += __popcll(__ballot(d >= 3u));
-        }
-        const int buf = round & 1;
-        if (lane == 0) {
-            red[buf][0][w] = c1;
-            red[buf][1][w] = c2;
-            red[buf][2][w] = c3;
-        }
-        __syncthreads();
-        int t1 = 0, t2 = 0, t3 = 0;
-#pragma unroll
-        for (int v = 0; v < NW; v++) {
-            t1 += red[buf][0][v];
-            t2 += red[buf][1][v];
-            t3 += red[buf][2][v];
-        }
-        K |= (t3 >= H ? 3u : t2 >= H ? 2u : t1 >= H ? 1u : 0u) << sh;
-    }
-    __syncthreads();
-    return K;
-}
-
-// Top-H by filter then select. The maxima of the row's groups of 64
-// consecutive logits give tau = the H-th largest group maximum; at least H
-// logits are >= tau, so the H-th largest logit is too. The logits >= tau are
-// compacted in index order into LDS (at most CAP of them) and the exact
-// selection of topk_reg_row runs on that list alone (a threshold <= tau
-// counts every candidate, >= H, exactly as over the whole row). A row with
-// more than CAP candidates takes topk_reg_row over all its keys. Same output
-// as topk_reg_kernel.
-template <int PER, int NT, int CAP>
-__global__ void __launch_bounds__(NT) topk_filter_kernel(int N, int H, const float* __restrict__ Z, long ldz,
-                                                          int* __restrict__ out, const float* __restrict__ lse,
-                                                          float* __restrict__ mass, float* __restrict__ lse_out,
-                                                          float* __restrict__ post_out) {
-    constexpr int NW = NT / 64, G = PER * NW, GP = (G + NT - 1) / NT, CPT = CAP / NT;
-    __shared__ unsigned int gm[G];
-    __shared__ int cnt[PER][NW];
-    __shared__ unsigned int ckey[CAP];
-    __shared__ int cidx[CAP];
-    __shared__ int wsum[2][NW];
-    __shared__ int total;
-    __shared__ float fred[NW];
-    const int r = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const float* z = Z + (long)r * ldz;
-    unsigned int key[PER];
-#pragma unroll
-    for (int j = 0; j < PER; j++) {
-        const int n = tid + NT * j;
-        key[j] = n < N ? fkey(z[n]) : 0u;
-    }
-    // group (j, w): the 64 keys of wave w at step j
-#pragma unroll
-    for (int j = 0; j < PER; j++) {
-        unsigned int m = key[j];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, off, 64));
-        if (lane == 0) gm[j * NW + w] = m;
-    }
-    __syncthreads();
-    unsigned int gk[GP];
-#pragma unroll
-    for (int i = 0; i < GP; i++) {
-        const int g = tid + NT * i;
-        gk[i] = g < G ? gm[g] : 0u;
-    }
-    float l = lse ? lse[r] : 0.0f;
-    if (lse_out) {  // the row's log-sum-exp and largest posterior (softmax_ce_kernel's sums)
-        __shared__ float sred[NW];
-        unsigned int mk = 0u;
-#pragma unroll
-        for (int i = 0; i < GP; i++) mk = max(mk, gk[i]);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) mk = max(mk, (unsigned int)__shfl_xor((int)mk, off, 64));
-        if (lane == 0) sred[w] = __uint_as_float(mk);
-        __syncthreads();
-        unsigned int rk = 0u;
-#pragma unroll
-        for (int v = 0; v < NW; v++) rk = max(rk, __float_as_uint(sred[v]));
-        const float mx = key_value(rk);
-        __syncthreads();
-        float sum = 0.0f;
-#pragma unroll
-        for (int j = 0; j < PER; j++)
-            if (tid + NT * j < N) sum += expf(key_value(key[j]) - mx);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
-        if (lane == 0) sred[w] = sum;
-        __syncthreads();
-        float tot_s = 0.0f;
-#pragma unroll
-        for (int v = 0; v < NW; v++) tot_s += sred[v];
-        l = mx + logf(tot_s);
-        if (tid == 0) {
-            lse_out[r] = l;
-            if (post_out) post_out[r] = expf(mx - l);
-        }
-    }
-    const unsigned int tau = G >= H ? block_kth<GP, NT>(gk, H) : 0u;
-    // candidates in index order: per-(j, wave) counts, scanned in (j, wave) order
-#pragma unroll
-    for (int j = 0; j < PER; j++) {
-        const bool f = tid + NT * j < N && key[j] >= tau;
-        const unsigned long long b = __ballot(f);
-        if (lane == 0) cnt[j][w] = __popcll(b);
-    }
-    __syncthreads();
-    if (w == 0) {
-        int* a = &cnt[0][0];
-        constexpr int M = PER * NW, PL = (M + 63) / 64;
-        const int b = lane * PL, e = min(M, b + PL);
-        int sum = 0;
-        for (int i = b; i < e; i++) sum += a[i];
-        int incl = sum;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int t = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += t;
-        }
-        int run = incl - sum;
-        for (int i = b; i < e; i++) {
-            const int v = a[i];
-            a[i] = run;
-            run += v;
-        }
-        if (lane == 63) total = incl;
-    }
-    __syncthreads();
-    const int M = total;
-    if (M > CAP) {  // block-uniform
-        topk_reg_row<PER, NT>(r, N, H, key, out, l, mass);
-        return;
-    }
-    const unsigned long long below = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int j = 0; j < PER; j++) {
-        const int n = tid + NT * j;
-        const bool f = n < N && key[j] >= tau;
-        const unsigned long long b = __ballot(f);
-        if (f) {
-            const int pos = cnt[j][w] + __popcll(b & below);
-            ckey[pos] = key[j];
-            cidx[pos] = n;
-        }
-    }
-    __syncthreads();
-    // thread t holds the contiguous candidates [t CPT, t CPT + CPT)
-    unsigned int ck[CPT];
-#pragma unroll
-    for (int i = 0; i < CPT; i++) {
-        const int c = tid * CPT + i;
-        ck[i] = c < M ? ckey[c] : 0u;
-    }
-    const unsigned int K = block_kth<CPT, NT>(ck, H);
-    int gt = 0, eq = 0;
-#pragma unroll
-    for (int i = 0; i < CPT; i++) {
-        const bool v = tid * CPT + i < M;
-        gt += v && ck[i] > K ? 1 : 0;
-        eq += v && ck[i] == K ? 1 : 0;
-    }
-    // exclusive block scan of (gt, eq) in thread order (= index order)
-    int ig = gt, ie = eq;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int a = __shfl_up(ig, off, 64), b = __shfl_up(ie, off, 64);
-        if (lane >= off) {
-            ig += a;
-            ie += b;
-        }
-    }
-    if (lane == 63) {
-        wsum[0][w] = ig;
-        wsum[1][w] = ie;
-    }
-    __syncthreads();
-    int bg = ig - gt, be = ie - eq, tg = 0;
-#pragma unroll
-    for (int v = 0; v < NW; v++) {
-        if (v < w) {
-            bg += wsum[0][v];
-            be += wsum[1][v];
-        }
-        tg += wsum[0][v];
-    }
-    const int need = H - tg;  // ties at K taken, lowest index first
-    int* dst = out + (long)r * H;
-    float ms = 0.0f;
-#pragma unroll
-    for (int i = 0; i < CPT; i++) {
-        const int c = tid * CPT + i;
-        if (c >= M) break;
-        const unsigned int k = ck[i];
-        const bool g = k > K, e = k == K;
-        const int o = bg + min(be, need);
-        if ((g || (e && be < need)) && o < H) {
-            dst[o] = cidx[c];
-            ms += expf(__uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k) - l);
-        }
-        bg += g ? 1 : 0;
-        be += e ? 1 : 0;
-    }
-    if (mass) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) ms += __shfl_xor(ms, off, 64);
-        if (lane == 0) fred[w] = ms;
-        __syncthreads();
-        if (tid == 0) {
-            float m = 0.0f;
-#pragma unroll
-            for (int v = 0; v < NW; v++) m += fred[v];
-            mass[r] = m;
-        }
-    }
-}
-
 // Wt[n][k] = bf16(W[k][n]) for W [rows][cols] FP32.
 __global__ void transpose_bf16_kernel(int rows, int cols, const float* __restrict__ W, __bf16* __restrict__ Wt) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -851,12 +618,199 @@ hipError_t pack_input(int rows, int din, int dpad, const double* x, const int* i
     return hipGetLastError();
 }
 
+namespace {
+
+constexpr int kWaveRows = 4;     // rows per workgroup, one wavefront each
+constexpr int kWaveGroup = 16;   // logits per group maximum
+constexpr int kWaveCap = 1024;   // candidates a wavefront keeps in LDS
+
+__device__ inline void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// H-th largest of the keys the wavefront's lanes produce through get(i) for
+// i < n (0 = absent), two bits per round from ballot counts: no barrier.
+template <typename Get>
+__device__ inline unsigned int wave_kth(int n, int H, Get get) {
+    unsigned int K = 0;
+    for (int round = 0; round < 16; round++) {
+        const int sh = 30 - 2 * round;
+        int c1 = 0, c2 = 0, c3 = 0;
+        for (int b = 0; b < n; b += 64) {
+            const int i = b + (int)(threadIdx.x & 63);
+            const unsigned int k = i < n ? get(i) : 0u;
+            const unsigned int d = k >= K ? (k - K) >> sh : 0u;
+            c1 += __popcll(__ballot(d >= 1u));
+            c2 += __popcll(__ballot(d >= 2u));
+            c3 += __popcll(__ballot(d >= 3u));
+        }
+        K |= (c3 >= H ? 3u : c2 >= H ? 2u : c1 >= H ? 1u : 0u) << sh;
+    }
+    return K;
+}
+
+// Top-H with one wavefront per row, in two streaming passes over the row.
+// Pass 1 stores the maxima of groups of 16 consecutive logits in LDS and the
+// row's log-sum-exp (online, per lane, then over the wave). tau = the H-th
+// largest group maximum bounds the H-th largest logit from below (the H
+// group maxima >= tau are H logits). Pass 2 compacts the logits >= tau in
+// index order into LDS; the exact H-th largest key K is selected among them
+// (thresholds <= tau count every candidate, >= H, as over the whole row), or,
+// when more than kWaveCap logits are >= tau, by the same rounds over the
+// whole row. Output: the logits > K and the first ties at K, in index order.
+__global__ void __launch_bounds__(64 * kWaveRows) topk_wave_kernel(int rows, int N, int H, const float* __restrict__ Z,
+                                                                   long ldz, int* __restrict__ out,
+                                                                   const float* __restrict__ lse_in,
+                                                                   float* __restrict__ mass, float* __restrict__ lse_out,
+                                                                   float* __restrict__ post_out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = blockIdx.x * kWaveRows + wv;
+    if (r >= rows) return;  // whole wavefront
+    const int G = (N + kWaveGroup - 1) / kWaveGroup;
+    const size_t per_wave = (size_t)G * 4 + (size_t)kWaveCap * 8;
+    unsigned int* gm = (unsigned int*)(smem + wv * per_wave);
+    unsigned int* ckey = gm + G;
+    int* cidx = (int*)(ckey + kWaveCap);
+    const float* z = Z + (long)r * ldz;
+    const unsigned long long below = (1ull << lane) - 1ull;
+
+    // pass 1: group maxima, and the log-sum-exp; kUnroll loads in flight per lane
+    constexpr int kUnroll = 16;
+    float m = -INFINITY, sacc = 0.0f;
+    for (int b0 = 0; b0 < N; b0 += 64 * kUnroll) {
+        float vv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const int n = b0 + 64 * u + lane;
+            vv[u] = n < N ? z[n] : -INFINITY;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const int n = b0 + 64 * u + lane;
+            const float v = vv[u];
+            unsigned int k = n < N ? fkey(v) : 0u;
+#pragma unroll
+            for (int off = 1; off < kWaveGroup; off <<= 1) k = max(k, (unsigned int)__shfl_xor((int)k, off, 64));
+            if ((lane & (kWaveGroup - 1)) == 0 && n < N) gm[n / kWaveGroup] = k;
+            if (n < N) {
+                if (v > m) {
+                    sacc = sacc * expf(m - v) + 1.0f;
+                    m = v;
+                } else {
+                    sacc += expf(v - m);
+                }
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const float m2 = __shfl_xor(m, off, 64), s2 = __shfl_xor(sacc, off, 64);
+        const float mm = fmaxf(m, m2);
+        sacc = (m == -INFINITY ? 0.0f : sacc * expf(m - mm)) + (m2 == -INFINITY ? 0.0f : s2 * expf(m2 - mm));
+        m = mm;
+    }
+    float l = lse_in ? lse_in[r] : 0.0f;
+    if (lse_out) {
+        l = m + logf(sacc);
+        if (lane == 0) {
+            lse_out[r] = l;
+            if (post_out) post_out[r] = expf(m - l);
+        }
+    }
+    wsync();
+    const unsigned int tau = G >= H ? wave_kth(G, H, [&](int i) { return gm[i]; }) : 0u;
+
+    // pass 2: the logits >= tau, in index order
+    int M = 0;
+    for (int b0 = 0; b0 < N; b0 += 64 * kUnroll) {
+        float vv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const int n = b0 + 64 * u + lane;
+            vv[u] = n < N ? z[n] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const int n = b0 + 64 * u + lane;
+            const unsigned int k = n < N ? fkey(vv[u]) : 0u;
+            const bool f = n < N && k >= tau;
+            const unsigned long long bal = __ballot(f);
+            const int pos = M + __popcll(bal & below);
+            if (f && pos < kWaveCap) {
+                ckey[pos] = k;
+                cidx[pos] = n;
+            }
+            M += __popcll(bal);
+        }
+    }
+    wsync();
+    const bool listed = M <= kWaveCap;
+    unsigned int K;
+    if (listed)
+        K = wave_kth(M, H, [&](int i) { return ckey[i]; });
+    else
+        K = wave_kth(N, H, [&](int i) { return fkey(z[i]); });
+
+    // output in index order: keys > K, then ties at K, lowest index first
+    int ngt = 0;
+    const int nsrc = listed ? M : N;
+    for (int b = 0; b < nsrc; b += 64) {
+        const int i = b + lane;
+        const unsigned int k = i < nsrc ? (listed ? ckey[i] : fkey(z[i])) : 0u;
+        ngt += __popcll(__ballot(i < nsrc && k > K));
+    }
+    const int need = H - ngt;
+    int* dst = out + (long)r * H;
+    int o = 0, eqs = 0;
+    float ms = 0.0f;
+    for (int b = 0; b < nsrc && o < H; b += 64) {
+        const int i = b + lane;
+        const bool v = i < nsrc;
+        const unsigned int k = v ? (listed ? ckey[i] : fkey(z[i])) : 0u;
+        const bool g = v && k > K, e = v && k == K;
+        const unsigned long long be = __ballot(e);
+        const bool sel = g || (e && eqs + __popcll(be & below) < need);
+        const unsigned long long bs = __ballot(sel);
+        const int pos = o + __popcll(bs & below);
+        if (sel && pos < H) {
+            dst[pos] = listed ? cidx[i] : i;
+            ms += expf(key_value(k) - l);
+        }
+        o += __popcll(bs);
+        eqs += __popcll(be);
+    }
+    if (mass) {
+        for (int off = 32; off > 0; off >>= 1) ms += __shfl_xor(ms, off, 64);
+        if (lane == 0) mass[r] = ms;
+    }
+}
+
+}  // namespace
+
+// One wavefront per row (rows of up to 32768 logits, H <= 1024).
+static hipError_t topk_wave(int rows, int N, int H, const float* Z, long ldz, int* out, const float* lse_in,
+                            float* mass, float* lse_out, float* post_out, hipStream_t s) {
+    const int G = (N + kWaveGroup - 1) / kWaveGroup;
+    const size_t lds = (size_t)kWaveRows * ((size_t)G * 4 + (size_t)kWaveCap * 8);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)topk_wave_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(topk_wave_kernel, dim3((rows + kWaveRows - 1) / kWaveRows), dim3(64 * kWaveRows), lds, s, rows,
+                       N, H, Z, ldz, out, lse_in, mass, lse_out, post_out);
+    return hipGetLastError();
+}
+
 hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, const float* lse, float* mass,
                 hipStream_t s) {
     if (rows <= 0) return hipSuccess;
-    if (N > H) {  // register-resident rows up to 32768 logits
-        // at most 32 keys per thread: 256 threads per row up to 8192 logits,
-        // 1024 threads per row beyond
+    if (N > 8192 && N <= 32768 && N > H && H <= 1024)  // one wavefront per row (larger rows: topk_kernel)
+        return topk_wave(rows, N, H, Z, ldz, out, lse, mass, nullptr, nullptr, s);
+    if (N > H) {  // register-resident rows
+        // at most 32 keys per thread, 256 threads per row (up to 8192 logits)
 #define WP_TOPK_REG(P, NT)                                                                                      \
     if (N <= (NT) * (P)) {                                                                                      \
         hipLaunchKernelGGL((topk_reg_kernel<P, NT>), dim3(rows), dim3(NT), 0, s, N, H, Z, ldz, out, lse, mass);  \
@@ -867,17 +821,6 @@ hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, cons
         WP_TOPK_REG(16, 256)
         WP_TOPK_REG(32, 256)
 #undef WP_TOPK_REG
-        // filter then select, the full register search as its fallback
-#define WP_TOPK_FILTER(P)                                                                                       \
-    if (N <= 1024 * (P) && H <= 1024) {                                                                         \
-        hipLaunchKernelGGL((topk_filter_kernel<P, 1024, 4096>), dim3(rows), dim3(1024), 0, s, N, H, Z, ldz, out,  \
-                           lse, mass, nullptr, nullptr);                                                        \
-        return hipGetLastError();                                                                               \
-    }
-        WP_TOPK_FILTER(12)
-        WP_TOPK_FILTER(20)
-        WP_TOPK_FILTER(32)
-#undef WP_TOPK_FILTER
     }
     hipLaunchKernelGGL(topk_kernel, dim3(rows), dim3(256), 0, s, N, H, Z, ldz, out, lse, mass);
     return hipGetLastError();
@@ -886,18 +829,8 @@ hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, cons
 hipError_t topk_softmax(int rows, int N, int H, const float* Z, long ldz, int* out, float* lse, float* max_post,
                         float* mass, hipStream_t s) {
     if (rows <= 0) return hipSuccess;
-    if (N > 8192 && N > H && H <= 1024) {  // one pass over the logits
-#define WP_TOPK_FUSED(P)                                                                                        \
-    if (N <= 1024 * (P)) {                                                                                      \
-        hipLaunchKernelGGL((topk_filter_kernel<P, 1024, 4096>), dim3(rows), dim3(1024), 0, s, N, H, Z, ldz, out,  \
-                           nullptr, mass, lse, max_post);                                                       \
-        return hipGetLastError();                                                                               \
-    }
-        WP_TOPK_FUSED(12)
-        WP_TOPK_FUSED(20)
-        WP_TOPK_FUSED(32)
-#undef WP_TOPK_FUSED
-    }
+    if (N > 8192 && N <= 32768 && N > H && H <= 1024)  // one wavefront per row, softmax statistics included
+        return topk_wave(rows, N, H, Z, ldz, out, nullptr, mass, lse, max_post, s);
     hipError_t e = softmax_ce(rows, N, Z, ldz, nullptr, 1.0f, nullptr, nullptr, max_post, lse, s);
     if (e != hipSuccess) return e;
     return topk(rows, N, H, Z, ldz, out, lse, mass, s);
