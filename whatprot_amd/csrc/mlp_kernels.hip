@@ -47,7 +47,58 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
     // a vector that would cross the matrix edge is loaded element-wise
     const int a_mode = (g.sak == 1 && g.sam % 8 == 0) ? 1 : (g.sam == 1 && g.sak % 8 == 0) ? 2 : 0;
     const int b_mode = (g.sbk == 1 && g.sbn % 8 == 0) ? 1 : (g.sbn == 1 && g.sbk % 8 == 0) ? 2 : 0;
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
+    auto mfma_tile = [&]() {
+        bf16x8 af[4], bfr[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) af[i] = *(const bf16x8*)&As[(wm + 16 * i + lr) * LDK + 8 * lg];
+#pragma unroll
+        for (int j = 0; j < 4; j++) bfr[j] = *(const bf16x8*)&Bs[(wn + 16 * j + lr) * LDK + 8 * lg];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    };
+    if (a_mode == 1 && b_mode == 1) {
+        // both operands k-contiguous (the forward product): the next K tile's
+        // 16-byte loads are in flight while the current tile's MFMAs run
+        bf16x8 ra[2], rb[2];
+        auto fetch = [&](int k0) {
+#pragma unroll
+            for (int it = 0; it < 2; it++) {
+                const int e = tid + it * 256, row = e >> 2, kv = (e & 3) * 8, gk = k0 + kv;
+                const int gm = m0 + row, gn = n0 + row;
+                if (gm < g.M && gk + 8 <= g.K) {
+                    ra[it] = *(const bf16x8*)&g.A[(long)gm * g.sam + gk];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        ra[it][q] = (gm < g.M && gk + q < g.K) ? g.A[(long)gm * g.sam + gk + q] : (__bf16)0.0f;
+                }
+                if (gn < g.N && gk + 8 <= g.K) {
+                    rb[it] = *(const bf16x8*)&g.B[(long)gn * g.sbn + gk];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        rb[it][q] = (gn < g.N && gk + q < g.K) ? g.B[(long)gn * g.sbn + gk + q] : (__bf16)0.0f;
+                }
+            }
+        };
+        fetch(0);
+        for (int k0 = 0; k0 < g.K; k0 += BK) {
+#pragma unroll
+            for (int it = 0; it < 2; it++) {
+                const int e = tid + it * 256, row = e >> 2, kv = (e & 3) * 8;
+                *(bf16x8*)&As[row * LDK + kv] = ra[it];
+                *(bf16x8*)&Bs[row * LDK + kv] = rb[it];
+            }
+            __syncthreads();
+            if (k0 + BK < g.K) fetch(k0 + BK);
+            mfma_tile();
+            __syncthreads();
+        }
+    }
+    for (int k0 = 0; k0 < g.K && !(a_mode == 1 && b_mode == 1); k0 += BK) {
         if (a_mode == 1) {
 #pragma unroll
             for (int it = 0; it < 2; it++) {
@@ -135,16 +186,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
             }
         }
         __syncthreads();
-        bf16x8 af[4], bfr[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) af[i] = *(const bf16x8*)&As[(wm + 16 * i + lr) * LDK + 8 * lg];
-#pragma unroll
-        for (int j = 0; j < 4; j++) bfr[j] = *(const bf16x8*)&Bs[(wn + 16 * j + lr) * LDK + 8 * lg];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        mfma_tile();
         __syncthreads();
     }
     // epilogue
