@@ -555,17 +555,51 @@ int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, cons
     hipFree(d_rads2);
     hipFree(d_cand2);
     if (rc) return rc;
-    if (!unsure.empty()) {  // full HMM fallback
-        std::vector<double> x(unsure.size() * (size_t)d);
+    if (!unsure.empty()) {
+        // Full HMM fallback. A few reads against every dye-seq would each be
+        // one workgroup working through S candidates while the rest of the
+        // GPU idles, so each read's dye-seqs are scored in chunks as separate
+        // candidate lists and the chunks are merged in order: the largest
+        // score with the earlier candidate winning ties (a later chunk must be
+        // strictly larger), and the totals by log-sum-exp over the chunks
+        // that hold a non-empty pair (hmm-classifier.h:45-78 over all S).
+        const int chunk = 512, nch = (S + chunk - 1) / chunk;
+        const size_t nv = unsure.size() * (size_t)nch;
+        std::vector<double> x(nv * (size_t)d);
+        std::vector<int> off(nv + 1), idx((size_t)S * unsure.size());
+        size_t o = 0;
         for (size_t i = 0; i < unsure.size(); i++)
-            std::memcpy(&x[i * d], radiometries + (size_t)unsure[i] * d, sizeof(double) * d);
-        std::vector<int> id(unsure.size());
-        std::vector<double> adj(unsure.size());
-        rc = wp_hmm_classify(clf, (int)unsure.size(), x.data(), id.data(), adj.data(), nullptr, nullptr);
+            for (int ch = 0; ch < nch; ch++) {
+                const size_t v = i * nch + ch;
+                std::memcpy(&x[v * d], radiometries + (size_t)unsure[i] * d, sizeof(double) * d);
+                off[v] = (int)o;
+                for (int k = ch * chunk; k < std::min(S, (ch + 1) * chunk); k++) idx[o++] = k;
+            }
+        off[nv] = (int)o;
+        std::vector<int> id(nv);
+        std::vector<double> adj(nv), ls(nv), lt(nv);
+        rc = wp_hmm_classify_candidates(clf, (int)nv, x.data(), off.data(), idx.data(), id.data(), adj.data(),
+                                        ls.data(), lt.data());
         if (rc) return rc;
         for (size_t i = 0; i < unsure.size(); i++) {
-            out_id[unsure[i]] = id[i];
-            out_adjusted[unsure[i]] = adj[i];
+            const size_t v0 = i * nch;
+            int best_id = id[v0];
+            double best = ls[v0], tot = -INFINITY;
+            for (int ch = 0; ch < nch; ch++) {
+                const size_t v = v0 + ch;
+                if (ls[v] > best) {
+                    best = ls[v];
+                    best_id = id[v];
+                }
+                if (ls[v] > -INFINITY) {  // an all-empty chunk holds no mass
+                    const double hi = std::max(tot, lt[v]);
+                    tot = hi + std::log(std::exp(tot - hi) + std::exp(lt[v] - hi));
+                }
+            }
+            double a = tot > -INFINITY ? std::exp(best - tot) : 0.0;
+            if (std::isnan(a)) a = 0.0;
+            out_id[unsure[i]] = best_id;
+            out_adjusted[unsure[i]] = a;
         }
     }
     if (out_num_fallback) *out_num_fallback = (int)unsure.size();
