@@ -201,13 +201,14 @@ int wp_hmm_last_timing(wp_hmm_classifier* clf, double* prep_ms, double* classify
  * at most 16 states run the quad-rows kernel in one of two variants with
  * identical outputs: 0 computes every timestep at the bin's box capacity,
  * 1 at each timestep's own box class. mode -1 (the default) times both on the
- * first call's reads and keeps the faster per bin; 0 or 1 forces one. Setting
- * the mode discards an earlier choice. */
+ * first call with at least 20,000 reads and keeps the faster per bin (smaller
+ * calls before it use 0 for bins of up to 4 states and 1 above); 0 or 1 forces
+ * one. Setting the mode discards an earlier choice. */
 int wp_hmm_set_variant(wp_hmm_classifier* clf, int mode);
 
-/* The per-bin choice (0/1, or -1 before the first call) and, when the first
- * call timed them, each variant's best time in ms ([bin][2], -1 if untimed).
- * Writes up to cap bins; returns the number of bins of this classifier. */
+/* The per-bin variant in use (0/1, or -1 before the first call) and, once a
+ * call has timed them, each variant's best time in ms ([bin][2], -1 if
+ * untimed). Writes up to cap bins; returns the number of bins. */
 int wp_hmm_get_variants(const wp_hmm_classifier* clf, int* out_variant, double* out_ms, int cap);
 
 /* Graph replay (no reference counterpart). The *_device entry points capture

@@ -52,6 +52,13 @@ def test_variants_identical_and_autotuned(gpu):
             assert (v == mode).all()
     _same(res[0], res[1])
     _same(res[0], res[-1])
+    # a call of 20,000 reads or more times both variants of each quad-rows bin
+    big, d_big, rbig = _setup(w, N=20000, S=200, seed=9)
+    big.set_variant(-1)
+    _run(big, d_big, rbig.shape[0], stream)
+    v, ms = big.variants()
+    print("tuned", v.tolist(), ms.tolist(), flush=True)
+    assert (v >= 0).all() and (ms[:3] >= 0).all()
     # the host entry point (eager launches) agrees with the device one
     ids, adj, ls, lt = clf.classify_arrays(rads)
     _same(res[0], [ids, adj, ls, lt])

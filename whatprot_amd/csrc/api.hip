@@ -107,6 +107,7 @@ struct wp_hmm_classifier {
     // call (variant_mode -1) or forced (0 bin-wide boxes, 1 per-timestep classes)
     int variant_mode = -1;
     int tuned = 0;
+    int variants_set = 0;  // variant[] holds the defaults or the tuned choice
     int variant[wp::kMaxBins] = {0};
     float tune_ms[wp::kMaxBins][2] = {{-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f},
                                        {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}};
@@ -370,6 +371,11 @@ int enqueue_call(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream, bo
     return WP_OK;
 }
 
+// Calls with fewer reads use the default variants (bin-wide boxes for bins of
+// up to 4 states, per-timestep classes above, as measured on the 10^5-read
+// benchmark) and leave the tuning to the first call with at least this many.
+constexpr int kTuneMinReads = 20000;
+
 // Picks each quad-rows bin's variant by timing both on this call's reads
 // (two runs each, the faster second run wins; both give identical outputs).
 int autotune(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream) {
@@ -432,8 +438,17 @@ int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_c
     if (rc) return rc;
     const CallArgs x{N, d_rads, d_cand_off, d_cand_idx, d_id, d_adj, d_ls, d_lt, d_pair};
     if (!c->tuned) {
-        rc = autotune(c, x, stream);
-        if (rc) return rc;
+        if (c->variant_mode < 0 && N < kTuneMinReads) {
+            // too few reads to time the variants reliably: the measured
+            // defaults until a large call tunes them
+            const std::vector<Bin> bins = make_bins(c);
+            for (int b = 0; b < (int)bins.size(); b++) c->variant[b] = bins[b].hi >= 8 ? 1 : 0;
+            c->variants_set = 1;
+        } else {
+            rc = autotune(c, x, stream);
+            if (rc) return rc;
+            c->variants_set = 1;
+        }
     }
     if (!graph_ok || !c->use_graph || c->timing) return enqueue_call(c, x, stream, c->timing != 0);
     std::vector<const void*> key = {(const void*)(intptr_t)N, d_rads, d_cand_off, d_cand_idx, d_id, d_adj,
@@ -765,6 +780,7 @@ int wp_hmm_set_variant(wp_hmm_classifier* c, int mode) {
     }
     c->variant_mode = mode;
     c->tuned = 0;
+    c->variants_set = 0;
     for (int b = 0; b < wp::kMaxBins; b++) c->variant[b] = 0;
     for (int b = 0; b < wp::kMaxBins; b++) c->tune_ms[b][0] = c->tune_ms[b][1] = -1.f;
     drop_graph(c);
@@ -778,7 +794,7 @@ int wp_hmm_get_variants(const wp_hmm_classifier* c, int* out_variant, double* ou
     }
     const int nb = (int)make_bins(c).size();
     for (int b = 0; b < std::min(nb, cap); b++) {
-        if (out_variant) out_variant[b] = c->tuned ? c->variant[b] : -1;
+        if (out_variant) out_variant[b] = c->variants_set ? c->variant[b] : -1;
         if (out_ms) {
             out_ms[2 * b] = c->tune_ms[b][0];
             out_ms[2 * b + 1] = c->tune_ms[b][1];
