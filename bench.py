@@ -47,9 +47,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3
 # HBM bytes of one classify step of the default workload (range, bounds, binning and
-# classify kernels): FETCH_SIZE 63.6 MiB x 2.0 + WRITE_SIZE 30.5 MiB x 1.0, each scaled by a
+# classify kernels): FETCH_SIZE 63.3 MiB x 2.0 + WRITE_SIZE 32.7 MiB x 1.0, each scaled by a
 # 512 MiB device copy in the same rocprofv3 --pmc runs.
-CLASSIFY_HBM_BYTES = 162_000_000
+CLASSIFY_HBM_BYTES = 163_000_000
 
 
 def log(msg):
