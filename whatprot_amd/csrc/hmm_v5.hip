@@ -108,6 +108,11 @@ __device__ __forceinline__ void v5_step(const V5Step& st, const int (&cix)[R], c
 #pragma unroll
     for (int k = 0; k < R; k++) {
         if (st.t < G * k) continue;  // every row of slot k starts after t (uniform)
+        // Every slot reads the same Wt; without this the compiler keeps the
+        // previous slot's Wt loads live across slots, which costs registers
+        // (spills under the occupancy bounds below) for no LDS traffic saved
+        // worth having: the reads are broadcasts.
+        if constexpr (K > 1) asm volatile("" ::: "memory");
         const float* W1 = Wt + (1 + cix[k]) * MS;
         float BA[K], BB[K], BA1[K];
 #pragma unroll
@@ -475,13 +480,22 @@ __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsi
 // One workgroup per read of the bin's compacted list: a persistent loop over
 // a queue costs this kernel registers (the compiler keeps read-invariant
 // values live across reads), which lowers its occupancy.
-// Boxes of up to 8 states keep at least 2 waves per SIMD (the register
-// allocator otherwise drifts to 1 for the per-timestep-class form).
-template <int NB>
-constexpr int v5_min_waves() { return NB <= 8 ? 2 : 1; }
+// Waves per SIMD. The step is latency-bound at one wave per SIMD, so up to 3
+// row slots (T <= 12) the bounds trade register spills for occupancy, as
+// measured on 10^5 reads x 1,000 dye-seqs (C=3, T=12): boxes of up to 16
+// states at 2 waves (unbounded, 390 registers and 1 wave; bounded, ~100
+// spilled: 19.3 -> 11.3 ms), up to 8 states at 3 waves (253 registers and 2
+// waves; bounded, 86 spilled: 8.4 -> 7.3 ms), up to 4 states at 2. More row
+// slots would spill their state itself, so they keep 2 waves up to 8 states
+// and 1 above.
+template <int NB, int R>
+constexpr int v5_min_waves() {
+    if (R <= 3) return NB <= 4 ? 2 : NB <= 8 ? 3 : 2;
+    return NB <= 8 ? 2 : 1;
+}
 
 template <int C, int NB, int R, bool kClass>
-__global__ void __launch_bounds__(kV5Threads, v5_min_waves<NB>()) classify_v5_kernel(const ClassifyArgs a) {
+__global__ void __launch_bounds__(kV5Threads, (v5_min_waves<NB, R>())) classify_v5_kernel(const ClassifyArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     if ((int)blockIdx.x >= *a.list_count) return;
     v5_read<C, NB, R, kClass>(a, a.list[blockIdx.x], smem);
