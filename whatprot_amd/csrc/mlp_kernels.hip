@@ -693,17 +693,31 @@ __device__ inline unsigned int wave_kth(int n, int H, Get get) {
     return K;
 }
 
-// Top-H with one wavefront per row, in two streaming passes over the row.
-// Pass 1 stores the maxima of groups of 16 consecutive logits in LDS and the
-// row's log-sum-exp (online, per lane, then over the wave). tau = the H-th
-// largest group maximum bounds the H-th largest logit from below (the H
-// group maxima >= tau are H logits). Pass 2 compacts the logits >= tau in
-// index order into LDS; the exact H-th largest key K is selected among them
-// (thresholds <= tau count every candidate, >= H, as over the whole row), or,
-// when more than kWaveCap logits are >= tau, by the same rounds over the
-// whole row. Output: the logits > K and the first ties at K, in index order.
+// Online log-sum-exp state of one lane: the running maximum m and the sum of
+// exp(v - m) over the values seen.
+__device__ inline void lse_add(float v, float& m, float& sacc) {
+    if (v > m) {
+        sacc = sacc * expf(m - v) + 1.0f;
+        m = v;
+    } else {
+        sacc += expf(v - m);
+    }
+}
+
+// Top-H with one wavefront per row. Pass 1 streams the row once, storing the
+// maxima of groups of 16 consecutive logits in LDS and the row's
+// log-sum-exp (online, per lane, then over the wave). tau = the H-th largest
+// group maximum bounds the H-th largest logit from below (the H group
+// maxima >= tau are H logits), so only the groups whose maximum is >= tau
+// (at least H of them, usually few more) can hold a top-H logit: pass 2
+// reads just those groups and compacts their logits >= tau in index order
+// into LDS. The exact H-th largest key K is selected among them (thresholds
+// <= tau count every candidate, >= H, as over the whole row), or, when more
+// than kWaveCap logits are >= tau, by the same rounds over the whole row.
+// Output: the logits > K and the first ties at K, in index order. vec: the
+// rows are 16-byte aligned (pass 1 loads four logits per lane).
 __global__ void __launch_bounds__(64 * kWaveRows) topk_wave_kernel(int rows, int N, int H, const float* __restrict__ Z,
-                                                                   long ldz, int* __restrict__ out,
+                                                                   long ldz, int vec, int* __restrict__ out,
                                                                    const float* __restrict__ lse_in,
                                                                    float* __restrict__ mass, float* __restrict__ lse_out,
                                                                    float* __restrict__ post_out) {
@@ -719,31 +733,59 @@ __global__ void __launch_bounds__(64 * kWaveRows) topk_wave_kernel(int rows, int
     const float* z = Z + (long)r * ldz;
     const unsigned long long below = (1ull << lane) - 1ull;
 
-    // pass 1: group maxima, and the log-sum-exp; kUnroll loads in flight per lane
-    constexpr int kUnroll = 16;
+    // pass 1: group maxima, and the log-sum-exp
     float m = -INFINITY, sacc = 0.0f;
-    for (int b0 = 0; b0 < N; b0 += 64 * kUnroll) {
-        float vv[kUnroll];
+    if (vec) {
+        // lane l holds logits n..n+3 of a span of 256: a group is 4 lanes
+        constexpr int kU = 8;
+        for (int b0 = 0; b0 < N; b0 += 256 * kU) {
+            float4 vv[kU];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const int n = b0 + 64 * u + lane;
-            vv[u] = n < N ? z[n] : -INFINITY;
-        }
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const int n = b0 + 64 * u + lane;
-            const float v = vv[u];
-            unsigned int k = n < N ? fkey(v) : 0u;
-#pragma unroll
-            for (int off = 1; off < kWaveGroup; off <<= 1) k = max(k, (unsigned int)__shfl_xor((int)k, off, 64));
-            if ((lane & (kWaveGroup - 1)) == 0 && n < N) gm[n / kWaveGroup] = k;
-            if (n < N) {
-                if (v > m) {
-                    sacc = sacc * expf(m - v) + 1.0f;
-                    m = v;
+            for (int u = 0; u < kU; u++) {
+                const int n = b0 + 256 * u + 4 * lane;
+                if (n + 3 < N) {
+                    vv[u] = *(const float4*)(z + n);
                 } else {
-                    sacc += expf(v - m);
+                    vv[u].x = n < N ? z[n] : -INFINITY;
+                    vv[u].y = n + 1 < N ? z[n + 1] : -INFINITY;
+                    vv[u].z = n + 2 < N ? z[n + 2] : -INFINITY;
+                    vv[u].w = -INFINITY;
                 }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const int n = b0 + 256 * u + 4 * lane;
+                const float v[4] = {vv[u].x, vv[u].y, vv[u].z, vv[u].w};
+                unsigned int k = 0u;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (n + j < N) {
+                        k = max(k, fkey(v[j]));
+                        lse_add(v[j], m, sacc);
+                    }
+                }
+                k = max(k, (unsigned int)__shfl_xor((int)k, 1, 64));
+                k = max(k, (unsigned int)__shfl_xor((int)k, 2, 64));
+                if ((lane & 3) == 0 && n < N) gm[n / kWaveGroup] = k;
+            }
+        }
+    } else {
+        constexpr int kUnroll = 16;  // loads in flight per lane
+        for (int b0 = 0; b0 < N; b0 += 64 * kUnroll) {
+            float vv[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const int n = b0 + 64 * u + lane;
+                vv[u] = n < N ? z[n] : -INFINITY;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const int n = b0 + 64 * u + lane;
+                unsigned int k = n < N ? fkey(vv[u]) : 0u;
+#pragma unroll
+                for (int off = 1; off < kWaveGroup; off <<= 1) k = max(k, (unsigned int)__shfl_xor((int)k, off, 64));
+                if ((lane & (kWaveGroup - 1)) == 0 && n < N) gm[n / kWaveGroup] = k;
+                if (n < N) lse_add(vv[u], m, sacc);
             }
         }
     }
@@ -764,18 +806,36 @@ __global__ void __launch_bounds__(64 * kWaveRows) topk_wave_kernel(int rows, int
     wsync();
     const unsigned int tau = G >= H ? wave_kth(G, H, [&](int i) { return gm[i]; }) : 0u;
 
-    // pass 2: the logits >= tau, in index order
+    // the groups whose maximum is >= tau, in index order, compacted in place
+    // (every lane reads its entry before any lane writes, and writes land at
+    // or below the entries read)
+    int NG = 0;
+    for (int b = 0; b < G; b += 64) {
+        const int i = b + lane;
+        const bool f = i < G && gm[i] >= tau;
+        const unsigned long long bal = __ballot(f);
+        if (f) gm[NG + __popcll(bal & below)] = (unsigned int)i;
+        NG += __popcll(bal);
+    }
+    wsync();
+
+    // pass 2: the logits >= tau of those groups, four groups per load
+    // (16 lanes each), in index order
     int M = 0;
-    for (int b0 = 0; b0 < N; b0 += 64 * kUnroll) {
-        float vv[kUnroll];
+    constexpr int kU2 = 16;
+    for (int b = 0; b < NG; b += 4 * kU2) {
+        float vv[kU2];
+        int nn[kU2];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const int n = b0 + 64 * u + lane;
-            vv[u] = n < N ? z[n] : 0.0f;
+        for (int u = 0; u < kU2; u++) {
+            const int sl = b + 4 * u + (lane >> 4);
+            nn[u] = sl < NG ? (int)gm[sl] * kWaveGroup + (lane & (kWaveGroup - 1)) : N;
         }
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const int n = b0 + 64 * u + lane;
+        for (int u = 0; u < kU2; u++) vv[u] = nn[u] < N ? z[nn[u]] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kU2; u++) {
+            const int n = nn[u];
             const unsigned int k = n < N ? fkey(vv[u]) : 0u;
             const bool f = n < N && k >= tau;
             const unsigned long long bal = __ballot(f);
@@ -841,8 +901,9 @@ static hipError_t topk_wave(int rows, int N, int H, const float* Z, long ldz, in
                                            (int)lds);
         if (e != hipSuccess) return e;
     }
+    const int vec = ldz % 4 == 0 && reinterpret_cast<uintptr_t>(Z) % 16 == 0;
     hipLaunchKernelGGL(topk_wave_kernel, dim3((rows + kWaveRows - 1) / kWaveRows), dim3(64 * kWaveRows), lds, s, rows,
-                       N, H, Z, ldz, out, lse_in, mass, lse_out, post_out);
+                       N, H, Z, ldz, vec, out, lse_in, mass, lse_out, post_out);
     return hipGetLastError();
 }
 
