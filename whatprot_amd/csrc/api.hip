@@ -121,6 +121,10 @@ struct wp_hmm_classifier {
     int timing = 0;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     int ev_recorded = 0;
+    // the bins of a call run as parallel branches (the caller's stream and
+    // these), so one bin's tail of heavy reads overlaps the others' work
+    hipStream_t branch[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -359,10 +363,30 @@ int enqueue_call(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream, bo
     enqueue_prep(c, bins, x, stream);
     WP_HIP(hipGetLastError());
     if (events) WP_HIP(hipEventRecord(c->ev[1], stream));
+    // Bins are independent once the prep has built their lists: every read's
+    // outputs, wide lists and counters belong to one bin. Bin b runs on
+    // branch b mod 3 (branch 0 = the caller's stream); the caller's stream
+    // then waits for every branch.
+    constexpr int kBranches = 3;
+    const int nbr = std::min(kBranches, (int)bins.size());
+    if (nbr > 1) {
+        if (!c->fork) WP_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+        WP_HIP(hipEventRecord(c->fork, stream));
+        for (int k = 0; k + 1 < nbr; k++) {
+            if (!c->branch[k]) WP_HIP(hipStreamCreateWithFlags(&c->branch[k], hipStreamNonBlocking));
+            if (!c->join[k]) WP_HIP(hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming));
+            WP_HIP(hipStreamWaitEvent(c->branch[k], c->fork, 0));
+        }
+    }
     for (int b = 0; b < (int)bins.size(); b++) {
-        WP_HIP(launch_bin_main(c, bins, b, x, c->variant[b], stream));
-        if (bins[b].kernel != kRowPerLane) WP_HIP(launch_bin_wide(c, bins, b, x, 1, stream));
-        if (bins[b].kernel == kGather) WP_HIP(launch_bin_wide(c, bins, b, x, 2, stream));
+        hipStream_t s = b % nbr == 0 ? stream : c->branch[b % nbr - 1];
+        WP_HIP(launch_bin_main(c, bins, b, x, c->variant[b], s));
+        if (bins[b].kernel != kRowPerLane) WP_HIP(launch_bin_wide(c, bins, b, x, 1, s));
+        if (bins[b].kernel == kGather) WP_HIP(launch_bin_wide(c, bins, b, x, 2, s));
+    }
+    for (int k = 0; k + 1 < nbr; k++) {
+        WP_HIP(hipEventRecord(c->join[k], c->branch[k]));
+        WP_HIP(hipStreamWaitEvent(stream, c->join[k], 0));
     }
     if (events) {
         WP_HIP(hipEventRecord(c->ev[2], stream));
@@ -621,6 +645,11 @@ void wp_hmm_classifier_destroy(wp_hmm_classifier* c) {
         if (c->ev[k]) hipEventDestroy(c->ev[k]);
     drop_graph(c);
     if (c->cap_stream) hipStreamDestroy(c->cap_stream);
+    for (int k = 0; k < 2; k++) {
+        if (c->branch[k]) hipStreamDestroy(c->branch[k]);
+        if (c->join[k]) hipEventDestroy(c->join[k]);
+    }
+    if (c->fork) hipEventDestroy(c->fork);
     hipFree(c->wLists);
     hipFree(c->wWideLists);
     hipFree(c->wWide2Lists);
