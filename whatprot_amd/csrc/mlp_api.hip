@@ -486,14 +486,16 @@ int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, cons
     // everything stays on the device: radiometries, shortlists, confidences
     double* d_rads = nullptr;
     int *d_cand = nullptr, *d_off = nullptr, *d_id = nullptr, *d_sel = nullptr, *d_cand2 = nullptr;
-    float* d_conf = nullptr;
+    float *d_post = nullptr, *d_mass = nullptr, *d_lse = nullptr;
     double *d_adj = nullptr, *d_rads2 = nullptr;
     std::vector<int> off(N + 1);
     for (int r = 0; r <= N; r++) off[r] = r * H;
     rc = dalloc(&d_rads, (size_t)N * d);
     if (!rc) rc = dalloc(&d_cand, (size_t)N * H);
     if (!rc) rc = dalloc(&d_off, (size_t)N + 1);
-    if (!rc) rc = dalloc(&d_conf, (size_t)N);
+    if (!rc) rc = dalloc(&d_post, (size_t)N);
+    if (!rc) rc = dalloc(&d_mass, (size_t)N);
+    if (!rc) rc = dalloc(&d_lse, (size_t)N);
     if (!rc) rc = dalloc(&d_id, (size_t)N);
     if (!rc) rc = dalloc(&d_adj, (size_t)N);
     auto hip_ok = [&](hipError_t e, const char* what) {
@@ -505,18 +507,19 @@ int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, cons
     };
     if (!rc) hip_ok(hipMemcpy(d_rads, radiometries, sizeof(double) * (size_t)N * d, hipMemcpyHostToDevice), "upload");
     if (!rc) hip_ok(hipMemcpy(d_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice), "upload");
+    // the chunks' statistics go straight to per-read arrays: no copy, and no
+    // host synchronisation, between chunks
     for (int r0 = 0; r0 < N && !rc; r0 += chunk) {
         const int B = std::min(chunk, N - r0);
         if (!hip_ok(wp::pack_input(B, d, m->in_pad, d_rads + (size_t)r0 * d, nullptr, m->X, nullptr), "pack"))
             break;
         rc = forward(m, B);
         if (rc) break;
-        hip_ok(wp::topk_softmax(B, m->out_dim, H, m->Z, m->out_dim, d_cand + (size_t)r0 * H, m->lse, m->post,
-                                m->mass, nullptr),
+        hip_ok(wp::topk_softmax(B, m->out_dim, H, m->Z, m->out_dim, d_cand + (size_t)r0 * H, d_lse + r0,
+                                d_post + r0, d_mass + r0, nullptr),
                "top-k");
-        hip_ok(hipMemcpy(d_conf + r0, criterion == WP_SHORTLIST_MASS ? m->mass : m->post, sizeof(float) * B,
-                         hipMemcpyDeviceToDevice), "copy");
     }
+    const float* d_conf = criterion == WP_SHORTLIST_MASS ? d_mass : d_post;
     std::vector<float> conf(N);
     if (!rc) hip_ok(hipMemcpy(conf.data(), d_conf, sizeof(float) * N, hipMemcpyDeviceToHost), "download");
     std::vector<int> sure, unsure;
@@ -548,7 +551,9 @@ int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, cons
     hipFree(d_rads);
     hipFree(d_cand);
     hipFree(d_off);
-    hipFree(d_conf);
+    hipFree(d_post);
+    hipFree(d_mass);
+    hipFree(d_lse);
     hipFree(d_id);
     hipFree(d_adj);
     hipFree(d_sel);
