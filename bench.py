@@ -47,9 +47,11 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3
 # HBM bytes of one classify step of the default workload (range, bounds, binning and
-# classify kernels): FETCH_SIZE 63.3 MiB x 2.0 + WRITE_SIZE 32.7 MiB x 1.0, each scaled by a
-# 512 MiB device copy in the same rocprofv3 --pmc runs.
-CLASSIFY_HBM_BYTES = 163_000_000
+# classify kernels): FETCH_SIZE 4,364,105 KiB x 2.0 + WRITE_SIZE 5,939,536 KiB x 1.0, each
+# scaled by a 512 MiB device copy in the same rocprofv3 --pmc runs. All but about 160 MB of it
+# is the register spill traffic of the quad-rows kernels of boxes (4,8] and (8,16] (scratch),
+# which run at 3 and 2 waves per SIMD: faster than unspilled at 2 and 1.
+CLASSIFY_HBM_BYTES = 15_020_000_000
 
 
 def log(msg):
