@@ -89,3 +89,32 @@ def test_graph_replay_matches_eager(gpu):
     n2 = N // 2
     part = _run(clf, d_rads[:n2].contiguous(), n2, stream)
     _same([e[:n2] for e in eager], part)
+
+
+def test_outputs_ordered_on_the_callers_stream(gpu):
+    """A call's bins run on branches of their own; work enqueued on the
+    caller's stream after the call (here clones of the outputs, on a
+    non-default stream, with no device synchronisation in between) must see
+    every output, in graph replay and in eager launches."""
+    import torch
+    w = gpu
+    clf, d_rads, rads = _setup(w, N=4000, seed=13)
+    N = rads.shape[0]
+    want = _run(clf, d_rads, N, torch.cuda.current_stream())
+    dev = d_rads.device
+    side = torch.cuda.Stream(device=dev)
+    side.wait_stream(torch.cuda.current_stream())
+    out = [torch.empty(N, dtype=torch.int32, device=dev)] + [torch.empty(N, dtype=torch.float64, device=dev)
+                                                           for _ in range(3)]
+    for graph in (True, False, True):
+        clf.set_graph(graph)
+        with torch.cuda.stream(side):
+            for o in out:
+                o.fill_(-5)
+            clf.classify_device(N, d_rads.data_ptr(), *[o.data_ptr() for o in out], side.cuda_stream)
+            got = [o.clone() for o in out]
+            for o in out:
+                o.fill_(-9)  # overwritten on the stream after the clones
+        side.synchronize()
+        print("graph", graph, flush=True)
+        _same(want, [g.cpu().numpy() for g in got])
