@@ -102,7 +102,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(reads shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(reads would shard across GPUs by read index with no exchange between them)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "fit":
