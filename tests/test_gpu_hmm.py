@@ -206,3 +206,52 @@ def test_edge_cases(gpu, oracle, precision):
             assert (gid == oid).all() or precision == 1
             if cut == 5.0:
                 assert gid[1] == 0 and adj[1] == 0.0  # all-empty: first candidate, NaN guard
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_zero_reads_and_rejected_arguments(gpu, oracle, precision):
+    """No reads is a no-op on the host and device entry points; a negative read
+    count, decreasing candidate offsets and candidate indices outside the
+    dye-seq table return WP_ERR_ARG (-1) before anything is written, and the
+    classifier still matches the oracle afterwards."""
+    import torch
+
+    w = gpu
+    m = w.SequencingModel.uniform(2)
+    strings = ["10.01", "0", "1"]
+    T = 4
+    clf = make_clf(w, m, T, strings, 5.0, precision)
+    empty = np.zeros((0, T, 2))
+    for out in (clf.classify_arrays(empty), clf.classify_arrays(empty, candidates=[])):
+        assert all(len(x) == 0 for x in out)
+    assert clf.pair_log_probability(empty).shape == (0, len(strings))
+    d_rads = torch.zeros(T * 2, dtype=torch.float64, device="cuda")
+    d_id = torch.full((1,), 7, dtype=torch.int32, device="cuda")
+    clf.classify_device(0, d_rads.data_ptr(), d_id.data_ptr())
+    torch.cuda.synchronize()
+    assert int(d_id[0]) == 7
+
+    lib = w.lib()
+    rads = np.random.default_rng(11).uniform(0.0, 3.0, size=(2, T, 2))
+    ids = np.full(2, 99, dtype=np.int32)
+    adj, ls, lt = np.zeros(2), np.zeros(2), np.zeros(2)
+    outs = (ids.ctypes.data, adj.ctypes.data, ls.ctypes.data, lt.ctypes.data)
+    assert lib.wp_hmm_classify(clf._h, -1, rads.ctypes.data, *outs) == -1
+    idx = np.array([0, 1], dtype=np.int32)
+    decreasing = np.array([0, 2, 1], dtype=np.int32)
+    assert lib.wp_hmm_classify_candidates(clf._h, 2, rads.ctypes.data, decreasing.ctypes.data,
+                                          idx.ctypes.data, *outs) == -1
+    off = np.array([0, 1, 2], dtype=np.int32)
+    for bad in (-1, len(strings)):
+        idx = np.array([0, bad], dtype=np.int32)
+        assert lib.wp_hmm_classify_candidates(clf._h, 2, rads.ctypes.data, off.ctypes.data,
+                                              idx.ctypes.data, *outs) == -1
+    assert (ids == 99).all() and not adj.any()
+
+    got = clf.pair_log_probability(rads)
+    with np.errstate(divide="ignore"):
+        want = np.log(oracle.pair_probabilities(m, strings, rads, 5.0))
+    check_empty_pattern(got, want, precision)
+    err = log_err(got, want)
+    err[np.isinf(want)] = 0.0
+    assert err.max() <= (1e-12 if precision == 0 else FP32_LOGP_TOL)
