@@ -125,3 +125,26 @@ def test_search_matches_reference_larger_tree(gpu, ref):
     np.testing.assert_array_equal(got[0], want[0])
     np.testing.assert_array_equal(got[1], want[1])
     np.testing.assert_array_equal(got[2], want[2])
+
+
+def test_zero_reads_then_reads(gpu, ref, workload):
+    """No reads is a no-op for search, NN classify and hybrid classify (the
+    hybrid path once rejected it: its empty per-read result vectors reached the
+    HMM entry point as null outputs); the same classifiers then still match the
+    reference on real reads."""
+    C, T, model, seqs, tracks, rads = workload
+    empty = np.zeros((0, T, C))
+    nn = w.NNClassifier(T, C, model, 20, 0.5, tracks)
+    cnt, idx, dist = nn.search(empty)
+    assert len(cnt) == 0 and idx.size == 0 and dist.size == 0
+    assert all(len(x) == 0 for x in nn.classify_arrays(empty))
+    hc = w.HybridClassifier(T, C, model, w.SequencingSettings(5.0), 20, 0.5, tracks, 10, seqs,
+                            w.WP_PRECISION_FP64)
+    ids, adj = hc.classify_arrays(empty)
+    assert len(ids) == 0 and len(adj) == 0
+    some = rads[:17]
+    ids, adj = hc.classify_arrays(some)
+    rid, rsc, rtot = ref.hybrid_classify(model, tracks, 20, 0.5, 10, [s.dye_string for s in seqs],
+                                         [s.count for s in seqs], [s.id for s in seqs], some, 5.0)
+    np.testing.assert_array_equal(ids, rid)
+    np.testing.assert_allclose(adj, rsc / rtot, rtol=1e-12, atol=1e-300)

@@ -419,6 +419,7 @@ int wp_hybrid_classify(wp_hybrid_classifier* c, int num_reads, const double* rad
         return WP_ERR_ARG;
     }
     const int N = num_reads;
+    if (N == 0) return WP_OK;  // (the per-read vectors below would be empty, their data() null)
     wp_nn_classifier* nn = c->nn;
     std::vector<int> cnt, idx;
     std::vector<double> dist;
