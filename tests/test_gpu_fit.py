@@ -165,3 +165,19 @@ def test_bootstrap_rounds_are_fits_of_resamples(gpu):
     v = sorted(m.channel_models[1].p_bleach for m in res["models"])
     assert res["ci_min"].channel_models[1].p_bleach == v[lo_i]
     assert res["ci_max"].channel_models[1].p_bleach == v[hi_i]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_estep_zero_reads(gpu, precision):
+    """A fitter over no reads is valid: its E-step adds nothing (every statistic
+    and the sum of ln P are 0, as HMMFitter's loop over an empty vector leaves
+    them), and a fitter over reads on the same dye-seq is unaffected."""
+    C, T, ds = 2, 8, "..0.1..0.1"
+    f = w.HMMFitter(T, C, ds, np.zeros((0, T, C)), precision)
+    model = w.SequencingModel.uniform(C)
+    st, slp = f.estep(model)
+    assert st.size > 0 and not st.any() and slp == 0.0
+    _, rads = _data(C, T, ds, 8, 5)
+    g = w.HMMFitter(T, C, ds, rads, precision)
+    st2, slp2 = g.estep(model)
+    assert np.isfinite(slp2) and slp2 != 0.0 and st2.any()
