@@ -7,7 +7,7 @@
  * CPU-baseline leg of the benchmark may load it. Nothing in the product links
  * or calls it.
  *
- * Parity pins (see tests/test_oracle_*.py):
+ * Parity pins (tests/test_oracle.py, tests/test_boxes.py, tests/test_fit.py):
  *   - whole-HMM goldens from hmm/hmm/peptide-hmm.test.cc:188,227,266,305,344;
  *   - ChannelModel::pdf goldens from channel-model.test.cc;
  *   - binomial table vs the closed form C(i,j)(1-q)^j q^(i-j);
