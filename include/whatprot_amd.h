@@ -37,7 +37,8 @@ enum {
     WP_ERR_HIP = -2,         /* HIP runtime error */
     WP_ERR_UNSUPPORTED = -3, /* shape beyond the limits above */
     WP_ERR_NOMEM = -4,       /* allocation failed */
-    WP_ERR_NO_DEVICE = -5    /* no gfx950 device visible */
+    WP_ERR_NO_DEVICE = -5,   /* no gfx950 device visible */
+    WP_ERR_INTERNAL = -6     /* a kernel reported a broken invariant */
 };
 
 /* Arithmetic of the forward recursion.
@@ -216,6 +217,23 @@ int wp_hmm_get_variants(const wp_hmm_classifier* clf, int* out_variant, double* 
  * reads, the device pointers and the kernel choice repeat (default on);
  * enable 0 launches every kernel eagerly. */
 int wp_hmm_set_graph(wp_hmm_classifier* clf, int enable);
+
+/* FP64 decision of FP32 near-ties (no reference counterpart; it makes the FP32
+ * path pick the candidate HMMClassifier::classify_helper picks,
+ * classifiers/hmm-classifier.h:59-65). An FP32 classifier keeps, per read,
+ * the candidates whose FP32 ln P lies within W = abs + rel |ln P_best| of the
+ * best; when there are two or more, they are rescored in FP64 on the device
+ * and the strict '>' with the earlier candidate winning decides the id and the
+ * ln score (the normaliser stays the FP32 one). W must be at least twice the
+ * FP32 error of ln P; the defaults are 2e-5 and 1e-5. enable 0 keeps the FP32
+ * argmax. Returns WP_ERR_UNSUPPORTED for FP64 classifiers and FP32 ones whose
+ * FP64 boxes do not fit in LDS. */
+int wp_hmm_set_refine(wp_hmm_classifier* clf, int enable, double abs, double rel);
+
+/* After an FP32 call with the FP64 decision on (synchronises the device):
+ * out2[0] = reads whose near-best list was rescored, out2[1] = those among
+ * them rescored over every candidate (their window held more than 32). */
+int wp_hmm_refine_stats(wp_hmm_classifier* clf, int num_reads, int* out2);
 
 /* Algorithmic work of classifying these reads against every dye-seq (no
  * reference counterpart; used for roofline accounting): out3[0] = non-empty

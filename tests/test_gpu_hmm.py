@@ -144,7 +144,8 @@ def test_classify_vs_oracle(gpu, oracle, name, precision):
         raw = float((gid == oid).mean())
         tie = tie_aware_agreement(inv[gid], inv[oid], p64)
         print(f"{name} FP32 top-1 raw {raw:.4f} tie-aware {tie:.4f}", flush=True)
-        assert tie >= 0.999, (raw, tie)
+        # near-ties are decided in FP64: the ids are the oracle's
+        assert raw == 1.0, (raw, tie)
         np.testing.assert_allclose(adj, oadj, rtol=1e-3, atol=1e-6)
 
 
@@ -175,7 +176,7 @@ def test_candidates_vs_oracle(gpu, oracle, precision):
         assert (gid == oid).all()
         np.testing.assert_allclose(adj[1:], oadj[1:], rtol=1e-10, atol=1e-300)
     else:
-        assert (gid == oid).mean() >= 0.95
+        assert (gid == oid).all()
         np.testing.assert_allclose(adj[1:], oadj[1:], rtol=1e-3, atol=1e-6)
 
 
@@ -203,7 +204,7 @@ def test_edge_cases(gpu, oracle, precision):
             assert err.max() <= tol, (T, cut)
             gid, adj, ls, lt = clf.classify_arrays(rads)
             oid, osc, otot = oracle.classify(m, strings, np.ones(5), np.arange(5), rads, cut)
-            assert (gid == oid).all() or precision == 1
+            assert (gid == oid).all()
             if cut == 5.0:
                 assert gid[1] == 0 and adj[1] == 0.0  # all-empty: first candidate, NaN guard
 

@@ -379,6 +379,17 @@ class HMMClassifier:
         """Graph replay of the device entry points' launch sequence."""
         check(lib().wp_hmm_set_graph(self._h, 1 if enable else 0))
 
+    def set_refine(self, enable=True, abs=2e-5, rel=1e-5):
+        """FP32 classifiers: decide the candidates within abs + rel |ln P_best|
+        of the FP32 best in FP64 (default on, with these defaults)."""
+        check(lib().wp_hmm_set_refine(self._h, 1 if enable else 0, float(abs), float(rel)))
+
+    def refine_stats(self, num_reads):
+        """(reads rescored in FP64, of which over every candidate) of the last call."""
+        out = np.zeros(2, dtype=np.int32)
+        check(lib().wp_hmm_refine_stats(self._h, int(num_reads), out.ctypes.data))
+        return int(out[0]), int(out[1])
+
     def work(self, radiometries):
         """(non-empty pairs, state-timesteps, algorithmic FLOPs) of classifying these reads."""
         r = self._rads(radiometries)

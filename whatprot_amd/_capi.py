@@ -114,6 +114,10 @@ def lib():
     L.wp_hmm_get_variants.argtypes = [c_void_p, c_void_p, c_void_p, c_int]
     L.wp_hmm_set_graph.restype = c_int
     L.wp_hmm_set_graph.argtypes = [c_void_p, c_int]
+    L.wp_hmm_set_refine.restype = c_int
+    L.wp_hmm_set_refine.argtypes = [c_void_p, c_int, c_double, c_double]
+    L.wp_hmm_refine_stats.restype = c_int
+    L.wp_hmm_refine_stats.argtypes = [c_void_p, c_int, c_void_p]
     L.wp_hmm_work.restype = c_int
     L.wp_hmm_work.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
     L.wp_generate_dye_seqs.restype = c_int

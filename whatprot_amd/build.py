@@ -38,8 +38,16 @@ def _jobs():
     return max(1, min(16, n or os.cpu_count() or 4))
 
 
-def _compile(src, verbose):
+def _newest_header():
+    hs = glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(INCLUDE, "*.h"))
+    return max((os.path.getmtime(h) for h in hs), default=0.0)
+
+
+def _compile(src, verbose, incremental=False):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    if incremental and os.path.exists(obj) and \
+            os.path.getmtime(obj) > max(os.path.getmtime(src), _newest_header()):
+        return obj
     cmd = [HIPCC] + FLAGS + ["-c", "-o", obj, src]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -51,11 +59,13 @@ def _compile(src, verbose):
     return obj
 
 
-def build(verbose=False):
+def build(verbose=False, incremental=False):
+    """Compiles every source (incremental: only those older than their object
+    or than any header) and links the library and the command line."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     with concurrent.futures.ThreadPoolExecutor(_jobs()) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
+        objs = list(ex.map(lambda s: _compile(s, verbose, incremental), srcs))
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp"] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -72,5 +82,5 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose="-v" in sys.argv)
+    build(verbose="-v" in sys.argv, incremental="-i" in sys.argv)
     print(OUT)

@@ -72,6 +72,18 @@ void binomial_tables(const wp_model& m, int C, int M, double* out) {
     }
 }
 
+bool dye_channels_ok(const short* seqs, int n, int cap, const int* lens, int C) {
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < lens[i]; k++) {
+            if (seqs[(size_t)i * cap + k] >= C) {
+                set_error("dye-seq names a channel >= num_channels");
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
 int require_device() {
     if (wp_device_count() < 1) {
         set_error("no gfx950 device visible");
@@ -79,6 +91,35 @@ int require_device() {
     }
     return WP_OK;
 }
+
+}  // namespace wp
+
+namespace wp {
+
+// Device workspace of one call, carved from one block (arrays 256-byte
+// aligned): per-read boxes and bin lists, and for the FP32 path the near-best
+// lists and the FP64 rescoring pass's own lists and outputs.
+struct Workspace {
+    uint8_t* L = nullptr;        // [N][T][C] read-level box lows
+    uint8_t* H = nullptr;        // [N][T][C] read-level box highs
+    int* nb = nullptr;           // [N] box bound
+    uint8_t* wide = nullptr;     // [N] per read: handed to its bin's wide pass
+    int* lists = nullptr;        // [nbins][N] reads of each bin
+    int* wide_lists = nullptr;   // [nbins][N] reads each bin's main launch hands to its wide pass
+    int* wide2_lists = nullptr;  // [nbins][N] gather bins: reads the full-size separable pass hands on
+    int* ctr = nullptr;          // counters: bin sizes, wide-list sizes
+    // FP32 only
+    int* tie_n = nullptr;        // [N] near-best list length (0 none, -1 every candidate)
+    int* tie_idx = nullptr;      // [N][kTieMax]
+    int* rid = nullptr;          // [N] FP64 pass: id
+    double* rls = nullptr;       // [N] FP64 pass: ln score
+    double* lt = nullptr;        // [N] ln total when the caller does not ask for it
+    uint8_t* rwide = nullptr;    // FP64 pass: as wide, lists, wide_lists, wide2_lists, ctr
+    int* rlists = nullptr;
+    int* rwide_lists = nullptr;
+    int* rwide2_lists = nullptr;
+    int* rctr = nullptr;
+};
 
 }  // namespace wp
 
@@ -92,17 +133,18 @@ struct wp_hmm_classifier {
     int* dCount = nullptr;
     int* dId = nullptr;
     double* dBinom = nullptr;
-    // per-read workspace of the device path
-    uint8_t* wL = nullptr;
-    uint8_t* wH = nullptr;
-    int* wNb = nullptr;
-    uint8_t* wWide = nullptr;  // per read: box densities too wide for the quad-rows kernel
-    int* wLists = nullptr;      // [kMaxBins][N] reads of each bin
-    int* wWideLists = nullptr;  // [kMaxBins][N] reads each bin's main launch hands to its wide pass
-    int* wWide2Lists = nullptr; // [kMaxBins][N] gather bins: reads the full-size separable pass hands on
-    int* wCtr = nullptr;        // counters: bin sizes, wide-list sizes, one queue per launch
+    // 1: the device tables and the per-read boxes belong to the FP32
+    // classifier this FP64 sibling rescoring pass serves
+    int borrowed = 0;
+    // per-read workspace of the device path (one block, wcap reads)
+    wp::Workspace ws;
+    void* wbuf = nullptr;
     int wcap = 0;
     int maxbox = 1;  // prod_c (M+1), bound on any read's box
+    // FP32: near-best candidates decided in FP64 by the sibling's kernels
+    wp_hmm_classifier* r64 = nullptr;
+    int refine = 0;
+    double tie_abs = 0.0, tie_rel = 0.0;
     // quad-rows kernel variant per bin: chosen by timing both on the first
     // call (variant_mode -1) or forced (0 bin-wide boxes, 1 per-timestep classes)
     int variant_mode = -1;
@@ -220,31 +262,52 @@ void drop_graph(wp_hmm_classifier* c) {
     c->gkey.clear();
 }
 
+// Lays out the arrays of an N-read workspace from base (null: sizes only);
+// returns the bytes it spans.
+size_t carve_workspace(const wp_hmm_classifier* c, int N, char* base, wp::Workspace* w) {
+    size_t o = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* p = base ? base + o : nullptr;
+        o += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    const size_t n = (size_t)std::max(N, 1);
+    const size_t tc = n * c->T * c->C;
+    const size_t nbins = make_bins(c).size();
+    w->L = (uint8_t*)take(tc);
+    w->H = (uint8_t*)take(tc);
+    w->nb = (int*)take(sizeof(int) * n);
+    w->wide = (uint8_t*)take(n);
+    w->lists = (int*)take(sizeof(int) * nbins * n);
+    w->wide_lists = (int*)take(sizeof(int) * nbins * n);
+    w->wide2_lists = (int*)take(sizeof(int) * nbins * n);
+    w->ctr = (int*)take(sizeof(int) * kCtrLen);
+    if (c->r64) {
+        const size_t nb64 = make_bins(c->r64).size();
+        w->tie_n = (int*)take(sizeof(int) * n);
+        w->tie_idx = (int*)take(sizeof(int) * n * wp::kTieMax);
+        w->rid = (int*)take(sizeof(int) * n);
+        w->rls = (double*)take(sizeof(double) * n);
+        w->lt = (double*)take(sizeof(double) * n);
+        w->rwide = (uint8_t*)take(n);
+        w->rlists = (int*)take(sizeof(int) * nb64 * n);
+        w->rwide_lists = (int*)take(sizeof(int) * nb64 * n);
+        w->rwide2_lists = (int*)take(sizeof(int) * nb64 * n);
+        w->rctr = (int*)take(sizeof(int) * kCtrLen);
+    }
+    return o;
+}
+
 int ensure_workspace(wp_hmm_classifier* c, int N) {
     if (N <= c->wcap) return WP_OK;
     drop_graph(c);  // the captured launches hold the old workspace
-    hipFree(c->wL);
-    hipFree(c->wH);
-    hipFree(c->wNb);
-    hipFree(c->wWide);
-    hipFree(c->wLists);
-    hipFree(c->wWideLists);
-    hipFree(c->wWide2Lists);
-    hipFree(c->wCtr);
-    c->wL = c->wH = nullptr;
-    c->wNb = nullptr;
-    c->wWide = nullptr;
-    c->wLists = c->wWideLists = c->wWide2Lists = c->wCtr = nullptr;
+    hipFree(c->wbuf);
+    c->wbuf = nullptr;
+    c->ws = wp::Workspace();
     c->wcap = 0;
-    size_t tc = (size_t)N * c->T * c->C;
-    WP_HIP(hipMalloc(&c->wL, tc));
-    WP_HIP(hipMalloc(&c->wH, tc));
-    WP_HIP(hipMalloc(&c->wNb, sizeof(int) * (size_t)N));
-    WP_HIP(hipMalloc(&c->wWide, (size_t)N));
-    WP_HIP(hipMalloc(&c->wLists, sizeof(int) * (size_t)wp::kMaxBins * N));
-    WP_HIP(hipMalloc(&c->wWideLists, sizeof(int) * (size_t)wp::kMaxBins * N));
-    WP_HIP(hipMalloc(&c->wWide2Lists, sizeof(int) * (size_t)wp::kMaxBins * N));
-    WP_HIP(hipMalloc(&c->wCtr, sizeof(int) * kCtrLen));
+    const size_t bytes = carve_workspace(c, N, nullptr, &c->ws);
+    WP_HIP(hipMalloc(&c->wbuf, bytes));
+    carve_workspace(c, N, (char*)c->wbuf, &c->ws);
     c->wcap = N;
     return WP_OK;
 }
@@ -256,6 +319,10 @@ struct CallArgs {
     const int* d_cand_idx;
     int* d_id;
     double *d_adj, *d_ls, *d_lt, *d_pair;
+    int tie_mode = 0;  // ClassifyArgs::tie_mode of the launches
+    int* tie_n = nullptr;
+    int* tie_idx = nullptr;
+    double tie_abs = 0.0, tie_rel = 0.0;
 };
 
 wp::ClassifyArgs base_args(const wp_hmm_classifier* c, const CallArgs& x) {
@@ -264,10 +331,10 @@ wp::ClassifyArgs base_args(const wp_hmm_classifier* c, const CallArgs& x) {
     a.N = x.N;
     a.S = c->S;
     a.rads = x.d_rads;
-    a.Lr = c->wL;
-    a.Hr = c->wH;
-    a.nbmax = c->wNb;
-    a.wide = c->wWide;
+    a.Lr = c->ws.L;
+    a.Hr = c->ws.H;
+    a.nbmax = c->ws.nb;
+    a.wide = c->ws.wide;
     a.wide_pass = 0;
     a.list = nullptr;
     a.list_count = nullptr;
@@ -289,18 +356,23 @@ wp::ClassifyArgs base_args(const wp_hmm_classifier* c, const CallArgs& x) {
     a.out_log_score = x.d_ls;
     a.out_log_total = x.d_lt;
     a.out_pair_logp = x.d_pair;
+    a.tie_mode = x.tie_mode;
+    a.tie_n = x.tie_n;
+    a.tie_idx = x.tie_idx;
+    a.tie_abs = x.tie_abs;
+    a.tie_rel = x.tie_rel;
     return a;
 }
 
 // Counters cleared, emission ranges and boxes, reads appended to their bins.
 void enqueue_prep(wp_hmm_classifier* c, const std::vector<Bin>& bins, const CallArgs& x, hipStream_t stream) {
-    wp::launch_zero_ints(c->wCtr, kCtrLen, stream);
-    wp::launch_read_prep(c->dm, x.N, x.d_rads, c->wL, c->wH, c->wNb, stream);
+    wp::launch_zero_ints(c->ws.ctr, kCtrLen, stream);
+    wp::launch_read_prep(c->dm, x.N, x.d_rads, c->ws.L, c->ws.H, c->ws.nb, stream);
     wp::BinEdges e;
     e.nbins = (int)bins.size();
     e.edge[0] = 0;
     for (size_t b = 0; b < bins.size(); b++) e.edge[b + 1] = bins[b].hi;
-    wp::launch_bin_reads(x.N, c->wNb, e, c->wLists, c->wCtr + kCtrCount, stream);
+    wp::launch_bin_reads(x.N, c->ws.nb, e, c->ws.lists, c->ws.ctr + kCtrCount, stream);
 }
 
 // The main launch of bin b (for a quad-rows bin, the given variant).
@@ -313,10 +385,10 @@ hipError_t launch_bin_main(wp_hmm_classifier* c, const std::vector<Bin>& bins, i
     a.NB = bin.hi;
     a.cap = c->T * bin.hi;
     a.ngroups = bin.ngroups;
-    a.list = c->wLists + (size_t)b * x.N;
-    a.list_count = c->wCtr + kCtrCount + b;
-    a.wide_list = c->wWideLists + (size_t)b * x.N;
-    a.wide_count = c->wCtr + kCtrWide + b;
+    a.list = c->ws.lists + (size_t)b * x.N;
+    a.list_count = c->ws.ctr + kCtrCount + b;
+    a.wide_list = c->ws.wide_lists + (size_t)b * x.N;
+    a.wide_count = c->ws.ctr + kCtrWide + b;
     a.variant = variant;
     if (bin.kernel == kRowPerLane) return wp::launch_classify_v3(a, c->precision, stream);
     if (bin.kernel == kQuadRows) return wp::launch_classify_v5(a, stream);
@@ -342,18 +414,54 @@ hipError_t launch_bin_wide(wp_hmm_classifier* c, const std::vector<Bin>& bins, i
     a.ngroups = bin.ngroups;
     a.wide_pass = 1;
     a.ngroups = bin.ngroups_wide;  // few reads: every group the LDS holds works on each
-    a.list = (stage == 2 ? c->wWide2Lists : c->wWideLists) + (size_t)b * x.N;
-    a.list_count = c->wCtr + (stage == 2 ? kCtrWide2 : kCtrWide) + b;
+    a.list = (stage == 2 ? c->ws.wide2_lists : c->ws.wide_lists) + (size_t)b * x.N;
+    a.list_count = c->ws.ctr + (stage == 2 ? kCtrWide2 : kCtrWide) + b;
     if (bin.kernel == kGather && stage == 1) {
         a.variant = 1;
-        a.wide_list = c->wWide2Lists + (size_t)b * x.N;
-        a.wide_count = c->wCtr + kCtrWide2 + b;
+        a.wide_list = c->ws.wide2_lists + (size_t)b * x.N;
+        a.wide_count = c->ws.ctr + kCtrWide2 + b;
         return wp::launch_classify(a, c->precision, c->G, stream);
     }
     if (wp::v3_supports(c->precision, c->T, bin.hi) &&
         wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, bin.hi) <= lds_hard_limit())
         return wp::launch_classify_v3(a, c->precision, stream);
     return wp::launch_classify(a, c->precision, c->G, stream);
+}
+
+// FP32 calls: the reads whose near-best list holds two or more candidates are
+// rescored over that list by the FP64 sibling's kernels (on the boxes this
+// call's prep computed), and their id and ln score are replaced by the FP64
+// decision.
+int enqueue_refine(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream) {
+    wp_hmm_classifier* s = c->r64;
+    s->ws = wp::Workspace();
+    s->ws.L = c->ws.L;
+    s->ws.H = c->ws.H;
+    s->ws.nb = c->ws.nb;
+    s->ws.wide = c->ws.rwide;
+    s->ws.lists = c->ws.rlists;
+    s->ws.wide_lists = c->ws.rwide_lists;
+    s->ws.wide2_lists = c->ws.rwide2_lists;
+    s->ws.ctr = c->ws.rctr;
+    const std::vector<Bin> bins = make_bins(s);
+    wp::launch_zero_ints(s->ws.ctr, kCtrLen, stream);
+    wp::BinEdges e;
+    e.nbins = (int)bins.size();
+    e.edge[0] = 0;
+    for (size_t b = 0; b < bins.size(); b++) e.edge[b + 1] = bins[b].hi;
+    wp::launch_bin_refine(x.N, c->ws.nb, c->ws.tie_n, e, s->ws.lists, s->ws.ctr + kCtrCount, stream);
+    CallArgs y{x.N, x.d_rads, x.d_cand_off, x.d_cand_idx, c->ws.rid, nullptr, c->ws.rls, nullptr, nullptr};
+    y.tie_mode = 2;
+    y.tie_n = c->ws.tie_n;
+    y.tie_idx = c->ws.tie_idx;
+    for (int b = 0; b < (int)bins.size(); b++) {
+        WP_HIP(launch_bin_main(s, bins, b, y, 0, stream));
+        if (bins[b].kernel != kRowPerLane) WP_HIP(launch_bin_wide(s, bins, b, y, 1, stream));
+        if (bins[b].kernel == kGather) WP_HIP(launch_bin_wide(s, bins, b, y, 2, stream));
+    }
+    wp::launch_refine_fix(x.N, c->ws.tie_n, c->ws.rid, c->ws.rls, x.d_lt, x.d_id, x.d_adj, x.d_ls, stream);
+    WP_HIP(hipGetLastError());
+    return WP_OK;
 }
 
 // The whole call: prep, binning, each bin's launch (and wide pass).
@@ -388,6 +496,10 @@ int enqueue_call(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream, bo
         WP_HIP(hipEventRecord(c->join[k], c->branch[k]));
         WP_HIP(hipStreamWaitEvent(stream, c->join[k], 0));
     }
+    if (x.tie_mode == 1) {
+        const int rc = enqueue_refine(c, x, stream);
+        if (rc) return rc;
+    }
     if (events) {
         WP_HIP(hipEventRecord(c->ev[2], stream));
         c->ev_recorded = 1;
@@ -399,6 +511,12 @@ int enqueue_call(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream, bo
 // up to 4 states, per-timestep classes above, as measured on the 10^5-read
 // benchmark) and leave the tuning to the first call with at least this many.
 constexpr int kTuneMinReads = 20000;
+
+// Default near-best window of the FP32 path, W(best) = abs + rel |best| in
+// ln P: twice the FP32 error bound eps(x) = 1e-5 + 5e-6 |x| (measured errors
+// on the 10^5 x 1,000 and 20,000 dye-seq workloads stay below a quarter of it).
+constexpr double kDefaultTieAbs = 2e-5;
+constexpr double kDefaultTieRel = 1e-5;
 
 // Picks each quad-rows bin's variant by timing both on this call's reads
 // (two runs each, the faster second run wins; both give identical outputs).
@@ -427,7 +545,7 @@ int autotune(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream) {
         for (int trial = 0; trial < 4 && !rc; trial++) {
             const int v = trial & 1;
             // a fresh wide list for each trial
-            wp::launch_zero_ints(c->wCtr + kCtrWide + b, 1, stream);
+            wp::launch_zero_ints(c->ws.ctr + kCtrWide + b, 1, stream);
             hipError_t e = hipEventRecord(e0, stream);
             if (e == hipSuccess) e = launch_bin_main(c, bins, b, x, v, stream);
             if (e == hipSuccess) e = hipEventRecord(e1, stream);
@@ -460,7 +578,15 @@ int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_c
     if (N == 0) return WP_OK;
     int rc = ensure_workspace(c, N);
     if (rc) return rc;
-    const CallArgs x{N, d_rads, d_cand_off, d_cand_idx, d_id, d_adj, d_ls, d_lt, d_pair};
+    CallArgs x{N, d_rads, d_cand_off, d_cand_idx, d_id, d_adj, d_ls, d_lt, d_pair};
+    if (c->r64 && c->refine) {
+        x.tie_mode = 1;
+        x.tie_n = c->ws.tie_n;
+        x.tie_idx = c->ws.tie_idx;
+        x.tie_abs = c->tie_abs;
+        x.tie_rel = c->tie_rel;
+        if (!x.d_lt) x.d_lt = c->ws.lt;  // the FP64 decision's adjusted score needs the normaliser
+    }
     if (!c->tuned) {
         if (c->variant_mode < 0 && N < kTuneMinReads) {
             // too few reads to time the variants reliably: the measured
@@ -476,7 +602,7 @@ int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_c
     }
     if (!graph_ok || !c->use_graph || c->timing) return enqueue_call(c, x, stream, c->timing != 0);
     std::vector<const void*> key = {(const void*)(intptr_t)N, d_rads, d_cand_off, d_cand_idx, d_id, d_adj,
-                                    d_ls, d_lt, d_pair};
+                                    d_ls, d_lt, d_pair, (const void*)(intptr_t)x.tie_mode};
     for (int b = 0; b < wp::kMaxBins; b++) key.push_back((const void*)(intptr_t)c->variant[b]);
     if (!c->gexec || key != c->gkey) {
         drop_graph(c);
@@ -635,6 +761,37 @@ int wp_hmm_classifier_create(unsigned int num_timesteps, unsigned int num_channe
         wp_hmm_classifier_destroy(c);
         return rc;
     }
+    if (precision == WP_PRECISION_FP32) {
+        // the FP64 rescoring pass of near-best candidates: the same tables,
+        // FP64 bins (left out if one of them does not fit in LDS)
+        wp_hmm_classifier* s = new wp_hmm_classifier();
+        s->T = T;
+        s->C = C;
+        s->S = S;
+        s->M = M;
+        s->precision = WP_PRECISION_FP64;
+        s->G = c->G;
+        s->tri = c->tri;
+        s->dm = c->dm;
+        s->maxbox = c->maxbox;
+        s->dD = c->dD;
+        s->dSeq = c->dSeq;
+        s->dTrack = c->dTrack;
+        s->dCount = c->dCount;
+        s->dId = c->dId;
+        s->dBinom = c->dBinom;
+        s->borrowed = 1;
+        bool fits = true;
+        for (const Bin& b : make_bins(s)) fits = fits && bin_lds_bytes(s, b) <= lds_hard_limit();
+        if (fits) {
+            c->r64 = s;
+            c->refine = 1;
+            c->tie_abs = kDefaultTieAbs;
+            c->tie_rel = kDefaultTieRel;
+        } else {
+            delete s;
+        }
+    }
     *out = c;
     return WP_OK;
 }
@@ -650,20 +807,16 @@ void wp_hmm_classifier_destroy(wp_hmm_classifier* c) {
         if (c->join[k]) hipEventDestroy(c->join[k]);
     }
     if (c->fork) hipEventDestroy(c->fork);
-    hipFree(c->wLists);
-    hipFree(c->wWideLists);
-    hipFree(c->wWide2Lists);
-    hipFree(c->wCtr);
-    hipFree(c->dD);
-    hipFree(c->dSeq);
-    hipFree(c->dTrack);
-    hipFree(c->dCount);
-    hipFree(c->dId);
-    hipFree(c->dBinom);
-    hipFree(c->wL);
-    hipFree(c->wH);
-    hipFree(c->wNb);
-    hipFree(c->wWide);
+    if (c->r64) wp_hmm_classifier_destroy(c->r64);
+    hipFree(c->wbuf);
+    if (!c->borrowed) {
+        hipFree(c->dD);
+        hipFree(c->dSeq);
+        hipFree(c->dTrack);
+        hipFree(c->dCount);
+        hipFree(c->dId);
+        hipFree(c->dBinom);
+    }
     delete c;
 }
 
@@ -842,6 +995,39 @@ int wp_hmm_set_graph(wp_hmm_classifier* c, int enable) {
     return WP_OK;
 }
 
+int wp_hmm_set_refine(wp_hmm_classifier* c, int enable, double abs, double rel) {
+    if (!c || !(abs >= 0.0) || !(rel >= 0.0)) {
+        wp::set_error("wp_hmm_set_refine: null classifier or negative window");
+        return WP_ERR_ARG;
+    }
+    if (!c->r64) {
+        wp::set_error("no FP64 decision for this classifier (FP64 itself, or FP64 boxes too large for LDS)");
+        return WP_ERR_UNSUPPORTED;
+    }
+    c->refine = enable ? 1 : 0;
+    c->tie_abs = abs;
+    c->tie_rel = rel;
+    drop_graph(c);
+    return WP_OK;
+}
+
+int wp_hmm_refine_stats(wp_hmm_classifier* c, int N, int* out2) {
+    if (!c || !out2 || N < 0 || N > c->wcap || !c->r64) {
+        wp::set_error("wp_hmm_refine_stats: null argument, no FP64 decision, or more reads than the last call");
+        return WP_ERR_ARG;
+    }
+    out2[0] = out2[1] = 0;
+    if (N == 0) return WP_OK;
+    std::vector<int> h(N);
+    WP_HIP(hipDeviceSynchronize());
+    WP_HIP(hipMemcpy(h.data(), c->ws.tie_n, sizeof(int) * N, hipMemcpyDeviceToHost));
+    for (int r = 0; r < N; r++) {
+        if (h[r] != 0) out2[0]++;
+        if (h[r] < 0) out2[1]++;
+    }
+    return WP_OK;
+}
+
 int wp_hmm_set_timing(wp_hmm_classifier* c, int enable) {
     if (!c) {
         wp::set_error("null classifier");
@@ -886,8 +1072,8 @@ int wp_hmm_work(wp_hmm_classifier* c, int N, const double* rads, double* out3) {
     if (e == hipSuccess) e = hipMemcpy(d_rads, rads, sizeof(double) * tc, hipMemcpyHostToDevice);
     std::vector<double> h(3 * (size_t)N);
     if (e == hipSuccess) {
-        wp::launch_read_prep(c->dm, N, d_rads, c->wL, c->wH, c->wNb, nullptr);
-        wp::launch_work(c->dm, N, c->S, c->wL, c->wH, c->wNb, c->dD, d_out, nullptr);
+        wp::launch_read_prep(c->dm, N, d_rads, c->ws.L, c->ws.H, c->ws.nb, nullptr);
+        wp::launch_work(c->dm, N, c->S, c->ws.L, c->ws.H, c->ws.nb, c->dD, d_out, nullptr);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(h.data(), d_out, sizeof(double) * 3 * N, hipMemcpyDeviceToHost);
@@ -930,6 +1116,7 @@ int wp_simulate_radiometries(const wp_model* seq_model, unsigned int num_timeste
     std::vector<short> seqs((size_t)num_dye_seqs * cap, -1);
     std::vector<int> lens(num_dye_seqs);
     for (int i = 0; i < num_dye_seqs; i++) lens[i] = wp::parse_dye_seq(dye_strings[i], &seqs[(size_t)i * cap], cap);
+    if (!wp::dye_channels_ok(seqs.data(), num_dye_seqs, cap, lens.data(), (int)num_channels)) return WP_ERR_ARG;
     wp::simulate_radiometries(*seq_model, (int)num_timesteps, (int)num_channels, num_dye_seqs, seqs.data(), cap,
                               lens.data(), weights, num_reads, seed, out_radiometries, out_dye_seq_index,
                               nullptr);
@@ -953,6 +1140,7 @@ int wp_simulate_radiometries_gpu(const wp_model* seq_model, unsigned int num_tim
     std::vector<short> seqs((size_t)num_dye_seqs * cap, -1);
     std::vector<int> lens(num_dye_seqs);
     for (int i = 0; i < num_dye_seqs; i++) lens[i] = wp::parse_dye_seq(dye_strings[i], &seqs[(size_t)i * cap], cap);
+    if (!wp::dye_channels_ok(seqs.data(), num_dye_seqs, cap, lens.data(), (int)num_channels)) return WP_ERR_ARG;
     wp::DevModel dm;
     wp::fill_dev_model(*seq_model, (int)num_timesteps, (int)num_channels, 1, DBL_MAX, &dm);
     return wp::gpu_simulate_radiometries(*seq_model, dm, (int)num_timesteps, (int)num_channels, num_dye_seqs,
@@ -980,6 +1168,7 @@ int wp_simulate_dye_tracks_gpu(const wp_model* seq_model, unsigned int num_times
     std::vector<short> seqs((size_t)num_dye_seqs * cap, -1);
     std::vector<int> lens(num_dye_seqs);
     for (int i = 0; i < num_dye_seqs; i++) lens[i] = wp::parse_dye_seq(dye_strings[i], &seqs[(size_t)i * cap], cap);
+    if (!wp::dye_channels_ok(seqs.data(), num_dye_seqs, cap, lens.data(), (int)num_channels)) return WP_ERR_ARG;
     return wp::gpu_simulate_dye_tracks(*seq_model, (int)num_timesteps, (int)num_channels, num_dye_seqs, seqs.data(),
                                        cap, lens.data(), dye_seq_counts, dye_seq_ids, per_peptide, seed, max_tracks,
                                        max_sources, out_num_tracks, out_num_sources, out_track_counts,
@@ -1005,6 +1194,7 @@ int wp_simulate_dye_tracks(const wp_model* seq_model, unsigned int num_timesteps
     std::vector<short> seqs((size_t)num_dye_seqs * cap, -1);
     std::vector<int> lens(num_dye_seqs);
     for (int i = 0; i < num_dye_seqs; i++) lens[i] = wp::parse_dye_seq(dye_strings[i], &seqs[(size_t)i * cap], cap);
+    if (!wp::dye_channels_ok(seqs.data(), num_dye_seqs, cap, lens.data(), (int)num_channels)) return WP_ERR_ARG;
     std::vector<short> tc;
     std::vector<int> off, id, cnt, hits;
     int n = wp::simulate_dye_tracks(*seq_model, (int)num_timesteps, (int)num_channels, num_dye_seqs, seqs.data(),

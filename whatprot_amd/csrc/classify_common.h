@@ -85,6 +85,90 @@ struct CandAcc {
     }
 };
 
+// Near-best candidates of the FP32 path. FP32 ln P carries an error of at most
+// eps(x) = a + b |x| against FP64 (measured, with margin), so the FP64 best of
+// a read lies within W = 2 eps of the FP32 best. Every pair group keeps the
+// candidates it scored within W of its own running best (the group's best only
+// rises, and the window's floor best - W rises with it, so what a group drops
+// is out of the final window too); at the end of the read the groups' lists
+// are merged against the read's best. A read with two or more candidates in
+// the window gets them, in candidate order, for an FP64 pass that applies the
+// reference's strict '>' with the earlier candidate winning
+// (hmm-classifier.h:59-65); a read whose window holds more than kTieMax, or
+// whose group list overflowed inside it, is rescored over all its candidates.
+constexpr int kTieSlots = 4;   // entries per pair group (kTieMax per read: hmm_kernels.h)
+// A read whose FP32 best lies below this ln P is rescored over every
+// candidate: there the FP64 recursion runs into the subnormal range (ln DBL_MIN
+// = -708, and intermediate states can sit e^33 below the final P when later
+// densities exceed 1), so the FP64 decision is no longer an ordering of the
+// FP32 values (an FP64 P of 0 everywhere picks the first candidate).
+constexpr double kTieFloorAll = -650.0;
+
+struct TieSlot {
+    double v[kTieSlots];
+    int kk[kTieSlots];
+    int n;
+    int pad;
+    double drop;  // largest value evicted from a full list (-inf if none)
+};
+
+__device__ inline double tie_floor(double best, double wa, double wr) { return best - (wa + wr * fabs(best)); }
+
+__device__ inline void tie_init(TieSlot* ts) {
+    ts->n = 0;
+    ts->drop = -INFINITY;
+}
+
+// One lane per group, after the group's accumulator took (logp, kk); best is
+// the group's best so far, logp included.
+__device__ inline void tie_note(TieSlot* ts, double best, double logp, int kk, double wa, double wr) {
+    if (!(logp > -INFINITY)) return;
+    const double lim = tie_floor(best, wa, wr);
+    if (logp < lim) return;
+    int n = 0;
+    const int n0 = ts->n;
+    for (int j = 0; j < n0; j++) {
+        const double v = ts->v[j];
+        if (v >= lim) {
+            ts->v[n] = v;
+            ts->kk[n] = ts->kk[j];
+            n++;
+        }
+    }
+    if (n < kTieSlots) {
+        ts->v[n] = logp;
+        ts->kk[n] = kk;
+        n++;
+    } else {
+        int jm = 0;
+        for (int j = 1; j < kTieSlots; j++)
+            if (ts->v[j] < ts->v[jm]) jm = j;
+        if (logp > ts->v[jm]) {
+            ts->drop = fmax(ts->drop, ts->v[jm]);
+            ts->v[jm] = logp;
+            ts->kk[jm] = kk;
+        } else {
+            ts->drop = fmax(ts->drop, logp);
+        }
+    }
+    ts->n = n;
+}
+
+// The candidate list of read r: its own list (or every dye-seq), or in the
+// FP64 rescoring pass (tie_mode 2) the near-best list the FP32 pass left
+// (tie_n > 0; -1 keeps the whole list).
+__device__ inline void read_candidates(const ClassifyArgs& a, int r, int* ncand, const int** cidx) {
+    *ncand = a.cand_off ? a.cand_off[r + 1] - a.cand_off[r] : a.S;
+    *cidx = a.cand_off ? a.cand_idx + a.cand_off[r] : nullptr;
+    if (a.tie_mode == 2) {
+        const int tn = a.tie_n[r];
+        if (tn > 0) {
+            *ncand = tn;
+            *cidx = a.tie_idx + (size_t)r * kTieMax;
+        }
+    }
+}
+
 // Compacts, in candidate order, the non-empty candidates among
 // [base, base + 4 * blockDim.x) of read r into list[]; returns how many. A
 // pair is non-empty iff the read is and D_c >= L[0][c] for every channel.
@@ -140,10 +224,13 @@ __device__ int compact_chunk(const ClassifyArgs& a, int r, int ncand, const int*
 }
 
 // Merges the groups' accumulators (gres: 4 doubles per group) and writes the
-// read's outputs, with the NaN guard of hmm-classifier.h:73-76. Block-wide.
+// read's outputs, with the NaN guard of hmm-classifier.h:73-76. In the FP32
+// collecting pass (tie_mode 1) it also writes the read's near-best list from
+// the groups' tie slots (tmp: kTieMax ints of free LDS). Block-wide.
 template <bool kRescale>
 __device__ void merge_and_store(const ClassifyArgs& a, int r, int ncand, const int* cidx, int ngroups,
-                                int group, int lane, const CandAcc<kRescale>& acc, double* gres) {
+                                int group, int lane, const CandAcc<kRescale>& acc, double* gres,
+                                const TieSlot* ties = nullptr, int* tmp = nullptr) {
     if (group < ngroups && lane == 0) {
         gres[group * 4 + 0] = acc.best_v;
         gres[group * 4 + 1] = (double)acc.best_k;
@@ -188,6 +275,41 @@ __device__ void merge_and_store(const ClassifyArgs& a, int r, int ncand, const i
         adjusted = 0.0;
         log_score = -INFINITY;
         log_total = 0.0;
+    }
+    if (kRescale && a.tie_mode == 1 && ties) {
+        int tn = 0;
+        if (bv > -INFINITY) {
+            const double lim = tie_floor(bv, a.tie_abs, a.tie_rel);
+            int cnt = 0;
+            bool over = false;
+            for (int g = 0; g < ngroups; g++) {
+                const TieSlot& s = ties[g];
+                if (s.drop >= lim) over = true;
+                for (int j = 0; j < s.n; j++) {
+                    if (s.v[j] >= lim) {
+                        if (cnt < kTieMax) tmp[cnt] = s.kk[j];
+                        cnt++;
+                    }
+                }
+            }
+            if (over || cnt > kTieMax || bv < kTieFloorAll) {
+                tn = -1;
+            } else if (cnt >= 2) {
+                for (int j = 1; j < cnt; j++) {  // candidate order
+                    const int v = tmp[j];
+                    int q = j - 1;
+                    while (q >= 0 && tmp[q] > v) {
+                        tmp[q + 1] = tmp[q];
+                        q--;
+                    }
+                    tmp[q + 1] = v;
+                }
+                int* out = a.tie_idx + (size_t)r * kTieMax;
+                for (int j = 0; j < cnt; j++) out[j] = cidx ? cidx[tmp[j]] : tmp[j];
+                tn = cnt;
+            }
+        }
+        a.tie_n[r] = tn;
     }
     int id = -1;
     if (ncand > 0) {

@@ -8,6 +8,9 @@
 
 namespace wp {
 
+// Near-best candidates per read handed from the FP32 pass to the FP64 pass.
+constexpr int kTieMax = 32;
+
 // Arguments of the fused classify kernel. Device pointers unless noted.
 struct ClassifyArgs {
     DevModel m;
@@ -44,6 +47,14 @@ struct ClassifyArgs {
     double* out_log_score;
     double* out_log_total;
     double* out_pair_logp;    // [N][S] or null
+    // FP32 near-best lists: tie_mode 1 (FP32 pass) writes tie_n[r] (0: the
+    // FP32 best stands; k >= 2: tie_idx[r][0..k) holds the dye-seqs within
+    // the window of the best, in candidate order; -1: rescore every
+    // candidate); tie_mode 2 (FP64 pass) scores those lists.
+    int tie_mode;
+    int* tie_n;               // [N]
+    int* tie_idx;             // [N][kTieMax]
+    double tie_abs, tie_rel;  // window W(best) = tie_abs + tie_rel |best| in ln P
 };
 
 // Largest number of read bins (box-size classes) of one classify call.
@@ -62,6 +73,16 @@ void launch_zero_ints(int* d, int n, hipStream_t stream);
 // Appends every read to its bin's list: lists[b * N + k], counts[b].
 void launch_bin_reads(int N, const int* d_nbmax, const BinEdges& e, int* d_lists, int* d_counts,
                       hipStream_t stream);
+
+// Appends every read with a near-best list (tie_n != 0) to its bin's list.
+void launch_bin_refine(int N, const int* d_nbmax, const int* d_tie_n, const BinEdges& e, int* d_lists,
+                       int* d_counts, hipStream_t stream);
+
+// Writes the FP64 decision of every read with a near-best list over the FP32
+// outputs: id and ln score from the FP64 pass, adjusted = exp(ln score - ln
+// total) with the FP32 normaliser (NaN guard as hmm-classifier.h:73-76).
+void launch_refine_fix(int N, const int* d_tie_n, const int* d_rid, const double* d_rls, const double* d_lt,
+                       int* d_id, double* d_adj, double* d_ls, hipStream_t stream);
 
 // Per-timestep emission ranges then pruned boxes for every read.
 void launch_read_prep(const DevModel& m, int N, const double* d_rads, uint8_t* d_Lr, uint8_t* d_Hr,

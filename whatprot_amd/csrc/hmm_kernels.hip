@@ -82,7 +82,7 @@ __host__ __device__ inline int group_stride(int cap) {
 }
 
 struct Layout {
-    size_t rad, L, H, eoff, estr, etab, eexp, binom, list, scan, gres, gseq, state, total;
+    size_t rad, L, H, eoff, estr, etab, eexp, binom, list, scan, gres, ties, gseq, state, total;
 };
 
 template <typename Real>
@@ -101,6 +101,7 @@ __host__ __device__ inline Layout make_layout(int T, int C, int tri, int NB, int
     l.list = o;  o = align16(o + sizeof(int) * nthreads * 4);
     l.scan = o;  o = align16(o + sizeof(int) * (nthreads / 64 + 2));
     l.gres = o;  o = align16(o + sizeof(double) * 4 * ngroups);
+    l.ties = o;  o = align16(o + (sizeof(Real) == 4 ? sizeof(TieSlot) * ngroups : 0));
     l.gseq = o;  o = align16(o + sizeof(int) * 3 * T * ngroups);
     l.state = o; o = align16(o + sizeof(Real) * (size_t)group_stride<Real>(cap) * ngroups);
     l.total = o;
@@ -842,11 +843,13 @@ __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, 
     int* list = (int*)(smem + lay.list);
     int* scan = (int*)(smem + lay.scan);
     double* gres = (double*)(smem + lay.gres);
+    TieSlot* ties = (TieSlot*)(smem + lay.ties);
     int* gseq_all = (int*)(smem + lay.gseq);  // per group: seq[T], K[T], 1/K[T]
     Real* state = (Real*)(smem + lay.state);
 
-    const int ncand = a.cand_off ? a.cand_off[r + 1] - a.cand_off[r] : a.S;
-    const int* cidx = a.cand_off ? a.cand_idx + a.cand_off[r] : nullptr;
+    int ncand;
+    const int* cidx;
+    read_candidates(a, r, &ncand, &cidx);
 
     // ---- prologue: read data, bounds, emission table, binomial tables ----
     for (int k = tid; k < T * C; k += nthreads) {
@@ -901,6 +904,8 @@ __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, 
     // The pair is non-empty iff D_c >= L[0][c] for every channel.
 
     CandAcc<kRescale> acc;
+    const bool collect = kRescale && a.tie_mode == 1;
+    if (collect && group < ngroups && lane == 0) tie_init(&ties[group]);
     const int cap = a.cap;
     if constexpr (kSep) {
         // a read one of whose candidates could outgrow the state buffers even
@@ -948,11 +953,12 @@ __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, 
                                                           bin, gseq, gK);
                 if (a.out_pair_logp && !cidx && lane == 0) a.out_pair_logp[(size_t)r * a.S + i] = pr.logp;
                 acc.add(pr.logp, pr.p, kk, a.dsCount[i]);
+                if (collect && lane == 0) tie_note(&ties[group], acc.best_v, pr.logp, kk, a.tie_abs, a.tie_rel);
             }
         }
         __syncthreads();
     }
-    merge_and_store<kRescale>(a, r, ncand, cidx, ngroups, group, lane, acc, gres);
+    merge_and_store<kRescale>(a, r, ncand, cidx, ngroups, group, lane, acc, gres, ties, list);
 }
 
 // One workgroup per read of the bin's compacted list.
@@ -1059,7 +1065,50 @@ __global__ void bin_reads_kernel(int N, const int* __restrict__ nbmax, BinEdges 
     lists[(size_t)b * N + atomicAdd(&counts[b], 1)] = r;
 }
 
+__global__ void bin_refine_kernel(int N, const int* __restrict__ nbmax, const int* __restrict__ tie_n, BinEdges e,
+                                  int* __restrict__ lists, int* __restrict__ counts) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N || tie_n[r] == 0) return;
+    const int nb = nbmax[r];
+    int b = -1;
+    for (int k = 0; k < e.nbins; k++)
+        if (nb > e.edge[k] && nb <= e.edge[k + 1]) b = k;
+    if (b < 0) return;
+    lists[(size_t)b * N + atomicAdd(&counts[b], 1)] = r;
+}
+
+__global__ void refine_fix_kernel(int N, const int* __restrict__ tie_n, const int* __restrict__ rid,
+                                  const double* __restrict__ rls, const double* __restrict__ lt, int* __restrict__ id,
+                                  double* __restrict__ adj, double* __restrict__ ls) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N || tie_n[r] == 0) return;
+    double s = rls[r];
+    double t = lt[r];
+    double v = exp(s - t);
+    if (isnan(v)) {
+        v = 0.0;
+        s = -INFINITY;
+    }
+    id[r] = rid[r];
+    if (adj) adj[r] = v;
+    if (ls) ls[r] = s;
+}
+
 }  // namespace
+
+void launch_bin_refine(int N, const int* d_nbmax, const int* d_tie_n, const BinEdges& e, int* d_lists,
+                       int* d_counts, hipStream_t stream) {
+    if (N > 0)
+        hipLaunchKernelGGL(bin_refine_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, N, d_nbmax, d_tie_n, e,
+                           d_lists, d_counts);
+}
+
+void launch_refine_fix(int N, const int* d_tie_n, const int* d_rid, const double* d_rls, const double* d_lt,
+                       int* d_id, double* d_adj, double* d_ls, hipStream_t stream) {
+    if (N > 0)
+        hipLaunchKernelGGL(refine_fix_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, N, d_tie_n, d_rid, d_rls,
+                           d_lt, d_id, d_adj, d_ls);
+}
 
 void launch_zero_ints(int* d, int n, hipStream_t stream) {
     if (n > 0) hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, d, n);

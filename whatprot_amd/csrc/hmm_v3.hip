@@ -37,7 +37,7 @@ using namespace cc;
 
 template <typename Real>
 struct V3Layout {
-    size_t rad, L, H, nbt, bstr, coord, etab, eexp, W, binom, list, scan, gres, total;
+    size_t rad, L, H, nbt, bstr, coord, etab, eexp, W, binom, list, scan, gres, ties, total;
 };
 
 // W block per timestep: (1 + C) matrices of NB x NB: Wt then W1t[c].
@@ -59,6 +59,7 @@ __host__ __device__ inline V3Layout<Real> v3_layout(int T, int C, int tri, int N
     l.list = o;  o = align16(o + sizeof(int) * nthreads * 4);
     l.scan = o;  o = align16(o + sizeof(int) * (nthreads / 64 + 2));
     l.gres = o;  o = align16(o + sizeof(double) * 4 * ngroups);
+    l.ties = o;  o = align16(o + (sizeof(Real) == 4 ? sizeof(TieSlot) * ngroups : 0));
     l.total = o;
     return l;
 }
@@ -116,8 +117,10 @@ __device__ __forceinline__ void v3_read(const ClassifyArgs& a, const int r, unsi
     int* list = (int*)(smem + lay.list);
     int* scan = (int*)(smem + lay.scan);
     double* gres = (double*)(smem + lay.gres);
-    const int ncand = a.cand_off ? a.cand_off[r + 1] - a.cand_off[r] : a.S;
-    const int* cidx = a.cand_off ? a.cand_idx + a.cand_off[r] : nullptr;
+    TieSlot* ties = (TieSlot*)(smem + lay.ties);
+    int ncand;
+    const int* cidx;
+    read_candidates(a, r, &ncand, &cidx);
     const bool read_empty = nbr == 0;
 
     // ---- prologue: read-level boxes, coordinates, emissions, bleach matrices
@@ -201,6 +204,8 @@ __device__ __forceinline__ void v3_read(const ClassifyArgs& a, const int r, unsi
     __syncthreads();
 
     CandAcc<kRescale> acc;
+    const bool collect = kRescale && a.tie_mode == 1;
+    if (collect && lane == 0) tie_init(&ties[group]);
     const Real pf = (Real)a.m.pf;
     const Real one_m_pf = (Real)(1.0 - a.m.pf);
     const double* dud = bin;
@@ -407,10 +412,11 @@ __device__ __forceinline__ void v3_read(const ClassifyArgs& a, const int r, unsi
             }
             if (a.out_pair_logp && !cidx && lane == 0) a.out_pair_logp[(size_t)r * a.S + i] = logp;
             acc.add(logp, p, kk, cntw);
+            if (collect && lane == 0) tie_note(&ties[group], acc.best_v, logp, kk, a.tie_abs, a.tie_rel);
         }
         __syncthreads();
     }
-    merge_and_store<kRescale>(a, r, ncand, cidx, ngroups, group, lane, acc, gres);
+    merge_and_store<kRescale>(a, r, ncand, cidx, ngroups, group, lane, acc, gres, ties, list);
 }
 
 // One workgroup per read of the bin's compacted list.

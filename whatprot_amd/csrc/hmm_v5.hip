@@ -40,7 +40,7 @@ constexpr int kV5Lanes = 4;  // lanes per candidate: one DPP quad
 constexpr int kV5MaxSpan = 90;  // largest binary-exponent spread of one box's densities
 
 struct V5Layout {
-    size_t rad, L, H, nbt, kcl, woff, bstr, coord, kcf, ev, etab, xt, W, binom, list, scan, gres, total;
+    size_t rad, L, H, nbt, kcl, woff, bstr, coord, kcf, ev, etab, xt, W, binom, list, scan, gres, ties, total;
 };
 
 // W block per timestep: (1 + C) matrices (Wt, then W1[c]) of K_t x K_t with a
@@ -68,6 +68,7 @@ __host__ __device__ inline V5Layout v5_layout(int T, int C, int tri, int NB) {
     l.list = o;  o = align16(o + sizeof(int) * kV5Threads * 4);
     l.scan = o;  o = align16(o + sizeof(int) * (kV5Threads / 64 + 2));
     l.gres = o;  o = align16(o + sizeof(double) * 4 * ngroups);
+    l.ties = o;  o = align16(o + sizeof(TieSlot) * ngroups);
     l.total = o;
     return l;
 }
@@ -215,8 +216,10 @@ __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsi
     int* list = (int*)(smem + lay.list);
     int* scan = (int*)(smem + lay.scan);
     double* gres = (double*)(smem + lay.gres);
-    const int ncand = a.cand_off ? a.cand_off[r + 1] - a.cand_off[r] : a.S;
-    const int* cidx = a.cand_off ? a.cand_idx + a.cand_off[r] : nullptr;
+    TieSlot* ties = (TieSlot*)(smem + lay.ties);
+    int ncand;
+    const int* cidx;
+    read_candidates(a, r, &ncand, &cidx);
     const bool read_empty = nbr == 0;
 
     // ---- prologue: read-level boxes, coordinates, emission densities
@@ -345,6 +348,8 @@ __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsi
     __syncthreads();
 
     CandAcc<true> acc;
+    const bool collect = a.tie_mode == 1;
+    if (collect && p == 0) tie_init(&ties[group]);
     const float pf = (float)a.m.pf;
     const float opf = (float)(1.0 - a.m.pf);
     const float pcb = (float)a.m.p_cb;
@@ -471,10 +476,11 @@ __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsi
             const double logp = s > 0.0f ? log((double)s) + (double)K * 0.69314718055994530942 : -INFINITY;
             if (a.out_pair_logp && !cidx && p == 0) a.out_pair_logp[(size_t)r * a.S + i] = logp;
             acc.add(logp, exp(logp), kk, a.dsCount[i]);
+            if (collect && p == 0) tie_note(&ties[group], acc.best_v, logp, kk, a.tie_abs, a.tie_rel);
         }
         __syncthreads();
     }
-    merge_and_store<true>(a, r, ncand, cidx, NGR, group, p, acc, gres);
+    merge_and_store<true>(a, r, ncand, cidx, NGR, group, p, acc, gres, ties, list);
 }
 
 // One workgroup per read of the bin's compacted list: a persistent loop over

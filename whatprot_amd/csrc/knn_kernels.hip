@@ -121,6 +121,7 @@ __global__ void __launch_bounds__(64 * kWaves) knn_kernel(const KnnArgs a) {
     if (lane == 0) {
         L.st[0] = 0;  // heap size
         L.st[1] = 0;  // hits
+        L.st[2] = 0;  // heap overflow
         L.kth[0] = DBL_MAX;
     }
     wave_sync();
@@ -192,6 +193,10 @@ __global__ void __launch_bounds__(64 * kWaves) knn_kernel(const KnnArgs a) {
                     cand &= cand - 1;
                     double dist = L.cd[jj];
                     if (dist >= kth) continue;
+                    if (hn > k) {  // no room for a push: only points with no hits get here
+                        L.st[2] = 1;
+                        break;
+                    }
                     int pos = nd.begin + c0 + jj;
                     hits += a.hits[pos];
                     heap_push(L.hd, L.hp, hn, dist, pos);
@@ -226,7 +231,8 @@ __global__ void __launch_bounds__(64 * kWaves) knn_kernel(const KnnArgs a) {
         int hn = L.st[0];
         int* oi = a.out_index + (size_t)qi * k;
         double* od = a.out_dist_sq + (size_t)qi * k;
-        a.out_count[qi] = hn;
+        a.out_count[qi] = L.st[2] ? -1 : hn;
+        if (hn > k) hn = k;  // (overflow only) the k nearest still fit the output row
         int j = 0;
         while (hn > 0) {
             od[j] = L.hd[0];

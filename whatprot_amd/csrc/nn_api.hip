@@ -119,6 +119,12 @@ int search(wp_nn_classifier* c, int N, const double* rads, std::vector<int>* cnt
     WP_HIP(hipMemcpy(cnt->data(), c->d_cnt, sizeof(int) * N, hipMemcpyDeviceToHost));
     WP_HIP(hipMemcpy(idx->data(), c->d_idx, sizeof(int) * (size_t)N * c->k, hipMemcpyDeviceToHost));
     WP_HIP(hipMemcpy(dist->data(), c->d_dist, sizeof(double) * (size_t)N * c->k, hipMemcpyDeviceToHost));
+    for (int r = 0; r < N; r++) {
+        if ((*cnt)[r] < 0) {
+            wp::set_error("kNN search: k-best heap full (a point with no hits)");
+            return WP_ERR_INTERNAL;
+        }
+    }
     return WP_OK;
 }
 
@@ -238,6 +244,21 @@ int wp_nn_classifier_create(unsigned int num_timesteps, unsigned int num_channel
         wp::set_error("unsupported shape: need 1 <= T <= 64, 1 <= C <= 8, model channels == C, k >= 1");
         return WP_ERR_UNSUPPORTED;
     }
+    // KBest::insert (k-best.h:26-43) pops only while the remaining hits cover
+    // k, so with a zero-hit point the heap could grow past k + 1 entries; the
+    // search kernel's LDS heap is sized for k + 1, so such input is refused.
+    for (int i = 0; i < num_dye_tracks; i++) {
+        if (source_offsets[i + 1] <= source_offsets[i]) {
+            wp::set_error("every dye-track needs at least one source");
+            return WP_ERR_ARG;
+        }
+        for (int j = source_offsets[i]; j < source_offsets[i + 1]; j++) {
+            if (source_hits[j] < 1) {
+                wp::set_error("every dye-track source needs hits >= 1");
+                return WP_ERR_ARG;
+            }
+        }
+    }
     int rc = wp::require_device();
     if (rc) return rc;
     wp_nn_classifier* c = new wp_nn_classifier();
@@ -350,7 +371,7 @@ int wp_nn_classify(wp_nn_classifier* c, int num_reads, const double* radiometrie
 
 int wp_nn_classify_top(wp_nn_classifier* c, int num_reads, const double* radiometries, int h, int* out_count,
                        int* out_ids, double* out_scores, double* out_total) {
-    if (!c || !valid_rads(num_reads, radiometries) || h < 0 || !out_count || !out_ids || !out_scores ||
+    if (!c || !valid_rads(num_reads, radiometries) || h < 1 || !out_count || !out_ids || !out_scores ||
         !out_total) {
         wp::set_error("null argument");
         return WP_ERR_ARG;
@@ -379,7 +400,7 @@ int wp_hybrid_classifier_create(unsigned int num_timesteps, unsigned int num_cha
                                 const int* source_counts, const int* source_hits, int h, int num_dye_seqs,
                                 const char* const* dye_strings, const int* dye_seq_counts,
                                 const int* dye_seq_ids, int precision, wp_hybrid_classifier** out) {
-    if (!out || h < 0) {
+    if (!out || h < 1) {
         wp::set_error("bad argument");
         return WP_ERR_ARG;
     }

@@ -42,6 +42,9 @@ inline double sigma(const wp_model& m, int channel, double amu) {
 // trailing '.'; seq[i] = channel digit or -1.
 int parse_dye_seq(const char* s, short* seq, int cap);
 
+// True iff every parsed dye names a channel < C (else sets the error text).
+bool dye_channels_ok(const short* seqs, int n, int cap, const int* lens, int C);
+
 void generate_dye_seqs(int C, int n, int len_min, int len_max, int k_min, int k_max,
                        uint64_t seed, char* out, int stride);
 void simulate_radiometries(const wp_model& m, int T, int C, int nds, const short* seqs, int cap,
