@@ -169,6 +169,29 @@ int wp_hmm_classify_device(wp_hmm_classifier* clf,
                            double* d_out_log_total,
                            void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments and outputs as wp_hmm_classify_device, and
+ * the same kernels, but the call's workspace (read boxes, bin lists, counters,
+ * near-best lists) lives in a 64 MiB device array of the library that each
+ * GPU holds until the process ends, so no hipMalloc, hipFree, stream, event
+ * or graph is created. Launches go eagerly onto `stream` in order (no
+ * parallel branches, no graph replay; the quad-rows kernel variants are the
+ * defaults, or the autotuned ones once another entry point tuned them). A
+ * call whose workspace exceeds 64 MiB (about 160,000 reads at C = 3, T = 12
+ * in FP32) returns WP_ERR_ARG before launching anything. Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (another stream,
+ * another classifier, another thread) race and must be serialised by the
+ * caller. */
+int wp_hmm_classify_device_noalloc(wp_hmm_classifier* clf,
+                                   int num_reads,
+                                   const double* d_radiometries,
+                                   int* d_out_id,
+                                   double* d_out_adjusted,
+                                   double* d_out_log_score,
+                                   double* d_out_log_total,
+                                   void* stream);
+
 /* Device-resident form of wp_hmm_classify_candidates: cand_offsets (N + 1)
  * and cand_indices in device memory; an index outside [0, num_dye_seqs)
  * scores as an empty pair (P = 0, id -1 if it is the pick). */

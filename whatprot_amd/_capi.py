@@ -102,6 +102,9 @@ def lib():
     L.wp_hmm_classify_device.restype = c_int
     L.wp_hmm_classify_device.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                          c_void_p, c_void_p, c_void_p]
+    L.wp_hmm_classify_device_noalloc.restype = c_int
+    L.wp_hmm_classify_device_noalloc.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                                 c_void_p, c_void_p, c_void_p]
     L.wp_hmm_pair_log_probability.restype = c_int
     L.wp_hmm_pair_log_probability.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
     L.wp_hmm_set_timing.restype = c_int

@@ -7,6 +7,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -121,6 +122,25 @@ struct Workspace {
     int* rctr = nullptr;
 };
 
+enum BinKernel { kGather = 0, kRowPerLane = 3, kQuadRows = 5 };
+
+struct Bin {
+    int lo, hi, ngroups;
+    int kernel;  // BinKernel: quad-rows (FP32, boxes <= 16), row-per-lane (FP64, <= 8) or gather
+    int cap_sep, ngroups_sep;  // gather bins: state capacity and groups of the separable launch
+    int ngroups_wide;          // groups of the wide passes (few reads: as many groups as LDS allows)
+};
+
+// The bins of a classifier, fixed at creation (no heap use per call).
+struct BinSet {
+    Bin b[kMaxBins];
+    int n = 0;
+    int size() const { return n; }
+    const Bin& operator[](int i) const { return b[i]; }
+    const Bin* begin() const { return b; }
+    const Bin* end() const { return b + n; }
+};
+
 }  // namespace wp
 
 struct wp_hmm_classifier {
@@ -141,6 +161,10 @@ struct wp_hmm_classifier {
     void* wbuf = nullptr;
     int wcap = 0;
     int maxbox = 1;  // prod_c (M+1), bound on any read's box
+    wp::BinSet bins;  // read bins by box bound, with their kernels and LDS sizing
+    // near-best list lengths of the last call (its workspace) and its reads
+    const int* last_tie_n = nullptr;
+    int last_n = 0;
     // FP32: near-best candidates decided in FP64 by the sibling's kernels
     wp_hmm_classifier* r64 = nullptr;
     int refine = 0;
@@ -171,14 +195,11 @@ struct wp_hmm_classifier {
 
 namespace {
 
-enum BinKernel { kGather = 0, kRowPerLane = 3, kQuadRows = 5 };
-
-struct Bin {
-    int lo, hi, ngroups;
-    int kernel;  // BinKernel: quad-rows (FP32, boxes <= 16), row-per-lane (FP64, <= 8) or gather
-    int cap_sep, ngroups_sep;  // gather bins: state capacity and groups of the separable launch
-    int ngroups_wide;          // groups of the wide passes (few reads: as many groups as LDS allows)
-};
+using wp::Bin;
+using wp::BinSet;
+using wp::kGather;
+using wp::kQuadRows;
+using wp::kRowPerLane;
 
 // The separable gather launch sizes its state buffers at 1/kSepCapDiv of the
 // T x NB entries the bin could need, so more pairs fit in LDS; a read that
@@ -205,9 +226,9 @@ size_t lds_hard_limit() {
 }
 
 // Bins of reads by box bound, each with its LDS sizing.
-std::vector<Bin> make_bins(const wp_hmm_classifier* c) {
-    std::vector<int> edges = {4, 8, 16, 32, 64, 128, c->maxbox};
-    std::vector<Bin> bins;
+BinSet make_bins(const wp_hmm_classifier* c) {
+    const int edges[] = {4, 8, 16, 32, 64, 128, c->maxbox};
+    BinSet bins;
     int lo = 0;
     for (int hi : edges) {
         if (hi <= lo) continue;
@@ -232,7 +253,7 @@ std::vector<Bin> make_bins(const wp_hmm_classifier* c) {
         else if (wp::v3_supports(c->precision, c->T, hi) &&
                  wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, hi) <= lds_hard_limit())
             kernel = kRowPerLane;
-        bins.push_back({lo, hi, ng, kernel, cap_sep, ng_sep, ng_wide});
+        bins.b[bins.n++] = Bin{lo, hi, ng, kernel, cap_sep, ng_sep, ng_wide};
         lo = hi;
         if (lo >= c->maxbox) break;
     }
@@ -273,7 +294,7 @@ size_t carve_workspace(const wp_hmm_classifier* c, int N, char* base, wp::Worksp
     };
     const size_t n = (size_t)std::max(N, 1);
     const size_t tc = n * c->T * c->C;
-    const size_t nbins = make_bins(c).size();
+    const size_t nbins = (size_t)c->bins.size();
     w->L = (uint8_t*)take(tc);
     w->H = (uint8_t*)take(tc);
     w->nb = (int*)take(sizeof(int) * n);
@@ -283,7 +304,7 @@ size_t carve_workspace(const wp_hmm_classifier* c, int N, char* base, wp::Worksp
     w->wide2_lists = (int*)take(sizeof(int) * nbins * n);
     w->ctr = (int*)take(sizeof(int) * kCtrLen);
     if (c->r64) {
-        const size_t nb64 = make_bins(c->r64).size();
+        const size_t nb64 = (size_t)c->r64->bins.size();
         w->tie_n = (int*)take(sizeof(int) * n);
         w->tie_idx = (int*)take(sizeof(int) * n * wp::kTieMax);
         w->rid = (int*)take(sizeof(int) * n);
@@ -305,6 +326,8 @@ int ensure_workspace(wp_hmm_classifier* c, int N) {
     c->wbuf = nullptr;
     c->ws = wp::Workspace();
     c->wcap = 0;
+    c->last_tie_n = nullptr;
+    c->last_n = 0;
     const size_t bytes = carve_workspace(c, N, nullptr, &c->ws);
     WP_HIP(hipMalloc(&c->wbuf, bytes));
     carve_workspace(c, N, (char*)c->wbuf, &c->ws);
@@ -365,7 +388,7 @@ wp::ClassifyArgs base_args(const wp_hmm_classifier* c, const CallArgs& x) {
 }
 
 // Counters cleared, emission ranges and boxes, reads appended to their bins.
-void enqueue_prep(wp_hmm_classifier* c, const std::vector<Bin>& bins, const CallArgs& x, hipStream_t stream) {
+void enqueue_prep(wp_hmm_classifier* c, const BinSet& bins, const CallArgs& x, hipStream_t stream) {
     wp::launch_zero_ints(c->ws.ctr, kCtrLen, stream);
     wp::launch_read_prep(c->dm, x.N, x.d_rads, c->ws.L, c->ws.H, c->ws.nb, stream);
     wp::BinEdges e;
@@ -376,7 +399,7 @@ void enqueue_prep(wp_hmm_classifier* c, const std::vector<Bin>& bins, const Call
 }
 
 // The main launch of bin b (for a quad-rows bin, the given variant).
-hipError_t launch_bin_main(wp_hmm_classifier* c, const std::vector<Bin>& bins, int b, const CallArgs& x,
+hipError_t launch_bin_main(wp_hmm_classifier* c, const BinSet& bins, int b, const CallArgs& x,
                            int variant, hipStream_t stream) {
     const Bin& bin = bins[b];
     wp::ClassifyArgs a = base_args(c, x);
@@ -403,7 +426,7 @@ hipError_t launch_bin_main(wp_hmm_classifier* c, const std::vector<Bin>& bins, i
 // per-entry exponent kernels. Gather bins: boxes too large for the
 // half-size separable launch; stage 1 runs the separable passes at the
 // bin's full capacity, stage 2 the composite form on what still does not fit.
-hipError_t launch_bin_wide(wp_hmm_classifier* c, const std::vector<Bin>& bins, int b, const CallArgs& x,
+hipError_t launch_bin_wide(wp_hmm_classifier* c, const BinSet& bins, int b, const CallArgs& x,
                            int stage, hipStream_t stream) {
     const Bin& bin = bins[b];
     wp::ClassifyArgs a = base_args(c, x);
@@ -443,7 +466,7 @@ int enqueue_refine(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream) 
     s->ws.wide_lists = c->ws.rwide_lists;
     s->ws.wide2_lists = c->ws.rwide2_lists;
     s->ws.ctr = c->ws.rctr;
-    const std::vector<Bin> bins = make_bins(s);
+    const BinSet& bins = s->bins;
     wp::launch_zero_ints(s->ws.ctr, kCtrLen, stream);
     wp::BinEdges e;
     e.nbins = (int)bins.size();
@@ -465,8 +488,9 @@ int enqueue_refine(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream) 
 }
 
 // The whole call: prep, binning, each bin's launch (and wide pass).
-int enqueue_call(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream, bool events) {
-    const std::vector<Bin> bins = make_bins(c);
+int enqueue_call(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream, bool events,
+                 bool branches = true) {
+    const BinSet& bins = c->bins;
     if (events) WP_HIP(hipEventRecord(c->ev[0], stream));
     enqueue_prep(c, bins, x, stream);
     WP_HIP(hipGetLastError());
@@ -476,7 +500,7 @@ int enqueue_call(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream, bo
     // branch b mod 3 (branch 0 = the caller's stream); the caller's stream
     // then waits for every branch.
     constexpr int kBranches = 3;
-    const int nbr = std::min(kBranches, (int)bins.size());
+    const int nbr = branches ? std::min(kBranches, (int)bins.size()) : 1;
     if (nbr > 1) {
         if (!c->fork) WP_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
         WP_HIP(hipEventRecord(c->fork, stream));
@@ -521,7 +545,7 @@ constexpr double kDefaultTieRel = 1e-5;
 // Picks each quad-rows bin's variant by timing both on this call's reads
 // (two runs each, the faster second run wins; both give identical outputs).
 int autotune(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream) {
-    const std::vector<Bin> bins = make_bins(c);
+    const BinSet& bins = c->bins;
     bool any = false;
     for (const Bin& b : bins) any = any || b.kernel == kQuadRows;
     if (!any || c->variant_mode >= 0) {
@@ -568,6 +592,28 @@ int autotune(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream) {
     return rc;
 }
 
+void default_variants(wp_hmm_classifier* c) {
+    if (c->variant_mode >= 0) {
+        for (int b = 0; b < wp::kMaxBins; b++) c->variant[b] = c->variant_mode;
+    } else {
+        for (int b = 0; b < c->bins.size(); b++) c->variant[b] = c->bins[b].hi >= 8 ? 1 : 0;
+    }
+    c->variants_set = 1;
+}
+
+// FP32 near-best settings of a call (the normaliser goes to the workspace
+// when the caller does not ask for it: the FP64 decision's adjusted score
+// needs it).
+void set_tie_args(wp_hmm_classifier* c, CallArgs* x) {
+    if (!(c->r64 && c->refine)) return;
+    x->tie_mode = 1;
+    x->tie_n = c->ws.tie_n;
+    x->tie_idx = c->ws.tie_idx;
+    x->tie_abs = c->tie_abs;
+    x->tie_rel = c->tie_rel;
+    if (!x->d_lt) x->d_lt = c->ws.lt;
+}
+
 int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_cand_off,
                const int* d_cand_idx, int* d_id, double* d_adj, double* d_ls, double* d_lt,
                double* d_pair, hipStream_t stream, bool graph_ok = false) {
@@ -579,21 +625,14 @@ int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_c
     int rc = ensure_workspace(c, N);
     if (rc) return rc;
     CallArgs x{N, d_rads, d_cand_off, d_cand_idx, d_id, d_adj, d_ls, d_lt, d_pair};
-    if (c->r64 && c->refine) {
-        x.tie_mode = 1;
-        x.tie_n = c->ws.tie_n;
-        x.tie_idx = c->ws.tie_idx;
-        x.tie_abs = c->tie_abs;
-        x.tie_rel = c->tie_rel;
-        if (!x.d_lt) x.d_lt = c->ws.lt;  // the FP64 decision's adjusted score needs the normaliser
-    }
+    set_tie_args(c, &x);
+    c->last_tie_n = x.tie_n;
+    c->last_n = N;
     if (!c->tuned) {
         if (c->variant_mode < 0 && N < kTuneMinReads) {
             // too few reads to time the variants reliably: the measured
             // defaults until a large call tunes them
-            const std::vector<Bin> bins = make_bins(c);
-            for (int b = 0; b < (int)bins.size(); b++) c->variant[b] = bins[b].hi >= 8 ? 1 : 0;
-            c->variants_set = 1;
+            default_variants(c);
         } else {
             rc = autotune(c, x, stream);
             if (rc) return rc;
@@ -743,7 +782,8 @@ int wp_hmm_classifier_create(unsigned int num_timesteps, unsigned int num_channe
     std::vector<double> binom((size_t)2 * C * c->tri);
     wp::binomial_tables(*seq_model, C, M, binom.data());
     // every bin must fit at least one pair group
-    for (const Bin& b : make_bins(c)) {
+    c->bins = make_bins(c);
+    for (const Bin& b : c->bins) {
         if (bin_lds_bytes(c, b) > lds_hard_limit()) {
             wp::set_error("state box too large for LDS at this precision (max_num_dyes/channels)");
             delete c;
@@ -782,7 +822,8 @@ int wp_hmm_classifier_create(unsigned int num_timesteps, unsigned int num_channe
         s->dBinom = c->dBinom;
         s->borrowed = 1;
         bool fits = true;
-        for (const Bin& b : make_bins(s)) fits = fits && bin_lds_bytes(s, b) <= lds_hard_limit();
+        s->bins = make_bins(s);
+        for (const Bin& b : s->bins) fits = fits && bin_lds_bytes(s, b) <= lds_hard_limit();
         if (fits) {
             c->r64 = s;
             c->refine = 1;
@@ -843,6 +884,77 @@ int wp_hmm_classify_device(wp_hmm_classifier* c, int N, const double* d_rads, in
     }
     return run_device(c, N, d_rads, nullptr, nullptr, d_id, d_adj, d_ls, d_lt, nullptr,
                       (hipStream_t)stream, true);
+}
+
+}  // extern "C"
+
+// Workspace of wp_hmm_classify_device_noalloc: a file-scope device array, so
+// that the entry point allocates nothing. Each device holds its own copy until
+// the process ends. Host code addresses it only through the pointer
+// hipGetSymbolAddress returns for the current device (the array's name is a
+// host-side stand-in).
+static constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {nullptr};
+
+// The current device's scratch address, looked up once per device.
+int scratch_for_current_device(void** out) {
+    int dev = 0;
+    WP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) {
+        wp::set_error("device index beyond the scratch table");
+        return WP_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        WP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return WP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wp_hmm_classify_device_noalloc(wp_hmm_classifier* c, int N, const double* d_rads, int* d_id, double* d_adj,
+                                   double* d_ls, double* d_lt, void* stream) {
+    if (!c || N < 0 || (N > 0 && (!d_rads || !d_id))) {
+        wp::set_error("null argument or num_reads < 0");
+        return WP_ERR_ARG;
+    }
+    if (N == 0) return WP_OK;
+    wp::Workspace ws;
+    const size_t need = carve_workspace(c, N, nullptr, &ws);
+    if (need > kScratchBytes) {
+        wp::set_error("wp_hmm_classify_device_noalloc: " + std::to_string(N) + " reads need " +
+                      std::to_string(need) + " bytes of workspace, more than its " +
+                      std::to_string(kScratchBytes) + "-byte scratch");
+        return WP_ERR_ARG;
+    }
+    void* base = nullptr;
+    int rc = scratch_for_current_device(&base);
+    if (rc) return rc;
+    carve_workspace(c, N, (char*)base, &ws);
+    if (!c->variants_set) default_variants(c);  // no autotuning here: it times launches with events
+    // the call's launches read the classifier's workspace pointers when they
+    // are enqueued; the classifier's own workspace is put back afterwards
+    const wp::Workspace own = c->ws;
+    c->ws = ws;
+    CallArgs x{N, d_rads, nullptr, nullptr, d_id, d_adj, d_ls, d_lt, nullptr};
+    set_tie_args(c, &x);
+    rc = enqueue_call(c, x, (hipStream_t)stream, false, false);
+    c->ws = own;
+    c->last_tie_n = x.tie_n;
+    c->last_n = N;
+    return rc;
 }
 
 static int classify_host(wp_hmm_classifier* c, int N, const double* rads, const int* cand_off,
@@ -974,7 +1086,7 @@ int wp_hmm_get_variants(const wp_hmm_classifier* c, int* out_variant, double* ou
         wp::set_error("null classifier");
         return WP_ERR_ARG;
     }
-    const int nb = (int)make_bins(c).size();
+    const int nb = c->bins.size();
     for (int b = 0; b < std::min(nb, cap); b++) {
         if (out_variant) out_variant[b] = c->variants_set ? c->variant[b] : -1;
         if (out_ms) {
@@ -1012,7 +1124,7 @@ int wp_hmm_set_refine(wp_hmm_classifier* c, int enable, double abs, double rel) 
 }
 
 int wp_hmm_refine_stats(wp_hmm_classifier* c, int N, int* out2) {
-    if (!c || !out2 || N < 0 || N > c->wcap || !c->r64) {
+    if (!c || !out2 || N < 0 || N > c->last_n || (N > 0 && !c->last_tie_n)) {
         wp::set_error("wp_hmm_refine_stats: null argument, no FP64 decision, or more reads than the last call");
         return WP_ERR_ARG;
     }
@@ -1020,7 +1132,7 @@ int wp_hmm_refine_stats(wp_hmm_classifier* c, int N, int* out2) {
     if (N == 0) return WP_OK;
     std::vector<int> h(N);
     WP_HIP(hipDeviceSynchronize());
-    WP_HIP(hipMemcpy(h.data(), c->ws.tie_n, sizeof(int) * N, hipMemcpyDeviceToHost));
+    WP_HIP(hipMemcpy(h.data(), c->last_tie_n, sizeof(int) * N, hipMemcpyDeviceToHost));
     for (int r = 0; r < N; r++) {
         if (h[r] != 0) out2[0]++;
         if (h[r] < 0) out2[1]++;
