@@ -15,7 +15,8 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("dims,B", [((36, 64, 3, 100), 64), ((20, 200, 2, 300), 150), ((36, 512, 3, 1000), 256)])
+@pytest.mark.parametrize("dims,B", [((36, 64, 3, 100), 64), ((20, 200, 2, 300), 150), ((36, 512, 3, 1000), 256),
+                                    ((36, 512, 3, 20000), 256)])
 def test_forward_matches_oracle(gpu, dims, B):
     din, hid, nh, dout = dims
     print(f"mlp forward {dims} B={B}", flush=True)

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "hmm_kernels.h"
@@ -192,6 +193,17 @@ struct wp_hmm_classifier {
     hipStream_t branch[2] = {nullptr, nullptr};
     hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
 };
+
+namespace wp {
+
+bool dye_seq_index_of_ids(const wp_hmm_classifier* c, std::unordered_map<int, int>* out) {
+    out->clear();
+    for (int i = 0; i < c->S; i++)
+        if (!out->emplace(c->ids[i], i).second) return false;
+    return true;
+}
+
+}  // namespace wp
 
 namespace {
 

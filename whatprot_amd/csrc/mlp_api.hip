@@ -12,6 +12,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "mlp_kernels.h"
@@ -586,6 +587,18 @@ int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, cons
         rc = wp_hmm_classify_candidates(clf, (int)nv, x.data(), off.data(), idx.data(), id.data(), adj.data(),
                                         ls.data(), lt.data());
         if (rc) return rc;
+        // The read's pick is the best of its chunks' picks. They are decided
+        // by a second call over the chunk winners in chunk order (one read per
+        // fallback read), so the kernels' own rule picks among them: strict
+        // '>' with the earlier candidate winning, near-ties in FP64 on the
+        // FP32 path. The winners' dye-seq indices come from their ids, which
+        // needs unique ids; otherwise the chunks' ln scores decide directly.
+        std::unordered_map<int, int> index_of;
+        const bool by_index = wp::dye_seq_index_of_ids(clf, &index_of);
+        std::vector<int> off2(unsure.size() + 1, 0), idx2;
+        std::vector<double> tots(unsure.size(), -INFINITY);
+        std::vector<int> best_ids(unsure.size());
+        std::vector<double> bests(unsure.size());
         for (size_t i = 0; i < unsure.size(); i++) {
             const size_t v0 = i * nch;
             int best_id = id[v0];
@@ -599,8 +612,32 @@ int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, cons
                 if (ls[v] > -INFINITY) {  // an all-empty chunk holds no mass
                     const double hi = std::max(tot, lt[v]);
                     tot = hi + std::log(std::exp(tot - hi) + std::exp(lt[v] - hi));
+                    if (by_index) idx2.push_back(index_of.at(id[v]));
                 }
             }
+            off2[i + 1] = (int)idx2.size();
+            tots[i] = tot;
+            best_ids[i] = best_id;
+            bests[i] = best;
+        }
+        std::vector<int> id2(unsure.size());
+        std::vector<double> adj2(unsure.size()), ls2(unsure.size()), lt2(unsure.size());
+        if (by_index && !idx2.empty()) {
+            std::vector<double> x2(unsure.size() * (size_t)d);
+            for (size_t i = 0; i < unsure.size(); i++)
+                std::memcpy(&x2[i * d], radiometries + (size_t)unsure[i] * d, sizeof(double) * d);
+            rc = wp_hmm_classify_candidates(clf, (int)unsure.size(), x2.data(), off2.data(), idx2.data(), id2.data(),
+                                            adj2.data(), ls2.data(), lt2.data());
+            if (rc) return rc;
+        }
+        for (size_t i = 0; i < unsure.size(); i++) {
+            int best_id = best_ids[i];
+            double best = bests[i];
+            if (by_index && off2[i + 1] > off2[i]) {
+                best_id = id2[i];
+                best = ls2[i];
+            }
+            const double tot = tots[i];
             double a = tot > -INFINITY ? std::exp(best - tot) : 0.0;
             if (std::isnan(a)) a = 0.0;
             out_id[unsure[i]] = best_id;

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/whatprot_amd.h"
@@ -41,6 +42,10 @@ inline double sigma(const wp_model& m, int channel, double amu) {
 // DyeSeq(C, string) (dye-seq.cc:23-40): returns the length after trimming
 // trailing '.'; seq[i] = channel digit or -1.
 int parse_dye_seq(const char* s, short* seq, int cap);
+
+// Dye-seq index of each id of an HMM classifier's table; false (map
+// unspecified) when two dye-seqs share an id.
+bool dye_seq_index_of_ids(const wp_hmm_classifier* c, std::unordered_map<int, int>* out);
 
 // True iff every parsed dye names a channel < C (else sets the error text).
 bool dye_channels_ok(const short* seqs, int n, int cap, const int* lens, int C);
