@@ -1,5 +1,6 @@
 # Builds the reference's own C++ sources for the HMM path in place (never
-# copied) into oracle/_ref/libwpref.so, with oracle/ref_shim.cc exposing a
+# copied) into oracle/_ref/libwpref.so with the release flags (-O2 -flto,
+# cc_code/makefile:121), with oracle/ref_shim.cc exposing a
 # plain C entry set for the tests. levmar is compiled from its C sources. The
 # prebuilt MR-MPI objects and the dedup code that needs them are not used.
 REF_SRC = $(REF)/src
@@ -25,5 +26,5 @@ $(REF_OUT)/liblevmar_pic.a: $(LEVMAR_C)
 	ar crs $@ $(REF_OUT)/levmar_obj/*.o
 
 $(REF_OUT)/libwpref.so: $(REF_CC) $(HERE)ref_shim.cc $(REF_OUT)/liblevmar_pic.a
-	$(CXX) -std=gnu++17 -O2 -fPIC -fopenmp -shared $(REF_INC) -I$(HERE)../include \
+	$(CXX) -std=gnu++17 -O2 -flto -fPIC -fopenmp -shared $(REF_INC) -I$(HERE)../include \
 	    -o $@ $(REF_CC) $(HERE)ref_shim.cc $(REF_OUT)/liblevmar_pic.a -lm
