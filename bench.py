@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--no-north-star", action="store_true",
                     help="leave out the learned-shortlist leg (10^5 reads x 20,000 dye-seqs) of the default line")
     ap.add_argument("--shortlist-reads", type=int, default=100000)
+    ap.add_argument("--labels", choices=["hmm", "truth"], default="hmm",
+                    help="shortlist training labels: the HMM's pick among same-dye-count dye-seqs, or the source")
     ap.add_argument("--workload", choices=["classify", "fit", "hybrid", "shortlist", "simulate"], default="classify")
     ap.add_argument("--peptides-per-dye-seq", type=int, default=1000)
     ap.add_argument("--train-reads", type=int, default=10000000)
@@ -308,9 +310,19 @@ def shortlist_leg(args, steps, warmup=1, fp64_steps=2):
     log(f"shortlist: simulating {args.train_reads} training reads and {N} query reads")
     t0 = time.perf_counter()
     tx, ty = w.simulate_radiometries(model, T, strings, args.train_reads, 4003, gpu=True)
-    tx = tx.reshape(len(tx), -1).astype(np.float32)
     t_sim = time.perf_counter() - t0
     qx, qy = w.simulate_radiometries(model, T, strings, N, 3002, gpu=True)
+    clf64 = w.HMMClassifier(T, C, model, w.SequencingSettings(cutoff), entries, w.WP_PRECISION_FP64)
+    clf32 = w.HMMClassifier(T, C, model, w.SequencingSettings(cutoff), entries, w.WP_PRECISION_FP32)
+    t_label = 0.0
+    if args.labels == "hmm":
+        log("shortlist: HMM labels (the FP32 HMM's pick among each read's same-dye-count dye-seqs)")
+        t0 = time.perf_counter()
+        ty_h = w.hmm_labels(clf32, tx, ty)
+        t_label = time.perf_counter() - t0
+        log(f"shortlist: {np.mean(ty_h != ty):.4f} of the labels moved ({t_label:.1f} s)")
+        ty = ty_h
+    tx = tx.reshape(len(tx), -1).astype(np.float32)
     net = w.ShortlistMLP(T * C, 512, 3, S, seed=1)
     log(f"shortlist: training {args.epochs} epochs ({t_sim:.0f} s simulating)")
     torch.cuda.synchronize()
@@ -321,8 +333,6 @@ def shortlist_leg(args, steps, warmup=1, fp64_steps=2):
     del tx, ty
     flops_step = 2 * 4096 * (64 * 512 + 2 * 512 * 512 + 512 * S) * 3
     train_tflops = flops_step * (args.train_reads // 4096) * args.epochs / t_train / 1e12
-    clf64 = w.HMMClassifier(T, C, model, w.SequencingSettings(cutoff), entries, w.WP_PRECISION_FP64)
-    clf32 = w.HMMClassifier(T, C, model, w.SequencingSettings(cutoff), entries, w.WP_PRECISION_FP32)
     log("shortlist: plain FP64 GPU HMM over every dye-seq")
     res64 = {}
 
@@ -360,6 +370,8 @@ def shortlist_leg(args, steps, warmup=1, fp64_steps=2):
         "fallback_fraction": nf / N,
         "timing": f"host arrays in and out on both sides, {warmup} untimed step(s), then {steps} (shortlist) and "
                   f"{fp64_steps} (FP64) steps between synchronizations",
+        "labels": args.labels,
+        "label_s": t_label,
         "train_s": t_train,
         "train_tflops_bf16": train_tflops,
         "epoch_loss": [float(x) for x in losses],

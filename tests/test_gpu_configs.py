@@ -48,12 +48,20 @@ def test_shortlist_fp32_h256_mass_criterion(gpu):
     strings = w.generate_dye_seqs(C, S, 6, 30, 0, 5, 1202)
     entries = [w.DyeSeqEntry(s, 1, i) for i, s in enumerate(strings)]
     tx, ty = w.simulate_radiometries(model, T, strings, 1_000_000, 1203, gpu=True)
-    net = w.ShortlistMLP(T * C, 512, 3, S, seed=1)
-    losses = net.train(tx.reshape(len(tx), -1).astype(np.float32), ty, batch=4096, epochs=2, lr=1e-3, seed=9)
-    del tx, ty
-    qx, _ = w.simulate_radiometries(model, T, strings, 2000, 1204)
     clf32 = w.HMMClassifier(T, C, model, w.SequencingSettings(5.0), entries, w.WP_PRECISION_FP32)
     clf64 = w.HMMClassifier(T, C, model, w.SequencingSettings(5.0), entries, w.WP_PRECISION_FP64)
+    # HMM labels: the pick among each read's same-dye-count dye-seqs; checked
+    # against the FP64 classifier on a sample
+    th = w.hmm_labels(clf32, tx, ty)
+    D = w.dye_counts(strings, C)
+    for r in range(0, 1_000_000, 50_000):
+        same = np.nonzero((D == D[ty[r]]).all(axis=1))[0]
+        assert th[r] == clf64.classify_arrays(tx[r:r + 1], candidates=[list(same)])[0][0]
+    print(f"labels moved by the HMM: {np.mean(th != ty):.4f}", flush=True)
+    net = w.ShortlistMLP(T * C, 512, 3, S, seed=1)
+    losses = net.train(tx.reshape(len(tx), -1).astype(np.float32), th, batch=4096, epochs=2, lr=1e-3, seed=9)
+    del tx, ty
+    qx, _ = w.simulate_radiometries(model, T, strings, 2000, 1204)
     ids, adj, nf = net.classify(clf32, qx, H, 0.5, "mass")
     idx, _, mass = net.shortlist(qx, H, with_mass=True)
     fall = mass < 0.5

@@ -338,6 +338,19 @@ class HMMClassifier:
                                                    ls.ctypes.data, lt.ctypes.data))
         return ids, adj, ls, lt
 
+    def classify_csr(self, radiometries, offsets, indices):
+        """classify_arrays over candidate lists given as CSR arrays (offsets
+        [N + 1], indices into the dye-seq table)."""
+        r = self._rads(radiometries)
+        N = r.shape[0]
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        idx = np.ascontiguousarray(indices if len(indices) else np.zeros(1), dtype=np.int32)
+        ids = np.zeros(N, dtype=np.int32)
+        adj, ls, lt = np.zeros(N), np.zeros(N), np.zeros(N)
+        check(lib().wp_hmm_classify_candidates(self._h, N, r.ctypes.data, off.ctypes.data, idx.ctypes.data,
+                                               ids.ctypes.data, adj.ctypes.data, ls.ctypes.data, lt.ctypes.data))
+        return ids, adj, ls, lt
+
     def classify(self, radiometries, candidates=None) -> List[ScoredClassification]:
         """HMMClassifier::classify(vector<Radiometry>) (hmm-classifier.cc:84-93),
         or per-read candidate_indices (hmm-classifier.cc:79-82)."""
@@ -799,6 +812,51 @@ class HybridClassifier:
     def classify(self, radiometries) -> List[ScoredClassification]:
         ids, adj = self.classify_arrays(radiometries)
         return [ScoredClassification(int(i), float(a), math.nan, math.nan) for i, a in zip(ids, adj)]
+
+
+def dye_counts(dye_strings, C):
+    """Dyes of each channel per dye-seq, [S][C] (DyeSeq digits, dye-seq.cc:23-40)."""
+    out = np.zeros((len(dye_strings), C), dtype=np.int32)
+    for i, s in enumerate(dye_strings):
+        for ch in s:
+            if ch.isdigit() and int(ch) < C:
+                out[i, int(ch)] += 1
+    return out
+
+
+def hmm_labels(classifier: "HMMClassifier", radiometries, labels, chunk=250000):
+    """Training labels from the HMM (no reference counterpart): for each read
+    simulated from dye-seq labels[r], the classifier's pick among the dye-seqs
+    with the same dye count in every channel (in table order, so exact ties go
+    to the lowest index, as HMMClassifier decides them). Reads of a dye-seq
+    that a pruned read cannot tell from other dye-seqs (equal P) are labelled
+    with the one the HMM returns, which the shortlist then learns to propose.
+    Returns dye-seq indices (the classifier's ids mapped back to indices)."""
+    strings = [e.dye_string for e in classifier.dye_seqs]
+    S = len(strings)
+    D = dye_counts(strings, classifier.num_channels)
+    _, group = np.unique(D, axis=0, return_inverse=True)
+    group = group.reshape(-1)
+    order = np.lexsort((np.arange(S), group))       # by group, then index
+    gsize = np.bincount(group)
+    gstart = np.concatenate([[0], np.cumsum(gsize)[:-1]])
+    index_of = {int(e.id): i for i, e in enumerate(classifier.dye_seqs)}
+    if len(index_of) != S:
+        raise WhatprotError("hmm_labels needs unique dye-seq ids")
+    rads = np.asarray(radiometries, dtype=np.float64).reshape(len(labels), classifier.num_timesteps,
+                                                               classifier.num_channels)
+    labels = np.asarray(labels, dtype=np.int64)
+    out = np.empty(len(labels), dtype=np.int32)
+    for r0 in range(0, len(labels), chunk):
+        lab = labels[r0:r0 + chunk]
+        g = group[lab]
+        cnt = gsize[g]
+        off = np.zeros(len(lab) + 1, dtype=np.int64)
+        off[1:] = np.cumsum(cnt)
+        pos = np.arange(off[-1]) - np.repeat(off[:-1], cnt) + np.repeat(gstart[g], cnt)
+        ids = classifier.classify_csr(rads[r0:r0 + chunk], off, order[pos])[0]
+        out[r0:r0 + chunk] = [index_of[int(i)] for i in ids]
+    return out
 
 
 class ShortlistMLP:
