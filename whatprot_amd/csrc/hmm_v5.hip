@@ -17,6 +17,21 @@
 // row exponents meet only where the Edman success crosses rows and in the
 // final sum of a pair.
 //
+// Blocked and detached mass. The N-terminally blocked tensor B and the
+// detached share are never Edman-shifted: from one step to the next they see
+// Detach, Bleach and Emit only, which are the read's, not the candidate's. So
+// a per-read backward pass gives, for every timestep t, what a unit of A at
+// (t - 1, q) will add to P through CyclicBlock and Detach(t - 1) at step t,
+//   g_t[q] = pd_t phi_t + pcb (1 - pd_t) sum_n Wt[q][n] E_t[n] beta_t[n],
+// with beta_t[n] the final worth of a unit of blocked mass at (t, n) after
+// Emit(t) and phi_t that of a unit of detached mass entering at step t
+// (the product of E_u(0) over u >= t where box(u) holds the zero count, else
+// 0); and the initial blocked share p_ib Dud(D -> n) E_0[n] is worth
+// beta_0[n]. Each candidate then runs A alone and adds row . g_t at every
+// step: P = sum A_(T-1) + those sums, which is the reference's
+// sum A + sum B + detached share (peptide-state-vector.cc:52-54) with the same
+// step order, regrouped.
+//
 // Per-timestep box classes. A read's box shrinks as its dyes leave (most
 // timesteps of most reads have a one-state box), so each timestep t runs at
 // the class K_t = the power of two >= max(|box(t-1)|, |box(t)|), not at the
@@ -38,9 +53,11 @@ using namespace cc;
 constexpr int kV5Threads = 256;
 constexpr int kV5Lanes = 4;  // lanes per candidate: one DPP quad
 constexpr int kV5MaxSpan = 90;  // largest binary-exponent spread of one box's densities
+constexpr int kV5MaxBackSpread = 800;  // largest spread of one level of the blocked/detached backward pass
 
 struct V5Layout {
-    size_t rad, L, H, nbt, kcl, woff, bstr, coord, kcf, ev, etab, xt, W, binom, list, scan, gres, ties, total;
+    size_t rad, L, H, nbt, kcl, woff, bstr, coord, kcf, ev, etab, xt, W, binom, list, scan, gres, ties, gtab, gx,
+        b0, bd, total;
 };
 
 // W block per timestep: (1 + C) matrices (Wt, then W1[c]) of K_t x K_t with a
@@ -69,6 +86,10 @@ __host__ __device__ inline V5Layout v5_layout(int T, int C, int tri, int NB) {
     l.scan = o;  o = align16(o + sizeof(int) * (kV5Threads / 64 + 2));
     l.gres = o;  o = align16(o + sizeof(double) * 4 * ngroups);
     l.ties = o;  o = align16(o + sizeof(TieSlot) * ngroups);
+    l.gtab = o;  o = align16(o + sizeof(double) * (size_t)T * NB);
+    l.gx = o;    o = align16(o + sizeof(int) * (T + 1));
+    l.b0 = o;    o = align16(o + sizeof(double) * NB);
+    l.bd = o;    o = align16(o + sizeof(double) * NB);
     l.total = o;
     return l;
 }
@@ -84,24 +105,35 @@ struct V5Step {
     const float* Wt;   // K x K composite Bleach, then the C Edman-shifted forms, K * K + 4 apart
     const float* Et;   // densities of box(t) relative to 2^Xt
     const float* kct;  // [NB][C] counts of box(t) as floats
-    int nbp, t, Xt, p;
-    float pd, cAA, cB, pf, pcb;
-    bool allow;        // box(t) contains the all-zero count (the detached share keeps E_t(0))
+    const double* gt;  // g_t over box(t - 1), relative to 2^Gx
+    int nbp, t, Xt, Gx, p;
+    float cAA, pf;
 };
+
+// s 2^e += v 2^ev, s kept in [0.5, 1) (s = 0 holds any e). FP64: a row's
+// share of g_t can lie far below the row's own scale.
+__device__ __forceinline__ void side_add(double& s, int& e, double v, int ev) {
+    if (v > 0.0) {
+        const int mx = max(e, ev);
+        int k;
+        s = frexp(ldexp(s, e - mx) + ldexp(v, ev - mx), &k);
+        e = mx + k;
+    }
+}
 
 // One step t >= 1 of a candidate's recursion (CyclicBlock, Detach(t-1),
 // Bleach_c, Edman, Emit(t)) on a box of class K (both box(t-1) and box(t) fit
-// in K entries). Entries >= K of A and B are stale and never read: inputs stop
-// at |box(t-1)| and outputs at K.
+// in K entries), A only: the shares CyclicBlock and Detach take from each row
+// go to the row's side sum (sm, se) through g_t. Entries >= K of A are stale
+// and never read: inputs stop at |box(t-1)| and outputs at K.
 template <int C, int NB, int R, int K>
 __device__ __forceinline__ void v5_step(const V5Step& st, const int (&cix)[R], const float (&aK)[R],
                                         const float (&cst)[R], const float (&Kf)[R], float (&A)[R][NB],
-                                        float (&B)[R][NB], int (&kx)[R], float (&dl)[R], float (&Sr)[R]) {
+                                        int (&kx)[R], double (&sm)[R], int (&se)[R]) {
     constexpr int G = kV5Lanes;
     constexpr int MS = K * K + 4;
     const float* Wt = st.Wt;
     const int p = st.p;
-    const float E0 = st.Et[0];
     float succ_prev[K];  // this lane's slot k - 1 Edman success (old scale)
     int kprev = kNoExp;  // and slot k - 1's exponent before this step
 #pragma unroll
@@ -111,40 +143,36 @@ __device__ __forceinline__ void v5_step(const V5Step& st, const int (&cix)[R], c
         if (st.t < G * k) continue;  // every row of slot k starts after t (uniform)
         // Every slot reads the same Wt; without this the compiler keeps the
         // previous slot's Wt loads live across slots, which costs registers
-        // (spills under the occupancy bounds below) for no LDS traffic saved
-        // worth having: the reads are broadcasts.
+        // for no LDS traffic saved worth having: the reads are broadcasts.
         if constexpr (K > 1) asm volatile("" ::: "memory");
         const float* W1 = Wt + (1 + cix[k]) * MS;
-        float BA[K], BB[K], BA1[K];
+        float BA[K], BA1[K];
 #pragma unroll
         for (int q2 = 0; q2 < K; q2++) {
             BA[q2] = 0.0f;
-            BB[q2] = 0.0f;
             BA1[q2] = 0.0f;
         }
+        double sv = 0.0;  // row . g_t: the row's blocked and detached shares of this step
 #pragma unroll
         for (int q = 0; q < K; q++) {
             if (q >= st.nbp) continue;  // uniform: the previous box's size
-            const float av = A[k][q], bv = B[k][q];
+            const float av = A[k][q];
+            sv = fma((double)av, st.gt[q], sv);
 #pragma unroll
             for (int q2 = 0; q2 < K; q2++) {
-                const float w = Wt[q * K + q2];
-                BA[q2] += w * av;
-                BB[q2] += w * bv;
+                BA[q2] += Wt[q * K + q2] * av;
                 BA1[q2] += W1[q * K + q2] * av;
             }
         }
         // exponents of this row and of the row below it (e - 1)
         const int ko = kx[k];
+        if (ko != kNoExp) side_add(sm[k], se[k], sv, ko + st.Gx);
         const int kfrom = quad_rot_i(p == G - 1 ? kprev : ko);
         int kn = max(ko, kfrom);
         if (kn == kNoExp) kn = 0;
         const float sown = ko == kNoExp ? 0.0f : ldexpf(1.0f, ko - kn);
         const float sfrom = kfrom == kNoExp ? 0.0f : ldexpf(1.0f, kfrom - kn);
         const float fAo = st.cAA * st.pf * sown, fAf = st.cAA * sfrom;
-        const float fB = st.cB * sown, fBA = st.cB * st.pcb * sown;
-        // Detach(t-1): pd times the row's previous sum.
-        float dlp = (dl[k] + st.pd * Sr[k]) * sown;
         float m = 0.0f;
 #pragma unroll
         for (int q2 = 0; q2 < K; q2++) {
@@ -155,28 +183,15 @@ __device__ __forceinline__ void v5_step(const V5Step& st, const int (&cix)[R], c
             const float from = quad_rot(p == G - 1 ? succ_prev[q2] : succ);
             succ_prev[q2] = succ;
             // Emit(t) (peptide-emission.cc:212-243)
-            const float em = st.Et[q2];
-            const float na = fmaf(fAf, from, fAo * BA[q2]) * em;
-            const float nb = fmaf(fBA, BA[q2], fB * BB[q2]) * em;
-            m = fmaxf(m, fmaxf(na, nb));
+            const float na = fmaf(fAf, from, fAo * BA[q2]) * st.Et[q2];
+            m = fmaxf(m, na);
             A[k][q2] = na;
-            B[k][q2] = nb;
         }
-        // the detached share keeps density E_t(0) only if the box includes zero
-        dlp = st.allow ? dlp * E0 : 0.0f;
-        m = fmaxf(m, dlp);
         kprev = ko;
         const int km = exp_of(m);
         const int kf = km == kNoExp ? 0 : km;
-        float S = 0.0f;
 #pragma unroll
-        for (int q2 = 0; q2 < K; q2++) {
-            A[k][q2] = ldexpf(A[k][q2], -kf);
-            B[k][q2] = ldexpf(B[k][q2], -kf);
-            S += A[k][q2] + B[k][q2];
-        }
-        Sr[k] = S;
-        dl[k] = ldexpf(dlp, -kf);
+        for (int q2 = 0; q2 < K; q2++) A[k][q2] = ldexpf(A[k][q2], -kf);
         kx[k] = km == kNoExp ? kNoExp : kn + st.Xt + kf;
     }
 }
@@ -217,6 +232,10 @@ __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsi
     int* scan = (int*)(smem + lay.scan);
     double* gres = (double*)(smem + lay.gres);
     TieSlot* ties = (TieSlot*)(smem + lay.ties);
+    double* gtab = (double*)(smem + lay.gtab);
+    int* gx = (int*)(smem + lay.gx);  // [T]: exponents; [T]: spread flag
+    double* b0d = (double*)(smem + lay.b0);
+    double* bd = (double*)(smem + lay.bd);
     int ncand;
     const int* cidx;
     read_candidates(a, r, &ncand, &cidx);
@@ -345,14 +364,100 @@ __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsi
         }
         Wall[(kClass ? woff[t] : t * (1 + C) * MSNB) + mat * ms + rem] = (float)v;
     }
+    // Blocked and detached mass (one wave, lane q = box entry q): from level
+    // t + 1 = u, beta_u = bq 2^eb and phi_(u+1) = ph 2^eb, to level t, both
+    // relative to 2^(eb + X_u) (densities E = Ehat 2^X), in FP64:
+    //   phi_u = Ehat_u(0) ph (box(u) holds the zero count) or 0,
+    //   v[q] = sum_m Bleach(q -> m) Ehat_u[m] bq[m]  (Bleach_c, Detach and Emit of step u),
+    //   beta_t[q] = pd phi_u + (1 - pd) v[q],  g_u[q] = pd phi_u + pcb (1 - pd) v[q],
+    // then level t is renormalised by a power of two. A level whose nonzero
+    // entries spread over more than 2^kV5MaxBackSpread hands the read to the
+    // wide pass: below that every product of the next level (Bleach >= 2^-65,
+    // relative densities >= 2^-kV5MaxSpan) is a normal double, so no share
+    // underflows unseen.
+    if (tid < 64) {
+        const int q = tid;
+        double bq = q < nbt[T - 1] ? 1.0 : 0.0;
+        double ph = 1.0;
+        int eb = 0;
+        bool spread = false;
+        const double pcb_d = a.m.p_cb;
+        for (int t = T - 2; t >= 0; t--) {
+            const int u = t + 1;
+            if (q < NB) bd[q] = bq;
+            wave_sync();
+            bool zero_in = true;
+#pragma unroll
+            for (int c = 0; c < C; c++) zero_in = zero_in && sL[u * C + c] == 0;
+            ph = zero_in ? ldexp(ev[u * NB], -xt[u]) * ph : 0.0;
+            double v = 0.0;
+            if (q < nbt[t]) {
+                for (int m2 = 0; m2 < nbt[u]; m2++) {
+                    double wv = ldexp(ev[u * NB + m2], -xt[u]) * bd[m2];
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        const int from = coord[(t * NB + q) * C + c];
+                        const int to = coord[(u * NB + m2) * C + c];
+                        wv = from < to ? 0.0 : wv * ble[c * tri + from * (from + 1) / 2 + to];
+                    }
+                    v += wv;
+                }
+            }
+            const double pd = a.m.p_detach[u - 1];
+            const double g = q < nbt[t] ? pd * ph + pcb_d * (1.0 - pd) * v : 0.0;
+            bq = q < nbt[t] ? pd * ph + (1.0 - pd) * v : 0.0;
+            double gm = g, bm = fmax(bq, ph);
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                gm = fmax(gm, __shfl_xor(gm, off, 64));
+                bm = fmax(bm, __shfl_xor(bm, off, 64));
+            }
+            int ge = 0, be = 0;
+            if (gm > 0.0) frexp(gm, &ge);
+            if (bm > 0.0) frexp(bm, &be);
+            // smallest nonzero exponents of the level
+            int gl = 100000, bl = 100000, k2;
+            if (g > 0.0) {
+                frexp(g, &k2);
+                gl = k2;
+            }
+            if (bq > 0.0) {
+                frexp(bq, &k2);
+                bl = k2;
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                gl = min(gl, __shfl_xor(gl, off, 64));
+                bl = min(bl, __shfl_xor(bl, off, 64));
+            }
+            spread = spread || (gm > 0.0 && ge - gl > kV5MaxBackSpread) || (bm > 0.0 && be - bl > kV5MaxBackSpread);
+            if (q < NB) gtab[u * NB + q] = ldexp(g, -ge);
+            if (q == 0) gx[u] = eb + xt[u] + ge;
+            bq = ldexp(bq, -be);
+            ph = ldexp(ph, -be);
+            eb += xt[u] + be;
+            wave_sync();
+        }
+        if (q < NB) b0d[q] = bq;
+        if (q == 0) {
+            gx[0] = eb;  // beta_0 = b0d 2^gx[0]
+            gx[T] = spread ? 1 : 0;
+        }
+    }
     __syncthreads();
+    if (gx[T]) {
+        if (tid == 0) {
+            a.wide[r] = 1;
+            a.wide_list[atomicAdd(a.wide_count, 1)] = r;
+        }
+        return;
+    }
 
     CandAcc<true> acc;
     const bool collect = a.tie_mode == 1;
     if (collect && p == 0) tie_init(&ties[group]);
     const float pf = (float)a.m.pf;
     const float opf = (float)(1.0 - a.m.pf);
-    const float pcb = (float)a.m.p_cb;
     const double* dud = bin;
 
     for (int base = 0; base < ncand; base += kV5Threads * 4) {
@@ -378,54 +483,44 @@ __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsi
                 cst[k] = ce >= 0 ? 0.0f : opf;
                 Kf[k] = (float)K;
             }
-            float A[R][NB], B[R][NB];
+            float A[R][NB];
             int kx[R];     // slot k holds values x 2^-kx[k]
-            float dl[R];   // detached share of slot k's row
-            float Sr[R];   // sum of slot k's row after the last emission
+            double sm[R];  // side sum of slot k's row (blocked and detached shares), sm 2^se
+            int se[R];
 #pragma unroll
             for (int k = 0; k < R; k++) {
 #pragma unroll
-                for (int q = 0; q < NB; q++) {
-                    A[k][q] = 0.0f;
-                    B[k][q] = 0.0f;
-                }
+                for (int q = 0; q < NB; q++) A[k][q] = 0.0f;
                 kx[k] = kNoExp;
-                dl[k] = 0.0f;
-                Sr[k] = 0.0f;
+                sm[k] = 0.0;
+                se[k] = kNoExp;
             }
-            // t = 0: InitialBlock, Dud_c, Emit(0): row 0 (lane 0, slot 0)
+            // t = 0: InitialBlock, Dud_c, Emit(0): row 0 (lane 0, slot 0); the
+            // blocked share p_ib Dud(D -> n) E_0[n] is worth beta_0[n]
             if (p == 0) {
                 const float pib = (float)a.m.p_ib;
                 float m = 0.0f;
+                double b0 = 0.0;
 #pragma unroll
                 for (int q = 0; q < NB; q++) {
                     bool ok = true;
-                    float xa = 1.0f - pib, xb = pib;
+                    float xa = 1.0f;
 #pragma unroll
                     for (int c = 0; c < C; c++) {
                         const int n = coord[q * C + c];
                         ok = ok && n <= D[c];
-                        if (ok) {
-                            const float pd = (float)dud[c * tri + D[c] * (D[c] + 1) / 2 + n];
-                            xa = pd * xa;
-                            xb = pd * xb;
-                        }
+                        if (ok) xa *= (float)dud[c * tri + D[c] * (D[c] + 1) / 2 + n];
                     }
-                    const float em = ok ? etab[q] : 0.0f;
-                    A[0][q] = xa * em;
-                    B[0][q] = xb * em;
-                    m = fmaxf(m, fmaxf(A[0][q], B[0][q]));
+                    const float em = ok ? xa * etab[q] : 0.0f;
+                    A[0][q] = (1.0f - pib) * em;
+                    b0 = fma((double)(pib * em), b0d[q], b0);
+                    m = fmaxf(m, A[0][q]);
                 }
+                side_add(sm[0], se[0], b0, xt[0] + gx[0]);
                 const int km = exp_of(m);
                 const int kf = km == kNoExp ? 0 : km;
-                float S = 0.0f;
 #pragma unroll
-                for (int q = 0; q < NB; q++) {
-                    A[0][q] = ldexpf(A[0][q], -kf);
-                    B[0][q] = ldexpf(B[0][q], -kf);
-                    S += A[0][q] + B[0][q];
-                }
-                Sr[0] = S;
+                for (int q = 0; q < NB; q++) A[0][q] = ldexpf(A[0][q], -kf);
                 kx[0] = km == kNoExp ? kNoExp : xt[0] + kf;
             }
 
@@ -434,46 +529,50 @@ __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsi
                 st.Wt = Wall + (kClass ? woff[t] : t * (1 + C) * MSNB);
                 st.nbp = nbt[t - 1];
                 st.t = t;
-                st.pd = (float)a.m.p_detach[t - 1];
-                st.cAA = (1.0f - st.pd) * (1.0f - pcb);
-                st.cB = 1.0f - st.pd;
+                const float pd = (float)a.m.p_detach[t - 1];
+                st.cAA = (1.0f - pd) * (1.0f - (float)a.m.p_cb);
                 st.pf = pf;
-                st.pcb = pcb;
                 st.Et = etab + (size_t)t * NB;
                 st.Xt = xt[t];
-                bool allow = true;
-#pragma unroll
-                for (int c = 0; c < C; c++) allow = allow && sL[t * C + c] == 0;
-                st.allow = allow;
+                st.gt = gtab + (size_t)t * NB;
+                st.Gx = gx[t];
                 st.kct = kcf + (size_t)t * NB * C;
                 st.p = p;
                 if constexpr (!kClass) {
-                    v5_step<C, NB, R, NB>(st, cix, aK, cst, Kf, A, B, kx, dl, Sr);
+                    v5_step<C, NB, R, NB>(st, cix, aK, cst, Kf, A, kx, sm, se);
                     continue;
                 }
                 const int Kt = kcl[t];  // uniform over the workgroup
-                if (Kt <= 1) v5_step<C, NB, R, 1>(st, cix, aK, cst, Kf, A, B, kx, dl, Sr);
-                else if (Kt == 2) v5_step<C, NB, R, 2>(st, cix, aK, cst, Kf, A, B, kx, dl, Sr);
-                else if (Kt == 4 || NB == 4) v5_step<C, NB, R, (NB < 4 ? NB : 4)>(st, cix, aK, cst, Kf, A, B, kx, dl, Sr);
-                else if (Kt == 8 || NB == 8) v5_step<C, NB, R, (NB < 8 ? NB : 8)>(st, cix, aK, cst, Kf, A, B, kx, dl, Sr);
-                else v5_step<C, NB, R, NB>(st, cix, aK, cst, Kf, A, B, kx, dl, Sr);
+                if (Kt <= 1) v5_step<C, NB, R, 1>(st, cix, aK, cst, Kf, A, kx, sm, se);
+                else if (Kt == 2) v5_step<C, NB, R, 2>(st, cix, aK, cst, Kf, A, kx, sm, se);
+                else if (Kt == 4 || NB == 4) v5_step<C, NB, R, (NB < 4 ? NB : 4)>(st, cix, aK, cst, Kf, A, kx, sm, se);
+                else if (Kt == 8 || NB == 8) v5_step<C, NB, R, (NB < 8 ? NB : 8)>(st, cix, aK, cst, Kf, A, kx, sm, se);
+                else v5_step<C, NB, R, NB>(st, cix, aK, cst, Kf, A, kx, sm, se);
             }
 
-            // P = sum over rows of (row sum + detached share) (peptide-state-vector.cc:52-54)
+            // P = sum A over the last box + the rows' side sums
+            const int nlast = nbt[T - 1];
+            float rs[R];
             int kl = kNoExp;
 #pragma unroll
-            for (int k = 0; k < R; k++)
-                if (Sr[k] + dl[k] > 0.0f) kl = max(kl, kx[k]);
-            float part = 0.0f;
+            for (int k = 0; k < R; k++) {
+                rs[k] = 0.0f;
+#pragma unroll
+                for (int q = 0; q < NB; q++)
+                    if (q < nlast) rs[k] += A[k][q];
+                if (rs[k] > 0.0f && kx[k] != kNoExp) kl = max(kl, kx[k]);
+                if (sm[k] > 0.0) kl = max(kl, se[k]);
+            }
+            double part = 0.0;
 #pragma unroll
             for (int k = 0; k < R; k++) {
-                const float own = Sr[k] + dl[k];
-                if (own > 0.0f && kx[k] != kNoExp) part += ldexpf(own, kx[k] - kl);
+                if (rs[k] > 0.0f && kx[k] != kNoExp) part += ldexp((double)rs[k], kx[k] - kl);
+                if (sm[k] > 0.0) part += ldexp(sm[k], se[k] - kl);
             }
             const int K = group_max_int<G>(kl);
-            part = (K == kNoExp || kl == kNoExp) ? 0.0f : ldexpf(part, kl - K);
-            const float s = group_sum<float, G>(part);
-            const double logp = s > 0.0f ? log((double)s) + (double)K * 0.69314718055994530942 : -INFINITY;
+            part = (K == kNoExp || kl == kNoExp) ? 0.0 : ldexp(part, kl - K);
+            const double s = group_sum<double, G>(part);
+            const double logp = s > 0.0 ? log(s) + (double)K * 0.69314718055994530942 : -INFINITY;
             if (a.out_pair_logp && !cidx && p == 0) a.out_pair_logp[(size_t)r * a.S + i] = logp;
             acc.add(logp, exp(logp), kk, a.dsCount[i]);
             if (collect && p == 0) tie_note(&ties[group], acc.best_v, logp, kk, a.tie_abs, a.tie_rel);
