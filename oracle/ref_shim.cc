@@ -32,6 +32,7 @@
 #include "levmar.h"
 #include "parameterization/model/sequencing-model.h"
 #include "parameterization/settings/sequencing-settings.h"
+#include "simulation/generate-dye-track.h"
 #include "whatprot_amd.h"
 
 using namespace whatprot;
@@ -147,6 +148,23 @@ std::vector<Radiometry> make_rads(unsigned int T, unsigned int C, int N, const d
 }  // namespace
 
 extern "C" {
+
+// generate_dye_track (simulation/generate-dye-track.cc:29-110) run n times
+// on one std::default_random_engine seeded with seed (the reference seeds it
+// from the clock, util/time.cc:28-30): out[n][T][C].
+void wpref_generate_dye_tracks(const wp_model* m, unsigned int T, unsigned int C, const char* dye_string, int n,
+                               unsigned int seed, short* out) {
+    SequencingModel sm;
+    fill_model(m, C, &sm);
+    DyeSeq ds(C, std::string(dye_string));
+    std::default_random_engine gen(seed);
+    for (int k = 0; k < n; k++) {
+        DyeTrack dt(T, C);
+        generate_dye_track(sm, ds, T, C, &gen, &dt);
+        for (unsigned int t = 0; t < T; t++)
+            for (unsigned int c = 0; c < C; c++) out[((size_t)k * T + t) * C + c] = dt(t, c);
+    }
+}
 
 double wpref_pdf(const wp_model* m, int channel, double observed, const unsigned int* counts) {
     SequencingModel sm;

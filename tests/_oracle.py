@@ -213,6 +213,8 @@ class Reference:
                               c_void_p, c_void_p]
         L.wpref_kdtree_order.restype = None
         L.wpref_kdtree_order.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p]
+        L.wpref_generate_dye_tracks.restype = None
+        L.wpref_generate_dye_tracks.argtypes = [P(ModelC), c_uint, c_uint, ctypes.c_char_p, c_int, c_uint, c_void_p]
         L.wpref_classify.restype = None
         L.wpref_classify.argtypes = [P(ModelC), c_uint, c_uint, c_int, P(ctypes.c_char_p), c_void_p, c_void_p,
                                      c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p]
@@ -223,6 +225,14 @@ class Reference:
         if not os.path.exists(REF_SO):
             return None
         return cls(ctypes.CDLL(REF_SO))
+
+    def generate_dye_tracks(self, model, T, dye_string, n, seed):
+        """The reference's generate_dye_track n times from one seeded engine: [n, T, C] counts."""
+        C = model.num_channels
+        out = np.zeros((n, T, C), dtype=np.int16)
+        self.L.wpref_generate_dye_tracks(ctypes.byref(Oracle._m(model)), T, C, dye_string.encode(), n, seed,
+                                         _ptr(out))
+        return out
 
     def pdf(self, model, channel, observed, counts):
         m = Oracle._m(model)
