@@ -566,6 +566,50 @@ int wp_simulate_dye_tracks_gpu(const wp_model* seq_model, unsigned int num_times
                                short* out_track_counts, int* out_source_offsets,
                                int* out_source_ids, int* out_source_counts, int* out_source_hits);
 
+
+/* ---------------------------------------------------------------------------
+ * Text files. The reference's formats, read in parallel and written
+ * byte-identically (17 significant digits where the reference writes
+ * setprecision(17)).
+ * ------------------------------------------------------------------------- */
+typedef struct wp_radiometries {
+    int num_timesteps, num_channels, num_reads;
+    double* values; /* [read][t][c], divided by the model's mu_c; library-allocated */
+} wp_radiometries;
+
+/* read_radiometries (io/radiometries-io.cc:33-56): T, C, N, then the N*T*C
+ * intensities, each divided by its channel's mu in seq_model (NULL: by 1).
+ * Free with wp_free_radiometries. */
+int wp_read_radiometries(const char* path, const wp_model* seq_model, wp_radiometries* out);
+void wp_free_radiometries(wp_radiometries* r);
+
+/* write_radiometries (io/radiometries-io.cc:58-81): each value times mu_c of
+ * seq_model (NULL: 1), tab-separated, one read per line, setprecision(17). */
+int wp_write_radiometries(const char* path, const wp_model* seq_model, int num_timesteps, int num_channels,
+                          int num_reads, const double* values);
+
+typedef struct wp_dye_tracks {
+    int num_timesteps, num_channels, num_dye_tracks, num_sources;
+    short* counts;       /* [track][t][c] */
+    int* source_offsets; /* [num_dye_tracks + 1] */
+    int* source_ids;     /* [num_sources]: SourceCountHits id, count, hits */
+    int* source_counts;
+    int* source_hits;
+} wp_dye_tracks;
+
+/* read_dye_tracks (io/dye-tracks-io.cc:32-64). Free with wp_free_dye_tracks. */
+int wp_read_dye_tracks(const char* path, wp_dye_tracks* out);
+void wp_free_dye_tracks(wp_dye_tracks* t);
+
+/* write_dye_tracks (io/dye-tracks-io.cc:66-103). */
+int wp_write_dye_tracks(const char* path, int num_timesteps, int num_channels, int num_dye_tracks,
+                        const short* counts, const int* source_offsets, const int* source_ids,
+                        const int* source_counts, const int* source_hits);
+
+/* write_scored_classifications_raw (io/scored-classifications-io.cc:54-68):
+ * header, then "i,id,score" with the score at setprecision(17). */
+int wp_write_scored_classifications(const char* path, int num_reads, const int* ids, const double* scores);
+
 #ifdef __cplusplus
 }
 #endif

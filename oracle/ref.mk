@@ -12,7 +12,9 @@ REF_CC = $(filter-out %.test.cc,$(foreach d,$(REF_DIRS),$(wildcard $(REF_SRC)/$(
          $(REF_SRC)/util/range.cc $(REF_SRC)/util/time.cc \
          $(REF_SRC)/simulation/generate-dye-track.cc \
          $(REF_SRC)/simulation/generate-radiometry.cc \
-         $(REF_SRC)/simulation/generate-radiometries.cc
+         $(REF_SRC)/simulation/generate-radiometries.cc \
+         $(REF_SRC)/io/radiometries-io.cc $(REF_SRC)/io/dye-tracks-io.cc \
+         $(REF_SRC)/io/scored-classifications-io.cc
 LEVMAR_DIR = $(REF)/extern/levmar-2.6
 LEVMAR_C = $(addprefix $(LEVMAR_DIR)/,lm.c Axb.c misc.c lmlec.c lmbc.c lmblec.c lmbleic.c)
 REF_OUT = $(HERE)_ref

@@ -33,6 +33,9 @@
 #include "parameterization/model/sequencing-model.h"
 #include "parameterization/settings/sequencing-settings.h"
 #include "simulation/generate-dye-track.h"
+#include "io/dye-tracks-io.h"
+#include "io/radiometries-io.h"
+#include "io/scored-classifications-io.h"
 #include "whatprot_amd.h"
 
 using namespace whatprot;
@@ -164,6 +167,84 @@ void wpref_generate_dye_tracks(const wp_model* m, unsigned int T, unsigned int C
         for (unsigned int t = 0; t < T; t++)
             for (unsigned int c = 0; c < C; c++) out[((size_t)k * T + t) * C + c] = dt(t, c);
     }
+}
+
+// write_radiometries (io/radiometries-io.cc:58-81) of values already divided by mu.
+void wpref_write_radiometries(const char* path, const wp_model* m, unsigned int T, unsigned int C, int n,
+                              const double* values) {
+    SequencingModel sm;
+    fill_model(m, C, &sm);
+    std::vector<SourcedData<Radiometry, SourceCount<int>>> rads;
+    for (int i = 0; i < n; i++) {
+        Radiometry r(T, C);
+        for (unsigned int k = 0; k < T * C; k++) r.intensities[k] = values[(size_t)i * T * C + k];
+        rads.push_back(SourcedData<Radiometry, SourceCount<int>>(r, SourceCount<int>(0, 1)));
+    }
+    write_radiometries(std::string(path), sm, T, C, rads);
+}
+
+// read_radiometries (io/radiometries-io.cc:33-56): returns N; fills out[cap].
+int wpref_read_radiometries(const char* path, const wp_model* m, double* out, int cap) {
+    SequencingModel sm;
+    fill_model(m, (unsigned int)m->num_channels, &sm);
+    unsigned int T, C, N;
+    std::vector<Radiometry> rads;
+    read_radiometries(std::string(path), sm, &T, &C, &N, &rads);
+    for (unsigned int i = 0; i < N; i++)
+        for (unsigned int k = 0; k < T * C && (int)(i * T * C + k) < cap; k++) out[(size_t)i * T * C + k] = rads[i].intensities[k];
+    return (int)N;
+}
+
+// write_dye_tracks (io/dye-tracks-io.cc:66-103).
+void wpref_write_dye_tracks(const char* path, unsigned int T, unsigned int C, int n, const short* counts,
+                            const int* off, const int* ids, const int* cnts, const int* hits) {
+    std::vector<SourcedData<DyeTrack, SourceCountHitsList<int>>> tracks;
+    for (int i = 0; i < n; i++) {
+        DyeTrack dt(T, C);
+        for (unsigned int k = 0; k < T * C; k++) dt.counts[k] = counts[(size_t)i * T * C + k];
+        const int ns = off[i + 1] - off[i];
+        SourceCountHits<int>** src = new SourceCountHits<int>*[ns];
+        for (int j = 0; j < ns; j++) src[j] = new SourceCountHits<int>(ids[off[i] + j], cnts[off[i] + j], hits[off[i] + j]);
+        tracks.push_back(SourcedData<DyeTrack, SourceCountHitsList<int>>(dt, SourceCountHitsList<int>(ns, src)));
+    }
+    write_dye_tracks(std::string(path), T, C, tracks);
+}
+
+// read_dye_tracks (io/dye-tracks-io.cc:32-64): returns n and the number of
+// sources; fills counts[n*T*C], off[n+1] and the source triples (capacities
+// cap_t tracks, cap_s sources).
+int wpref_read_dye_tracks(const char* path, short* counts, int* off, int* ids, int* cnts, int* hits, int cap_t,
+                          int cap_s, int* out_sources) {
+    unsigned int T, C;
+    std::vector<SourcedData<DyeTrack, SourceCountHitsList<int>>> tracks;
+    read_dye_tracks(std::string(path), &T, &C, &tracks);
+    int s = 0;
+    if (cap_t >= 0) off[0] = 0;
+    for (size_t i = 0; i < tracks.size(); i++) {
+        if ((int)i < cap_t)
+            for (unsigned int k = 0; k < T * C; k++) counts[i * T * C + k] = tracks[i].value.counts[k];
+        for (int j = 0; j < tracks[i].source.num_sources; j++, s++) {
+            if (s < cap_s) {
+                ids[s] = tracks[i].source.sources[j]->source;
+                cnts[s] = tracks[i].source.sources[j]->count;
+                hits[s] = tracks[i].source.sources[j]->hits;
+            }
+        }
+        if ((int)i < cap_t) off[i + 1] = s;
+    }
+    *out_sources = s;
+    return (int)tracks.size();
+}
+
+// write_scored_classifications_raw (io/scored-classifications-io.cc:54-68).
+void wpref_write_scored(const char* path, int n, const int* ids, const double* scores) {
+    int* i2 = new int[n > 0 ? n : 1];
+    double* s2 = new double[n > 0 ? n : 1];
+    for (int i = 0; i < n; i++) {
+        i2[i] = ids[i];
+        s2[i] = scores[i];
+    }
+    write_scored_classifications_raw(std::string(path), n, i2, s2);  // frees i2, s2
 }
 
 double wpref_pdf(const wp_model* m, int channel, double observed, const unsigned int* counts) {

@@ -215,6 +215,16 @@ class Reference:
         L.wpref_kdtree_order.argtypes = [c_int, c_int, c_int, c_void_p, c_void_p]
         L.wpref_generate_dye_tracks.restype = None
         L.wpref_generate_dye_tracks.argtypes = [P(ModelC), c_uint, c_uint, ctypes.c_char_p, c_int, c_uint, c_void_p]
+        L.wpref_write_radiometries.restype = None
+        L.wpref_write_radiometries.argtypes = [ctypes.c_char_p, P(ModelC), c_uint, c_uint, c_int, c_void_p]
+        L.wpref_read_radiometries.restype = c_int
+        L.wpref_read_radiometries.argtypes = [ctypes.c_char_p, P(ModelC), c_void_p, c_int]
+        L.wpref_write_dye_tracks.restype = None
+        L.wpref_write_dye_tracks.argtypes = [ctypes.c_char_p, c_uint, c_uint, c_int] + [c_void_p] * 5
+        L.wpref_read_dye_tracks.restype = c_int
+        L.wpref_read_dye_tracks.argtypes = [ctypes.c_char_p] + [c_void_p] * 5 + [c_int, c_int, c_void_p]
+        L.wpref_write_scored.restype = None
+        L.wpref_write_scored.argtypes = [ctypes.c_char_p, c_int, c_void_p, c_void_p]
         L.wpref_classify.restype = None
         L.wpref_classify.argtypes = [P(ModelC), c_uint, c_uint, c_int, P(ctypes.c_char_p), c_void_p, c_void_p,
                                      c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p]
@@ -225,6 +235,36 @@ class Reference:
         if not os.path.exists(REF_SO):
             return None
         return cls(ctypes.CDLL(REF_SO))
+
+    def write_radiometries(self, path, model, rads):
+        r = np.ascontiguousarray(rads, dtype=np.float64)
+        N, T, C = r.shape
+        self.L.wpref_write_radiometries(str(path).encode(), ctypes.byref(Oracle._m(model)), T, C, N, _ptr(r))
+
+    def read_radiometries(self, path, model, T, C, cap):
+        out = np.zeros(cap * T * C)
+        n = self.L.wpref_read_radiometries(str(path).encode(), ctypes.byref(Oracle._m(model)), _ptr(out), out.size)
+        return out[:n * T * C].reshape(n, T, C)
+
+    def write_dye_tracks(self, path, tracks):
+        c, off, ids, cnt, hits = tracks._arrays()
+        self.L.wpref_write_dye_tracks(str(path).encode(), tracks.num_timesteps, tracks.num_channels, len(tracks),
+                                      _ptr(c), _ptr(off), _ptr(ids), _ptr(cnt), _ptr(hits))
+
+    def read_dye_tracks(self, path, T, C, cap_t, cap_s):
+        counts = np.zeros((cap_t, T, C), dtype=np.int16)
+        off = np.zeros(cap_t + 1, dtype=np.int32)
+        ids, cnt, hits = (np.zeros(cap_s, dtype=np.int32) for _ in range(3))
+        ns = ctypes.c_int()
+        n = self.L.wpref_read_dye_tracks(str(path).encode(), _ptr(counts), _ptr(off), _ptr(ids), _ptr(cnt),
+                                         _ptr(hits), cap_t, cap_s, ctypes.byref(ns))
+        k = ns.value
+        return counts[:n], off[:n + 1], ids[:k], cnt[:k], hits[:k]
+
+    def write_scored(self, path, ids, scores):
+        i = np.ascontiguousarray(ids, dtype=np.int32)
+        s = np.ascontiguousarray(scores, dtype=np.float64)
+        self.L.wpref_write_scored(str(path).encode(), len(i), _ptr(i), _ptr(s))
 
     def generate_dye_tracks(self, model, T, dye_string, n, seed):
         """The reference's generate_dye_track n times from one seeded engine: [n, T, C] counts."""

@@ -16,6 +16,7 @@ argument meaning follow whatprot's cc_code/src):
 All scoring runs in libwhatprot_amd.so on the GPU; there is no CPU path.
 """
 import ctypes
+import os
 import json
 import math
 import sys
@@ -25,8 +26,8 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _capi
-from ._capi import (WP_MAX_CHANNELS, WP_PRECISION_FP32, WP_PRECISION_FP64, FitSettingsC, ModelC,
-                    WhatprotError, check, fit_stats_len, lib)
+from ._capi import (WP_MAX_CHANNELS, WP_PRECISION_FP32, WP_PRECISION_FP64, DyeTracksC, FitSettingsC, ModelC,
+                    RadiometriesC, WhatprotError, check, fit_stats_len, lib)
 
 DBL_MAX = sys.float_info.max
 
@@ -208,35 +209,36 @@ def write_dye_seqs(path, C, entries: Sequence[DyeSeqEntry]):
 
 
 def read_radiometries(path, seq_model: SequencingModel):
-    """radiometries-io.cc:33-56: T, C, N then N rows of T*C values (t outer,
-    c inner), each divided by that channel's mu. Returns (T, C, array[N,T,C])."""
-    with open(path) as f:
-        T = int(f.readline())
-        C = int(f.readline())
-        N = int(f.readline())
-        data = np.loadtxt(f, dtype=np.float64, ndmin=2) if N else np.zeros((0, T * C))
-    data = data.reshape(N, T, C)
-    mus = np.array([cm.mu for cm in seq_model.channel_models[:C]], dtype=np.float64)
-    return T, C, np.ascontiguousarray(data / mus[None, None, :])
+    """read_radiometries (radiometries-io.cc:33-56) through wp_read_radiometries:
+    T, C, N then N rows of T*C values (t outer, c inner), each divided by that
+    channel's mu. Returns (T, C, array[N,T,C])."""
+    out = RadiometriesC()
+    m = seq_model.to_c()
+    check(lib().wp_read_radiometries(os.fsencode(path), ctypes.byref(m), ctypes.byref(out)))
+    try:
+        T, C, N = out.num_timesteps, out.num_channels, out.num_reads
+        data = np.ctypeslib.as_array(out.values, shape=(max(N * T * C, 1),))[:N * T * C].copy()
+    finally:
+        lib().wp_free_radiometries(ctypes.byref(out))
+    return T, C, data.reshape(N, T, C)
 
 
 def write_radiometries(path, seq_model: SequencingModel, rads):
-    """radiometries-io.cc:58-82 (values multiplied back by mu, 17 digits)."""
-    N, T, C = rads.shape
-    mus = np.array([cm.mu for cm in seq_model.channel_models[:C]])
-    with open(path, "w") as f:
-        f.write(f"{T}\n{C}\n{N}\n")
-        for r in range(N):
-            vals = (rads[r] * mus[None, :]).reshape(-1)
-            f.write("\t".join(repr(float(v)) if False else f"{v:.17g}" for v in vals) + "\n")
+    """write_radiometries (radiometries-io.cc:58-82) through
+    wp_write_radiometries: values multiplied back by mu, 17 digits."""
+    r = np.ascontiguousarray(rads, dtype=np.float64)
+    N, T, C = r.shape
+    m = seq_model.to_c()
+    check(lib().wp_write_radiometries(os.fsencode(path), ctypes.byref(m), T, C, N, r.ctypes.data))
 
 
 def write_scored_classifications(path, results: Sequence[ScoredClassification]):
-    """scored-classifications-io.cc:30-68."""
-    with open(path, "w") as f:
-        f.write("radmat_iz,best_pep_iz,best_pep_score\n")
-        for i, r in enumerate(results):
-            f.write(f"{i},{r.id},{r.adjusted_score:.17g}\n")
+    """write_scored_classifications (scored-classifications-io.cc:30-68)."""
+    ids = np.ascontiguousarray([r.id for r in results], dtype=np.int32)
+    sc = np.ascontiguousarray([r.adjusted_score for r in results], dtype=np.float64)
+    check(lib().wp_write_scored_classifications(os.fsencode(path), len(results),
+                                                ids.ctypes.data if len(ids) else None,
+                                                sc.ctypes.data if len(sc) else None))
 
 
 def generate_dye_seqs(C, n, len_min, len_max, dyes_min, dyes_max, seed):
@@ -615,44 +617,31 @@ class DyeTracks:
 
 
 def read_dye_tracks(path) -> DyeTracks:
-    """read_dye_tracks (io/dye-tracks-io.cc:32-62): T, C, n, then per track
-    T*C counts, the number of sources and (id, count, hits) per source."""
-    with open(path) as f:
-        tok = f.read().split()
-    pos = 0
+    """read_dye_tracks (io/dye-tracks-io.cc:32-62) through wp_read_dye_tracks:
+    T, C, n, then per track T*C counts, the number of sources and
+    (id, count, hits) per source."""
+    out = DyeTracksC()
+    check(lib().wp_read_dye_tracks(os.fsencode(path), ctypes.byref(out)))
+    try:
+        T, C, n, ns = out.num_timesteps, out.num_channels, out.num_dye_tracks, out.num_sources
 
-    def nxt():
-        nonlocal pos
-        pos += 1
-        return int(tok[pos - 1])
+        def arr(p, k, dt):
+            return np.ctypeslib.as_array(p, shape=(max(k, 1),))[:k].astype(dt, copy=True)
 
-    T, C, n = nxt(), nxt(), nxt()
-    counts = np.zeros((n, T, C), dtype=np.int16)
-    offsets = [0]
-    ids, cnts, hits = [], [], []
-    for i in range(n):
-        for j in range(T * C):
-            counts[i].flat[j] = nxt()
-        for _ in range(nxt()):
-            ids.append(nxt())
-            cnts.append(nxt())
-            hits.append(nxt())
-        offsets.append(len(ids))
-    return DyeTracks(T, C, counts, np.array(offsets, np.int32), np.array(ids, np.int32), np.array(cnts, np.int32),
-                     np.array(hits, np.int32))
+        tracks = DyeTracks(T, C, arr(out.counts, n * T * C, np.int16).reshape(n, T, C),
+                           arr(out.source_offsets, n + 1, np.int32), arr(out.source_ids, ns, np.int32),
+                           arr(out.source_counts, ns, np.int32), arr(out.source_hits, ns, np.int32))
+    finally:
+        lib().wp_free_dye_tracks(ctypes.byref(out))
+    return tracks
 
 
 def write_dye_tracks(path, tracks: DyeTracks):
-    """write_dye_tracks (io/dye-tracks-io.cc:64-103), tab separated."""
-    with open(path, "w") as f:
-        f.write(f"{tracks.num_timesteps}\n{tracks.num_channels}\n{len(tracks)}\n")
-        for i in range(len(tracks)):
-            parts = [str(int(x)) for x in tracks.counts[i].ravel()]
-            b, e = int(tracks.offsets[i]), int(tracks.offsets[i + 1])
-            src = []
-            for j in range(b, e):
-                src.append(f"{int(tracks.ids[j])}\t{int(tracks.src_counts[j])}\t{int(tracks.hits[j])}")
-            f.write("\t".join(parts) + "\t" + str(e - b) + "\t" + "\t".join(src) + "\n")
+    """write_dye_tracks (io/dye-tracks-io.cc:64-103) through wp_write_dye_tracks."""
+    c, off, ids, cnt, hits = tracks._arrays()
+    check(lib().wp_write_dye_tracks(os.fsencode(path), tracks.num_timesteps, tracks.num_channels, len(tracks),
+                                    c.ctypes.data, off.ctypes.data, ids.ctypes.data, cnt.ctypes.data,
+                                    hits.ctypes.data))
 
 
 def simulate_dye_tracks(seq_model: SequencingModel, T, dye_seqs: Sequence[DyeSeqEntry], per_peptide, seed,

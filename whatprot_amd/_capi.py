@@ -64,6 +64,18 @@ def fit_stats_len(T, C):
 _lib = None
 
 
+class RadiometriesC(ctypes.Structure):
+    _fields_ = [("num_timesteps", ctypes.c_int), ("num_channels", ctypes.c_int), ("num_reads", ctypes.c_int),
+                ("values", ctypes.POINTER(ctypes.c_double))]
+
+
+class DyeTracksC(ctypes.Structure):
+    _fields_ = [("num_timesteps", ctypes.c_int), ("num_channels", ctypes.c_int), ("num_dye_tracks", ctypes.c_int),
+                ("num_sources", ctypes.c_int), ("counts", ctypes.POINTER(ctypes.c_short)),
+                ("source_offsets", ctypes.POINTER(ctypes.c_int)), ("source_ids", ctypes.POINTER(ctypes.c_int)),
+                ("source_counts", ctypes.POINTER(ctypes.c_int)), ("source_hits", ctypes.POINTER(ctypes.c_int))]
+
+
 class WhatprotError(RuntimeError):
     pass
 
@@ -117,6 +129,20 @@ def lib():
     L.wp_hmm_get_variants.argtypes = [c_void_p, c_void_p, c_void_p, c_int]
     L.wp_hmm_set_graph.restype = c_int
     L.wp_hmm_set_graph.argtypes = [c_void_p, c_int]
+    L.wp_read_radiometries.restype = c_int
+    L.wp_read_radiometries.argtypes = [ctypes.c_char_p, c_void_p, P(RadiometriesC)]
+    L.wp_free_radiometries.restype = None
+    L.wp_free_radiometries.argtypes = [P(RadiometriesC)]
+    L.wp_write_radiometries.restype = c_int
+    L.wp_write_radiometries.argtypes = [ctypes.c_char_p, c_void_p, c_int, c_int, c_int, c_void_p]
+    L.wp_read_dye_tracks.restype = c_int
+    L.wp_read_dye_tracks.argtypes = [ctypes.c_char_p, P(DyeTracksC)]
+    L.wp_free_dye_tracks.restype = None
+    L.wp_free_dye_tracks.argtypes = [P(DyeTracksC)]
+    L.wp_write_dye_tracks.restype = c_int
+    L.wp_write_dye_tracks.argtypes = [ctypes.c_char_p, c_int, c_int, c_int] + [c_void_p] * 5
+    L.wp_write_scored_classifications.restype = c_int
+    L.wp_write_scored_classifications.argtypes = [ctypes.c_char_p, c_int, c_void_p, c_void_p]
     L.wp_hmm_set_refine.restype = c_int
     L.wp_hmm_set_refine.argtypes = [c_void_p, c_int, c_double, c_double]
     L.wp_hmm_refine_stats.restype = c_int

@@ -253,71 +253,45 @@ DyeSeqs read_dye_seqs(const std::string& path) {
     return d;
 }
 
+void check(int rc) {
+    if (rc != WP_OK) throw std::runtime_error(std::string("whatprot_amd: ") + wp_last_error());
+}
+
 struct Rads {
     int T = 0, C = 0, N = 0;
     std::vector<double> v;
 };
 
-// read_radiometries (radiometries-io.cc:33-56): T, C, N, then N rows of T*C
-// intensities, each divided by its channel's mu of the model as read.
+// read_radiometries (radiometries-io.cc:33-56) through the library's parallel
+// reader: T, C, N, then N rows of T*C intensities, each divided by its
+// channel's mu of the model as read.
 Rads read_radiometries(const std::string& path, const wp_model& true_model) {
-    std::ifstream f(path);
-    if (!f) throw std::runtime_error("cannot open " + path);
-    Rads r;
-    f >> r.T >> r.C >> r.N;
-    r.v.resize((size_t)r.N * r.T * r.C);
-    for (int i = 0; i < r.N; i++)
-        for (int t = 0; t < r.T; t++)
-            for (int c = 0; c < r.C; c++) {
-                double x;
-                f >> x;
-                r.v[((size_t)i * r.T + t) * r.C + c] = x / true_model.channels[c].mu;
-            }
-    return r;
+    wp_radiometries r;
+    check(wp_read_radiometries(path.c_str(), &true_model, &r));
+    Rads o;
+    o.T = r.num_timesteps;
+    o.C = r.num_channels;
+    o.N = r.num_reads;
+    o.v.assign(r.values, r.values + (size_t)o.N * o.T * o.C);
+    wp_free_radiometries(&r);
+    return o;
 }
 
+// read_dye_tracks (dye-tracks-io.cc:32-62) through the library's parallel
+// reader; the arrays stay in the library's allocation.
 struct Tracks {
-    int T = 0, C = 0, n = 0;
-    std::vector<short> counts;
-    std::vector<int> off{0}, ids, cnts, hits;
+    wp_dye_tracks d;
+    Tracks() { std::memset(&d, 0, sizeof(d)); }
+    Tracks(const Tracks&) = delete;
+    Tracks& operator=(const Tracks&) = delete;
+    ~Tracks() { wp_free_dye_tracks(&d); }
 };
 
-// read_dye_tracks (dye-tracks-io.cc:32-62).
-Tracks read_dye_tracks(const std::string& path) {
-    std::ifstream f(path);
-    if (!f) throw std::runtime_error("cannot open " + path);
-    Tracks t;
-    f >> t.T >> t.C >> t.n;
-    t.counts.resize((size_t)t.n * t.T * t.C);
-    for (int i = 0; i < t.n; i++) {
-        for (int j = 0; j < t.T * t.C; j++) f >> t.counts[(size_t)i * t.T * t.C + j];
-        int ns;
-        f >> ns;
-        for (int j = 0; j < ns; j++) {
-            int id, c, h;
-            f >> id >> c >> h;
-            t.ids.push_back(id);
-            t.cnts.push_back(c);
-            t.hits.push_back(h);
-        }
-        t.off.push_back((int)t.ids.size());
-    }
-    return t;
-}
+void read_dye_tracks(const std::string& path, Tracks* t) { check(wp_read_dye_tracks(path.c_str(), &t->d)); }
 
 // write_scored_classifications_raw (scored-classifications-io.cc:54-68).
 void write_results(const std::string& path, const std::vector<int>& ids, const std::vector<double>& scores) {
-    std::ofstream f(path);
-    f << "radmat_iz,best_pep_iz,best_pep_score\n";
-    for (size_t i = 0; i < ids.size(); i++) {
-        f << i << ",";
-        f << ids[i] << ",";
-        f << std::setprecision(17) << scores[i] << "\n";
-    }
-}
-
-void check(int rc) {
-    if (rc != WP_OK) throw std::runtime_error(std::string("whatprot_amd: ") + wp_last_error());
+    check(wp_write_scored_classifications(path.c_str(), (int)ids.size(), ids.data(), scores.data()));
 }
 
 double now() {
@@ -453,24 +427,24 @@ int classify_hmm(const Args& a) {
     return 0;
 }
 
-Tracks load_tracks(const Args& a, int C) {
-    Tracks t = read_dye_tracks(a.get('T'));
-    if (t.C != C) throw std::runtime_error("dye-tracks and model disagree on the number of channels");
-    std::cout << "Read " << t.n << " dye tracks.\n";
-    return t;
+void load_tracks(const Args& a, int C, Tracks* t) {
+    read_dye_tracks(a.get('T'), t);
+    if (t->d.num_channels != C) throw std::runtime_error("dye-tracks and model disagree on the number of channels");
+    std::cout << "Read " << t->d.num_dye_tracks << " dye tracks.\n";
 }
 
 int classify_nn(const Args& a) {
     double t0 = now();
     wp_model truth = read_seq_model(a.get('P')), model;
     wp_model_with_mu_as_one(&truth, &model);
-    Tracks tr = load_tracks(a, model.num_channels);
+    Tracks tr;
+    load_tracks(a, model.num_channels, &tr);
     Rads r = read_radiometries(a.get('R'), truth);
     need_rads_shape(r, model.num_channels);
     wp_nn_classifier* nn = nullptr;
-    check(wp_nn_classifier_create(tr.T, tr.C, &model, std::stoi(a.get('k')), std::stod(a.get('s')), tr.n,
-                                  tr.counts.data(), tr.off.data(), tr.ids.data(), tr.cnts.data(), tr.hits.data(),
-                                  &nn));
+    check(wp_nn_classifier_create(tr.d.num_timesteps, tr.d.num_channels, &model, std::stoi(a.get('k')),
+                                  std::stod(a.get('s')), tr.d.num_dye_tracks, tr.d.counts, tr.d.source_offsets,
+                                  tr.d.source_ids, tr.d.source_counts, tr.d.source_hits, &nn));
     std::vector<int> id(r.N);
     std::vector<double> sc(r.N), tot(r.N), adj(r.N);
     double t1 = now();
@@ -489,15 +463,17 @@ int classify_hybrid(const Args& a) {
     wp_model_with_mu_as_one(&truth, &model);
     double cutoff = a.has('p') ? std::stod(a.get('p')) : DBL_MAX;
     DyeSeqs ds = read_dye_seqs(a.get('S'));
-    Tracks tr = load_tracks(a, model.num_channels);
+    Tracks tr;
+    load_tracks(a, model.num_channels, &tr);
     Rads r = read_radiometries(a.get('R'), truth);
     need_rads_shape(r, model.num_channels);
     auto strs = cstrs(ds.strings);
     wp_hybrid_classifier* hc = nullptr;
-    check(wp_hybrid_classifier_create(tr.T, tr.C, &model, cutoff, std::stoi(a.get('k')), std::stod(a.get('s')), tr.n,
-                                      tr.counts.data(), tr.off.data(), tr.ids.data(), tr.cnts.data(),
-                                      tr.hits.data(), std::stoi(a.get('H')), (int)ds.strings.size(), strs.data(),
-                                      ds.counts.data(), ds.ids.data(), precision_of(a), &hc));
+    check(wp_hybrid_classifier_create(tr.d.num_timesteps, tr.d.num_channels, &model, cutoff, std::stoi(a.get('k')),
+                                      std::stod(a.get('s')), tr.d.num_dye_tracks, tr.d.counts, tr.d.source_offsets,
+                                      tr.d.source_ids, tr.d.source_counts, tr.d.source_hits, std::stoi(a.get('H')),
+                                      (int)ds.strings.size(), strs.data(), ds.counts.data(), ds.ids.data(),
+                                      precision_of(a), &hc));
     std::vector<int> id(r.N);
     std::vector<double> adj(r.N);
     double t1 = now();
@@ -592,15 +568,7 @@ int simulate(const Args& a, bool rad) {
         std::vector<int> idx(g);
         check(wp_simulate_radiometries_gpu(&truth, T, C, (int)ds.strings.size(), strs.data(), ds.counts.data(), g, a.seed,
                                        v.data(), idx.data()));
-        std::ofstream f(a.get('R'));
-        f << T << "\n" << C << "\n" << g << "\n";
-        for (int i = 0; i < g; i++) {
-            for (int j = 0; j < T * C; j++) {
-                char buf[64];
-                std::snprintf(buf, sizeof(buf), "%.17g", v[(size_t)i * T * C + j]);
-                f << buf << (j + 1 < T * C ? "\t" : "\n");
-            }
-        }
+        check(wp_write_radiometries(a.get('R').c_str(), nullptr, T, C, g, v.data()));
         std::ofstream y(a.get('Y'));  // write_ys (radiometries-io.cc:104-114)
         y << g << "\n";
         for (int i = 0; i < g; i++) y << ds.ids[idx[i]] << "\n";
@@ -615,17 +583,9 @@ int simulate(const Args& a, bool rad) {
     check(wp_simulate_dye_tracks_gpu(&truth, T, C, (int)ds.strings.size(), strs.data(), ds.counts.data(), ds.ids.data(), g,
                                      a.seed, cap, cap, &n, &ns, counts.data(), off.data(), ids.data(), cnts.data(),
                                      hits.data()));
-    std::ofstream f(a.get('T'));  // write_dye_tracks (dye-tracks-io.cc:75-103)
-    f << T << "\n" << C << "\n" << n << "\n";
-    for (int i = 0; i < n; i++) {
-        for (int j = 0; j < T * C; j++) f << counts[(size_t)i * T * C + j] << "\t";
-        f << off[i + 1] - off[i] << "\t";
-        for (int j = off[i]; j < off[i + 1]; j++) {
-            f << ids[j] << "\t" << cnts[j] << "\t" << hits[j];
-            if (j + 1 < off[i + 1]) f << "\t";
-        }
-        f << "\n";
-    }
+    // write_dye_tracks (dye-tracks-io.cc:66-103)
+    check(wp_write_dye_tracks(a.get('T').c_str(), T, C, n, counts.data(), off.data(), ids.data(), cnts.data(),
+                              hits.data()));
     return 0;
 }
 
