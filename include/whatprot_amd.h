@@ -466,6 +466,13 @@ int wp_hmm_fit(wp_hmm_fitter* fitter, const wp_model* seq_model, const wp_fit_se
                double stopping_threshold, double max_runtime, int max_iterations,
                wp_model* out_model, double* out_step_size, double* out_log_l, int* out_iterations);
 
+/* Bootstrap rounds in one batch (no reference counterpart; default on): the
+ * rounds' EM loops run in lockstep and every iteration's E-steps of all
+ * running rounds are one launch, each round's statistics identical bit for
+ * bit to a launch of its own; the wall-clock limit then counts from the start
+ * of the batch. enable 0 runs the rounds one after another. */
+int wp_hmm_fitter_set_batched(wp_hmm_fitter* fitter, int enable);
+
 /* Bootstrap fit (fitters/bootstrap-fit.cc:37-173, as run by run-fit.cc:
  * 107-130): the fit of wp_hmm_fit on every read (out_best_model,
  * out_step_size, out_log_l), then num_rounds fits from seq_model on

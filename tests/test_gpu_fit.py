@@ -219,3 +219,28 @@ def test_estep_zero_reads(gpu, precision):
     g = w.HMMFitter(T, C, ds, rads, precision)
     st2, slp2 = g.estep(model)
     assert np.isfinite(slp2) and slp2 != 0.0 and st2.any()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batched_bootstrap_equals_sequential_rounds(gpu, precision):
+    """All rounds' E-steps in one launch per EM iteration give every round the
+    models of its own run bit for bit (same reads, same accumulation order);
+    wall times of both printed."""
+    import time
+
+    C, T, ds = 3, 12, "0.1.2..1.0..2"
+    truth = w.SequencingModel.uniform(C, p_edman_failure=0.08, p_bleach=0.04, p_dud=0.1, p_detach=0.03)
+    rads, _ = w.simulate_radiometries(truth, T, [ds], 20000, 41, gpu=True)
+    start = w.SequencingModel.uniform(C)
+    f = w.HMMFitter(T, C, ds, rads, precision)
+    res = {}
+    for batched in (True, False):
+        f.set_batched(batched)
+        t0 = time.perf_counter()
+        res[batched] = f.bootstrap_fit(start, 1e-4, 8, 0.9, seed=5)
+        print(f"precision {precision} batched={batched}: {time.perf_counter() - t0:.2f} s", flush=True)
+    a, b = res[True], res[False]
+    assert np.array_equal(a["log_ls"], b["log_ls"])
+    for ma, mb in zip(a["models"], b["models"]):
+        assert _params(ma) == _params(mb)
+    assert _params(a["ci_min"]) == _params(b["ci_min"]) and _params(a["ci_max"]) == _params(b["ci_max"])

@@ -545,6 +545,10 @@ class HMMFitter:
                                ctypes.byref(out), ctypes.byref(step), ctypes.byref(log_l), ctypes.byref(it)))
         return SequencingModel.from_c(out), step.value, log_l.value, it.value
 
+    def set_batched(self, enable=True):
+        """Bootstrap rounds as one batch (default) or one after another."""
+        check(lib().wp_hmm_fitter_set_batched(self._h, 1 if enable else 0))
+
     def bootstrap_fit(self, seq_model: SequencingModel, stopping_threshold, num_rounds, confidence_interval,
                       seed=1, max_runtime=float("inf"), settings: Optional[FitSettings] = None):
         """bootstrap_fit (fitters/bootstrap-fit.cc:37-173): returns a dict with

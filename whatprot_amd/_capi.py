@@ -143,6 +143,8 @@ def lib():
     L.wp_write_dye_tracks.argtypes = [ctypes.c_char_p, c_int, c_int, c_int] + [c_void_p] * 5
     L.wp_write_scored_classifications.restype = c_int
     L.wp_write_scored_classifications.argtypes = [ctypes.c_char_p, c_int, c_void_p, c_void_p]
+    L.wp_hmm_fitter_set_batched.restype = c_int
+    L.wp_hmm_fitter_set_batched.argtypes = [c_void_p, c_int]
     L.wp_hmm_set_refine.restype = c_int
     L.wp_hmm_set_refine.argtypes = [c_void_p, c_int, c_double, c_double]
     L.wp_hmm_refine_stats.restype = c_int

@@ -31,7 +31,16 @@ struct wp_hmm_fitter {
     size_t hist_stride = 0;
     double* d_partial = nullptr;
     double* d_out = nullptr;
-    int* d_weight = nullptr;  // bootstrap multiplicities [N]
+    int* d_weight = nullptr;  // bootstrap multiplicities [rounds][N]
+    int weight_rows = 0;
+    // batched bootstrap: every active round's E-step in one launch
+    int batched = 1;
+    int cap_slots = 0;  // capacity of the arrays below
+    wp::DevModel* d_models = nullptr;
+    int* d_slot_round = nullptr;
+    double* d_binom_b = nullptr;
+    double* d_partial_b = nullptr;
+    double* d_out_b = nullptr;
 };
 
 namespace {
@@ -47,6 +56,11 @@ void free_fitter(wp_hmm_fitter* f) {
     hipFree(f->d_partial);
     hipFree(f->d_out);
     hipFree(f->d_weight);
+    hipFree(f->d_models);
+    hipFree(f->d_slot_round);
+    hipFree(f->d_binom_b);
+    hipFree(f->d_partial_b);
+    hipFree(f->d_out_b);
     delete f;
 }
 
@@ -128,6 +142,9 @@ int estep(wp_hmm_fitter* f, const wp_model* model, double* stats, double* sum_lo
     a.partial = f->d_partial;
     a.L = f->L;
     a.weight = d_weight;
+    a.nslots = 1;
+    a.models = nullptr;
+    a.slot_round = nullptr;
     std::vector<double> out((size_t)f->L + 1, 0.0);
     if (f->N > 0) {
         WP_HIP(wp::launch_fit_estep(a, f->precision, f->grid, f->d_out, nullptr));
@@ -135,6 +152,71 @@ int estep(wp_hmm_fitter* f, const wp_model* model, double* stats, double* sum_lo
     }
     if (stats) std::memcpy(stats, out.data(), sizeof(double) * f->L);
     if (sum_log_p) *sum_log_p = out[f->L];
+    return WP_OK;
+}
+
+// The E-steps of several bootstrap rounds in one launch: slot k runs under
+// models[k] with weight row rounds[k] of f->d_weight; out[k][0..L] as
+// estep's stats and sum of ln P. Each slot's result is estep's bit for bit.
+int estep_batch(wp_hmm_fitter* f, const std::vector<wp_model>& models, const std::vector<int>& rounds,
+                std::vector<double>* out) {
+    const int K = (int)models.size();
+    out->assign((size_t)K * (f->L + 1), 0.0);
+    if (K == 0 || f->N == 0) return WP_OK;
+    if (K > f->cap_slots) {
+        hipFree(f->d_models);
+        hipFree(f->d_slot_round);
+        hipFree(f->d_binom_b);
+        hipFree(f->d_partial_b);
+        hipFree(f->d_out_b);
+        f->d_models = nullptr;
+        f->d_slot_round = nullptr;
+        f->d_binom_b = f->d_partial_b = f->d_out_b = nullptr;
+        f->cap_slots = 0;
+        WP_HIP(hipMalloc(&f->d_models, sizeof(wp::DevModel) * K));
+        WP_HIP(hipMalloc(&f->d_slot_round, sizeof(int) * K));
+        WP_HIP(hipMalloc(&f->d_binom_b, sizeof(double) * (size_t)K * 2 * f->C * f->tri));
+        WP_HIP(hipMalloc(&f->d_partial_b, sizeof(double) * (size_t)K * f->grid * (f->L + 1)));
+        WP_HIP(hipMalloc(&f->d_out_b, sizeof(double) * (size_t)K * (f->L + 1)));
+        f->cap_slots = K;
+    }
+    std::vector<wp::DevModel> dm(K);
+    std::vector<double> binom((size_t)K * 2 * f->C * f->tri);
+    for (int k = 0; k < K; k++) {
+        if (models[k].num_channels != f->C) {
+            wp::set_error("model channels != fitter channels");
+            return WP_ERR_ARG;
+        }
+        wp::fill_dev_model(models[k], f->T, f->C, f->M, DBL_MAX, &dm[k]);
+        wp::binomial_tables(models[k], f->C, f->M, binom.data() + (size_t)k * 2 * f->C * f->tri);
+    }
+    WP_HIP(hipMemcpy(f->d_models, dm.data(), sizeof(wp::DevModel) * K, hipMemcpyHostToDevice));
+    WP_HIP(hipMemcpy(f->d_slot_round, rounds.data(), sizeof(int) * K, hipMemcpyHostToDevice));
+    WP_HIP(hipMemcpy(f->d_binom_b, binom.data(), sizeof(double) * binom.size(), hipMemcpyHostToDevice));
+    wp::FitArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.m = dm[0];  // T and C for the launch; each slot reads its own model
+    a.N = f->N;
+    a.rads = f->d_rads;
+    a.NB = f->NB;
+    for (int c = 0; c < f->C; c++) a.D[c] = f->D[c];
+    a.crd = f->d_crd;
+    a.fbase = f->d_fbase;
+    a.nfib = f->nfib;
+    a.seq = f->d_seq;
+    a.track = f->d_track;
+    a.binom = f->d_binom_b;
+    a.tri = f->tri;
+    a.hist = f->d_hist;
+    a.hist_stride = f->hist_stride;
+    a.partial = f->d_partial_b;
+    a.L = f->L;
+    a.weight = f->d_weight;
+    a.nslots = K;
+    a.models = f->d_models;
+    a.slot_round = f->d_slot_round;
+    WP_HIP(wp::launch_fit_estep(a, f->precision, f->grid, f->d_out_b, nullptr));
+    WP_HIP(hipMemcpy(out->data(), f->d_out_b, sizeof(double) * out->size(), hipMemcpyDeviceToHost));
     return WP_OK;
 }
 
@@ -371,6 +453,57 @@ int run_em(wp_hmm_fitter* f, const wp_model* model, const wp_fit_settings* setti
     return WP_OK;
 }
 
+// The EM loops of R bootstrap rounds in lockstep (each round as run_em with
+// weight row i): one batched E-step launch per iteration over the rounds
+// still running, then each round's M-step. Every round follows the same
+// sequence of models as its own run_em, bit for bit; the wall-clock limit
+// counts from the start of the batch.
+int run_em_rounds(wp_hmm_fitter* f, const wp_model* model, const wp_fit_settings* settings,
+                  double stopping_threshold, double max_runtime, int R, wp_model* out_models, double* out_steps,
+                  double* out_log_ls) {
+    std::string ds;
+    for (int i = 0; i < f->len; i++) ds += f->seq[i] < 0 ? '.' : (char)('0' + f->seq[i]);
+    std::vector<wp_model> sm(R, *model), next(R, *model);
+    std::vector<double> step(R, 0.0), out;
+    std::vector<char> done(R, 0);
+    auto start = std::chrono::steady_clock::now();
+    int left = R;
+    while (left > 0) {
+        std::vector<wp_model> ms;
+        std::vector<int> rs;
+        for (int i = 0; i < R; i++)
+            if (!done[i]) {
+                ms.push_back(sm[i]);
+                rs.push_back(i);
+            }
+        int rc = estep_batch(f, ms, rs, &out);
+        if (rc) return rc;
+        const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        for (size_t k = 0; k < rs.size(); k++) {
+            const int i = rs[k];
+            rc = wp_fit_mstep(f->T, f->C, ds.c_str(), &sm[i], f->N, out.data() + k * (f->L + 1), settings, &next[i],
+                              &step[i]);
+            if (rc) return rc;
+            if (step[i] < stopping_threshold || elapsed > max_runtime) {
+                done[i] = 1;
+                left--;
+            } else {
+                sm[i] = next[i];
+            }
+        }
+    }
+    std::vector<int> all(R);
+    for (int i = 0; i < R; i++) all[i] = i;
+    int rc = estep_batch(f, next, all, &out);  // HMMFitter::log_likelihood under each fitted model
+    if (rc) return rc;
+    for (int i = 0; i < R; i++) {
+        out_models[i] = next[i];
+        out_steps[i] = step[i];
+        out_log_ls[i] = out[(size_t)i * (f->L + 1) + f->L];
+    }
+    return WP_OK;
+}
+
 // Order statistic i of one parameter over the rounds' models.
 double order_stat(std::vector<double> v, size_t i) {
     std::sort(v.begin(), v.end());
@@ -390,6 +523,15 @@ int wp_hmm_fit(wp_hmm_fitter* f, const wp_model* model, const wp_fit_settings* s
     }
     return run_em(f, model, settings, stopping_threshold, max_runtime, max_iterations, nullptr, out_model,
                   out_step_size, out_log_l, out_iterations);
+}
+
+int wp_hmm_fitter_set_batched(wp_hmm_fitter* f, int enable) {
+    if (!f) {
+        wp::set_error("null fitter");
+        return WP_ERR_ARG;
+    }
+    f->batched = enable ? 1 : 0;
+    return WP_OK;
 }
 
 int wp_fit_bootstrap_counts(int num_reads, uint64_t seed, int round, int* out_counts) {
@@ -418,20 +560,32 @@ int wp_hmm_fit_bootstrap(wp_hmm_fitter* f, const wp_model* model, const wp_fit_s
                     out_log_l, nullptr);
     if (rc) return rc;
     if (num_rounds == 0) return WP_OK;
-    if (!f->d_weight) {
-        WP_HIP(hipMalloc(&f->d_weight, sizeof(int) * std::max(f->N, 1)));
+    // bootstrap-fit.cc:76-93: each round a resample of N reads with
+    // replacement, as per-read multiplicities, all rounds resident
+    if (num_rounds > f->weight_rows) {
+        hipFree(f->d_weight);
+        f->d_weight = nullptr;
+        f->weight_rows = 0;
+        WP_HIP(hipMalloc(&f->d_weight, sizeof(int) * (size_t)num_rounds * std::max(f->N, 1)));
+        f->weight_rows = num_rounds;
     }
-    std::vector<int> w(std::max(f->N, 1));
+    std::vector<int> w((size_t)num_rounds * std::max(f->N, 1));
+    for (int i = 0; i < num_rounds; i++) wp_fit_bootstrap_counts(f->N, seed, i, w.data() + (size_t)i * f->N);
+    WP_HIP(hipMemcpy(f->d_weight, w.data(), sizeof(int) * (size_t)num_rounds * f->N, hipMemcpyHostToDevice));
     double worst = 0.0;
-    for (int i = 0; i < num_rounds; i++) {  // bootstrap-fit.cc:76-93: a resample of N reads with replacement
-        wp_fit_bootstrap_counts(f->N, seed, i, w.data());
-        WP_HIP(hipMemcpy(f->d_weight, w.data(), sizeof(int) * f->N, hipMemcpyHostToDevice));
-        double step = 0.0;
-        rc = run_em(f, model, settings, stopping_threshold, max_runtime, 0, f->d_weight, &out_models[i], &step,
-                    &out_log_ls[i], nullptr);
+    std::vector<double> steps(num_rounds, 0.0);
+    if (f->batched) {
+        rc = run_em_rounds(f, model, settings, stopping_threshold, max_runtime, num_rounds, out_models, steps.data(),
+                           out_log_ls);
         if (rc) return rc;
-        worst = std::max(worst, step);
+    } else {
+        for (int i = 0; i < num_rounds; i++) {
+            rc = run_em(f, model, settings, stopping_threshold, max_runtime, 0, f->d_weight + (size_t)i * f->N,
+                        &out_models[i], &steps[i], &out_log_ls[i], nullptr);
+            if (rc) return rc;
+        }
     }
+    for (double st : steps) worst = std::max(worst, st);
     if (out_worst_step) *out_worst_step = worst;
     // confidence bounds: per parameter, the order statistics at (1 - ci) / 2
     // and 1 - (1 - ci) / 2 of the rounds (bootstrap-fit.cc:98-167)

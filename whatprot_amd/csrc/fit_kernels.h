@@ -26,7 +26,13 @@ struct FitArgs {
     size_t hist_stride;           // elements per workgroup
     double* partial;              // [grid][L + 1] per-workgroup totals
     int L;                        // WP_FIT_STATS_LEN(T, C)
-    const int* weight;            // [N] multiplicity of each read (bootstrap resamples), or null = 1
+    const int* weight;            // [rounds][N] multiplicity of each read (bootstrap resamples), or null = 1
+    // Several E-steps in one launch (bootstrap rounds): nslots models,
+    // binom holds nslots tables back to back, partial nslots x grid totals,
+    // d_out nslots x (L + 1) results.
+    int nslots;                   // 1 for a plain E-step
+    const DevModel* models;       // [nslots] device copies, or null: a.m
+    const int* slot_round;        // [nslots] weight row of each slot, or null: row 0
 };
 
 // Dynamic LDS of one workgroup ((size_t)-1 for an unsupported C).

@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "fit_kernels.h"
 
 namespace wp {
@@ -160,10 +162,12 @@ __device__ inline Fiber fiber_at(int f, int R, int G, int NB, const unsigned sho
     return r;
 }
 
+// One workgroup's share of an E-step under model m: the reads r = blockIdx.x
+// + k gridDim.x, weighted by weight (null: 1), with the binomial tables binom;
+// the workgroup's totals go to out[0..L].
 template <typename Real, int C>
-__global__ void __launch_bounds__(kThreads) fit_estep_kernel(const FitArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const DevModel& m = a.m;
+__device__ void fit_estep_one(const FitArgs& a, const DevModel& m, const double* binom, const int* weight,
+                              double* out, unsigned char* smem) {
     const int T = m.T, NB = a.NB, tid = threadIdx.x;
     const size_t S = (size_t)T * NB;
     Smem<Real> L;
@@ -174,7 +178,7 @@ __global__ void __launch_bounds__(kThreads) fit_estep_kernel(const FitArgs a) {
     const bool split = sizeof(Real) == 4;
 
     // per-fitter constants into LDS
-    for (int i = tid; i < 2 * C * a.tri; i += kThreads) L.bin[i] = (Real)a.binom[i];
+    for (int i = tid; i < 2 * C * a.tri; i += kThreads) L.bin[i] = (Real)binom[i];
     for (int i = tid; i < NB * C; i += kThreads) L.crd[i] = a.crd[i];
     for (int i = tid; i < a.nfib; i += kThreads) L.fbase[i] = a.fbase[i];
     for (int i = tid; i < T; i += kThreads) L.seq[i] = a.seq[i];
@@ -289,7 +293,7 @@ __global__ void __launch_bounds__(kThreads) fit_estep_kernel(const FitArgs a) {
     for (int r = blockIdx.x; r < a.N; r += gridDim.x) {
         // A read drawn w times adds w times its statistics and ln P; reads
         // not drawn are skipped (uniform over the workgroup).
-        const int wr = a.weight ? a.weight[r] : 1;
+        const int wr = weight ? weight[r] : 1;
         if (wr == 0) continue;
         // ---- emission table of this read (peptide-emission.cc:165-181)
         const double* rad = a.rads + (size_t)r * T * C;
@@ -529,7 +533,6 @@ __global__ void __launch_bounds__(kThreads) fit_estep_kernel(const FitArgs a) {
     }
 
     // ---- this workgroup's totals
-    double* out = a.partial + (size_t)blockIdx.x * (a.L + 1);
     auto put = [&](int idx, double v0, double v1) {
         block_sum2(v0, v1, L.red);
         if (tid == 0) {
@@ -552,13 +555,33 @@ __global__ void __launch_bounds__(kThreads) fit_estep_kernel(const FitArgs a) {
     if (tid == 0) out[a.L] = sum_logp;
 }
 
-// Sums the per-workgroup totals in workgroup order (deterministic).
+// The E-steps of a.nslots models in one launch. Each workgroup runs its share
+// of every slot in turn, exactly as a launch of that slot alone would (same
+// reads, same order, own totals), so every slot's result is the single
+// launch's bit for bit. Slot k uses models[k] (a.m when there is one slot and
+// no array), binom + k * 2C * tri and weight row slot_round[k] (row 0 when
+// there is no map).
+template <typename Real, int C>
+__global__ void __launch_bounds__(kThreads) fit_estep_kernel(const FitArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    for (int k = 0; k < a.nslots; k++) {
+        const int row = a.slot_round ? a.slot_round[k] : 0;
+        fit_estep_one<Real, C>(a, a.models ? a.models[k] : a.m, a.binom + (size_t)k * 2 * C * a.tri,
+                               a.weight ? a.weight + (size_t)row * a.N : nullptr,
+                               a.partial + ((size_t)k * gridDim.x + blockIdx.x) * (a.L + 1), smem);
+        __syncthreads();
+    }
+}
+
+// Sums the per-workgroup totals of slot blockIdx.y in workgroup order
+// (deterministic).
 __global__ void fit_reduce_kernel(const double* partial, int nblocks, int L, double* out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > L) return;
+    const size_t k = blockIdx.y;
     double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += partial[(size_t)b * (L + 1) + i];
-    out[i] = s;
+    for (int b = 0; b < nblocks; b++) s += partial[(k * nblocks + b) * (L + 1) + i];
+    out[k * (L + 1) + i] = s;
 }
 
 template <typename Real, int C>
@@ -605,8 +628,8 @@ hipError_t launch_fit_estep(const FitArgs& a, int precision, int grid, double* d
     }
     if (e != hipSuccess) return e;
     int n = a.L + 1;
-    hipLaunchKernelGGL(fit_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a.partial, grid, a.L,
-                       d_out);
+    hipLaunchKernelGGL(fit_reduce_kernel, dim3((n + 255) / 256, std::max(a.nslots, 1)), dim3(256), 0, stream,
+                       a.partial, grid, a.L, d_out);
     return hipGetLastError();
 }
 
