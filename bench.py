@@ -233,7 +233,7 @@ def main():
         "wall_s": wall,
         "launch": {"graph_replay": True, "per_bin_read_lists": True,
                    "quad_rows_variant_per_bin": [int(v) for v in variants],
-                   "autotune_ms_per_bin": [[round(float(a), 3), round(float(b), 3)] for a, b in tune_ms]},
+                   "autotune_ms_per_bin": [[round(float(x), 3) for x in row] for row in tune_ms]},
         "nonempty_pairs": pairs_nonempty,
         "state_timesteps": states,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

@@ -123,11 +123,13 @@ struct Workspace {
     int* rctr = nullptr;
 };
 
-enum BinKernel { kGather = 0, kRowPerLane = 3, kQuadRows = 5 };
+enum BinKernel { kGather = 0, kRowPerLane = 3, kQuadRows = 5, kLaneGroup = 6 };
 
 struct Bin {
     int lo, hi, ngroups;
-    int kernel;  // BinKernel: quad-rows (FP32, boxes <= 16), row-per-lane (FP64, <= 8) or gather
+    // BinKernel: lane-group (FP32, boxes <= 16, T <= 12), quad-rows (FP32,
+    // boxes <= 16), row-per-lane (FP64, <= 8) or gather
+    int kernel;
     int cap_sep, ngroups_sep;  // gather bins: state capacity and groups of the separable launch
     int ngroups_wide;          // groups of the wide passes (few reads: as many groups as LDS allows)
 };
@@ -176,8 +178,10 @@ struct wp_hmm_classifier {
     int tuned = 0;
     int variants_set = 0;  // variant[] holds the defaults or the tuned choice
     int variant[wp::kMaxBins] = {0};
-    float tune_ms[wp::kMaxBins][2] = {{-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f},
-                                       {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}, {-1.f, -1.f}};
+    float tune_ms[wp::kMaxBins][2];  // autotuner's best time per variant (-1 untimed)
+    wp_hmm_classifier() {
+        for (int b = 0; b < wp::kMaxBins; b++) tune_ms[b][0] = tune_ms[b][1] = -1.f;
+    }
     // the device entry points replay a captured graph of the launch sequence
     // while the call's shape and pointers repeat
     int use_graph = 1;
@@ -210,6 +214,7 @@ namespace {
 using wp::Bin;
 using wp::BinSet;
 using wp::kGather;
+using wp::kLaneGroup;
 using wp::kQuadRows;
 using wp::kRowPerLane;
 
@@ -221,6 +226,12 @@ constexpr int kSepCapDiv = 2;
 size_t bin_lds_bytes(const wp_hmm_classifier* c, const Bin& b);
 
 constexpr size_t kLdsBudget = 48 * 1024;
+
+// Workgroups of the row-per-lane and gather launches, which stride over their
+// read list: most of their lists (wide passes, the FP64 decision, the bins of
+// large boxes) hold a small part of the call's reads, and a grid of one
+// workgroup per read spent about 1.5 ms per launch on 10^5 exits.
+constexpr int kListGrid = 8192;
 
 // Largest dynamic LDS one workgroup may take on the current device (160 KB
 // on gfx950), queried once.
@@ -259,7 +270,10 @@ BinSet make_bins(const wp_hmm_classifier* c) {
         const int ng_sep = groups_for(cap_sep, kLdsBudget);
         const int ng_wide = groups_for(c->T * hi, lds_hard_limit() - 1024);
         int kernel = kGather;
-        if (wp::v5_supports(c->precision, c->T, hi) &&
+        if (wp::v6_supports(c->precision, c->T, hi) &&
+            wp::v6_lds_bytes(c->T, c->C, c->tri, hi) <= lds_hard_limit())
+            kernel = kLaneGroup;
+        else if (wp::v5_supports(c->precision, c->T, hi) &&
             wp::v5_lds_bytes(c->T, c->C, c->tri, hi) <= lds_hard_limit())
             kernel = kQuadRows;
         else if (wp::v3_supports(c->precision, c->T, hi) &&
@@ -276,6 +290,7 @@ size_t bin_lds_bytes(const wp_hmm_classifier* c, const Bin& b) {
     switch (b.kernel) {
     case kRowPerLane: return wp::v3_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi);
     case kQuadRows: return wp::v5_lds_bytes(c->T, c->C, c->tri, b.hi);
+    case kLaneGroup: return wp::v6_lds_bytes(c->T, c->C, c->tri, b.hi);
     default:
         return std::max(
             wp::classify_lds_bytes(c->precision, c->T, c->C, c->tri, b.hi, c->T * b.hi, b.ngroups_wide, c->G),
@@ -377,6 +392,7 @@ wp::ClassifyArgs base_args(const wp_hmm_classifier* c, const CallArgs& x) {
     a.wide_count = nullptr;
     a.variant = 0;
     a.cap = 0;
+    a.grid_cap = kListGrid;
     a.dsD = c->dD;
     a.dsSeq = c->dSeq;
     a.dsTrack = c->dTrack;
@@ -426,6 +442,7 @@ hipError_t launch_bin_main(wp_hmm_classifier* c, const BinSet& bins, int b, cons
     a.wide_count = c->ws.ctr + kCtrWide + b;
     a.variant = variant;
     if (bin.kernel == kRowPerLane) return wp::launch_classify_v3(a, c->precision, stream);
+    if (bin.kernel == kLaneGroup) return wp::launch_classify_v6(a, stream);
     if (bin.kernel == kQuadRows) return wp::launch_classify_v5(a, stream);
     a.variant = 1;  // gather bins: separable Bleach passes; reads that outgrow them go to the wide pass
     a.cap = bin.cap_sep;

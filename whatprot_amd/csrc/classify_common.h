@@ -322,5 +322,145 @@ __device__ void merge_and_store(const ClassifyArgs& a, int r, int ncand, const i
     if (a.out_log_total) a.out_log_total[r] = log_total;
 }
 
+// x takes in another FP32 accumulator (best ln P and candidate, log-sum-exp
+// pair): the larger ln P wins, ties to the earlier candidate.
+__device__ inline void acc_merge(CandAcc<true>& x, double bv, int bk, double ta, double tb) {
+    if (bv > x.best_v || (bv == x.best_v && bk < x.best_k)) {
+        x.best_v = bv;
+        x.best_k = bk;
+    }
+    if (ta > x.tot_a) {
+        x.tot_b = x.tot_b * exp(x.tot_a - ta) + tb;
+        x.tot_a = ta;
+    } else if (ta > -INFINITY) {
+        x.tot_b += tb * exp(ta - x.tot_a);
+    }
+}
+
+// merge_and_store for many small groups (L lanes each, lane 0 holding the
+// group's accumulator and tie slot): each wave first reduces its groups'
+// accumulators by shuffles, then the waves are merged; the near-best list is
+// gathered from every group's tie slot in parallel (a block prefix sum over
+// the groups' counts). red: 16 ints of free LDS; gres: at least 20 doubles;
+// tmp: kTieMax ints of free LDS. Block-wide.
+template <int L>
+__device__ void merge_lanes_and_store(const ClassifyArgs& a, int r, int ncand, const int* cidx, int ngroups,
+                                      int group, int p, CandAcc<true> acc, double* gres, const TieSlot* ties,
+                                      int* tmp, int* red) {
+    const int tid = threadIdx.x, wl = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+    const bool lead = p == 0 && group < ngroups;
+    double bv = lead ? acc.best_v : -INFINITY;
+    int bk = lead ? acc.best_k : 0x7fffffff;
+    double ta = lead ? acc.tot_a : -INFINITY;
+    double tb = lead ? acc.tot_b : 0.0;
+    CandAcc<true> x;
+    x.best_v = bv;
+    x.best_k = bk;
+    x.tot_a = ta;
+    x.tot_b = tb;
+#pragma unroll
+    for (int off = 32; off >= L; off >>= 1) {
+        acc_merge(x, __shfl_xor(x.best_v, off, 64), __shfl_xor(x.best_k, off, 64), __shfl_xor(x.tot_a, off, 64),
+                  __shfl_xor(x.tot_b, off, 64));
+    }
+    if (wl == 0) {
+        gres[wv * 4 + 0] = x.best_v;
+        gres[wv * 4 + 1] = (double)x.best_k;
+        gres[wv * 4 + 2] = x.tot_a;
+        gres[wv * 4 + 3] = x.tot_b;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        CandAcc<true> y;
+        y.best_v = gres[0];
+        y.best_k = (int)gres[1];
+        y.tot_a = gres[2];
+        y.tot_b = gres[3];
+        for (int w = 1; w < nw; w++) acc_merge(y, gres[w * 4], (int)gres[w * 4 + 1], gres[w * 4 + 2], gres[w * 4 + 3]);
+        gres[16] = y.best_v;
+        gres[17] = (double)y.best_k;
+        gres[18] = y.tot_a;
+        gres[19] = y.tot_b;
+        red[8] = 0;  // near-best overflow flag
+    }
+    __syncthreads();
+    const double best = gres[16];
+    const bool collect = a.tie_mode == 1 && ties && best > -INFINITY;
+    if (collect) {
+        const double lim = tie_floor(best, a.tie_abs, a.tie_rel);
+        int cnt = 0;
+        if (lead) {
+            const TieSlot& s = ties[group];
+            if (s.drop >= lim) atomicOr(&red[8], 1);
+            for (int j = 0; j < s.n; j++) cnt += s.v[j] >= lim ? 1 : 0;
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(incl, off, 64);
+            if (wl >= off) incl += y;
+        }
+        if (wl == 63) red[wv] = incl;
+        __syncthreads();
+        int pos = incl - cnt;
+        for (int w = 0; w < wv; w++) pos += red[w];
+        if (lead) {
+            const TieSlot& s = ties[group];
+            for (int j = 0; j < s.n; j++)
+                if (s.v[j] >= lim) {
+                    if (pos < kTieMax) tmp[pos] = s.kk[j];
+                    pos++;
+                }
+        }
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    bv = gres[16];
+    bk = (int)gres[17];
+    ta = gres[18];
+    tb = gres[19];
+    double log_score = bv;
+    double log_total = tb > 0 ? ta + log(tb) : -INFINITY;
+    double adjusted = exp(log_score - log_total);
+    if (isnan(adjusted)) {
+        adjusted = 0.0;
+        log_score = -INFINITY;
+        log_total = 0.0;
+    }
+    if (a.tie_mode == 1 && ties) {
+        int tn = 0;
+        if (collect) {
+            int cnt = 0;
+            for (int w = 0; w < nw; w++) cnt += red[w];
+            if (red[8] || cnt > kTieMax || bv < kTieFloorAll) {
+                tn = -1;
+            } else if (cnt >= 2) {
+                for (int j = 1; j < cnt; j++) {  // candidate order
+                    const int v = tmp[j];
+                    int q = j - 1;
+                    while (q >= 0 && tmp[q] > v) {
+                        tmp[q + 1] = tmp[q];
+                        q--;
+                    }
+                    tmp[q + 1] = v;
+                }
+                int* out = a.tie_idx + (size_t)r * kTieMax;
+                for (int j = 0; j < cnt; j++) out[j] = cidx ? cidx[tmp[j]] : tmp[j];
+                tn = cnt;
+            }
+        }
+        a.tie_n[r] = tn;
+    }
+    int id = -1;
+    if (ncand > 0) {
+        const int i = cidx ? cidx[bk] : bk;
+        id = (i >= 0 && i < a.S) ? a.dsId[i] : -1;
+    }
+    a.out_id[r] = id;
+    if (a.out_adjusted) a.out_adjusted[r] = adjusted;
+    if (a.out_log_score) a.out_log_score[r] = log_score;
+    if (a.out_log_total) a.out_log_total[r] = log_total;
+}
+
 }  // namespace cc
 }  // namespace wp

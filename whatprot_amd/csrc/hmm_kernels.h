@@ -23,8 +23,9 @@ struct ClassifyArgs {
     int nb_lo, nb_hi;         // this launch takes reads with nb_lo < nbmax <= nb_hi
     uint8_t* wide;            // [N] set by the quad-rows kernel: densities too wide for it
     int wide_pass;            // 1: score only the reads flagged in wide[]
-    // Reads of this launch: workgroup k scores read list[k] for k < *list_count
-    // (the list is built on the device; later workgroups exit at once).
+    // Reads of this launch: list[k] for k < *list_count (the list is built on
+    // the device); workgroup k scores list[k] (quad-rows and lane-group
+    // kernels: later workgroups exit at once) or strides over it (grid_cap).
     const int* list;
     const int* list_count;
     int* wide_list;           // quad-rows: reads handed to the wide pass are appended here
@@ -33,6 +34,10 @@ struct ClassifyArgs {
     int NB;                   // per-group state capacity in count-box entries
     int cap;                  // gather kernel: entries of each of a group's four state buffers
     int ngroups;              // pair groups per workgroup
+    // Workgroups of the launch: 0 = one per read of the list (grid a.N);
+    // k > 0 = min(a.N, k) workgroups striding over the list (for lists that
+    // hold few of the N reads, so the grid does not idle through N exits).
+    int grid_cap;
     const uint8_t* dsD;       // [S][C] dyes per channel
     const int8_t* dsSeq;      // [S][T] channel of position e (-1 = none)
     const uint8_t* dsTrack;   // [S][T][C] dyes of channel c at positions >= e
@@ -56,6 +61,9 @@ struct ClassifyArgs {
     int* tie_idx;             // [N][kTieMax]
     double tie_abs, tie_rel;  // window W(best) = tie_abs + tie_rel |best| in ln P
 };
+
+// Workgroups of a launch over a read list (ClassifyArgs::grid_cap).
+inline int list_grid(const ClassifyArgs& a) { return a.grid_cap > 0 && a.grid_cap < a.N ? a.grid_cap : a.N; }
 
 // Largest number of read bins (box-size classes) of one classify call.
 constexpr int kMaxBins = 16;
@@ -103,6 +111,12 @@ hipError_t launch_classify_v3(const ClassifyArgs& a, int precision, hipStream_t 
 bool v5_supports(int precision, int T, int NB);
 size_t v5_lds_bytes(int T, int C, int tri, int NB);
 hipError_t launch_classify_v5(const ClassifyArgs& a, hipStream_t stream);
+
+// Lane-group classify (FP32): 1, 2 or 4 lanes per candidate, one scale per
+// candidate, per-timestep box classes; NB in {4, 8, 16}, T <= 12.
+bool v6_supports(int precision, int T, int NB);
+size_t v6_lds_bytes(int T, int C, int tri, int NB);
+hipError_t launch_classify_v6(const ClassifyArgs& a, hipStream_t stream);
 
 // Per-read algorithmic work (non-empty pairs, state-timesteps, FLOPs) -> [N][3].
 void launch_work(const DevModel& m, int N, int S, const uint8_t* d_Lr, const uint8_t* d_Hr,

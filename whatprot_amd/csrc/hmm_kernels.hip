@@ -961,13 +961,18 @@ __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, 
     merge_and_store<kRescale>(a, r, ncand, cidx, ngroups, group, lane, acc, gres, ties, list);
 }
 
-// One workgroup per read of the bin's compacted list.
+// Workgroups stride over the bin's compacted list (one read each when the
+// grid spans the list); the barrier keeps a read's LDS tables from being
+// overwritten while the previous read's outputs are written.
 template <typename Real, int C, int G, bool kSep>
 __global__ void __launch_bounds__(kBlockThreads) WP_CLASSIFY_OCCUPANCY
 classify_kernel(const ClassifyArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    if ((int)blockIdx.x >= *a.list_count) return;
-    gather_read<Real, C, G, kSep>(a, a.list[blockIdx.x], smem);
+    const int n = *a.list_count;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        if (k != (int)blockIdx.x) __syncthreads();
+        gather_read<Real, C, G, kSep>(a, a.list[k], smem);
+    }
 }
 
 template <typename Real, int C, int G, bool kSep>
@@ -981,7 +986,7 @@ hipError_t launch_t(const ClassifyArgs& a, hipStream_t stream) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((classify_kernel<Real, C, G, kSep>), dim3(a.N), dim3(nthreads), l.total, stream, a);
+    hipLaunchKernelGGL((classify_kernel<Real, C, G, kSep>), dim3(list_grid(a)), dim3(nthreads), l.total, stream, a);
     return hipGetLastError();
 }
 

@@ -1,5 +1,6 @@
 // Classify kernel (FP32 rescaled) for reads whose count box holds at most 16
-// states: a quad of lanes per candidate, several Edman rows per lane.
+// states: a quad of lanes per candidate, several Edman rows per lane, one
+// power-of-two exponent per row.
 //
 // Same recursion as the row-per-lane kernel (read-level boxes, composite
 // Bleach matrices Wt and Edman-shifted W1[c] shared by every candidate of
@@ -10,95 +11,25 @@
 // Edman success of row e - 1 reaches row e through one quad DPP rotation:
 // lanes 1..3 take slot k of the lane below, lane 0 takes slot k - 1 of lane 3.
 //
-// Scaling. Emission densities are stored per timestep relative to the
-// largest density of that timestep's box (E = Ehat 2^X_t), and every row
-// carries one power-of-two exponent, so the per-entry work of a step is the
+// Every row carries its own exponent, so the per-entry work of a step is the
 // Edman coefficients, the two updates, the emission product and a rescale;
 // row exponents meet only where the Edman success crosses rows and in the
-// final sum of a pair.
-//
-// Blocked and detached mass. The N-terminally blocked tensor B and the
-// detached share are never Edman-shifted: from one step to the next they see
-// Detach, Bleach and Emit only, which are the read's, not the candidate's. So
-// a per-read backward pass gives, for every timestep t, what a unit of A at
-// (t - 1, q) will add to P through CyclicBlock and Detach(t - 1) at step t,
-//   g_t[q] = pd_t phi_t + pcb (1 - pd_t) sum_n Wt[q][n] E_t[n] beta_t[n],
-// with beta_t[n] the final worth of a unit of blocked mass at (t, n) after
-// Emit(t) and phi_t that of a unit of detached mass entering at step t
-// (the product of E_u(0) over u >= t where box(u) holds the zero count, else
-// 0); and the initial blocked share p_ib Dud(D -> n) E_0[n] is worth
-// beta_0[n]. Each candidate then runs A alone and adds row . g_t at every
-// step: P = sum A_(T-1) + those sums, which is the reference's
-// sum A + sum B + detached share (peptide-state-vector.cc:52-54) with the same
-// step order, regrouped.
-//
-// Per-timestep box classes. A read's box shrinks as its dyes leave (most
-// timesteps of most reads have a one-state box), so each timestep t runs at
-// the class K_t = the power of two >= max(|box(t-1)|, |box(t)|), not at the
-// bin's capacity NB: the composite matrices of t are stored K_t x K_t and the
-// step body is instantiated per class, chosen uniformly per read.
+// final sum of a pair. The read's tables (read_tables.h) are shared with the
+// lane-group kernel, which serves T <= 12; this kernel serves longer runs.
 #include <float.h>
 #include <math.h>
 
 #include <algorithm>
 
-#include "classify_common.h"
-#include "hmm_kernels.h"
+#include "read_tables.h"
 
 namespace wp {
 namespace {
 
 using namespace cc;
+using namespace rt;
 
-constexpr int kV5Threads = 256;
 constexpr int kV5Lanes = 4;  // lanes per candidate: one DPP quad
-constexpr int kV5MaxSpan = 90;  // largest binary-exponent spread of one box's densities
-constexpr int kV5MaxBackSpread = 800;  // largest spread of one level of the blocked/detached backward pass
-
-struct V5Layout {
-    size_t rad, L, H, nbt, kcl, woff, bstr, coord, kcf, ev, etab, xt, W, binom, list, scan, gres, ties, gtab, gx,
-        b0, bd, total;
-};
-
-// W block per timestep: (1 + C) matrices (Wt, then W1[c]) of K_t x K_t with a
-// stride of K_t * K_t + 4 entries, so the shifted matrices of different
-// channels start in different LDS banks; sized for K_t = NB at every t.
-__host__ __device__ inline V5Layout v5_layout(int T, int C, int tri, int NB) {
-    V5Layout l;
-    const size_t ms = (size_t)NB * NB + 4;
-    const int ngroups = kV5Threads / kV5Lanes;
-    size_t o = 0;
-    l.rad = o;   o = align16(o + sizeof(double) * T * C);
-    l.L = o;     o = align16(o + sizeof(int) * T * C);
-    l.H = o;     o = align16(o + sizeof(int) * T * C);
-    l.nbt = o;   o = align16(o + sizeof(int) * T);
-    l.kcl = o;   o = align16(o + sizeof(int) * T);
-    l.woff = o;  o = align16(o + sizeof(int) * T);
-    l.bstr = o;  o = align16(o + sizeof(int) * T * C);
-    l.coord = o; o = align16(o + sizeof(int) * (size_t)T * NB * C);
-    l.kcf = o;   o = align16(o + sizeof(float) * (size_t)T * NB * C);
-    l.ev = o;    o = align16(o + sizeof(double) * (size_t)T * NB);
-    l.etab = o;  o = align16(o + sizeof(float) * (size_t)T * NB);
-    l.xt = o;    o = align16(o + sizeof(int) * T);
-    l.W = o;     o = align16(o + sizeof(float) * (size_t)T * (1 + C) * ms);
-    l.binom = o; o = align16(o + sizeof(double) * 2 * C * tri);
-    l.list = o;  o = align16(o + sizeof(int) * kV5Threads * 4);
-    l.scan = o;  o = align16(o + sizeof(int) * (kV5Threads / 64 + 2));
-    l.gres = o;  o = align16(o + sizeof(double) * 4 * ngroups);
-    l.ties = o;  o = align16(o + sizeof(TieSlot) * ngroups);
-    l.gtab = o;  o = align16(o + sizeof(double) * (size_t)T * NB);
-    l.gx = o;    o = align16(o + sizeof(int) * (T + 1));
-    l.b0 = o;    o = align16(o + sizeof(double) * NB);
-    l.bd = o;    o = align16(o + sizeof(double) * NB);
-    l.total = o;
-    return l;
-}
-
-// Lane p of each quad receives the value of lane (p + 3) % 4.
-__device__ inline int quad_rot_i(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, 3 | (0 << 2) | (1 << 4) | (2 << 6), 0xf, 0xf, false);
-}
-__device__ inline float quad_rot(float v) { return __int_as_float(quad_rot_i(__float_as_int(v))); }
 
 // Per-timestep constants of one step t >= 1, shared by every candidate.
 struct V5Step {
@@ -109,17 +40,6 @@ struct V5Step {
     int nbp, t, Xt, Gx, p;
     float cAA, pf;
 };
-
-// s 2^e += v 2^ev, s kept in [0.5, 1) (s = 0 holds any e). FP64: a row's
-// share of g_t can lie far below the row's own scale.
-__device__ __forceinline__ void side_add(double& s, int& e, double v, int ev) {
-    if (v > 0.0) {
-        const int mx = max(e, ev);
-        int k;
-        s = frexp(ldexp(s, e - mx) + ldexp(v, ev - mx), &k);
-        e = mx + k;
-    }
-}
 
 // One step t >= 1 of a candidate's recursion (CyclicBlock, Detach(t-1),
 // Bleach_c, Edman, Emit(t)) on a box of class K (both box(t-1) and box(t) fit
@@ -197,273 +117,51 @@ __device__ __forceinline__ void v5_step(const V5Step& st, const int (&cix)[R], c
 }
 
 // One read: prologue (boxes, densities, composite matrices) then its
-// candidates. kClass: each timestep at its own box class; otherwise every
-// timestep at the bin's capacity NB. Block-uniform returns only.
+// candidates, a quad of lanes each. kClass: each timestep at its own box
+// class; otherwise every timestep at the bin's capacity NB. Block-uniform
+// returns only.
 template <int C, int NB, int R, bool kClass>
 __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsigned char* smem) {
     constexpr int G = kV5Lanes;
-    constexpr int NGR = kV5Threads / G;
+    constexpr int NGR = kReadThreads / G;
     constexpr int MSNB = NB * NB + 4;
-    if (r >= a.N) return;
-    const int nbr = a.nbmax[r];
-    if (!((nbr > a.nb_lo && nbr <= a.nb_hi) || (nbr == 0 && a.nb_lo == 0))) return;
-
+    if (!in_bin(a, r)) return;
     const int T = a.m.T;
     const int tri = a.tri;
     const int tid = threadIdx.x;
     const int group = tid / G;
     const int p = tid % G;
-    const V5Layout lay = v5_layout(T, C, tri, NB);
-    double* s_rad = (double*)(smem + lay.rad);
-    int* sL = (int*)(smem + lay.L);
-    int* sH = (int*)(smem + lay.H);
-    int* nbt = (int*)(smem + lay.nbt);
-    int* kcl = (int*)(smem + lay.kcl);
-    int* woff = (int*)(smem + lay.woff);
-    int* bstr = (int*)(smem + lay.bstr);
-    int* coord = (int*)(smem + lay.coord);
-    float* kcf = (float*)(smem + lay.kcf);
-    double* ev = (double*)(smem + lay.ev);
-    float* etab = (float*)(smem + lay.etab);
-    int* xt = (int*)(smem + lay.xt);
-    float* Wall = (float*)(smem + lay.W);
-    double* bin = (double*)(smem + lay.binom);
-    int* list = (int*)(smem + lay.list);
-    int* scan = (int*)(smem + lay.scan);
-    double* gres = (double*)(smem + lay.gres);
-    TieSlot* ties = (TieSlot*)(smem + lay.ties);
-    double* gtab = (double*)(smem + lay.gtab);
-    int* gx = (int*)(smem + lay.gx);  // [T]: exponents; [T]: spread flag
-    double* b0d = (double*)(smem + lay.b0);
-    double* bd = (double*)(smem + lay.bd);
+    const ReadLayout lay = read_layout(T, C, tri, NB, NGR);
+    const ReadTables s = read_views(smem, lay);
+    if (!read_prologue<C, NB, kClass>(a, r, s, (int)((lay.binom - lay.W) / sizeof(float)))) return;
+    const int* sL = s.sL;
+    const int* nbt = s.nbt;
+    const int* kcl = s.kcl;
+    const int* woff = s.woff;
+    const int* coord = s.coord;
+    const float* kcf = s.kcf;
+    const float* etab = s.etab;
+    const int* xt = s.xt;
+    const float* Wall = s.Wall;
+    const double* gtab = s.gtab;
+    const int* gx = s.gx;
+    const double* b0d = s.b0d;
     int ncand;
     const int* cidx;
     read_candidates(a, r, &ncand, &cidx);
-    const bool read_empty = nbr == 0;
-
-    // ---- prologue: read-level boxes, coordinates, emission densities
-    for (int k = tid; k < T * C; k += kV5Threads) {
-        s_rad[k] = a.rads[(size_t)r * T * C + k];
-        sL[k] = a.Lr[(size_t)r * T * C + k];
-        sH[k] = a.Hr[(size_t)r * T * C + k];
-    }
-    for (int k = tid; k < 2 * C * tri; k += kV5Threads) bin[k] = a.binom[k];
-    if (tid == 0) scan[0] = 0;
-    __syncthreads();
-    if (tid < T) {
-        const int t = tid;
-        int s = 1;
-        for (int c = C - 1; c >= 0; c--) {
-            bstr[t * C + c] = s;
-            const int w = sH[t * C + c] - sL[t * C + c];
-            s *= w > 0 ? w : 0;
-        }
-        nbt[t] = read_empty ? 0 : s;
-    }
-    __syncthreads();
-    if (kClass && tid == 0) {  // per-timestep classes and the offsets of their W blocks
-        int o = 0;
-        for (int t = 0; t < T; t++) {
-            const int need = t == 0 ? nbt[0] : max(nbt[t - 1], nbt[t]);
-            int k = 1;
-            while (k < need) k *= 2;
-            kcl[t] = k;
-            woff[t] = o;
-            if (t >= 1) o += (1 + C) * (k * k + 4);
-        }
-    }
-    __syncthreads();
-    // peptide-emission.cc:165-180
-    for (int k = tid; k < T * NB; k += kV5Threads) {
-        const int t = k / NB, q = k % NB;
-        unsigned int n[WP_MAX_CHANNELS];
-        double v = 0.0;
-        if (q < nbt[t]) {
-            int rem = q;
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                const int d = rem / bstr[t * C + c];
-                rem -= d * bstr[t * C + c];
-                n[c] = (unsigned int)(sL[t * C + c] + d);
-            }
-            v = 1.0;
-#pragma unroll
-            for (int c = 0; c < C; c++) v *= dev_pdf<C>(a.m, c, s_rad[t * C + c], n);
-        } else {
-#pragma unroll
-            for (int c = 0; c < C; c++) n[c] = 0;
-        }
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            coord[k * C + c] = (int)n[c];
-            kcf[k * C + c] = (float)n[c];
-        }
-        ev[k] = v;
-    }
-    __syncthreads();
-    if (tid < T) {  // X_t: the largest binary exponent of the box's densities
-        int x = kNoExp, lo = -kNoExp;
-        for (int q = 0; q < NB; q++) {
-            const double v = ev[tid * NB + q];
-            if (v > 0.0) {
-                int ex;
-                frexp(v, &ex);
-                x = max(x, ex);
-                lo = min(lo, ex);
-            }
-        }
-        xt[tid] = x == kNoExp ? 0 : x;
-        if (x != kNoExp && x - lo > kV5MaxSpan) scan[0] = 1;  // cleared before the barrier above
-    }
-    __syncthreads();
-    // Densities spanning more than 2^kV5MaxSpan in one box could fall below
-    // FP32's range relative to the box maximum: such a read is flagged and
-    // left to the kernels that keep one exponent per density.
-    const bool wide = scan[0] != 0;
-    if (tid == 0) {
-        a.wide[r] = wide ? 1 : 0;
-        if (wide) a.wide_list[atomicAdd(a.wide_count, 1)] = r;
-    }
-    if (wide) return;
-    for (int k = tid; k < T * NB; k += kV5Threads) etab[k] = (float)ldexp(ev[k], -xt[k / NB]);
-    // Composite Bleach between the boxes of t-1 and t and its Edman-shifted
-    // forms: W1[c][q][q2] maps onto n'(q2) + 1_c and is zero at or above
-    // U = min(Hr[t] + 1, Hr[t-1]) (edman-transition.cc:28-60). Timestep t's
-    // matrices are K_t x K_t, K_t + 4 entries apart (different LDS banks for
-    // the shifted matrices of different channels).
-    const double* ble = bin + C * tri;
-    // entries of the W blocks of t >= 1: (1 + C) matrices of K_t x K_t each
-    const int wtotal = kClass ? woff[T - 1] + (T > 1 ? (1 + C) * (kcl[T - 1] * kcl[T - 1] + 4) : 0)
-                              : (T - 1) * (1 + C) * MSNB;
-    for (int k = tid; k < wtotal; k += kV5Threads) {
-        int t = 1, K = NB, rem;
-        if constexpr (kClass) {
-            while (t + 1 < T && woff[t + 1] <= k) t++;
-            K = kcl[t];
-            rem = k - woff[t];
-        } else {
-            t = 1 + k / ((1 + C) * MSNB);
-            rem = k - (t - 1) * (1 + C) * MSNB;
-        }
-        const int ms = K * K + 4;
-        const int mat = rem / ms;
-        rem -= mat * ms;
-        if (rem >= K * K) continue;  // the 4-entry gap between matrices
-        const int q = rem / K, q2 = rem % K;
-        double v = 0.0;
-        if (t >= 1 && q < nbt[t - 1] && q2 < nbt[t]) {
-            v = 1.0;
-            const int sc = mat - 1;  // shifted channel, -1 for Wt
-            for (int c = 0; c < C; c++) {
-                const int from = coord[((t - 1) * NB + q) * C + c];
-                const int to = coord[(t * NB + q2) * C + c] + (c == sc ? 1 : 0);
-                if (c == sc && to >= min(sH[t * C + c] + 1, sH[(t - 1) * C + c])) v = 0.0;
-                if (from < to) v = 0.0;
-                else v *= ble[c * tri + from * (from + 1) / 2 + to];
-            }
-        }
-        Wall[(kClass ? woff[t] : t * (1 + C) * MSNB) + mat * ms + rem] = (float)v;
-    }
-    // Blocked and detached mass (one wave, lane q = box entry q): from level
-    // t + 1 = u, beta_u = bq 2^eb and phi_(u+1) = ph 2^eb, to level t, both
-    // relative to 2^(eb + X_u) (densities E = Ehat 2^X), in FP64:
-    //   phi_u = Ehat_u(0) ph (box(u) holds the zero count) or 0,
-    //   v[q] = sum_m Bleach(q -> m) Ehat_u[m] bq[m]  (Bleach_c, Detach and Emit of step u),
-    //   beta_t[q] = pd phi_u + (1 - pd) v[q],  g_u[q] = pd phi_u + pcb (1 - pd) v[q],
-    // then level t is renormalised by a power of two. A level whose nonzero
-    // entries spread over more than 2^kV5MaxBackSpread hands the read to the
-    // wide pass: below that every product of the next level (Bleach >= 2^-65,
-    // relative densities >= 2^-kV5MaxSpan) is a normal double, so no share
-    // underflows unseen.
-    if (tid < 64) {
-        const int q = tid;
-        double bq = q < nbt[T - 1] ? 1.0 : 0.0;
-        double ph = 1.0;
-        int eb = 0;
-        bool spread = false;
-        const double pcb_d = a.m.p_cb;
-        for (int t = T - 2; t >= 0; t--) {
-            const int u = t + 1;
-            if (q < NB) bd[q] = bq;
-            wave_sync();
-            bool zero_in = true;
-#pragma unroll
-            for (int c = 0; c < C; c++) zero_in = zero_in && sL[u * C + c] == 0;
-            ph = zero_in ? ldexp(ev[u * NB], -xt[u]) * ph : 0.0;
-            double v = 0.0;
-            if (q < nbt[t]) {
-                for (int m2 = 0; m2 < nbt[u]; m2++) {
-                    double wv = ldexp(ev[u * NB + m2], -xt[u]) * bd[m2];
-#pragma unroll
-                    for (int c = 0; c < C; c++) {
-                        const int from = coord[(t * NB + q) * C + c];
-                        const int to = coord[(u * NB + m2) * C + c];
-                        wv = from < to ? 0.0 : wv * ble[c * tri + from * (from + 1) / 2 + to];
-                    }
-                    v += wv;
-                }
-            }
-            const double pd = a.m.p_detach[u - 1];
-            const double g = q < nbt[t] ? pd * ph + pcb_d * (1.0 - pd) * v : 0.0;
-            bq = q < nbt[t] ? pd * ph + (1.0 - pd) * v : 0.0;
-            double gm = g, bm = fmax(bq, ph);
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                gm = fmax(gm, __shfl_xor(gm, off, 64));
-                bm = fmax(bm, __shfl_xor(bm, off, 64));
-            }
-            int ge = 0, be = 0;
-            if (gm > 0.0) frexp(gm, &ge);
-            if (bm > 0.0) frexp(bm, &be);
-            // smallest nonzero exponents of the level
-            int gl = 100000, bl = 100000, k2;
-            if (g > 0.0) {
-                frexp(g, &k2);
-                gl = k2;
-            }
-            if (bq > 0.0) {
-                frexp(bq, &k2);
-                bl = k2;
-            }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                gl = min(gl, __shfl_xor(gl, off, 64));
-                bl = min(bl, __shfl_xor(bl, off, 64));
-            }
-            spread = spread || (gm > 0.0 && ge - gl > kV5MaxBackSpread) || (bm > 0.0 && be - bl > kV5MaxBackSpread);
-            if (q < NB) gtab[u * NB + q] = ldexp(g, -ge);
-            if (q == 0) gx[u] = eb + xt[u] + ge;
-            bq = ldexp(bq, -be);
-            ph = ldexp(ph, -be);
-            eb += xt[u] + be;
-            wave_sync();
-        }
-        if (q < NB) b0d[q] = bq;
-        if (q == 0) {
-            gx[0] = eb;  // beta_0 = b0d 2^gx[0]
-            gx[T] = spread ? 1 : 0;
-        }
-    }
-    __syncthreads();
-    if (gx[T]) {
-        if (tid == 0) {
-            a.wide[r] = 1;
-            a.wide_list[atomicAdd(a.wide_count, 1)] = r;
-        }
-        return;
-    }
+    const bool read_empty = a.nbmax[r] == 0;
 
     CandAcc<true> acc;
     const bool collect = a.tie_mode == 1;
-    if (collect && p == 0) tie_init(&ties[group]);
+    if (collect && p == 0) tie_init(&s.ties[group]);
     const float pf = (float)a.m.pf;
     const float opf = (float)(1.0 - a.m.pf);
-    const double* dud = bin;
+    const double* dud = s.bin;
 
-    for (int base = 0; base < ncand; base += kV5Threads * 4) {
-        const int nlive = compact_chunk<C>(a, r, ncand, cidx, base, read_empty, sL, list, scan);
+    for (int base = 0; base < ncand; base += kReadThreads * 4) {
+        const int nlive = compact_chunk<C>(a, r, ncand, cidx, base, read_empty, sL, s.list, s.scan);
         for (int q0 = group; q0 < nlive; q0 += NGR) {
-            const int kk = list[q0];
+            const int kk = s.list[q0];
             const int i = cidx ? cidx[kk] : kk;
             int D[C];
 #pragma unroll
@@ -571,15 +269,15 @@ __device__ __forceinline__ void v5_read(const ClassifyArgs& a, const int r, unsi
             }
             const int K = group_max_int<G>(kl);
             part = (K == kNoExp || kl == kNoExp) ? 0.0 : ldexp(part, kl - K);
-            const double s = group_sum<double, G>(part);
-            const double logp = s > 0.0 ? log(s) + (double)K * 0.69314718055994530942 : -INFINITY;
+            const double sum = group_sum<double, G>(part);
+            const double logp = sum > 0.0 ? log(sum) + (double)K * 0.69314718055994530942 : -INFINITY;
             if (a.out_pair_logp && !cidx && p == 0) a.out_pair_logp[(size_t)r * a.S + i] = logp;
             acc.add(logp, exp(logp), kk, a.dsCount[i]);
-            if (collect && p == 0) tie_note(&ties[group], acc.best_v, logp, kk, a.tie_abs, a.tie_rel);
+            if (collect && p == 0) tie_note(&s.ties[group], acc.best_v, logp, kk, a.tie_abs, a.tie_rel);
         }
         __syncthreads();
     }
-    merge_and_store<true>(a, r, ncand, cidx, NGR, group, p, acc, gres, ties, list);
+    merge_and_store<true>(a, r, ncand, cidx, NGR, group, p, acc, s.gres, s.ties, s.list);
 }
 
 // One workgroup per read of the bin's compacted list: a persistent loop over
@@ -600,7 +298,7 @@ constexpr int v5_min_waves() {
 }
 
 template <int C, int NB, int R, bool kClass>
-__global__ void __launch_bounds__(kV5Threads, (v5_min_waves<NB, R>())) classify_v5_kernel(const ClassifyArgs a) {
+__global__ void __launch_bounds__(kReadThreads, (v5_min_waves<NB, R>())) classify_v5_kernel(const ClassifyArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     if ((int)blockIdx.x >= *a.list_count) return;
     v5_read<C, NB, R, kClass>(a, a.list[blockIdx.x], smem);
@@ -608,14 +306,14 @@ __global__ void __launch_bounds__(kV5Threads, (v5_min_waves<NB, R>())) classify_
 
 template <int C, int NB, int R, bool kClass>
 hipError_t launch_v5_v(const ClassifyArgs& a, hipStream_t stream) {
-    const V5Layout l = v5_layout(a.m.T, C, a.tri, NB);
+    const ReadLayout l = read_layout(a.m.T, C, a.tri, NB, kReadThreads / kV5Lanes);
     if (l.total > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)classify_v5_kernel<C, NB, R, kClass>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
         if (e != hipSuccess) return e;
     }
     // one workgroup per read of the bin's list (workgroups past its length exit)
-    hipLaunchKernelGGL((classify_v5_kernel<C, NB, R, kClass>), dim3(a.N), dim3(kV5Threads), l.total, stream, a);
+    hipLaunchKernelGGL((classify_v5_kernel<C, NB, R, kClass>), dim3(a.N), dim3(kReadThreads), l.total, stream, a);
     return hipGetLastError();
 }
 
@@ -656,7 +354,7 @@ bool v5_supports(int precision, int T, int NB) {
     return NB == 16 && T <= 16;
 }
 
-size_t v5_lds_bytes(int T, int C, int tri, int NB) { return v5_layout(T, C, tri, NB).total; }
+size_t v5_lds_bytes(int T, int C, int tri, int NB) { return read_layout(T, C, tri, NB, kReadThreads / kV5Lanes).total; }
 
 hipError_t launch_classify_v5(const ClassifyArgs& a, hipStream_t stream) {
     if (a.N <= 0) return hipSuccess;
