@@ -441,6 +441,9 @@ hipError_t launch_bin_main(wp_hmm_classifier* c, const BinSet& bins, int b, cons
     a.wide_list = c->ws.wide_lists + (size_t)b * x.N;
     a.wide_count = c->ws.ctr + kCtrWide + b;
     a.variant = variant;
+    // the bins of up to 32 states hold most reads: one workgroup per read
+    // (the FP64 decision's lists are short in every bin)
+    if (bin.hi <= 32 && x.tie_mode != 2) a.grid_cap = 0;
     if (bin.kernel == kRowPerLane) return wp::launch_classify_v3(a, c->precision, stream);
     if (bin.kernel == kLaneGroup) return wp::launch_classify_v6(a, stream);
     if (bin.kernel == kQuadRows) return wp::launch_classify_v5(a, stream);

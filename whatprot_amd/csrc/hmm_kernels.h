@@ -113,7 +113,7 @@ size_t v5_lds_bytes(int T, int C, int tri, int NB);
 hipError_t launch_classify_v5(const ClassifyArgs& a, hipStream_t stream);
 
 // Lane-group classify (FP32): 1, 2 or 4 lanes per candidate, one scale per
-// candidate, per-timestep box classes; NB in {4, 8, 16}, T <= 12.
+// candidate, per-timestep box classes; NB in {4, 8, 16, 32}, T <= 12.
 bool v6_supports(int precision, int T, int NB);
 size_t v6_lds_bytes(int T, int C, int tri, int NB);
 hipError_t launch_classify_v6(const ClassifyArgs& a, hipStream_t stream);

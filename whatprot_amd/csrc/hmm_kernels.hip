@@ -961,17 +961,25 @@ __device__ __forceinline__ void gather_read(const ClassifyArgs& a, const int r, 
     merge_and_store<kRescale>(a, r, ncand, cidx, ngroups, group, lane, acc, gres, ties, list);
 }
 
-// Workgroups stride over the bin's compacted list (one read each when the
-// grid spans the list); the barrier keeps a read's LDS tables from being
-// overwritten while the previous read's outputs are written.
-template <typename Real, int C, int G, bool kSep>
+// kStrided false: one workgroup per read of the bin's compacted list
+// (workgroups past its length exit). kStrided: workgroups stride over the
+// list, for lists that hold few of the call's reads; the barrier keeps a
+// read's LDS tables from being overwritten while the previous read's outputs
+// are written. Two instantiations: the loop holds read-invariant values
+// across reads, which costs registers where every read of a large list has
+// its own workgroup anyway.
+template <typename Real, int C, int G, bool kSep, bool kStrided>
 __global__ void __launch_bounds__(kBlockThreads) WP_CLASSIFY_OCCUPANCY
 classify_kernel(const ClassifyArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = *a.list_count;
-    for (int k = blockIdx.x; k < n; k += gridDim.x) {
-        if (k != (int)blockIdx.x) __syncthreads();
-        gather_read<Real, C, G, kSep>(a, a.list[k], smem);
+    if constexpr (!kStrided) {
+        if ((int)blockIdx.x < n) gather_read<Real, C, G, kSep>(a, a.list[blockIdx.x], smem);
+    } else {
+        for (int k = blockIdx.x; k < n; k += gridDim.x) {
+            if (k != (int)blockIdx.x) __syncthreads();
+            gather_read<Real, C, G, kSep>(a, a.list[k], smem);
+        }
     }
 }
 
@@ -981,12 +989,18 @@ hipError_t launch_t(const ClassifyArgs& a, hipStream_t stream) {
     if (nthreads < 64) nthreads = 64;
     nthreads = (nthreads + 63) / 64 * 64;
     Layout l = make_layout<Real>(a.m.T, C, a.tri, a.NB, a.cap, a.ngroups, G, nthreads);
-    const void* fn = (const void*)classify_kernel<Real, C, G, kSep>;
+    const bool strided = list_grid(a) < a.N;
+    const void* fn = strided ? (const void*)classify_kernel<Real, C, G, kSep, true>
+                             : (const void*)classify_kernel<Real, C, G, kSep, false>;
     if (l.total > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((classify_kernel<Real, C, G, kSep>), dim3(list_grid(a)), dim3(nthreads), l.total, stream, a);
+    if (strided)
+        hipLaunchKernelGGL((classify_kernel<Real, C, G, kSep, true>), dim3(list_grid(a)), dim3(nthreads), l.total, stream,
+                           a);
+    else
+        hipLaunchKernelGGL((classify_kernel<Real, C, G, kSep, false>), dim3(a.N), dim3(nthreads), l.total, stream, a);
     return hipGetLastError();
 }
 

@@ -61,7 +61,11 @@ __device__ inline float grp_max(float v) {
 // 32 KB (the classes of 10^5 simulated reads of that bin need 39 KB at most
 // and 11 KB at the median; the reserve for K_t = 16 at every timestep was
 // 50 KB, which held the kernel to 2 workgroups per CU).
-__host__ __device__ constexpr size_t v6_wcap(int NB) { return NB >= 16 ? 8192 : 0; }
+// Boxes of up to 32 states build each timestep's matrices just before its
+// step (kStaged): two timesteps' blocks at class NB.
+__host__ __device__ constexpr size_t v6_wcap(int NB, int C) {
+    return NB >= 32 ? 2 * (1 + C) * ((size_t)NB * NB + 4) : NB >= 16 ? 8192 : 0;
+}
 
 // Per-timestep constants of one step t >= 1 (lane-group kernel).
 struct V6Step {
@@ -148,21 +152,38 @@ __device__ __forceinline__ void v6_step(const V6Step& st, const int (&cix)[R], c
     ce += st.Xt + km;
 }
 
-// One read: the shared prologue, then its candidates, L lanes each.
-template <int C, int NB, int L, bool kClass>
+// Timestep t's (1 + C) matrices of class K, built by the whole workgroup.
+template <int C, int NB>
+__device__ inline void fill_step_w(const ReadTables& s, int tri, int t, int K, float* dst) {
+    const int ms = K * K + 4;
+    const int total = (1 + C) * ms;
+    for (int k = threadIdx.x; k < total; k += kReadThreads) {
+        const int mat = k / ms;
+        const int rem = k - mat * ms;
+        if (rem >= K * K) continue;  // the 4-entry gap between matrices
+        dst[k] = (float)w_entry<C, NB>(s, tri, t, mat, rem / K, rem % K);
+    }
+}
+
+// One read: the shared prologue, then its candidates, L lanes each, in
+// rounds of one candidate per group. kStaged: the composite matrices of each
+// timestep are built just before its step into one of two LDS buffers (the
+// groups step together, a barrier per timestep), instead of all timesteps'
+// in the prologue; for boxes whose classes' matrices outgrow the LDS.
+template <int C, int NB, int L, bool kStaged>
 __device__ __forceinline__ void v6_read(const ClassifyArgs& a, const int r, unsigned char* smem) {
     constexpr int NGR = kReadThreads / L;
     constexpr int R = 12 / L;  // T <= 12
-    constexpr int MSNB = NB * NB + 4;
+    constexpr int kWStep = (1 + C) * (NB * NB + 4);  // one timestep's block at class NB
     if (!in_bin(a, r)) return;
     const int T = a.m.T;
     const int tri = a.tri;
     const int tid = threadIdx.x;
     const int group = tid / L;
     const int p = tid % L;
-    const ReadLayout lay = read_layout(T, C, tri, NB, NGR, v6_wcap(NB));
+    const ReadLayout lay = read_layout(T, C, tri, NB, NGR, v6_wcap(NB, C));
     const ReadTables s = read_views(smem, lay);
-    if (!read_prologue<C, NB, kClass>(a, r, s, (int)((lay.binom - lay.W) / sizeof(float)))) return;
+    if (!read_prologue<C, NB, true, !kStaged>(a, r, s, (int)((lay.binom - lay.W) / sizeof(float)))) return;
     const int* sL = s.sL;
     const int* nbt = s.nbt;
     const int* kcl = s.kcl;
@@ -171,7 +192,7 @@ __device__ __forceinline__ void v6_read(const ClassifyArgs& a, const int r, unsi
     const float* kcf = s.kcf;
     const float* etab = s.etab;
     const int* xt = s.xt;
-    const float* Wall = s.Wall;
+    float* Wall = s.Wall;
     const double* gtab = s.gtab;
     const int* gx = s.gx;
     const double* b0d = s.b0d;
@@ -189,9 +210,14 @@ __device__ __forceinline__ void v6_read(const ClassifyArgs& a, const int r, unsi
 
     for (int base = 0; base < ncand; base += kReadThreads * 4) {
         const int nlive = compact_chunk<C>(a, r, ncand, cidx, base, read_empty, sL, s.list, s.scan);
-        for (int q0 = group; q0 < nlive; q0 += NGR) {
-            const int kk = s.list[q0];
-            const int i = cidx ? cidx[kk] : kk;
+        for (int r0 = 0; r0 < nlive; r0 += NGR) {
+            // staged: the previous round's last step may still read the
+            // buffer this round's first step fills
+            if (kStaged && r0 > 0) __syncthreads();
+            const int q0 = r0 + group;
+            const bool live = q0 < nlive;
+            const int kk = live ? s.list[q0] : 0;
+            const int i = live ? (cidx ? cidx[kk] : kk) : 0;
             int D[C];
 #pragma unroll
             for (int c = 0; c < C; c++) D[c] = a.dsD[(size_t)i * C + c];
@@ -250,9 +276,18 @@ __device__ __forceinline__ void v6_read(const ClassifyArgs& a, const int r, unsi
             }
 
             for (int t = 1; t < T; t++) {
-                if (ce == kNoExp) break;  // uniform over the group: every row is zero
+                const int Kt = kcl[t];  // uniform over the workgroup
+                float* Wt_t;
+                if constexpr (kStaged) {
+                    Wt_t = Wall + (t & 1) * kWStep;
+                    fill_step_w<C, NB>(s, tri, t, Kt, Wt_t);
+                    __syncthreads();
+                } else {
+                    Wt_t = Wall + woff[t];
+                }
+                if (!live || ce == kNoExp) continue;  // ce: uniform over the group (every row zero)
                 V6Step st;
-                st.Wt = Wall + (kClass ? woff[t] : t * (1 + C) * MSNB);
+                st.Wt = Wt_t;
                 st.nbp = nbt[t - 1];
                 st.t = t;
                 const float pd = (float)a.m.p_detach[t - 1];
@@ -264,15 +299,11 @@ __device__ __forceinline__ void v6_read(const ClassifyArgs& a, const int r, unsi
                 st.Gx = gx[t];
                 st.kct = kcf + (size_t)t * NB * C;
                 st.p = p;
-                if constexpr (!kClass) {
-                    v6_step<C, NB, L, R, NB>(st, cix, aK, cst, Kf, A, ce, sm, se);
-                    continue;
-                }
-                const int Kt = kcl[t];  // uniform over the workgroup
                 if (Kt <= 1) v6_step<C, NB, L, R, 1>(st, cix, aK, cst, Kf, A, ce, sm, se);
                 else if (Kt == 2) v6_step<C, NB, L, R, 2>(st, cix, aK, cst, Kf, A, ce, sm, se);
                 else if (Kt == 4 || NB == 4) v6_step<C, NB, L, R, (NB < 4 ? NB : 4)>(st, cix, aK, cst, Kf, A, ce, sm, se);
                 else if (Kt == 8 || NB == 8) v6_step<C, NB, L, R, (NB < 8 ? NB : 8)>(st, cix, aK, cst, Kf, A, ce, sm, se);
+                else if (Kt == 16 || NB == 16) v6_step<C, NB, L, R, (NB < 16 ? NB : 16)>(st, cix, aK, cst, Kf, A, ce, sm, se);
                 else v6_step<C, NB, L, R, NB>(st, cix, aK, cst, Kf, A, ce, sm, se);
             }
 
@@ -303,6 +334,7 @@ __device__ __forceinline__ void v6_read(const ClassifyArgs& a, const int r, unsi
                 sum = group_sum<double, L>(part);
             }
             const double logp = (sum > 0.0 && K != kNoExp) ? log(sum) + (double)K * 0.69314718055994530942 : -INFINITY;
+            if (!live) continue;
             if (a.out_pair_logp && !cidx && p == 0) a.out_pair_logp[(size_t)r * a.S + i] = logp;
             acc.add(logp, exp(logp), kk, a.dsCount[i]);
             if (collect && p == 0) tie_note(&s.ties[group], acc.best_v, logp, kk, a.tie_abs, a.tie_rel);
@@ -319,27 +351,29 @@ constexpr int v6_min_waves() {
 
 // Lanes per candidate by box capacity, as measured alone on 10^5 reads x
 // 1,000 dye-seqs (C = 3, T = 12): boxes <= 4 at 1 lane (3.6 ms against 4.0 at
-// 2), <= 8 at 2 (5.6 against 7.3 at 1), <= 16 at 4 (7.2 against 9.3 at 2).
+// 2), <= 8 at 2 (5.6 against 7.3 at 1), <= 16 at 4 (7.2 against 9.3 at 2);
+// boxes <= 32 at 4 (three rows of 32 entries per lane).
 constexpr int v6_lanes(int NB) { return NB <= 4 ? 1 : NB <= 8 ? 2 : 4; }
 
-template <int C, int NB, int L, bool kClass>
+template <int C, int NB, int L, bool kStaged>
 __global__ void __launch_bounds__(kReadThreads, (v6_min_waves<NB, L>())) classify_v6_kernel(const ClassifyArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     if ((int)blockIdx.x >= *a.list_count) return;
-    v6_read<C, NB, L, kClass>(a, a.list[blockIdx.x], smem);
+    v6_read<C, NB, L, kStaged>(a, a.list[blockIdx.x], smem);
 }
 
 template <int C, int NB>
 hipError_t launch_v6_nb(const ClassifyArgs& a, hipStream_t stream) {
     constexpr int L = v6_lanes(NB);
-    const ReadLayout l = read_layout(a.m.T, C, a.tri, NB, kReadThreads / L, v6_wcap(NB));
+    constexpr bool kStaged = NB >= 32;
+    const ReadLayout l = read_layout(a.m.T, C, a.tri, NB, kReadThreads / L, v6_wcap(NB, C));
     if (l.total > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)classify_v6_kernel<C, NB, L, true>,
+        hipError_t e = hipFuncSetAttribute((const void*)classify_v6_kernel<C, NB, L, kStaged>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
         if (e != hipSuccess) return e;
     }
     // one workgroup per read of the bin's list (workgroups past its length exit)
-    hipLaunchKernelGGL((classify_v6_kernel<C, NB, L, true>), dim3(a.N), dim3(kReadThreads), l.total, stream, a);
+    hipLaunchKernelGGL((classify_v6_kernel<C, NB, L, kStaged>), dim3(a.N), dim3(kReadThreads), l.total, stream, a);
     return hipGetLastError();
 }
 
@@ -349,6 +383,7 @@ hipError_t launch_v6_c(const ClassifyArgs& a, hipStream_t stream) {
     case 4: return launch_v6_nb<C, 4>(a, stream);
     case 8: return launch_v6_nb<C, 8>(a, stream);
     case 16: return launch_v6_nb<C, 16>(a, stream);
+    case 32: return launch_v6_nb<C, 32>(a, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -356,12 +391,12 @@ hipError_t launch_v6_c(const ClassifyArgs& a, hipStream_t stream) {
 }  // namespace
 
 bool v6_supports(int precision, int T, int NB) {
-    return precision == WP_PRECISION_FP32 && T <= 12 && (NB == 4 || NB == 8 || NB == 16);
+    return precision == WP_PRECISION_FP32 && T <= 12 && (NB == 4 || NB == 8 || NB == 16 || NB == 32);
 }
 
 size_t v6_lds_bytes(int T, int C, int tri, int NB) {
     const int L = v6_lanes(NB);
-    return read_layout(T, C, tri, NB, kReadThreads / L, v6_wcap(NB)).total;
+    return read_layout(T, C, tri, NB, kReadThreads / L, v6_wcap(NB, C)).total;
 }
 
 hipError_t launch_classify_v6(const ClassifyArgs& a, hipStream_t stream) {

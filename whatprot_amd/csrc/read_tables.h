@@ -171,7 +171,30 @@ __device__ inline bool in_bin(const ClassifyArgs& a, int r) {
 // Bleach matrices (per box class with kClass, else at NB) and the blocked /
 // detached backward tables. Returns false, uniformly over the workgroup, when
 // the read is handed to the wide pass instead. Block-wide.
-template <int C, int NB, bool kClass>
+// Entry (q, q2) of matrix mat of timestep t >= 1 (0: the composite Bleach
+// Wt between the boxes of t - 1 and t; 1 + c: its Edman-shifted form W1[c],
+// which maps onto n'(q2) + 1_c and is zero at or above
+// U = min(Hr[t] + 1, Hr[t-1]), edman-transition.cc:28-60), from the read's
+// box coordinates and bleach tables.
+template <int C, int NB>
+__device__ inline double w_entry(const ReadTables& s, int tri, int t, int mat, int q, int q2) {
+    if (!(q < s.nbt[t - 1] && q2 < s.nbt[t])) return 0.0;
+    const double* ble = s.bin + C * tri;
+    double v = 1.0;
+    const int sc = mat - 1;  // shifted channel, -1 for Wt
+    for (int c = 0; c < C; c++) {
+        const int from = s.coord[((t - 1) * NB + q) * C + c];
+        const int to = s.coord[(t * NB + q2) * C + c] + (c == sc ? 1 : 0);
+        if (c == sc && to >= min(s.sH[t * C + c] + 1, s.sH[(t - 1) * C + c])) v = 0.0;
+        if (from < to) v = 0.0;
+        else v *= ble[c * tri + from * (from + 1) / 2 + to];
+    }
+    return v;
+}
+
+// kFillW false: the W block is left to the caller (built per timestep), and
+// no W capacity applies.
+template <int C, int NB, bool kClass, bool kFillW = true>
 __device__ __forceinline__ bool read_prologue(const ClassifyArgs& a, const int r, const ReadTables& s, int wcap) {
     constexpr int MSNB = NB * NB + 4;
     const int T = a.m.T;
@@ -228,7 +251,7 @@ __device__ __forceinline__ bool read_prologue(const ClassifyArgs& a, const int r
             woff[t] = o;
             if (t >= 1) o += (1 + C) * (k * k + 4);
         }
-        if (o > wcap) scan[0] = 1;  // the classes' matrices exceed the W block: wide pass
+        if (kFillW && o > wcap) scan[0] = 1;  // the classes' matrices exceed the W block: wide pass
     }
     __syncthreads();
     // peptide-emission.cc:165-180
@@ -285,15 +308,14 @@ __device__ __forceinline__ bool read_prologue(const ClassifyArgs& a, const int r
     }
     if (wide) return false;
     for (int k = tid; k < T * NB; k += kReadThreads) etab[k] = (float)ldexp(ev[k], -xt[k / NB]);
-    // Composite Bleach between the boxes of t-1 and t and its Edman-shifted
-    // forms: W1[c][q][q2] maps onto n'(q2) + 1_c and is zero at or above
-    // U = min(Hr[t] + 1, Hr[t-1]) (edman-transition.cc:28-60). Timestep t's
-    // matrices are K_t x K_t, K_t + 4 entries apart (different LDS banks for
-    // the shifted matrices of different channels).
-    const double* ble = bin + C * tri;
+    // Composite Bleach matrices of every timestep (unless the caller builds
+    // each timestep's just before its step): timestep t's matrices are
+    // K_t x K_t, K_t * K_t + 4 entries apart (different LDS banks for the
+    // shifted matrices of different channels).
     // entries of the W blocks of t >= 1: (1 + C) matrices of K_t x K_t each
-    const int wtotal = kClass ? woff[T - 1] + (T > 1 ? (1 + C) * (kcl[T - 1] * kcl[T - 1] + 4) : 0)
-                              : (T - 1) * (1 + C) * MSNB;
+    const int wtotal = !kFillW ? 0
+                       : kClass ? woff[T - 1] + (T > 1 ? (1 + C) * (kcl[T - 1] * kcl[T - 1] + 4) : 0)
+                                : (T - 1) * (1 + C) * MSNB;
     for (int k = tid; k < wtotal; k += kReadThreads) {
         int t = 1, K = NB, rem;
         if constexpr (kClass) {
@@ -309,19 +331,7 @@ __device__ __forceinline__ bool read_prologue(const ClassifyArgs& a, const int r
         rem -= mat * ms;
         if (rem >= K * K) continue;  // the 4-entry gap between matrices
         const int q = rem / K, q2 = rem % K;
-        double v = 0.0;
-        if (t >= 1 && q < nbt[t - 1] && q2 < nbt[t]) {
-            v = 1.0;
-            const int sc = mat - 1;  // shifted channel, -1 for Wt
-            for (int c = 0; c < C; c++) {
-                const int from = coord[((t - 1) * NB + q) * C + c];
-                const int to = coord[(t * NB + q2) * C + c] + (c == sc ? 1 : 0);
-                if (c == sc && to >= min(sH[t * C + c] + 1, sH[(t - 1) * C + c])) v = 0.0;
-                if (from < to) v = 0.0;
-                else v *= ble[c * tri + from * (from + 1) / 2 + to];
-            }
-        }
-        Wall[(kClass ? woff[t] : t * (1 + C) * MSNB) + mat * ms + rem] = (float)v;
+        Wall[(kClass ? woff[t] : t * (1 + C) * MSNB) + mat * ms + rem] = (float)w_entry<C, NB>(s, tri, t, mat, q, q2);
     }
     // Blocked and detached mass (one wave, lane q = box entry q): from level
     // t + 1 = u, beta_u = bq 2^eb and phi_(u+1) = ph 2^eb, to level t, both
@@ -334,6 +344,7 @@ __device__ __forceinline__ bool read_prologue(const ClassifyArgs& a, const int r
     // wide pass: below that every product of the next level (Bleach >= 2^-65,
     // relative densities >= 2^-kMaxDensitySpan) is a normal double, so no share
     // underflows unseen.
+    const double* ble = bin + C * tri;
     if (tid < 64) {
         const int q = tid;
         double bq = q < nbt[T - 1] ? 1.0 : 0.0;
