@@ -296,8 +296,9 @@ def shortlist_leg(args, steps, warmup=1, fp64_steps=2):
     simulated reads of 20,000 dye-seqs, then N reads classified by FP32 HMM
     rescoring of each read's top-256 (full-HMM fallback below shortlist
     posterior mass 0.5), against the plain FP64 GPU HMM over every dye-seq.
-    Both sides are timed the same way: host arrays in and out, W untimed
-    steps, then K steps between synchronizations."""
+    Both sides are timed the same way: radiometries resident in device
+    memory, outputs to device memory, W untimed steps, then K steps between
+    synchronizations."""
     import numpy as np
     import torch
 
@@ -334,13 +335,17 @@ def shortlist_leg(args, steps, warmup=1, fp64_steps=2):
     flops_step = 2 * 4096 * (64 * 512 + 2 * 512 * 512 + 512 * S) * 3
     train_tflops = flops_step * (args.train_reads // 4096) * args.epochs / t_train / 1e12
     log("shortlist: plain FP64 GPU HMM over every dye-seq")
-    res64 = {}
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    d_qx = torch.from_numpy(np.ascontiguousarray(qx)).to(dev)
+    d_id = torch.empty(N, dtype=torch.int32, device=dev)
+    d_adj = torch.empty(N, dtype=torch.float64, device=dev)
 
     def step64():
-        res64["ids"] = clf64.classify_arrays(qx)[0]
+        clf64.classify_device(N, d_qx.data_ptr(), d_id.data_ptr(), d_adj.data_ptr(), 0, 0, stream.cuda_stream)
 
     t64 = timed_steps(step64, warmup, fp64_steps)
-    ids64 = res64["ids"]
+    ids64 = d_id.cpu().numpy()
     log(f"shortlist: FP64 {t64 * 1e3:.1f} ms/step; shortlist steps")
     idx, post = net.shortlist(qx, H)
     recall = float(np.mean([ids64[i] in idx[i] for i in range(N)]))
@@ -349,10 +354,11 @@ def shortlist_leg(args, steps, warmup=1, fp64_steps=2):
         res = {}
 
         def step_sl():
-            res["out"] = net.classify(clf32, qx, H, thr, crit)
+            res["nf"] = net.classify_device(clf32, N, d_qx.data_ptr(), H, d_id.data_ptr(), d_adj.data_ptr(), thr,
+                                            crit, stream.cuda_stream)
 
         times[key] = timed_steps(step_sl, warmup, steps)
-        out[key] = res["out"]
+        out[key] = (d_id.cpu().numpy(), d_adj.cpu().numpy(), res["nf"])
     ids, adj, nf = out["mass"]
     return {
         "workload": f"shortlist: MLP 36->512x3->{S} (BF16 MFMA) trained on {args.train_reads} simulated reads "
@@ -368,8 +374,8 @@ def shortlist_leg(args, steps, warmup=1, fp64_steps=2):
         "recall_fp64_top1_at_256": recall,
         "top1_vs_fp64": float(np.mean(ids == ids64)),
         "fallback_fraction": nf / N,
-        "timing": f"host arrays in and out on both sides, {warmup} untimed step(s), then {steps} (shortlist) and "
-                  f"{fp64_steps} (FP64) steps between synchronizations",
+        "timing": f"radiometries resident in device memory and device outputs on both sides, {warmup} untimed "
+                  f"step(s), then {steps} (shortlist) and {fp64_steps} (FP64) steps between synchronizations",
         "labels": args.labels,
         "label_s": t_label,
         "train_s": t_train,

@@ -241,6 +241,13 @@ int wp_hmm_get_variants(const wp_hmm_classifier* clf, int* out_variant, double* 
  * enable 0 launches every kernel eagerly. */
 int wp_hmm_set_graph(wp_hmm_classifier* clf, int enable);
 
+/* Parallel bins (no reference counterpart). Once its reads are binned, a call
+ * runs its bins as three parallel branches (the caller's stream and two of
+ * the classifier's, joined before the call returns); enable 0 runs them in
+ * order on the caller's stream, so each kernel's duration is its own (for
+ * profiling). Outputs are identical either way. */
+int wp_hmm_set_branches(wp_hmm_classifier* clf, int enable);
+
 /* FP64 decision of FP32 near-ties (no reference counterpart; it makes the FP32
  * path pick the candidate HMMClassifier::classify_helper picks,
  * classifiers/hmm-classifier.h:59-65). An FP32 classifier keeps, per read,
@@ -386,6 +393,14 @@ enum { WP_SHORTLIST_MAX_POSTERIOR = 0, WP_SHORTLIST_MASS = 1 };
 int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* mlp, int num_reads, const double* radiometries, int H,
                           double min_posterior, int criterion, int* out_id, double* out_adjusted,
                           int* out_num_fallback);
+
+/* Device-resident form of wp_shortlist_classify (no reference counterpart):
+ * radiometries and outputs in device memory. The call first waits for
+ * `stream`, and returns when the outputs are written; the MLP's and the
+ * shortlist's buffers are kept by the MLP between calls. */
+int wp_shortlist_classify_device(wp_hmm_classifier* clf, wp_mlp* mlp, int num_reads, const double* d_radiometries,
+                                 int H, double min_posterior, int criterion, int* d_out_id, double* d_out_adjusted,
+                                 int* out_num_fallback, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Baum-Welch fit. Replaces HMMFitter (fitters/hmm-fitter.h:30-151) as run by

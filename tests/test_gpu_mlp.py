@@ -100,6 +100,36 @@ def test_shortlist_classify(gpu, small_task):
     assert np.mean(ids == full_ids) > 0.95
 
 
+def test_shortlist_classify_device_matches_host(gpu, small_task):
+    """wp_shortlist_classify_device on device arrays gives the host entry
+    point's ids and adjusted scores bit for bit: every read sure, every read
+    falling back, and a threshold that splits them (sure reads gathered and
+    scattered back, fallback results scattered into place)."""
+    import torch
+    C, T, S, model, strings, train_x, train_y, test_x, test_y = small_task
+    net = w.ShortlistMLP(T * C, 128, 3, S, seed=11)
+    net.train(train_x.reshape(len(train_x), -1), train_y, batch=512, epochs=2, lr=2e-3)
+    clf = w.HMMClassifier(T, C, model, w.SequencingSettings(5.0), [w.DyeSeqEntry(s, 1, i) for i, s in
+                                                                   enumerate(strings)], w.WP_PRECISION_FP32)
+    N = len(test_x)
+    dev = torch.device("cuda:0")
+    d_x = torch.from_numpy(np.ascontiguousarray(test_x, dtype=np.float64)).to(dev)
+    d_id = torch.full((N,), -7, dtype=torch.int32, device=dev)
+    d_adj = torch.full((N,), 7.0, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    _, post, mass = net.shortlist(test_x, 8, with_mass=True)
+    split = float(np.median(mass))
+    for thr in (0.0, 2.0, split):
+        ids, adj, nf = net.classify(clf, test_x, 8, min_posterior=thr, criterion="mass")
+        nfd = net.classify_device(clf, N, d_x.data_ptr(), 8, d_id.data_ptr(), d_adj.data_ptr(), min_posterior=thr,
+                                  criterion="mass", stream=stream.cuda_stream)
+        assert nfd == nf
+        if thr == split:
+            assert 0 < nf < N
+        np.testing.assert_array_equal(d_id.cpu().numpy(), ids)
+        np.testing.assert_array_equal(d_adj.cpu().numpy(), adj)
+
+
 @pytest.mark.parametrize("S", [40, 700, 5000, 20000, 40000])
 def test_shortlist_topk_matches_logits(gpu, S):
     """The shortlist is the top H of the network's own logits (ties to the

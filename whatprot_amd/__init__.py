@@ -394,6 +394,12 @@ class HMMClassifier:
         """Graph replay of the device entry points' launch sequence."""
         check(lib().wp_hmm_set_graph(self._h, 1 if enable else 0))
 
+    def set_branches(self, enable=True):
+        """Bins as three parallel branches (default) or in order on the
+        caller's stream (identical outputs; per-kernel durations for
+        profiling)."""
+        check(lib().wp_hmm_set_branches(self._h, 1 if enable else 0))
+
     def set_refine(self, enable=True, abs=2e-5, rel=1e-5):
         """FP32 classifiers: decide the candidates within abs + rel |ln P_best|
         of the FP32 best in FP64 (default on, with these defaults)."""
@@ -952,3 +958,15 @@ class ShortlistMLP:
         check(lib().wp_shortlist_classify(classifier._h, self._h, N, r.ctypes.data, int(H), float(min_posterior),
                                           crit, ids.ctypes.data, adj.ctypes.data, ctypes.byref(nf)))
         return ids, adj, nf.value
+
+    def classify_device(self, classifier: "HMMClassifier", num_reads, d_rads, H, d_id, d_adjusted,
+                        min_posterior=0.5, criterion="max", stream=0):
+        """wp_shortlist_classify_device on device pointers (radiometries
+        [N][T][C] FP64 in, ids int32 and adjusted FP64 out); returns the
+        number of full-HMM fallbacks."""
+        nf = ctypes.c_int()
+        crit = {"max": 0, "mass": 1}[criterion]
+        check(lib().wp_shortlist_classify_device(classifier._h, self._h, int(num_reads), d_rads, int(H),
+                                                 float(min_posterior), crit, d_id, d_adjusted, ctypes.byref(nf),
+                                                 stream))
+        return nf.value

@@ -129,6 +129,8 @@ def lib():
     L.wp_hmm_get_variants.argtypes = [c_void_p, c_void_p, c_void_p, c_int]
     L.wp_hmm_set_graph.restype = c_int
     L.wp_hmm_set_graph.argtypes = [c_void_p, c_int]
+    L.wp_hmm_set_branches.restype = c_int
+    L.wp_hmm_set_branches.argtypes = [c_void_p, c_int]
     L.wp_read_radiometries.restype = c_int
     L.wp_read_radiometries.argtypes = [ctypes.c_char_p, c_void_p, P(RadiometriesC)]
     L.wp_free_radiometries.restype = None
@@ -231,6 +233,9 @@ def lib():
     L.wp_shortlist_classify.restype = c_int
     L.wp_shortlist_classify.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_int, c_double, c_int, c_void_p,
                                         c_void_p, P(c_int)]
+    L.wp_shortlist_classify_device.restype = c_int
+    L.wp_shortlist_classify_device.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_int, c_double, c_int,
+                                               c_void_p, c_void_p, P(c_int), c_void_p]
     L.wp_fit_offset_exponential.restype = c_int
     L.wp_fit_offset_exponential.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_void_p]
     _lib = L

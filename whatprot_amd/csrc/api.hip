@@ -185,6 +185,9 @@ struct wp_hmm_classifier {
     // the device entry points replay a captured graph of the launch sequence
     // while the call's shape and pointers repeat
     int use_graph = 1;
+    // the bins of a call run as three parallel branches (0: in order on the
+    // caller's stream, as the no-allocation entry point always does)
+    int branches = 1;
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<const void*> gkey;
@@ -671,7 +674,7 @@ int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_c
             c->variants_set = 1;
         }
     }
-    if (!graph_ok || !c->use_graph || c->timing) return enqueue_call(c, x, stream, c->timing != 0);
+    if (!graph_ok || !c->use_graph || c->timing) return enqueue_call(c, x, stream, c->timing != 0, c->branches != 0);
     std::vector<const void*> key = {(const void*)(intptr_t)N, d_rads, d_cand_off, d_cand_idx, d_id, d_adj,
                                     d_ls, d_lt, d_pair, (const void*)(intptr_t)x.tie_mode};
     for (int b = 0; b < wp::kMaxBins; b++) key.push_back((const void*)(intptr_t)c->variant[b]);
@@ -679,7 +682,7 @@ int run_device(wp_hmm_classifier* c, int N, const double* d_rads, const int* d_c
         drop_graph(c);
         if (!c->cap_stream) WP_HIP(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
         WP_HIP(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeRelaxed));
-        rc = enqueue_call(c, x, c->cap_stream, false);
+        rc = enqueue_call(c, x, c->cap_stream, false, c->branches != 0);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
         if (rc) {
@@ -1135,6 +1138,16 @@ int wp_hmm_set_graph(wp_hmm_classifier* c, int enable) {
         return WP_ERR_ARG;
     }
     c->use_graph = enable ? 1 : 0;
+    drop_graph(c);
+    return WP_OK;
+}
+
+int wp_hmm_set_branches(wp_hmm_classifier* c, int enable) {
+    if (!c) {
+        wp::set_error("null classifier");
+        return WP_ERR_ARG;
+    }
+    c->branches = enable ? 1 : 0;
     drop_graph(c);
     return WP_OK;
 }

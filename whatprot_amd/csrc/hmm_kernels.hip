@@ -1069,33 +1069,50 @@ __global__ void zero_ints_kernel(int* d, int n) {
 
 // Appends read r to its bin's list. The order inside a list follows the
 // atomics; every read's outputs are computed independently of that order.
+// Appends read r (lane's b >= 0) to list b: one atomic per wave and bin
+// (the lanes of a bin take consecutive slots in lane order). Every lane of
+// the wave calls it; b < 0 appends nothing.
+__device__ inline void append_wave(int b, int r, int N, int* __restrict__ lists, int* __restrict__ counts) {
+    const int lane = (int)__lane_id();
+    unsigned long long pending = __ballot(b >= 0);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int lb = __shfl(b, leader);
+        const unsigned long long mask = __ballot(b == lb);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counts[lb], (int)__popcll(mask));
+        base = __shfl(base, leader);
+        if (b == lb) lists[(size_t)lb * N + base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = r;
+        pending &= ~mask;
+    }
+}
+
 __global__ void bin_reads_kernel(int N, const int* __restrict__ nbmax, BinEdges e, int* __restrict__ lists,
                                  int* __restrict__ counts) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= N) return;
-    const int nb = nbmax[r];
-    int b = 0;
-    if (nb > 0) {
-        b = -1;
-        for (int k = 0; k < e.nbins; k++)
-            if (nb > e.edge[k] && nb <= e.edge[k + 1]) b = k;
+    int b = -1;  // outside every bin (or past N): no launch takes it
+    if (r < N) {
+        const int nb = nbmax[r];
+        b = 0;
+        if (nb > 0) {
+            b = -1;
+            for (int k = 0; k < e.nbins; k++)
+                if (nb > e.edge[k] && nb <= e.edge[k + 1]) b = k;
+        }
     }
-    if (b < 0) return;  // outside every bin: no launch takes it
-    lists[(size_t)b * N + atomicAdd(&counts[b], 1)] = r;
+    append_wave(b, r, N, lists, counts);
 }
-
 __global__ void bin_refine_kernel(int N, const int* __restrict__ nbmax, const int* __restrict__ tie_n, BinEdges e,
                                   int* __restrict__ lists, int* __restrict__ counts) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= N || tie_n[r] == 0) return;
-    const int nb = nbmax[r];
     int b = -1;
-    for (int k = 0; k < e.nbins; k++)
-        if (nb > e.edge[k] && nb <= e.edge[k + 1]) b = k;
-    if (b < 0) return;
-    lists[(size_t)b * N + atomicAdd(&counts[b], 1)] = r;
+    if (r < N && tie_n[r] != 0) {
+        const int nb = nbmax[r];
+        for (int k = 0; k < e.nbins; k++)
+            if (nb > e.edge[k] && nb <= e.edge[k + 1]) b = k;
+    }
+    append_wave(b, r, N, lists, counts);
 }
-
 __global__ void refine_fix_kernel(int N, const int* __restrict__ tie_n, const int* __restrict__ rid,
                                   const double* __restrict__ rls, const double* __restrict__ lt, int* __restrict__ id,
                                   double* __restrict__ adj, double* __restrict__ ls) {

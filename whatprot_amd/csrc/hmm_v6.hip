@@ -61,10 +61,11 @@ __device__ inline float grp_max(float v) {
 // 32 KB (the classes of 10^5 simulated reads of that bin need 39 KB at most
 // and 11 KB at the median; the reserve for K_t = 16 at every timestep was
 // 50 KB, which held the kernel to 2 workgroups per CU).
-// Boxes of up to 32 states build each timestep's matrices just before its
-// step (kStaged): two timesteps' blocks at class NB.
-__host__ __device__ constexpr size_t v6_wcap(int NB, int C) {
-    return NB >= 32 ? 2 * (1 + C) * ((size_t)NB * NB + 4) : NB >= 16 ? 8192 : 0;
+// Staged launches (boxes of up to 32 states, or variant 1 of the 16-state
+// bin) build each timestep's matrices just before its step: two timesteps'
+// blocks at class NB.
+__host__ __device__ constexpr size_t v6_wcap(int NB, int C, bool staged) {
+    return staged ? 2 * (1 + C) * ((size_t)NB * NB + 4) : NB >= 16 ? 8192 : 0;
 }
 
 // Per-timestep constants of one step t >= 1 (lane-group kernel).
@@ -181,7 +182,7 @@ __device__ __forceinline__ void v6_read(const ClassifyArgs& a, const int r, unsi
     const int tid = threadIdx.x;
     const int group = tid / L;
     const int p = tid % L;
-    const ReadLayout lay = read_layout(T, C, tri, NB, NGR, v6_wcap(NB, C));
+    const ReadLayout lay = read_layout(T, C, tri, NB, NGR, v6_wcap(NB, C, kStaged));
     const ReadTables s = read_views(smem, lay);
     if (!read_prologue<C, NB, true, !kStaged>(a, r, s, (int)((lay.binom - lay.W) / sizeof(float)))) return;
     const int* sL = s.sL;
@@ -362,11 +363,10 @@ __global__ void __launch_bounds__(kReadThreads, (v6_min_waves<NB, L>())) classif
     v6_read<C, NB, L, kStaged>(a, a.list[blockIdx.x], smem);
 }
 
-template <int C, int NB>
-hipError_t launch_v6_nb(const ClassifyArgs& a, hipStream_t stream) {
+template <int C, int NB, bool kStaged>
+hipError_t launch_v6_s(const ClassifyArgs& a, hipStream_t stream) {
     constexpr int L = v6_lanes(NB);
-    constexpr bool kStaged = NB >= 32;
-    const ReadLayout l = read_layout(a.m.T, C, a.tri, NB, kReadThreads / L, v6_wcap(NB, C));
+    const ReadLayout l = read_layout(a.m.T, C, a.tri, NB, kReadThreads / L, v6_wcap(NB, C, kStaged));
     if (l.total > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)classify_v6_kernel<C, NB, L, kStaged>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.total);
@@ -375,6 +375,15 @@ hipError_t launch_v6_nb(const ClassifyArgs& a, hipStream_t stream) {
     // one workgroup per read of the bin's list (workgroups past its length exit)
     hipLaunchKernelGGL((classify_v6_kernel<C, NB, L, kStaged>), dim3(a.N), dim3(kReadThreads), l.total, stream, a);
     return hipGetLastError();
+}
+
+template <int C, int NB>
+hipError_t launch_v6_nb(const ClassifyArgs& a, hipStream_t stream) {
+    if constexpr (NB >= 32) return launch_v6_s<C, NB, true>(a, stream);
+    if constexpr (NB >= 16) {
+        if (a.variant) return launch_v6_s<C, NB, true>(a, stream);
+    }
+    return launch_v6_s<C, NB, false>(a, stream);
 }
 
 template <int C>
@@ -396,7 +405,8 @@ bool v6_supports(int precision, int T, int NB) {
 
 size_t v6_lds_bytes(int T, int C, int tri, int NB) {
     const int L = v6_lanes(NB);
-    return read_layout(T, C, tri, NB, kReadThreads / L, v6_wcap(NB, C)).total;
+    return std::max(read_layout(T, C, tri, NB, kReadThreads / L, v6_wcap(NB, C, NB >= 32)).total,
+                    read_layout(T, C, tri, NB, kReadThreads / L, v6_wcap(NB, C, NB >= 16)).total);
 }
 
 hipError_t launch_classify_v6(const ClassifyArgs& a, hipStream_t stream) {

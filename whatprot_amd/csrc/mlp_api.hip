@@ -37,6 +37,17 @@ struct wp_mlp {
     float* lse = nullptr;
     float* mass = nullptr;
     float* csum = nullptr;  // row-chunk partial sums of the bias gradients
+    // shortlist classification workspace (reads x H candidates), kept across calls
+    int sl_cap = 0, sl_H = 0, sl_off_n = -1, sl_off_h = 0;
+    double* sl_rads = nullptr;  // the host entry point's upload
+    int* sl_cand = nullptr;
+    int* sl_off = nullptr;
+    float* sl_stat = nullptr;  // [3][cap]: largest posterior, mass, log-sum-exp
+    int* sl_id = nullptr;
+    double* sl_adj = nullptr;
+    int* sl_sel = nullptr;
+    double* sl_rads2 = nullptr;
+    int* sl_cand2 = nullptr;
 };
 
 namespace {
@@ -57,6 +68,9 @@ void free_mlp(wp_mlp* m) {
     hipFree(m->lse);
     hipFree(m->mass);
     hipFree(m->csum);
+    for (void* p : {(void*)m->sl_rads, (void*)m->sl_cand, (void*)m->sl_off, (void*)m->sl_stat, (void*)m->sl_id,
+                    (void*)m->sl_adj, (void*)m->sl_sel, (void*)m->sl_rads2, (void*)m->sl_cand2})
+        hipFree(p);
     delete m;
 }
 
@@ -462,43 +476,51 @@ int wp_mlp_shortlist(wp_mlp* m, int N, const double* radiometries, int H, int* o
     return rc;
 }
 
-int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, const double* radiometries, int H,
-                          double min_posterior, int criterion, int* out_id, double* out_adjusted,
-                          int* out_num_fallback) {
-    if (!clf || !m || num_reads < 0 || H < 1 || (num_reads > 0 && (!radiometries || !out_id || !out_adjusted))) {
-        wp::set_error("null argument");
-        return WP_ERR_ARG;
-    }
-    if (criterion != WP_SHORTLIST_MAX_POSTERIOR && criterion != WP_SHORTLIST_MASS) {
-        wp::set_error("unknown shortlist criterion");
-        return WP_ERR_ARG;
-    }
-    const int N = num_reads, S = wp_hmm_classifier_num_dye_seqs(clf), d = m->in_dim;
-    if (m->out_dim != S) {
-        wp::set_error("the MLP's outputs must be the classifier's dye-seqs");
-        return WP_ERR_ARG;
-    }
-    if (out_num_fallback) *out_num_fallback = 0;
-    if (N == 0) return WP_OK;
+}  // extern "C"
+
+namespace {
+
+// Grows the shortlist workspace to n reads of H candidates.
+int ensure_shortlist(wp_mlp* m, int n, int H) {
+    if (n <= m->sl_cap && H <= m->sl_H) return WP_OK;
+    const int cap = std::max(n, m->sl_cap), h = std::max(H, m->sl_H);
+    for (void* p : {(void*)m->sl_rads, (void*)m->sl_cand, (void*)m->sl_off, (void*)m->sl_stat, (void*)m->sl_id,
+                    (void*)m->sl_adj, (void*)m->sl_sel, (void*)m->sl_rads2, (void*)m->sl_cand2})
+        hipFree(p);
+    m->sl_rads = m->sl_rads2 = nullptr;
+    m->sl_cand = m->sl_off = m->sl_id = m->sl_sel = m->sl_cand2 = nullptr;
+    m->sl_stat = nullptr;
+    m->sl_adj = nullptr;
+    m->sl_cap = m->sl_H = 0;
+    m->sl_off_n = -1;
+    const size_t d = (size_t)m->in_dim;
+    int rc = dalloc(&m->sl_rads, (size_t)cap * d);
+    if (!rc) rc = dalloc(&m->sl_cand, (size_t)cap * h);
+    if (!rc) rc = dalloc(&m->sl_off, (size_t)cap + 1);
+    if (!rc) rc = dalloc(&m->sl_stat, (size_t)3 * cap);
+    if (!rc) rc = dalloc(&m->sl_id, (size_t)cap);
+    if (!rc) rc = dalloc(&m->sl_adj, (size_t)cap);
+    if (!rc) rc = dalloc(&m->sl_sel, (size_t)cap);
+    if (!rc) rc = dalloc(&m->sl_rads2, (size_t)cap * d);
+    if (!rc) rc = dalloc(&m->sl_cand2, (size_t)cap * h);
+    if (rc) return rc;
+    m->sl_cap = cap;
+    m->sl_H = h;
+    return WP_OK;
+}
+
+// The shortlist call on device radiometries (N x in_dim) into device outputs;
+// h_rads: the same radiometries on the host, or null (the fallback reads are
+// then copied down). Runs on the null stream and returns when the outputs
+// are written.
+int shortlist_run(wp_hmm_classifier* clf, wp_mlp* m, int N, const double* d_rads, const double* h_rads, int H,
+                  double min_posterior, int criterion, int* d_out_id, double* d_out_adj, int* out_num_fallback) {
+    const int S = wp_hmm_classifier_num_dye_seqs(clf), d = m->in_dim;
     H = std::min(H, S);
     const int chunk = std::max(1, std::min(N, (int)std::max<long>(1, (512l << 20) / (4l * m->out_dim))));
     int rc = ensure_batch(m, chunk);
+    if (!rc) rc = ensure_shortlist(m, N, H);
     if (rc) return rc;
-    // everything stays on the device: radiometries, shortlists, confidences
-    double* d_rads = nullptr;
-    int *d_cand = nullptr, *d_off = nullptr, *d_id = nullptr, *d_sel = nullptr, *d_cand2 = nullptr;
-    float *d_post = nullptr, *d_mass = nullptr, *d_lse = nullptr;
-    double *d_adj = nullptr, *d_rads2 = nullptr;
-    std::vector<int> off(N + 1);
-    for (int r = 0; r <= N; r++) off[r] = r * H;
-    rc = dalloc(&d_rads, (size_t)N * d);
-    if (!rc) rc = dalloc(&d_cand, (size_t)N * H);
-    if (!rc) rc = dalloc(&d_off, (size_t)N + 1);
-    if (!rc) rc = dalloc(&d_post, (size_t)N);
-    if (!rc) rc = dalloc(&d_mass, (size_t)N);
-    if (!rc) rc = dalloc(&d_lse, (size_t)N);
-    if (!rc) rc = dalloc(&d_id, (size_t)N);
-    if (!rc) rc = dalloc(&d_adj, (size_t)N);
     auto hip_ok = [&](hipError_t e, const char* what) {
         if (e != hipSuccess && !rc) {
             wp::set_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -506,8 +528,17 @@ int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, cons
         }
         return rc == WP_OK;
     };
-    if (!rc) hip_ok(hipMemcpy(d_rads, radiometries, sizeof(double) * (size_t)N * d, hipMemcpyHostToDevice), "upload");
-    if (!rc) hip_ok(hipMemcpy(d_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice), "upload");
+    if (m->sl_off_n != N || m->sl_off_h != H) {  // candidate offsets r * H, uploaded once per shape
+        std::vector<int> offs(N + 1);
+        for (int r = 0; r <= N; r++) offs[r] = r * H;
+        if (!hip_ok(hipMemcpy(m->sl_off, offs.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice), "upload"))
+            return rc;
+        m->sl_off_n = N;
+        m->sl_off_h = H;
+    }
+    float* d_post = m->sl_stat;
+    float* d_mass = m->sl_stat + m->sl_cap;
+    float* d_lse = m->sl_stat + 2 * (size_t)m->sl_cap;
     // the chunks' statistics go straight to per-read arrays: no copy, and no
     // host synchronisation, between chunks
     for (int r0 = 0; r0 < N && !rc; r0 += chunk) {
@@ -516,7 +547,7 @@ int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, cons
             break;
         rc = forward(m, B);
         if (rc) break;
-        hip_ok(wp::topk_softmax(B, m->out_dim, H, m->Z, m->out_dim, d_cand + (size_t)r0 * H, d_lse + r0,
+        hip_ok(wp::topk_softmax(B, m->out_dim, H, m->Z, m->out_dim, m->sl_cand + (size_t)r0 * H, d_lse + r0,
                                 d_post + r0, d_mass + r0, nullptr),
                "top-k");
     }
@@ -527,40 +558,41 @@ int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, cons
     for (int r = 0; r < N && !rc; r++) (conf[r] >= min_posterior ? sure : unsure).push_back(r);
     if (!rc && !sure.empty()) {  // HMM rescoring of the shortlists, on the device
         const int ns = (int)sure.size();
-        const double* rs = d_rads;
-        const int* cs = d_cand;
-        if (ns < N) {
-            rc = dalloc(&d_sel, (size_t)ns);
-            if (!rc) rc = dalloc(&d_rads2, (size_t)ns * d);
-            if (!rc) rc = dalloc(&d_cand2, (size_t)ns * H);
-            if (!rc) hip_ok(hipMemcpy(d_sel, sure.data(), sizeof(int) * ns, hipMemcpyHostToDevice), "upload");
-            if (!rc) hip_ok(wp::gather_rows(ns, d, d_rads, d_sel, d_rads2, nullptr), "gather");
-            if (!rc) hip_ok(wp::gather_rows(ns, H, d_cand, d_sel, d_cand2, nullptr), "gather");
-            rs = d_rads2;
-            cs = d_cand2;
-        }
-        if (!rc) rc = wp_hmm_classify_candidates_device(clf, ns, rs, d_off, cs, d_id, d_adj, nullptr, nullptr, nullptr);
-        std::vector<int> id(ns);
-        std::vector<double> adj(ns);
-        if (!rc) hip_ok(hipMemcpy(id.data(), d_id, sizeof(int) * ns, hipMemcpyDeviceToHost), "download");
-        if (!rc) hip_ok(hipMemcpy(adj.data(), d_adj, sizeof(double) * ns, hipMemcpyDeviceToHost), "download");
-        for (int i = 0; i < ns && !rc; i++) {
-            out_id[sure[i]] = id[i];
-            out_adjusted[sure[i]] = adj[i];
+        if (ns == N) {
+            rc = wp_hmm_classify_candidates_device(clf, N, d_rads, m->sl_off, m->sl_cand, d_out_id, d_out_adj, nullptr,
+                                                   nullptr, nullptr);
+        } else {
+            if (hip_ok(hipMemcpy(m->sl_sel, sure.data(), sizeof(int) * ns, hipMemcpyHostToDevice), "upload") &&
+                hip_ok(wp::gather_rows(ns, d, d_rads, m->sl_sel, m->sl_rads2, nullptr), "gather") &&
+                hip_ok(wp::gather_rows(ns, H, m->sl_cand, m->sl_sel, m->sl_cand2, nullptr), "gather"))
+                rc = wp_hmm_classify_candidates_device(clf, ns, m->sl_rads2, m->sl_off, m->sl_cand2, m->sl_id,
+                                                       m->sl_adj, nullptr, nullptr, nullptr);
+            if (!rc && hip_ok(wp::scatter(ns, m->sl_id, m->sl_sel, d_out_id, nullptr), "scatter"))
+                hip_ok(wp::scatter(ns, m->sl_adj, m->sl_sel, d_out_adj, nullptr), "scatter");
         }
     }
-    hipFree(d_rads);
-    hipFree(d_cand);
-    hipFree(d_off);
-    hipFree(d_post);
-    hipFree(d_mass);
-    hipFree(d_lse);
-    hipFree(d_id);
-    hipFree(d_adj);
-    hipFree(d_sel);
-    hipFree(d_rads2);
-    hipFree(d_cand2);
     if (rc) return rc;
+    if (out_num_fallback) *out_num_fallback = (int)unsure.size();
+    if (unsure.empty()) {
+        hip_ok(hipDeviceSynchronize(), "shortlist");
+        return rc;
+    }
+    // the fallback reads' radiometries on the host
+    std::vector<double> hx;
+    const double* radiometries = h_rads;
+    if (!radiometries) {
+        hx.assign((size_t)N * d, 0.0);
+        for (int r : unsure)
+            if (!hip_ok(hipMemcpy(&hx[(size_t)r * d], d_rads + (size_t)r * d, sizeof(double) * d,
+                                  hipMemcpyDeviceToHost),
+                        "download"))
+                return rc;
+        radiometries = hx.data();
+    }
+    std::vector<int> fid(unsure.size());
+    std::vector<double> fadj(unsure.size());
+    int* out_id = fid.data();  // per fallback read
+    double* out_adjusted = fadj.data();
     if (!unsure.empty()) {
         // Full HMM fallback. A few reads against every dye-seq would each be
         // one workgroup working through S candidates while the rest of the
@@ -640,12 +672,92 @@ int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, cons
             const double tot = tots[i];
             double a = tot > -INFINITY ? std::exp(best - tot) : 0.0;
             if (std::isnan(a)) a = 0.0;
-            out_id[unsure[i]] = best_id;
-            out_adjusted[unsure[i]] = a;
+            out_id[i] = best_id;
+            out_adjusted[i] = a;
         }
     }
-    if (out_num_fallback) *out_num_fallback = (int)unsure.size();
-    return WP_OK;
+    // into the outputs at the fallback reads' places
+    const int nu = (int)unsure.size();
+    if (hip_ok(hipMemcpy(m->sl_sel, unsure.data(), sizeof(int) * nu, hipMemcpyHostToDevice), "upload") &&
+        hip_ok(hipMemcpy(m->sl_id, fid.data(), sizeof(int) * nu, hipMemcpyHostToDevice), "upload") &&
+        hip_ok(hipMemcpy(m->sl_adj, fadj.data(), sizeof(double) * nu, hipMemcpyHostToDevice), "upload") &&
+        hip_ok(wp::scatter(nu, m->sl_id, m->sl_sel, d_out_id, nullptr), "scatter") &&
+        hip_ok(wp::scatter(nu, m->sl_adj, m->sl_sel, d_out_adj, nullptr), "scatter"))
+        hip_ok(hipDeviceSynchronize(), "shortlist");
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wp_shortlist_classify(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, const double* radiometries, int H,
+                          double min_posterior, int criterion, int* out_id, double* out_adjusted,
+                          int* out_num_fallback) {
+    if (!clf || !m || num_reads < 0 || H < 1 || (num_reads > 0 && (!radiometries || !out_id || !out_adjusted))) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    if (criterion != WP_SHORTLIST_MAX_POSTERIOR && criterion != WP_SHORTLIST_MASS) {
+        wp::set_error("unknown shortlist criterion");
+        return WP_ERR_ARG;
+    }
+    if (m->out_dim != wp_hmm_classifier_num_dye_seqs(clf)) {
+        wp::set_error("the MLP's outputs must be the classifier's dye-seqs");
+        return WP_ERR_ARG;
+    }
+    if (out_num_fallback) *out_num_fallback = 0;
+    const int N = num_reads;
+    if (N == 0) return WP_OK;
+    int rc = ensure_shortlist(m, N, std::min(H, m->out_dim));
+    if (rc) return rc;
+    const size_t d = (size_t)m->in_dim;
+    int* d_id = nullptr;
+    double* d_adj = nullptr;
+    hipError_t e = hipMemcpy(m->sl_rads, radiometries, sizeof(double) * (size_t)N * d, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&d_id, sizeof(int) * (size_t)N);
+    if (e == hipSuccess) e = hipMalloc(&d_adj, sizeof(double) * (size_t)N);
+    if (e == hipSuccess) {
+        rc = shortlist_run(clf, m, N, m->sl_rads, radiometries, H, min_posterior, criterion, d_id, d_adj,
+                           out_num_fallback);
+        if (!rc) e = hipMemcpy(out_id, d_id, sizeof(int) * (size_t)N, hipMemcpyDeviceToHost);
+        if (!rc && e == hipSuccess)
+            e = hipMemcpy(out_adjusted, d_adj, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost);
+    }
+    hipFree(d_id);
+    hipFree(d_adj);
+    if (!rc && e != hipSuccess) {
+        wp::set_error(std::string("shortlist: ") + hipGetErrorString(e));
+        rc = WP_ERR_HIP;
+    }
+    return rc;
+}
+
+int wp_shortlist_classify_device(wp_hmm_classifier* clf, wp_mlp* m, int num_reads, const double* d_radiometries,
+                                 int H, double min_posterior, int criterion, int* d_out_id, double* d_out_adjusted,
+                                 int* out_num_fallback, void* stream) {
+    if (!clf || !m || num_reads < 0 || H < 1 ||
+        (num_reads > 0 && (!d_radiometries || !d_out_id || !d_out_adjusted))) {
+        wp::set_error("null argument");
+        return WP_ERR_ARG;
+    }
+    if (criterion != WP_SHORTLIST_MAX_POSTERIOR && criterion != WP_SHORTLIST_MASS) {
+        wp::set_error("unknown shortlist criterion");
+        return WP_ERR_ARG;
+    }
+    if (m->out_dim != wp_hmm_classifier_num_dye_seqs(clf)) {
+        wp::set_error("the MLP's outputs must be the classifier's dye-seqs");
+        return WP_ERR_ARG;
+    }
+    if (out_num_fallback) *out_num_fallback = 0;
+    if (num_reads == 0) return WP_OK;
+    const hipError_t e = hipStreamSynchronize((hipStream_t)stream);  // the caller's inputs are ready
+    if (e != hipSuccess) {
+        wp::set_error(std::string("shortlist: ") + hipGetErrorString(e));
+        return WP_ERR_HIP;
+    }
+    return shortlist_run(clf, m, num_reads, d_radiometries, nullptr, H, min_posterior, criterion, d_out_id,
+                         d_out_adjusted, out_num_fallback);
 }
 
 }  // extern "C"

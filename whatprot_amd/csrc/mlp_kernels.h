@@ -35,6 +35,9 @@ hipError_t to_bf16(long n, const float* x, __bf16* y, hipStream_t s);
 hipError_t pack_input(int rows, int din, int dpad, const float* x, const int* idx, __bf16* y, hipStream_t s);
 hipError_t pack_input(int rows, int din, int dpad, const double* x, const int* idx, __bf16* y, hipStream_t s);
 hipError_t gather_rows(int rows, int w, const double* src, const int* idx, double* dst, hipStream_t s);
+// dst[idx[i]] = src[i], i < n.
+hipError_t scatter(int n, const int* src, const int* idx, int* dst, hipStream_t s);
+hipError_t scatter(int n, const double* src, const int* idx, double* dst, hipStream_t s);
 hipError_t gather_rows(int rows, int w, const int* src, const int* idx, int* dst, hipStream_t s);
 hipError_t gather_int(int n, const int* src, const int* idx, int* dst, hipStream_t s);
 hipError_t mean(int n, const float* x, float* out, hipStream_t s);

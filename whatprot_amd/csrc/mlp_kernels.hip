@@ -574,7 +574,25 @@ __global__ void gather_rows_kernel(int rows, int w, const T* __restrict__ src, c
     dst[i] = src[(long)idx[r] * w + c];
 }
 
+template <typename T>
+__global__ void scatter_kernel(int n, const T* __restrict__ src, const int* __restrict__ idx, T* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[idx[i]] = src[i];
+}
+
 }  // namespace
+
+hipError_t scatter(int n, const int* src, const int* idx, int* dst, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_kernel<int>, dim3((n + 255) / 256), dim3(256), 0, s, n, src, idx, dst);
+    return hipGetLastError();
+}
+
+hipError_t scatter(int n, const double* src, const int* idx, double* dst, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_kernel<double>, dim3((n + 255) / 256), dim3(256), 0, s, n, src, idx, dst);
+    return hipGetLastError();
+}
 
 hipError_t gather_rows(int rows, int w, const double* src, const int* idx, double* dst, hipStream_t s) {
     long n = (long)rows * w;
