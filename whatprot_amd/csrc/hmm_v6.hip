@@ -202,7 +202,17 @@ __device__ __forceinline__ void v6_read(const ClassifyArgs& a, const int r, unsi
     read_candidates(a, r, &ncand, &cidx);
     const bool read_empty = a.nbmax[r] == 0;
 
-    CandAcc<true> acc;
+    // The group's running argmax and normaliser live in its gres slot (four
+    // doubles: best ln P, best candidate, log-sum-exp pair), read and written
+    // once per candidate, so nothing of the accumulator is held in registers
+    // across a candidate's timestep loop.
+    double* gacc = s.gres + 4 * group;
+    if (p == 0) {
+        gacc[0] = -INFINITY;
+        gacc[1] = 0.0;
+        gacc[2] = -INFINITY;
+        gacc[3] = 0.0;
+    }
     const bool collect = a.tie_mode == 1;
     if (collect && p == 0) tie_init(&s.ties[group]);
     const float opf = (float)(1.0 - a.m.pf);
@@ -335,16 +345,42 @@ __device__ __forceinline__ void v6_read(const ClassifyArgs& a, const int r, unsi
                 sum = group_sum<double, L>(part);
             }
             const double logp = (sum > 0.0 && K != kNoExp) ? log(sum) + (double)K * 0.69314718055994530942 : -INFINITY;
-            if (!live) continue;
-            if (a.out_pair_logp && !cidx && p == 0) a.out_pair_logp[(size_t)r * a.S + i] = logp;
-            acc.add(logp, exp(logp), kk, a.dsCount[i]);
-            if (collect && p == 0) tie_note(&s.ties[group], acc.best_v, logp, kk, a.tie_abs, a.tie_rel);
+            if (!live || p != 0) continue;
+            // the candidate again from LDS, not held across the timestep loop
+            asm volatile("" ::: "memory");
+            const int kc = s.list[q0];
+            const int ic = cidx ? cidx[kc] : kc;
+            if (a.out_pair_logp && !cidx) a.out_pair_logp[(size_t)r * a.S + ic] = logp;
+            CandAcc<true> acc;
+            acc.best_v = gacc[0];
+            acc.best_k = (int)gacc[1];
+            acc.tot_a = gacc[2];
+            acc.tot_b = gacc[3];
+            acc.add(logp, 0.0, kc, a.dsCount[ic]);
+            gacc[0] = acc.best_v;
+            gacc[1] = (double)acc.best_k;
+            gacc[2] = acc.tot_a;
+            gacc[3] = acc.tot_b;
+            if (collect) tie_note(&s.ties[group], acc.best_v, logp, kc, a.tie_abs, a.tie_rel);
         }
         __syncthreads();
     }
+    CandAcc<true> acc;
+    if (p == 0) {
+        acc.best_v = gacc[0];
+        acc.best_k = (int)gacc[1];
+        acc.tot_a = gacc[2];
+        acc.tot_b = gacc[3];
+    }
+    __syncthreads();  // the merge below reuses gres
     merge_lanes_and_store<L>(a, r, ncand, cidx, NGR, group, p, acc, s.gres, s.ties, s.list, (int*)(s.gres + 24));
 }
 
+// Waves per SIMD: 3 where a lane's rows fit in 168 registers (L * 4 >= NB),
+// with some per-candidate values spilled around the timestep loop; at 2
+// waves nothing spills but every bin runs slower (10^5 reads x 1,000
+// dye-seqs, bins alone: 4.0 / 6.2 / 7.8 / 8.4 ms at 3 waves against
+// 4.6 / 7.1 / 9.2 / 8.8 ms at 2).
 template <int NB, int L>
 constexpr int v6_min_waves() {
     return L * 4 >= NB ? 3 : 2;
@@ -379,11 +415,13 @@ hipError_t launch_v6_s(const ClassifyArgs& a, hipStream_t stream) {
 
 template <int C, int NB>
 hipError_t launch_v6_nb(const ClassifyArgs& a, hipStream_t stream) {
-    if constexpr (NB >= 32) return launch_v6_s<C, NB, true>(a, stream);
-    if constexpr (NB >= 16) {
-        if (a.variant) return launch_v6_s<C, NB, true>(a, stream);
+    if constexpr (NB >= 32) {
+        return launch_v6_s<C, NB, true>(a, stream);
+    } else if constexpr (NB >= 16) {
+        return a.variant ? launch_v6_s<C, NB, true>(a, stream) : launch_v6_s<C, NB, false>(a, stream);
+    } else {
+        return launch_v6_s<C, NB, false>(a, stream);
     }
-    return launch_v6_s<C, NB, false>(a, stream);
 }
 
 template <int C>
