@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--train-reads", type=int, default=10000000)
     ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--tracks-per-dye-seq", type=int, default=1000)
+    ap.add_argument("--serial-bins", action="store_true",
+                    help="run each call's bins in order on one stream (profiling: each kernel's duration is its own)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -152,6 +154,8 @@ def main():
     entries = [w.DyeSeqEntry(s, 1, i) for i, s in enumerate(strings)]
     prec = w.WP_PRECISION_FP32 if args.precision == "fp32" else w.WP_PRECISION_FP64
     clf = w.HMMClassifier(T, C, model, w.SequencingSettings(cutoff), entries, prec)
+    if args.serial_bins:
+        clf.set_branches(False)
 
     dev = torch.device("cuda:0")
     d_rads = torch.from_numpy(rads).to(dev)

@@ -486,11 +486,48 @@ hipError_t launch_bin_wide(wp_hmm_classifier* c, const BinSet& bins, int b, cons
     return wp::launch_classify(a, c->precision, c->G, stream);
 }
 
+// Every bin's launches of classifier s (its main launch, then its wide
+// passes; s->variant for quad-rows bins) onto up to three branches: the
+// caller's stream and two streams of c, forked from and joined back into
+// `stream` by events (parallel branches of the graph under capture). Once the
+// reads are binned, the bins are independent: every read's outputs, wide
+// lists and counters belong to one bin. Bins are taken from the largest boxes
+// down, branch = position mod 3: the bins of large boxes hold few reads whose
+// workgroups run long, so they start first and overlap the dense bins.
+hipError_t launch_bins_branched(wp_hmm_classifier* c, wp_hmm_classifier* s, const CallArgs& x, hipStream_t stream,
+                                bool branches) {
+    const BinSet& bins = s->bins;
+    constexpr int kBranches = 3;
+    const int nbr = branches ? std::min(kBranches, (int)bins.size()) : 1;
+    hipError_t e = hipSuccess;
+    if (nbr > 1) {
+        if (!c->fork) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(c->fork, stream);
+        for (int k = 0; k + 1 < nbr && e == hipSuccess; k++) {
+            if (!c->branch[k]) e = hipStreamCreateWithFlags(&c->branch[k], hipStreamNonBlocking);
+            if (e == hipSuccess && !c->join[k]) e = hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipStreamWaitEvent(c->branch[k], c->fork, 0);
+        }
+    }
+    for (int pos = 0; pos < (int)bins.size() && e == hipSuccess; pos++) {
+        const int b = (int)bins.size() - 1 - pos;
+        hipStream_t st = pos % nbr == 0 ? stream : c->branch[pos % nbr - 1];
+        e = launch_bin_main(s, bins, b, x, s->variant[b], st);
+        if (e == hipSuccess && bins[b].kernel != kRowPerLane) e = launch_bin_wide(s, bins, b, x, 1, st);
+        if (e == hipSuccess && bins[b].kernel == kGather) e = launch_bin_wide(s, bins, b, x, 2, st);
+    }
+    for (int k = 0; k + 1 < nbr && e == hipSuccess; k++) {
+        e = hipEventRecord(c->join[k], c->branch[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, c->join[k], 0);
+    }
+    return e;
+}
+
 // FP32 calls: the reads whose near-best list holds two or more candidates are
 // rescored over that list by the FP64 sibling's kernels (on the boxes this
 // call's prep computed), and their id and ln score are replaced by the FP64
 // decision.
-int enqueue_refine(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream) {
+int enqueue_refine(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream, bool branches) {
     wp_hmm_classifier* s = c->r64;
     s->ws = wp::Workspace();
     s->ws.L = c->ws.L;
@@ -512,11 +549,7 @@ int enqueue_refine(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream) 
     y.tie_mode = 2;
     y.tie_n = c->ws.tie_n;
     y.tie_idx = c->ws.tie_idx;
-    for (int b = 0; b < (int)bins.size(); b++) {
-        WP_HIP(launch_bin_main(s, bins, b, y, 0, stream));
-        if (bins[b].kernel != kRowPerLane) WP_HIP(launch_bin_wide(s, bins, b, y, 1, stream));
-        if (bins[b].kernel == kGather) WP_HIP(launch_bin_wide(s, bins, b, y, 2, stream));
-    }
+    WP_HIP(launch_bins_branched(c, s, y, stream, branches));
     wp::launch_refine_fix(x.N, c->ws.tie_n, c->ws.rid, c->ws.rls, x.d_lt, x.d_id, x.d_adj, x.d_ls, stream);
     WP_HIP(hipGetLastError());
     return WP_OK;
@@ -530,33 +563,9 @@ int enqueue_call(wp_hmm_classifier* c, const CallArgs& x, hipStream_t stream, bo
     enqueue_prep(c, bins, x, stream);
     WP_HIP(hipGetLastError());
     if (events) WP_HIP(hipEventRecord(c->ev[1], stream));
-    // Bins are independent once the prep has built their lists: every read's
-    // outputs, wide lists and counters belong to one bin. Bin b runs on
-    // branch b mod 3 (branch 0 = the caller's stream); the caller's stream
-    // then waits for every branch.
-    constexpr int kBranches = 3;
-    const int nbr = branches ? std::min(kBranches, (int)bins.size()) : 1;
-    if (nbr > 1) {
-        if (!c->fork) WP_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
-        WP_HIP(hipEventRecord(c->fork, stream));
-        for (int k = 0; k + 1 < nbr; k++) {
-            if (!c->branch[k]) WP_HIP(hipStreamCreateWithFlags(&c->branch[k], hipStreamNonBlocking));
-            if (!c->join[k]) WP_HIP(hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming));
-            WP_HIP(hipStreamWaitEvent(c->branch[k], c->fork, 0));
-        }
-    }
-    for (int b = 0; b < (int)bins.size(); b++) {
-        hipStream_t s = b % nbr == 0 ? stream : c->branch[b % nbr - 1];
-        WP_HIP(launch_bin_main(c, bins, b, x, c->variant[b], s));
-        if (bins[b].kernel != kRowPerLane) WP_HIP(launch_bin_wide(c, bins, b, x, 1, s));
-        if (bins[b].kernel == kGather) WP_HIP(launch_bin_wide(c, bins, b, x, 2, s));
-    }
-    for (int k = 0; k + 1 < nbr; k++) {
-        WP_HIP(hipEventRecord(c->join[k], c->branch[k]));
-        WP_HIP(hipStreamWaitEvent(stream, c->join[k], 0));
-    }
+    WP_HIP(launch_bins_branched(c, c, x, stream, branches));
     if (x.tie_mode == 1) {
-        const int rc = enqueue_refine(c, x, stream);
+        const int rc = enqueue_refine(c, x, stream, branches);
         if (rc) return rc;
     }
     if (events) {
