@@ -37,6 +37,8 @@ struct wp_mlp {
     float* lse = nullptr;
     float* mass = nullptr;
     float* csum = nullptr;  // row-chunk partial sums of the bias gradients
+    unsigned int* gkey = nullptr;  // the output layer's row statistics for the top-k
+    float2* lsp = nullptr;
     // shortlist classification workspace (reads x H candidates), kept across calls
     int sl_cap = 0, sl_H = 0, sl_off_n = -1, sl_off_h = 0;
     double* sl_rads = nullptr;  // the host entry point's upload
@@ -68,6 +70,8 @@ void free_mlp(wp_mlp* m) {
     hipFree(m->lse);
     hipFree(m->mass);
     hipFree(m->csum);
+    hipFree(m->gkey);
+    hipFree(m->lsp);
     for (void* p : {(void*)m->sl_rads, (void*)m->sl_cand, (void*)m->sl_off, (void*)m->sl_stat, (void*)m->sl_id,
                     (void*)m->sl_adj, (void*)m->sl_sel, (void*)m->sl_rads2, (void*)m->sl_cand2})
         hipFree(p);
@@ -94,6 +98,8 @@ int ensure_batch(wp_mlp* m, int B) {
     hipFree(m->lse);
     hipFree(m->mass);
     hipFree(m->csum);
+    hipFree(m->gkey);
+    hipFree(m->lsp);
     m->H.assign(m->nh, nullptr);
     m->dH.assign(m->nh, nullptr);
     m->cap = 0;
@@ -108,12 +114,15 @@ int ensure_batch(wp_mlp* m, int B) {
     if (!rc) rc = dalloc(&m->lse, (size_t)B);
     if (!rc) rc = dalloc(&m->mass, (size_t)B);
     if (!rc) rc = dalloc(&m->csum, wp::colsum_work_floats(B, std::max(m->hidden, m->out_dim)));
+    if (!rc) rc = dalloc(&m->gkey, (size_t)B * wp::stat_groups(m->out_dim));
+    if (!rc) rc = dalloc(&m->lsp, (size_t)B * wp::stat_slabs(m->out_dim));
     if (!rc) m->cap = B;
     return rc;
 }
 
-// Z = MLP(X) for the first B rows of m->X.
-int forward(wp_mlp* m, int B) {
+// Z = MLP(X) for the first B rows of m->X; stats: the output layer also
+// writes the row statistics of Z (m->gkey, m->lsp) for topk_softmax_stats.
+int forward(wp_mlp* m, int B, bool stats = false) {
     const __bf16* in = m->X;
     for (int l = 0; l < m->L; l++) {
         wp::GemmArgs g;
@@ -134,6 +143,10 @@ int forward(wp_mlp* m, int B) {
             g.relu = 1;
         } else {
             g.Cf = m->Z;
+            if (stats) {
+                g.gkey = m->gkey;
+                g.lsp = m->lsp;
+            }
         }
         WP_HIP(wp::gemm(g, nullptr));
         if (l < m->nh) in = m->H[l];
@@ -458,10 +471,13 @@ int wp_mlp_shortlist(wp_mlp* m, int N, const double* radiometries, int H, int* o
             rc = WP_ERR_HIP;
             break;
         }
-        rc = forward(m, B);
+        const bool stats = wp::stats_topk_supported(m->out_dim, H);
+        rc = forward(m, B, stats);
         if (rc) break;
-        if (wp::topk_softmax(B, m->out_dim, H, m->Z, m->out_dim, didx, m->lse, m->post, m->mass, nullptr) !=
-                hipSuccess ||
+        if ((stats ? wp::topk_softmax_stats(B, m->out_dim, H, m->Z, m->out_dim, m->gkey, m->lsp, didx, m->lse,
+                                            m->post, m->mass, nullptr)
+                   : wp::topk_softmax(B, m->out_dim, H, m->Z, m->out_dim, didx, m->lse, m->post, m->mass,
+                                      nullptr)) != hipSuccess ||
             hipMemcpy(out_index + (size_t)r0 * H, didx, sizeof(int) * (size_t)B * H, hipMemcpyDeviceToHost) !=
                 hipSuccess ||
             (out_max_posterior &&
@@ -545,10 +561,14 @@ int shortlist_run(wp_hmm_classifier* clf, wp_mlp* m, int N, const double* d_rads
         const int B = std::min(chunk, N - r0);
         if (!hip_ok(wp::pack_input(B, d, m->in_pad, d_rads + (size_t)r0 * d, nullptr, m->X, nullptr), "pack"))
             break;
-        rc = forward(m, B);
+        const bool stats = wp::stats_topk_supported(m->out_dim, H);
+        rc = forward(m, B, stats);
         if (rc) break;
-        hip_ok(wp::topk_softmax(B, m->out_dim, H, m->Z, m->out_dim, m->sl_cand + (size_t)r0 * H, d_lse + r0,
-                                d_post + r0, d_mass + r0, nullptr),
+        int* cand = m->sl_cand + (size_t)r0 * H;
+        hip_ok(stats ? wp::topk_softmax_stats(B, m->out_dim, H, m->Z, m->out_dim, m->gkey, m->lsp, cand, d_lse + r0,
+                                              d_post + r0, d_mass + r0, nullptr)
+                     : wp::topk_softmax(B, m->out_dim, H, m->Z, m->out_dim, cand, d_lse + r0, d_post + r0,
+                                        d_mass + r0, nullptr),
                "top-k");
     }
     const float* d_conf = criterion == WP_SHORTLIST_MASS ? d_mass : d_post;

@@ -8,6 +8,11 @@ namespace wp {
 // C(m, n) = epilogue(sum_k A(m, k) B(k, n)); A(m, k) = A[m * sam + k * sak],
 // B(k, n) = B[k * sbk + n * sbn] (bf16). Epilogue: + bias[n], ReLU, zero where
 // mask(m, n) <= 0, then FP32 (Cf) and/or bf16 (Cb) stores with row stride ldc.
+// Optional row statistics of the stored values for the top-k (gkey non-null):
+// gkey[m * stat_groups(N) + n / 16] = the largest order-preserving key (fkey)
+// of columns 16 (n / 16) .. +15 (columns >= N excluded), and
+// lsp[m * stat_slabs(N) + n / 64] = (max, sum of exp(value - max)) over
+// columns 64 (n / 64) .. +63.
 struct GemmArgs {
     int M, N, K;
     const __bf16* A;
@@ -21,7 +26,13 @@ struct GemmArgs {
     int relu;
     const __bf16* mask;
     long ldm;
+    unsigned int* gkey;
+    float2* lsp;
 };
+
+// Row strides of the GEMM's row statistics.
+__host__ __device__ inline long stat_groups(int N) { return ((long)(N + 15) / 16 + 3) / 4 * 4; }
+__host__ __device__ inline long stat_slabs(int N) { return (long)(N + 63) / 64; }
 
 hipError_t gemm(const GemmArgs& g, hipStream_t s);
 hipError_t softmax_ce(int rows, int N, const float* Z, long ldz, const int* labels, float scale, __bf16* dZ,
@@ -47,6 +58,13 @@ hipError_t topk(int rows, int N, int H, const float* Z, long ldz, int* out, cons
 // in one pass over the logits where the row fits the filtered top-k.
 hipError_t topk_softmax(int rows, int N, int H, const float* Z, long ldz, int* out, float* lse, float* max_post,
                         float* mass, hipStream_t s);
+// topk_softmax over logits Z whose row statistics the GEMM wrote (gkey, lsp):
+// the same outputs without a pass over every logit. Returns
+// hipErrorNotSupported for shapes the one-wavefront-per-row kernel does not take
+// (stats_topk_supported).
+bool stats_topk_supported(int N, int H);
+hipError_t topk_softmax_stats(int rows, int N, int H, const float* Z, long ldz, const unsigned int* gkey,
+                              const float2* lsp, int* out, float* lse, float* max_post, float* mass, hipStream_t s);
 hipError_t transpose_bf16(int rows, int cols, const float* W, __bf16* Wt, hipStream_t s);
 
 }  // namespace wp

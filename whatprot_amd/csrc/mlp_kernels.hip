@@ -17,6 +17,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "mlp_kernels.h"
 
 namespace wp {
@@ -30,11 +32,155 @@ constexpr int BM = 128, BN = 128, BK = 32, PAD = 8, LDK = BK + PAD;
 __device__ inline __bf16 to_bf16(float x) { return (__bf16)x; }
 __device__ inline float from_bf16(__bf16 x) { return (float)x; }
 
-__global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
+// Order-preserving unsigned image of a float (larger float, larger key; -0 < +0).
+__device__ inline unsigned int fkey(float f) {
+    unsigned int u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// The logit an order-preserving key stands for (the inverse of fkey).
+__device__ inline float key_value(unsigned int k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// Tile (tx, ty) of workgroup w of an nx x ny grid, w in dispatch order.
+// Workgroups go to the 8 XCDs round-robin (XCD w % 8), so XCD x is given the
+// consecutive tiles [x per, (x + 1) per) of a band-major order -- bands of
+// ceil(nx / 8) column tiles, each swept row tile by row tile -- and its L2
+// holds one band of B and the few A row tiles in flight. A permutation of
+// the tiles: every tile is computed once, as before.
+__device__ inline void tile_of(int w, int nx, int ny, int& tx, int& ty) {
+    const int per = nx * ny / 8;
+    const int t = w < 8 * per ? (w % 8) * per + w / 8 : w;
+    const int bw = (nx + 7) / 8, full = nx / bw, tb = ny * bw;
+    if (t < full * tb) {
+        const int rem = t % tb;
+        ty = rem / bw;
+        tx = (t / tb) * bw + rem % bw;
+    } else {  // the narrower last band
+        const int wl = nx - full * bw, rem = t - full * tb;
+        ty = rem / wl;
+        tx = full * bw + rem % wl;
+    }
+}
+
+// Exchange with the partner lane of step S of the 16-lane transposing
+// reductions below: lane l ^ 8 (row_ror:8), l ^ 7 (row_half_mirror), l ^ 2,
+// l ^ 1 (quad permutations) -- four steps that together reach all 16 lanes
+// of a row.
+template <int S>
+__device__ inline unsigned int xchg(unsigned int v) {
+    constexpr int ctrl = S == 0 ? 0x128 : S == 1 ? 0x141 : S == 2 ? (2 | (3 << 2) | (0 << 4) | (1 << 6))
+                                                                  : (1 | (0 << 2) | (3 << 4) | (2 << 6));
+    return (unsigned int)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, 0xf, 0xf, false);
+}
+template <int S>
+__device__ inline float xchg(float v) { return __uint_as_float(xchg<S>(__float_as_uint(v))); }
+
+// Lane bit that step S tells apart (the partners of the step differ in it).
+template <int S>
+__device__ inline bool step_bit(int lr) { return (lr >> (3 - S)) & 1; }
+
+// Step S of a transposing reduction: of its 2n entries (entry e + n in the
+// upper half) a lane keeps the half whose index bit equals its lane bit and
+// combines it with the partner's copy of that half, which the partner sends.
+template <int S, int n, typename T, typename Op>
+__device__ inline void treduce(T* v, int lr, Op op) {
+    const bool b = step_bit<S>(lr);
+#pragma unroll
+    for (int e = 0; e < n; e++) {
+        const T keep = b ? v[e + n] : v[e];
+        const T send = b ? v[e] : v[e + n];
+        v[e] = op(keep, xchg<S>(send));
+    }
+}
+
+// Step S of the inverse (an all-gather): n entries become 2n, the lane's own
+// in the half its lane bit names and the partner's in the other.
+template <int S, int n>
+__device__ inline void tgather(float* v, int lr) {
+    const bool b = step_bit<S>(lr);
+#pragma unroll
+    for (int e = n - 1; e >= 0; e--) {
+        const float own = v[e], other = xchg<S>(v[e]);
+        v[e + n] = b ? own : other;
+        v[e] = b ? other : own;
+    }
+}
+
+// Row statistics of a wavefront's 64 x 64 quarter of C for the top-k (the
+// GemmArgs gkey / lsp outputs). In the accumulator layout the 16 lanes of a
+// row of lanes (l / 16 = lg) hold the 16 columns of one group for 16
+// (i, r) combinations q = 4 i + r. Each half of the combinations (i / 2 = h)
+// goes through a transposing reduction over lanes l ^ 7, l ^ 2, l ^ 1, which
+// leaves lane lr with combination q = 8 h + lr % 8 reduced over 8 lanes, and
+// a last exchange with l ^ 8 completes it on both lanes; the lanes with
+// lr / 8 = h store it: the row's four group keys as one 16-byte word, its
+// slab partial as one 8-byte word.
+__device__ inline void row_stats(const GemmArgs& g, const f32x4 (&acc)[4][4], int m0, int n0, int lr, int lg) {
+    const long G = stat_groups(g.N), NS = stat_slabs(g.N);
+    auto kmax = [](unsigned int a, unsigned int b) { return max(a, b); };
+    auto fadd = [](float a, float b) { return a + b; };
+    float mx[16];  // slab maxima of every combination, on every lane of the row
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        unsigned int key[32];  // [q % 8][j]
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    key[(4 * i + r) * 4 + j] = n0 + 16 * j + lr < g.N ? fkey(acc[2 * h + i][j][r]) : 0u;
+        treduce<1, 16>(key, lr, kmax);
+        treduce<2, 8>(key, lr, kmax);
+        treduce<3, 4>(key, lr, kmax);
+#pragma unroll
+        for (int j = 0; j < 4; j++) key[j] = max(key[j], xchg<0>(key[j]));
+        const int m = m0 + 16 * (2 * h + ((lr >> 2) & 1)) + 4 * lg + (lr & 3);
+        if ((lr >> 3) == h && m < g.M)
+            *(uint4*)&g.gkey[(long)m * G + n0 / 16] = uint4{key[0], key[1], key[2], key[3]};
+        float* x = mx + 8 * h;
+        x[0] = key_value(max(max(key[0], key[1]), max(key[2], key[3])));
+        tgather<3, 1>(x, lr);
+        tgather<2, 2>(x, lr);
+        tgather<1, 4>(x, lr);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        float sum[8];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int q = 4 * i + r;
+                float t = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (n0 + 16 * j + lr < g.N) t += __expf(acc[2 * h + i][j][r] - mx[8 * h + q]);
+                sum[q] = t;
+            }
+        treduce<1, 4>(sum, lr, fadd);
+        treduce<2, 2>(sum, lr, fadd);
+        treduce<3, 1>(sum, lr, fadd);
+        sum[0] += xchg<0>(sum[0]);
+        const int m = m0 + 16 * (2 * h + ((lr >> 2) & 1)) + 4 * lg + (lr & 3);
+        if ((lr >> 3) == h && m < g.M) g.lsp[(long)m * NS + n0 / 64] = float2{mx[8 * h + (lr & 7)], sum[0]};
+    }
+}
+
+// kKC: both operands k-contiguous with rows of a multiple of 8 (the forward
+// products; the host checks) -- only that load path is compiled, which keeps
+// the register count of the prefetching main loop down. kStats: the row
+// statistics epilogue (GemmArgs gkey, lsp).
+template <bool kKC, bool kStats>
+__device__ inline void gemm_body(const GemmArgs& g) {
     __shared__ __align__(16) __bf16 As[BM * LDK];
     __shared__ __align__(16) __bf16 Bs[BN * LDK];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    int tx, ty;
+    tile_of(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y, tx, ty);
+    const int m0 = ty * BM, n0 = tx * BN;
     const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
     const int lr = lane & 15, lg = lane >> 4;
     f32x4 acc[4][4];
@@ -45,8 +191,8 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
     // 16-byte loads: along k when k is contiguous (stored as is), else along
     // the row index (m or n) when that is contiguous (scattered into LDS);
     // a vector that would cross the matrix edge is loaded element-wise
-    const int a_mode = (g.sak == 1 && g.sam % 8 == 0) ? 1 : (g.sam == 1 && g.sak % 8 == 0) ? 2 : 0;
-    const int b_mode = (g.sbk == 1 && g.sbn % 8 == 0) ? 1 : (g.sbn == 1 && g.sbk % 8 == 0) ? 2 : 0;
+    const int a_mode = kKC ? 1 : (g.sak == 1 && g.sam % 8 == 0) ? 1 : (g.sam == 1 && g.sak % 8 == 0) ? 2 : 0;
+    const int b_mode = kKC ? 1 : (g.sbk == 1 && g.sbn % 8 == 0) ? 1 : (g.sbn == 1 && g.sbk % 8 == 0) ? 2 : 0;
     auto mfma_tile = [&]() {
         bf16x8 af[4], bfr[4];
 #pragma unroll
@@ -59,7 +205,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
             for (int j = 0; j < 4; j++)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     };
-    if (a_mode == 1 && b_mode == 1) {
+    if (kKC) {
         // both operands k-contiguous (the forward product): the next K tile's
         // 16-byte loads are in flight while the current tile's MFMAs run
         bf16x8 ra[2], rb[2];
@@ -98,7 +244,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
             __syncthreads();
         }
     }
-    for (int k0 = 0; k0 < g.K && !(a_mode == 1 && b_mode == 1); k0 += BK) {
+    for (int k0 = 0; k0 < g.K && !kKC; k0 += BK) {
         if (a_mode == 1) {
 #pragma unroll
             for (int it = 0; it < 2; it++) {
@@ -189,7 +335,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
         mfma_tile();
         __syncthreads();
     }
-    // epilogue
+    // epilogue: the stored values, kept in acc for the row statistics
 #pragma unroll
     for (int i = 0; i < 4; i++) {
 #pragma unroll
@@ -204,12 +350,20 @@ __global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) {
                 float v = acc[i][j][r] + bias;
                 if (g.relu) v = fmaxf(v, 0.0f);
                 if (g.mask && !(from_bf16(g.mask[(long)m * g.ldm + n]) > 0.0f)) v = 0.0f;
+                acc[i][j][r] = v;
                 if (g.Cf) g.Cf[(long)m * g.ldc + n] = v;
                 if (g.Cb) g.Cb[(long)m * g.ldc + n] = to_bf16(v);
             }
         }
     }
+    if (kStats && n0 + wn < g.N) row_stats(g, acc, m0 + wm, n0 + wn, lr, lg);
 }
+
+// The forward products run at 3 waves per SIMD (the statistics epilogue
+// included); the general strided form, for the backward products, at 2.
+__global__ void __launch_bounds__(256, 3) gemm_kc_kernel(const GemmArgs g) { gemm_body<true, false>(g); }
+__global__ void __launch_bounds__(256, 3) gemm_kc_stats_kernel(const GemmArgs g) { gemm_body<true, true>(g); }
+__global__ void __launch_bounds__(256) gemm_kernel(const GemmArgs g) { gemm_body<false, false>(g); }
 
 // Row-wise softmax cross-entropy over N logits (one workgroup per row):
 // loss_r = logsumexp(z_r) - z_r[y_r]; dZ = (softmax - onehot) * scale as
@@ -314,15 +468,7 @@ __global__ void pack_input_kernel(int rows, int din, int dpad, const T* __restri
 // H-th largest key over the order-preserving unsigned image of the floats,
 // then every index above it and the lowest-indexed ties at it, in
 // increasing index order.
-// The logit an order-preserving key stands for (the inverse of fkey).
-__device__ inline float key_value(unsigned int k) {
-    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
-}
 
-__device__ inline unsigned int fkey(float f) {
-    unsigned int u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
 
 __global__ void __launch_bounds__(256) topk_kernel(int N, int H, const float* __restrict__ Z, long ldz,
                                                     int* __restrict__ out, const float* __restrict__ lse,
@@ -624,7 +770,14 @@ hipError_t mean(int n, const float* x, float* out, hipStream_t s) {
 hipError_t gemm(const GemmArgs& g, hipStream_t s) {
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
     dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM);
-    hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, s, g);
+    const bool kc = g.sak == 1 && g.sam % 8 == 0 && g.sbk == 1 && g.sbn % 8 == 0;
+    if (g.gkey && !(kc && g.lsp)) return hipErrorInvalidValue;  // statistics only on the forward path
+    if (g.gkey)
+        hipLaunchKernelGGL(gemm_kc_stats_kernel, grid, dim3(256), 0, s, g);
+    else if (kc)
+        hipLaunchKernelGGL(gemm_kc_kernel, grid, dim3(256), 0, s, g);
+    else
+        hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, s, g);
     return hipGetLastError();
 }
 
@@ -907,6 +1060,236 @@ __global__ void __launch_bounds__(64 * kWaveRows) topk_wave_kernel(int rows, int
     }
 }
 
+constexpr int kStatWords = 8;    // 16-byte words of group keys per lane: 2048 groups, 32768 logits
+constexpr int kStatCand = 1024;  // largest candidate list (LDS), 16 keys per lane
+
+// Radix selection over keys held in registers, NR per lane (absent keys 0,
+// never counted): n keys count, n >= H >= 1, all in [lo, hi]. Two bits a round,
+// from the highest bit where lo and hi differ (the bits above it are common to
+// every key), each round by three thresholds, K + d 2^sh for d = 1, 2, 3,
+// counted with ballots. Returns K with count(keys >= K) = cnt >= H, stopping
+// as soon as cnt <= H + slack; with slack 0 either cnt == H or K is the exact
+// H-th largest key -- either way the keys > K and the first ties at K are the
+// top H.
+template <int NR>
+__device__ inline unsigned int radix_select(const unsigned int (&k)[NR], int nr, int n, int H, int slack,
+                                            unsigned int lo, unsigned int hi, int& cnt) {
+    cnt = n;
+    if (lo == hi) return hi;
+    const int hb = 31 - __builtin_clz(lo ^ hi);
+    int sh = hb & ~1;
+    unsigned int K = hi & ~((4u << sh) - 1u);  // 4u << 30 wraps to 0: every bit cleared
+    for (; sh >= 0 && cnt > H + slack; sh -= 2) {
+        const unsigned int t1 = K + (1u << sh), t2 = K + (2u << sh), t3 = K + (3u << sh);  // no wrap: K's low bits are 0
+        int c1 = 0, c2 = 0, c3 = 0;
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+            if (t < nr) {  // wave-uniform
+                c1 += __popcll(__ballot(k[t] >= t1));
+                c2 += __popcll(__ballot(k[t] >= t2));
+                c3 += __popcll(__ballot(k[t] >= t3));
+            }
+        }
+        const unsigned int d = c3 >= H ? 3u : c2 >= H ? 2u : c1 >= H ? 1u : 0u;
+        cnt = d == 3u ? c3 : d == 2u ? c2 : d == 1u ? c1 : cnt;
+        K += d << sh;
+    }
+    return K;
+}
+
+__device__ inline unsigned int wave_max_u32(unsigned int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (unsigned int)__shfl_xor((int)v, off, 64));
+    return v;
+}
+__device__ inline unsigned int wave_min_u32(unsigned int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (unsigned int)__shfl_xor((int)v, off, 64));
+    return v;
+}
+
+// Writes the H selected indices of row r (keys > K, then the first ties at K,
+// in index order) from the candidates in LDS (listed) or the whole row, and
+// returns the lane's share of their softmax mass.
+__device__ inline float emit_topk(bool listed, int M, int N, int H, unsigned int K, const unsigned int* ckey,
+                                  const int* cidx, const float* z, float l, int* dst) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int ngt = 0;
+    const int nsrc = listed ? M : N;
+    for (int b = 0; b < nsrc; b += 64) {
+        const int i = b + lane;
+        const unsigned int k = i < nsrc ? (listed ? ckey[i] : fkey(z[i])) : 0u;
+        ngt += __popcll(__ballot(i < nsrc && k > K));
+    }
+    const int need = H - ngt;
+    int o = 0, eqs = 0;
+    float ms = 0.0f;
+    for (int b = 0; b < nsrc && o < H; b += 64) {
+        const int i = b + lane;
+        const bool v = i < nsrc;
+        const unsigned int k = v ? (listed ? ckey[i] : fkey(z[i])) : 0u;
+        const bool g = v && k > K, e = v && k == K;
+        const unsigned long long be = __ballot(e);
+        const bool sel = g || (e && eqs + __popcll(be & below) < need);
+        const unsigned long long bs = __ballot(sel);
+        const int pos = o + __popcll(bs & below);
+        if (sel && pos < H) {
+            dst[pos] = listed ? cidx[i] : i;
+            ms += expf(key_value(k) - l);
+        }
+        o += __popcll(bs);
+        eqs += __popcll(be);
+    }
+    return ms;
+}
+
+// Top-H of logits whose row statistics the GEMM wrote (GemmArgs gkey, lsp),
+// one wavefront per row: no pass over every logit. The row's group keys are
+// held in registers (lane l: groups 4 (l + 64 t) .. +3); a radix selection
+// over them finds a threshold tau' <= the H-th largest group key (its top 24
+// bits; at least H group maxima, hence at least H logits, are >= tau'), the
+// groups whose key reaches tau' are compacted in index order, and their
+// logits >= tau' are gathered (16 lanes a group) into the candidate list, in
+// index order. The exact H-th largest key among them is the row's (every key
+// >= tau' is a candidate); when more than cap logits reach tau' the rounds run
+// over the whole row. The log-sum-exp merges the 64-column slab partials.
+__global__ void __launch_bounds__(64 * kWaveRows) topk_stats_kernel(int rows, int N, int H, const float* __restrict__ Z,
+                                                                    long ldz, const unsigned int* __restrict__ gk,
+                                                                    const float2* __restrict__ lsp, int cap,
+                                                                    int* __restrict__ out, float* __restrict__ mass,
+                                                                    float* __restrict__ lse_out,
+                                                                    float* __restrict__ post_out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = blockIdx.x * kWaveRows + wv;
+    if (r >= rows) return;  // whole wavefront
+    const int G = (N + kWaveGroup - 1) / kWaveGroup, G4 = (G + 3) / 4;
+    const size_t per_wave = (size_t)G * 4 + (size_t)cap * 8;
+    int* glist = (int*)(smem + wv * per_wave);
+    unsigned int* ckey = (unsigned int*)(glist + G);
+    int* cidx = (int*)(ckey + cap);
+    const float* z = Z + (long)r * ldz;
+    const unsigned long long below = (1ull << lane) - 1ull;
+
+    // group keys (the words past the row's last group hold keys 0) and slab partials
+    uint4 gv[kStatWords];
+    const uint4* g4 = (const uint4*)(gk + (long)r * stat_groups(N));
+#pragma unroll
+    for (int t = 0; t < kStatWords; t++) {
+        const int w = lane + 64 * t;
+        gv[t] = w < G4 ? g4[w] : uint4{0u, 0u, 0u, 0u};
+    }
+    float m = -INFINITY, sacc = 0.0f;
+    const float2* p = lsp + (long)r * stat_slabs(N);
+    for (int i = lane; i < G4; i += 64) {
+        const float2 q = p[i];
+        const float mm = fmaxf(m, q.x);
+        sacc = (m == -INFINITY ? 0.0f : sacc * expf(m - mm)) + q.y * expf(q.x - mm);
+        m = mm;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const float m2 = __shfl_xor(m, off, 64), s2 = __shfl_xor(sacc, off, 64);
+        const float mm = fmaxf(m, m2);
+        sacc = (m == -INFINITY ? 0.0f : sacc * expf(m - mm)) + (m2 == -INFINITY ? 0.0f : s2 * expf(m2 - mm));
+        m = mm;
+    }
+    const float l = m + logf(sacc);
+    if (lane == 0) {
+        lse_out[r] = l;
+        post_out[r] = expf(m - l);
+    }
+
+    // tau': a radix selection over the register-held group keys that stops
+    // once at most H + 32 groups reach it
+    unsigned int kk[4 * kStatWords];
+    unsigned int khi = 0u, klo = 0xffffffffu;
+#pragma unroll
+    for (int t = 0; t < kStatWords; t++) {
+        const unsigned int v4[4] = {gv[t].x, gv[t].y, gv[t].z, gv[t].w};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            kk[4 * t + c] = v4[c];
+            const bool valid = 4 * (lane + 64 * t) + c < G;
+            khi = valid ? max(khi, v4[c]) : khi;
+            klo = valid ? min(klo, v4[c]) : klo;
+        }
+    }
+    khi = wave_max_u32(khi);
+    klo = wave_min_u32(klo);
+    int ng_above = G;
+    const unsigned int tau =
+        G >= H ? radix_select(kk, 4 * ((G4 + 63) / 64), G, H, 32, klo, khi, ng_above) : 0u;
+
+    // the groups whose key is >= tau', in index order
+    int NG = 0;
+#pragma unroll
+    for (int t = 0; t < kStatWords; t++) {
+        if (64 * t >= G4) continue;  // wave-uniform
+        const int i0 = 4 * (lane + 64 * t);
+        const unsigned int kk[4] = {gv[t].x, gv[t].y, gv[t].z, gv[t].w};
+        bool f[4];
+        int cnt = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            f[c] = i0 + c < G && kk[c] >= tau;
+            cnt += f[c];
+        }
+        const unsigned long long b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4);
+        int pos = NG + __popcll(b0 & below) + 2 * __popcll(b1 & below) + 4 * __popcll(b2 & below);
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if (f[c]) glist[pos++] = i0 + c;
+        NG += __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
+    }
+    wsync();
+
+    // the logits >= tau' of those groups, four groups per load (16 lanes each)
+    int M = 0;
+    constexpr int kU2 = 16;
+    for (int b = 0; b < NG; b += 4 * kU2) {
+        float vv[kU2];
+        int nn[kU2];
+#pragma unroll
+        for (int u = 0; u < kU2; u++) {
+            const int sl = b + 4 * u + (lane >> 4);
+            nn[u] = sl < NG ? glist[sl] * kWaveGroup + (lane & (kWaveGroup - 1)) : N;
+        }
+#pragma unroll
+        for (int u = 0; u < kU2; u++) vv[u] = nn[u] < N ? z[nn[u]] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kU2; u++) {
+            const int n = nn[u];
+            const unsigned int k = n < N ? fkey(vv[u]) : 0u;
+            const bool f = n < N && k >= tau;
+            const unsigned long long bal = __ballot(f);
+            const int pos = M + __popcll(bal & below);
+            if (f && pos < cap) {
+                ckey[pos] = k;
+                cidx[pos] = n;
+            }
+            M += __popcll(bal);
+        }
+    }
+    wsync();
+    const bool listed = M <= cap;
+    unsigned int K = 0u;
+    if (listed) {
+        // the exact selection among the candidates (all in [tau', khi]), held in registers
+        unsigned int cv[kStatCand / 64];
+#pragma unroll
+        for (int t = 0; t < kStatCand / 64; t++) {
+            const int i = lane + 64 * t;
+            cv[t] = i < M ? ckey[i] : 0u;
+        }
+        int c;
+        K = radix_select(cv, (M + 63) / 64, M, H, 0, tau, khi, c);
+    } else {
+        K = wave_kth(N, H, [&](int i) { return fkey(z[i]); });
+    }
+    float ms = emit_topk(listed, M, N, H, K, ckey, cidx, z, l, out + (long)r * H);
+    for (int off = 32; off > 0; off >>= 1) ms += __shfl_xor(ms, off, 64);
+    if (lane == 0) mass[r] = ms;
+}
+
 }  // namespace
 
 // One wavefront per row (rows of up to 32768 logits, H <= 1024).
@@ -922,6 +1305,28 @@ static hipError_t topk_wave(int rows, int N, int H, const float* Z, long ldz, in
     const int vec = ldz % 4 == 0 && reinterpret_cast<uintptr_t>(Z) % 16 == 0;
     hipLaunchKernelGGL(topk_wave_kernel, dim3((rows + kWaveRows - 1) / kWaveRows), dim3(64 * kWaveRows), lds, s, rows,
                        N, H, Z, ldz, vec, out, lse_in, mass, lse_out, post_out);
+    return hipGetLastError();
+}
+
+bool stats_topk_supported(int N, int H) {
+    return N > 8192 && N <= 64 * kStatWords * 4 * kWaveGroup && N > H && H >= 1 && H <= kStatCand / 2;
+}
+
+hipError_t topk_softmax_stats(int rows, int N, int H, const float* Z, long ldz, const unsigned int* gkey,
+                              const float2* lsp, int* out, float* lse, float* max_post, float* mass, hipStream_t s) {
+    if (!stats_topk_supported(N, H)) return hipErrorNotSupported;
+    if (rows <= 0) return hipSuccess;
+    // candidate lists of 2H (at least 512) logits
+    const int cap = std::min(kStatCand, std::max(512, 2 * H));
+    const int G = (N + kWaveGroup - 1) / kWaveGroup;
+    const size_t lds = (size_t)kWaveRows * ((size_t)G * 4 + (size_t)cap * 8);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)topk_stats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(topk_stats_kernel, dim3((rows + kWaveRows - 1) / kWaveRows), dim3(64 * kWaveRows), lds, s, rows,
+                       N, H, Z, ldz, gkey, lsp, cap, out, mass, lse, max_post);
     return hipGetLastError();
 }
 
