@@ -394,6 +394,102 @@ def shortlist_leg(args, steps, warmup=1, fp64_steps=2):
     }
 
 
+def bench_hybrid(args):
+    import numpy as np
+    import torch
+
+    import whatprot_amd as w
+
+    T, C, S, N = 12, 3, 20000, args.reads
+    k, sig, H, cutoff = 100, 0.5, 1000, 5.0
+    model = w.SequencingModel.uniform(C)
+    log(f"generating {S} dye-seqs, {args.tracks_per_dye_seq} tracks each, {N} reads")
+    strings = w.generate_dye_seqs(C, S, 6, 30, 0, 5, 1002)
+    seqs = [w.DyeSeqEntry(s, 1, i) for i, s in enumerate(strings)]
+    t0 = time.perf_counter()
+    tracks = w.simulate_dye_tracks(model, T, seqs, args.tracks_per_dye_seq, 2002, gpu=True)
+    t_sim = time.perf_counter() - t0
+    rads, true_idx = w.simulate_radiometries(model, T, strings, N, 3002, gpu=True)
+    log(f"{len(tracks)} distinct dye-tracks ({t_sim:.1f} s); building the hybrid classifier (host kd-tree)")
+    prec = w.WP_PRECISION_FP32 if args.precision == "fp32" else w.WP_PRECISION_FP64
+    t0 = time.perf_counter()
+    hc = w.HybridClassifier(T, C, model, w.SequencingSettings(cutoff), k, sig, tracks, H, seqs, prec)
+    t_build = time.perf_counter() - t0
+    log(f"built in {t_build:.1f} s; building the kNN classifier for the search-only timing")
+    nn = w.NNClassifier(T, C, model, k, sig, tracks)
+    log("warmup")
+    for _ in range(args.warmup):
+        hc.classify_arrays(rads[: min(N, 2000)])
+    torch.cuda.synchronize()
+    log("kNN search alone")
+    t0 = time.perf_counter()
+    cnt, _, _ = nn.search(rads)
+    t_knn = time.perf_counter() - t0
+    log(f"timing {args.steps} hybrid classify calls")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ids, adj = hc.classify_arrays(rads)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    result = {
+        "metric": "hybrid classify reads/s on 1 MI355X (kNN k=100 + HMM over top 1000, -p 5)",
+        "value": N / (ms * 1e-3),
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic (seeded simulator, seqparams_atto647n_x3 priors with mu = 1)",
+        "config": {"workload": f"classify hybrid: {N} reads, {S} dye-seqs, {args.tracks_per_dye_seq} simulated "
+                               f"dye-tracks per dye-seq ({len(tracks)} distinct), C=3, T=12, k=100, sigma=0.5, "
+                               f"H=1000, -p 5", "reads": N, "dye_seqs": S, "dye_tracks": len(tracks),
+                   "precision": args.precision},
+        "knn_search_ms": t_knn * 1e3,
+        "knn_queries_per_s": N / t_knn,
+        "knn_mean_neighbours": float(np.mean(cnt)),
+        "hmm_pairs_per_step": int(N * H),
+        "build_s": t_build,
+        "top1_vs_truth": float((ids == true_idx).mean()),
+    }
+    if not args.no_cpu_baseline and len(tracks) > 2_000_000:
+        # the reference builds its kd-tree and precomputations single-threaded
+        # in one call that runs for minutes at this size
+        result["cpu_baseline"] = None
+        result["cpu_baseline_note"] = (f"not run at {len(tracks)} tracks (the reference's HybridClassifier "
+                                       "construction alone runs for minutes); see --tracks-per-dye-seq 100")
+    elif not args.no_cpu_baseline:
+        from tests._oracle import Oracle, Reference
+        ref = Reference.load()
+        if ref is not None:
+            n = min(N, 200)
+            log(f"CPU baseline: the reference's HybridClassifier on {n} reads (its kd-tree build first)")
+            # construction alone (kd-tree build, precomputations), then with n reads
+            t1 = time.perf_counter()
+            ref.hybrid_classify(model, tracks, k, sig, H, strings, [1] * S, list(range(S)), rads[:0], cutoff)
+            d0 = time.perf_counter() - t1
+            t1 = time.perf_counter()
+            rid, _, _ = ref.hybrid_classify(model, tracks, k, sig, H, strings, [1] * S, list(range(S)), rads[:n],
+                                            cutoff)
+            d1 = time.perf_counter() - t1 - d0
+            log(f"reference: construction {d0:.1f} s, {n} reads {d1:.1f} s")
+            result["cpu_baseline"] = {"value": n / d1, "unit": "reads/s", "cores": int(Oracle().L.wpo_num_threads()),
+                                      "kind": "reference",
+                                      "sample": f"{n} reads of the same workload through HybridClassifier::classify "
+                                                f"(OpenMP over reads), construction ({d0:.1f} s) excluded, "
+                                                f"{d1:.1f} s wall"}
+            result["cpu_agreement_ids"] = float((rid == ids[:n]).mean())
+            if prec == w.WP_PRECISION_FP32:
+                hc64 = w.HybridClassifier(T, C, model, w.SequencingSettings(cutoff), k, sig, tracks, H, seqs,
+                                          w.WP_PRECISION_FP64)
+                ids64, _ = hc64.classify_arrays(rads[:n])
+                result["cpu_agreement_ids_fp64"] = float((rid == ids64).mean())
+    print(json.dumps(result), flush=True)
+
+
 def bench_shortlist(args):
     r = shortlist_leg(args, args.steps)
     result = {

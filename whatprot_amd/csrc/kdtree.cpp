@@ -11,6 +11,7 @@
 // either side would hold fewer than k points. The points are permuted in
 // place (here: a permutation of point indices) exactly as the reference's
 // swaps permute its vector.
+#include <algorithm>
 #include <cfloat>
 #include <utility>
 #include <vector>
@@ -182,6 +183,38 @@ int build_kdtree(int k, int d, int n, const double* X, std::vector<int>* perm, s
         }
     }
     return depth;
+}
+
+void kdtree_boxes(int d, const double* X, const std::vector<int>& perm, const std::vector<KdNode>& nodes,
+                  std::vector<double>* boxes) {
+    boxes->assign(nodes.size() * 2 * (size_t)d, 0.0);
+    // children have larger ids than their parent: leaves from their points,
+    // then every internal node from its children
+    for (size_t id = nodes.size(); id-- > 0;) {
+        const KdNode& nd = nodes[id];
+        double* lo = &(*boxes)[id * 2 * d];
+        double* hi = lo + d;
+        if (nd.left < 0) {
+            for (int i = 0; i < d; i++) {
+                lo[i] = DBL_MAX;
+                hi[i] = -DBL_MAX;
+            }
+            for (int p = nd.begin; p < nd.end; p++) {
+                const double* x = X + (size_t)perm[p] * d;
+                for (int i = 0; i < d; i++) {
+                    lo[i] = std::min(lo[i], x[i]);
+                    hi[i] = std::max(hi[i], x[i]);
+                }
+            }
+        } else {
+            const double* l = &(*boxes)[(size_t)nd.left * 2 * d];
+            const double* r = &(*boxes)[(size_t)nd.right * 2 * d];
+            for (int i = 0; i < d; i++) {
+                lo[i] = std::min(l[i], r[i]);
+                hi[i] = std::max(l[d + i], r[d + i]);
+            }
+        }
+    }
 }
 
 }  // namespace wp

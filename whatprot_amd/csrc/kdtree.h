@@ -21,16 +21,23 @@ struct KdNode {
 // original index); returns the tree depth, -1 for n <= 0. Node 0 is the root.
 int build_kdtree(int k, int d, int n, const double* X, std::vector<int>* perm, std::vector<KdNode>* nodes);
 
+// Bounding boxes of the nodes: boxes[(node * 2) * d + i] = the smallest, and
+// boxes[(node * 2 + 1) * d + i] the largest, coordinate i of the node's points.
+void kdtree_boxes(int d, const double* X, const std::vector<int>& perm, const std::vector<KdNode>& nodes,
+                  std::vector<double>* boxes);
+
 struct KnnArgs {
     int N, d, k, depth;
     const double* queries;   // [N][d]
     const KdNode* nodes;
+    const double* boxes;     // kdtree_boxes, or null (no box test)
     const double* points;    // permuted order, each leaf's block [begin*d, end*d) dimension-major
     const int* hits;         // [n] in permuted order
     const int* perm;         // position -> original index
     int* out_count;          // [N]
     int* out_index;          // [N][k] original indices, farthest first
     double* out_dist_sq;     // [N][k]
+    int* next;               // device counter of the launch's query queue (launch_knn zeroes it)
 };
 
 size_t knn_lds_bytes(int d, int k, int depth);

@@ -16,8 +16,18 @@
 // pop when the farthest entry is not needed to keep k hits, and the k-th
 // distance set once the hits reach k. The heap lives in LDS and is updated
 // by lane 0.
+//
+// Box test: before a node's subtree is entered, the squared distance from the
+// query to the bounding box of its points is summed over the lanes; when it
+// reaches the k-th distance (by a relative margin of 1e-12, far above the
+// rounding of that sum or of any point's), every point below would be
+// rejected by the distance test above, so the subtree is left at once and
+// the heap, the k-th distance and the order of the far sides visited after
+// it are those of the full walk.
 #include <hip/hip_runtime.h>
 #include <float.h>
+
+#include <algorithm>
 
 #include "kdtree.h"
 
@@ -25,6 +35,7 @@ namespace wp {
 namespace {
 
 constexpr int kWaves = 4;
+constexpr int kKnnGrid = 2048;  // workgroups of a launch (8 per CU)
 
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -73,6 +84,22 @@ __device__ inline WaveLds carve(unsigned char* base, int d, int k, int depth) {
     return w;
 }
 
+// Whether the box [lo, hi] (box[0..d), box[d..2d)) is at least the k-th
+// distance from the query: no point inside can be admitted. Wave-uniform
+// (the butterfly leaves the same sum on every lane).
+__device__ inline bool box_rejects(const double* box, int d, const double* q, double kth) {
+#pragma clang fp contract(off)
+    if (kth == DBL_MAX) return false;
+    double t = 0.0;
+    for (int i = threadIdx.x & 63; i < d; i += 64) {
+        const double lo = box[i], hi = box[d + i], v = q[i];
+        const double g = v < lo ? lo - v : (v > hi ? v - hi : 0.0);
+        t += g * g;
+    }
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    return t >= kth * (1.0 + 1e-12);
+}
+
 // (d1, p1) < (d2, p2) lexicographically (std::pair ordering).
 __device__ inline bool less_dp(double d1, int p1, double d2, int p2) { return d1 < d2 || (d1 == d2 && p1 < p2); }
 
@@ -109,14 +136,11 @@ __device__ void heap_pop(double* hd, int* hp, int& n) {
     }
 }
 
-__global__ void __launch_bounds__(64 * kWaves) knn_kernel(const KnnArgs a) {
+// The search of query qi by one wavefront, in its LDS L.
+__device__ void search_query(const KnnArgs& a, const WaveLds& L, int qi) {
 #pragma clang fp contract(off)
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int qi = blockIdx.x * kWaves + wv;
-    if (qi >= a.N) return;  // whole wavefront
+    const int lane = threadIdx.x & 63;
     const int d = a.d, k = a.k;
-    WaveLds L = carve(smem + (size_t)wv * wave_bytes(d, k, a.depth), d, k, a.depth);
     for (int i = lane; i < d; i += 64) L.q[i] = a.queries[(size_t)qi * d + i];
     if (lane == 0) {
         L.st[0] = 0;  // heap size
@@ -128,10 +152,15 @@ __global__ void __launch_bounds__(64 * kWaves) knn_kernel(const KnnArgs a) {
     int sp = 0;
     int node = 0;
     bool done = false;
+    auto boxed_out = [&](int id) {
+        return a.boxes != nullptr && box_rejects(a.boxes + (size_t)id * 2 * d, d, L.q, L.kth[0]);
+    };
     while (!done) {
-        // descend to a leaf, stacking the far sides
+        // descend to a leaf, stacking the far sides; a subtree whose box
+        // is beyond the k-th distance is left at once
+        bool skip = boxed_out(node);
         KdNode nd = a.nodes[node];
-        while (nd.left >= 0) {
+        while (!skip && nd.left >= 0) {
             double qv = L.q[nd.s];
             int near, far;
             double gap;
@@ -150,12 +179,13 @@ __global__ void __launch_bounds__(64 * kWaves) knn_kernel(const KnnArgs a) {
             }
             sp++;
             node = near;
+            skip = boxed_out(node);
             nd = a.nodes[node];
         }
         // the leaf's points, 64 at a time; a leaf's block of the point array
         // is dimension-major (point j, dimension i at i * len + j), so the
         // lanes' loads coalesce
-        const int len = nd.end - nd.begin;
+        const int len = skip ? 0 : nd.end - nd.begin;
         const double* blk = a.points + (size_t)nd.begin * d;
         for (int c0 = 0; c0 < len; c0 += 64) {
             const int j = c0 + lane;
@@ -247,6 +277,26 @@ __global__ void __launch_bounds__(64 * kWaves) knn_kernel(const KnnArgs a) {
     }
 }
 
+// Wavefronts take queries in index order from a counter (a.next, zero at
+// launch) until none is left: a slow query holds one wavefront rather than
+// a share of the grid, so the launch ends with the last query, not with the
+// slowest workgroup of every round.
+__global__ void __launch_bounds__(64 * kWaves) knn_kernel(const KnnArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const WaveLds L = carve(smem + (size_t)wv * wave_bytes(a.d, a.k, a.depth), a.d, a.k, a.depth);
+    for (;;) {
+        int qi = 0;
+        if (lane == 0) qi = atomicAdd(a.next, 1);
+        qi = __shfl(qi, 0, 64);
+        if (qi >= a.N) break;  // whole wavefront
+        search_query(a, L, qi);
+        wave_sync();  // the next query's set-up rewrites what the last one read
+    }
+}
+
+__global__ void knn_zero_kernel(int* p) { *p = 0; }
+
 }  // namespace
 
 size_t knn_lds_bytes(int d, int k, int depth) { return kWaves * wave_bytes(d, k, depth); }
@@ -256,7 +306,8 @@ hipError_t launch_knn(const KnnArgs& a, hipStream_t stream) {
     size_t lds = knn_lds_bytes(a.d, a.k, a.depth);
     hipError_t e = hipFuncSetAttribute((const void*)knn_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    int grid = (a.N + kWaves - 1) / kWaves;
+    hipLaunchKernelGGL(knn_zero_kernel, dim3(1), dim3(1), 0, stream, a.next);
+    const int grid = std::min((a.N + kWaves - 1) / kWaves, kKnnGrid);
     hipLaunchKernelGGL(knn_kernel, dim3(grid), dim3(64 * kWaves), lds, stream, a);
     return hipGetLastError();
 }

@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <memory>
+#include <memory_resource>
 #include <queue>
 #include <string>
 #include <thread>
@@ -30,6 +32,7 @@ struct wp_nn_classifier {
     double two_sig_sq = 0.0;
     std::vector<int> src_off, src_id, src_count, src_hits;  // per original track
     wp::KdNode* d_nodes = nullptr;
+    double* d_boxes = nullptr;
     double* d_points = nullptr;
     int* d_hits = nullptr;
     int* d_perm = nullptr;
@@ -39,14 +42,48 @@ struct wp_nn_classifier {
     int* d_cnt = nullptr;
     int* d_idx = nullptr;
     double* d_dist = nullptr;
+    // pinned host copies of the results (the hybrid's pipelined fetches)
+    int* h_cnt = nullptr;
+    int* h_idx = nullptr;
+    double* h_dist = nullptr;
+    int* d_next = nullptr;  // the searches' query counters, one per stream slot
+    hipStream_t st[2] = {nullptr, nullptr};  // the hybrid's two search streams (created on first use)
 };
 
 struct wp_hybrid_classifier {
     wp_nn_classifier* nn = nullptr;
     wp_hmm_classifier* hmm = nullptr;
     int h = 0;
+    // pinned host lists of candidates ([N][h] slots, and packed), kept
+    // across calls
+    size_t list_cap = 0;
+    int* h_slots = nullptr;
+    int* h_packed = nullptr;
     std::unordered_map<int, int> id_index;  // hybrid-classifier.cc:37-40
     std::unordered_map<int, int> id_count;
+    // the same two maps as arrays over [id_lo, id_lo + flat_index.size()) when
+    // the ids are dense enough (index -1: no such id)
+    int id_lo = 0;
+    std::vector<int> flat_index, flat_count;
+    // table index and count of dye-seq id (0 and 0 for an unknown id, as the
+    // reference's map lookups with defaults give)
+    void lookup(int id, int* index, int* count) const {
+        if (!flat_index.empty()) {
+            const long o = (long)id - id_lo;
+            if (o >= 0 && o < (long)flat_index.size() && flat_index[o] >= 0) {
+                *index = flat_index[o];
+                *count = flat_count[o];
+            } else {
+                *index = 0;
+                *count = 0;
+            }
+            return;
+        }
+        auto it = id_count.find(id);
+        *count = it == id_count.end() ? 0 : it->second;
+        auto jt = id_index.find(id);
+        *index = jt == id_index.end() ? 0 : jt->second;
+    }
 };
 
 namespace {
@@ -62,6 +99,7 @@ bool operator>(const Scored& x, const Scored& y) { return x.score > y.score; }
 
 void free_nn(wp_nn_classifier* c) {
     hipFree(c->d_nodes);
+    hipFree(c->d_boxes);
     hipFree(c->d_points);
     hipFree(c->d_hits);
     hipFree(c->d_perm);
@@ -69,6 +107,12 @@ void free_nn(wp_nn_classifier* c) {
     hipFree(c->d_cnt);
     hipFree(c->d_idx);
     hipFree(c->d_dist);
+    hipHostFree(c->h_cnt);
+    hipHostFree(c->h_idx);
+    hipHostFree(c->h_dist);
+    hipFree(c->d_next);
+    for (hipStream_t s : c->st)
+        if (s) hipStreamDestroy(s);
     delete c;
 }
 
@@ -78,49 +122,78 @@ int ensure_cap(wp_nn_classifier* c, int N) {
     hipFree(c->d_cnt);
     hipFree(c->d_idx);
     hipFree(c->d_dist);
+    hipHostFree(c->h_cnt);
+    hipHostFree(c->h_idx);
+    hipHostFree(c->h_dist);
     c->d_q = nullptr;
     c->d_cnt = nullptr;
     c->d_idx = nullptr;
     c->d_dist = nullptr;
+    c->h_cnt = nullptr;
+    c->h_idx = nullptr;
+    c->h_dist = nullptr;
     c->cap = 0;
     WP_HIP(hipMalloc(&c->d_q, sizeof(double) * (size_t)N * c->d));
     WP_HIP(hipMalloc(&c->d_cnt, sizeof(int) * (size_t)N));
     WP_HIP(hipMalloc(&c->d_idx, sizeof(int) * (size_t)N * c->k));
     WP_HIP(hipMalloc(&c->d_dist, sizeof(double) * (size_t)N * c->k));
+    if (!c->d_next) WP_HIP(hipMalloc(&c->d_next, 2 * sizeof(int)));
+    WP_HIP(hipHostMalloc((void**)&c->h_cnt, sizeof(int) * (size_t)N, hipHostMallocDefault));
+    WP_HIP(hipHostMalloc((void**)&c->h_idx, sizeof(int) * (size_t)N * c->k, hipHostMallocDefault));
+    WP_HIP(hipHostMalloc((void**)&c->h_dist, sizeof(double) * (size_t)N * c->k, hipHostMallocDefault));
     c->cap = N;
     return WP_OK;
 }
 
-// KDTree::search for N queries: per query up to k (track, dist^2), farthest
-// first (k-best.h:45-55).
-int search(wp_nn_classifier* c, int N, const double* rads, std::vector<int>* cnt, std::vector<int>* idx,
-           std::vector<double>* dist) {
-    cnt->assign(N, 0);
-    idx->assign((size_t)N * c->k, -1);
-    dist->assign((size_t)N * c->k, 0.0);
-    if (N == 0 || c->n == 0) return WP_OK;
+// KDTree::search (kd-tree.h:40-46) in pieces: upload the N queries, launch
+// the search of reads [r0, r1) (asynchronous on the null stream), fetch its
+// results (waits for it): per query up to k (track, dist^2), farthest first
+// (k-best.h:45-55).
+int search_upload(wp_nn_classifier* c, int N, const double* rads) {
     int rc = ensure_cap(c, N);
     if (rc) return rc;
     WP_HIP(hipMemcpy(c->d_q, rads, sizeof(double) * (size_t)N * c->d, hipMemcpyHostToDevice));
+    return WP_OK;
+}
+
+int search_launch(wp_nn_classifier* c, int r0, int r1, hipStream_t stream = nullptr, int slot = 0) {
     wp::KnnArgs a;
-    a.N = N;
+    a.N = r1 - r0;
     a.d = c->d;
     a.k = c->k;
     a.depth = c->depth;
-    a.queries = c->d_q;
+    a.queries = c->d_q + (size_t)r0 * c->d;
     a.nodes = c->d_nodes;
+    a.boxes = c->d_boxes;
     a.points = c->d_points;
     a.hits = c->d_hits;
     a.perm = c->d_perm;
-    a.out_count = c->d_cnt;
-    a.out_index = c->d_idx;
-    a.out_dist_sq = c->d_dist;
-    WP_HIP(wp::launch_knn(a, nullptr));
-    WP_HIP(hipMemcpy(cnt->data(), c->d_cnt, sizeof(int) * N, hipMemcpyDeviceToHost));
-    WP_HIP(hipMemcpy(idx->data(), c->d_idx, sizeof(int) * (size_t)N * c->k, hipMemcpyDeviceToHost));
-    WP_HIP(hipMemcpy(dist->data(), c->d_dist, sizeof(double) * (size_t)N * c->k, hipMemcpyDeviceToHost));
-    for (int r = 0; r < N; r++) {
-        if ((*cnt)[r] < 0) {
+    a.out_count = c->d_cnt + r0;
+    a.out_index = c->d_idx + (size_t)r0 * c->k;
+    a.out_dist_sq = c->d_dist + (size_t)r0 * c->k;
+    a.next = c->d_next + slot;
+    WP_HIP(wp::launch_knn(a, stream));
+    return WP_OK;
+}
+
+// Queues the copies of reads [r0, r1)'s results into the pinned buffers
+// (valid once the stream has been synchronized).
+int search_queue_fetch(wp_nn_classifier* c, int r0, int r1, hipStream_t stream) {
+    const size_t n = (size_t)(r1 - r0), k = (size_t)c->k;
+    WP_HIP(hipMemcpyAsync(c->h_cnt + r0, c->d_cnt + r0, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+    WP_HIP(hipMemcpyAsync(c->h_idx + r0 * k, c->d_idx + r0 * k, sizeof(int) * n * k, hipMemcpyDeviceToHost, stream));
+    WP_HIP(hipMemcpyAsync(c->h_dist + r0 * k, c->d_dist + r0 * k, sizeof(double) * n * k, hipMemcpyDeviceToHost,
+                          stream));
+    return WP_OK;
+}
+
+int search_fetch(wp_nn_classifier* c, int r0, int r1, int* cnt, int* idx, double* dist) {
+    const size_t n = (size_t)(r1 - r0), k = (size_t)c->k;
+    WP_HIP(hipMemcpy(cnt + r0, c->d_cnt + r0, sizeof(int) * n, hipMemcpyDeviceToHost));
+    WP_HIP(hipMemcpy(idx + r0 * k, c->d_idx + r0 * k, sizeof(int) * n * k, hipMemcpyDeviceToHost));
+    WP_HIP(hipMemcpy(dist + r0 * k, c->d_dist + r0 * k, sizeof(double) * n * k, hipMemcpyDeviceToHost));
+    for (int r = r0; r < r1; r++) {
+        if (cnt[r] < 0) {
             wp::set_error("kNN search: k-best heap full (a point with no hits)");
             return WP_ERR_INTERNAL;
         }
@@ -128,9 +201,21 @@ int search(wp_nn_classifier* c, int N, const double* rads, std::vector<int>* cnt
     return WP_OK;
 }
 
+int search(wp_nn_classifier* c, int N, const double* rads, std::vector<int>* cnt, std::vector<int>* idx,
+           std::vector<double>* dist) {
+    cnt->assign(N, 0);
+    idx->assign((size_t)N * c->k, -1);
+    dist->assign((size_t)N * c->k, 0.0);
+    if (N == 0 || c->n == 0) return WP_OK;
+    int rc = search_upload(c, N, rads);
+    if (!rc) rc = search_launch(c, 0, N);
+    if (!rc) rc = search_fetch(c, 0, N, cnt->data(), idx->data(), dist->data());
+    return rc;
+}
+
 // NNClassifier::classify_helper (nn-classifier.cc:123-151).
-double vote(const wp_nn_classifier* c, int cnt, const int* idx, const double* dist,
-            std::unordered_map<int, double>* scores) {
+template <typename Map>
+double vote(const wp_nn_classifier* c, int cnt, const int* idx, const double* dist, Map* scores) {
     double total = 0.0;
     for (int i = 0; i < cnt; i++) {
         int t = idx[i];
@@ -169,10 +254,18 @@ Scored best_of(const wp_nn_classifier* c, int cnt, const int* idx, const double*
 
 // NNClassifier::classify(radiometry, h) (nn-classifier.cc:179-204): the h
 // best ids in ascending score order.
+// The score map is the reference's std::unordered_map<int, double> (its
+// iteration order decides which of equal scores the heap keeps), with its
+// nodes and buckets carved from a per-thread arena rewound for every read in
+// place of one heap allocation per dye-seq.
 std::vector<Scored> top_of(const wp_nn_classifier* c, int cnt, const int* idx, const double* dist, unsigned int h) {
-    std::unordered_map<int, double> scores;
+    thread_local std::vector<unsigned char> arena(8u << 20);
+    std::pmr::monotonic_buffer_resource res(arena.data(), arena.size());  // beyond it: new/delete
+    std::pmr::unordered_map<int, double> scores(&res);
     double total = vote(c, cnt, idx, dist, &scores);
-    std::priority_queue<Scored, std::vector<Scored>, std::greater<Scored>> pq;
+    std::vector<Scored> heap;
+    heap.reserve((size_t)h + 1);
+    std::priority_queue<Scored, std::vector<Scored>, std::greater<Scored>> pq(std::greater<Scored>(), std::move(heap));
     for (const auto& kv : scores) {
         double id = kv.first;
         double score = kv.second;
@@ -308,7 +401,10 @@ int wp_nn_classifier_create(unsigned int num_timesteps, unsigned int num_channel
                 for (int i = 0; i < d; i++) blk[(size_t)i * len + j] = X[(size_t)perm[nd.begin + j] * d + i];
         }
         for (int p = 0; p < n; p++) H[p] = hits[perm[p]];
+        std::vector<double> boxes;
+        wp::kdtree_boxes(d, X.data(), perm, nodes, &boxes);
         if (hipMalloc(&c->d_nodes, sizeof(wp::KdNode) * nodes.size()) != hipSuccess ||
+            hipMalloc(&c->d_boxes, sizeof(double) * boxes.size()) != hipSuccess ||
             hipMalloc(&c->d_points, sizeof(double) * P.size()) != hipSuccess ||
             hipMalloc(&c->d_hits, sizeof(int) * n) != hipSuccess || hipMalloc(&c->d_perm, sizeof(int) * n) != hipSuccess) {
             wp::set_error("cannot allocate the kd-tree on the device");
@@ -316,6 +412,7 @@ int wp_nn_classifier_create(unsigned int num_timesteps, unsigned int num_channel
             return WP_ERR_NOMEM;
         }
         if (hipMemcpy(c->d_nodes, nodes.data(), sizeof(wp::KdNode) * nodes.size(), hipMemcpyHostToDevice) ||
+            hipMemcpy(c->d_boxes, boxes.data(), sizeof(double) * boxes.size(), hipMemcpyHostToDevice) ||
             hipMemcpy(c->d_points, P.data(), sizeof(double) * P.size(), hipMemcpyHostToDevice) ||
             hipMemcpy(c->d_hits, H.data(), sizeof(int) * n, hipMemcpyHostToDevice) ||
             hipMemcpy(c->d_perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice)) {
@@ -422,12 +519,25 @@ int wp_hybrid_classifier_create(unsigned int num_timesteps, unsigned int num_cha
         c->id_index[dye_seq_ids[i]] = i;
         c->id_count[dye_seq_ids[i]] = dye_seq_counts[i];
     }
+    if (num_dye_seqs > 0) {
+        const int lo = *std::min_element(dye_seq_ids, dye_seq_ids + num_dye_seqs);
+        const int hi = *std::max_element(dye_seq_ids, dye_seq_ids + num_dye_seqs);
+        if ((long)hi - lo < 8l * num_dye_seqs + 1024) {
+            c->id_lo = lo;
+            c->flat_index.assign((size_t)hi - lo + 1, -1);
+            c->flat_count.assign((size_t)hi - lo + 1, 0);
+            for (const auto& kv : c->id_index) c->flat_index[kv.first - lo] = kv.second;
+            for (const auto& kv : c->id_count) c->flat_count[kv.first - lo] = kv.second;
+        }
+    }
     *out = c;
     return WP_OK;
 }
 
 void wp_hybrid_classifier_destroy(wp_hybrid_classifier* c) {
     if (!c) return;
+    hipHostFree(c->h_slots);
+    hipHostFree(c->h_packed);
     wp_hmm_classifier_destroy(c->hmm);
     wp_nn_classifier_destroy(c->nn);
     delete c;
@@ -442,50 +552,109 @@ int wp_hybrid_classify(wp_hybrid_classifier* c, int num_reads, const double* rad
     const int N = num_reads;
     if (N == 0) return WP_OK;  // (the per-read vectors below would be empty, their data() null)
     wp_nn_classifier* nn = c->nn;
-    std::vector<int> cnt, idx;
-    std::vector<double> dist;
-    int rc = search(nn, N, radiometries, &cnt, &idx, &dist);
-    if (rc) return rc;
-    // candidates per read (hybrid-classifier.cc:55-66), in the nn order
-    std::vector<std::vector<Scored>> cands(N);
-    std::vector<std::vector<int>> cidx(N);  // candidates as dye-seq table indices
-    std::vector<double> subfraction(N, 0.0);
-    parallel_for(N, [&](int b, int e) {
-        for (int r = b; r < e; r++) {
-            cands[r] = top_of(nn, cnt[r], &idx[(size_t)r * nn->k], &dist[(size_t)r * nn->k], (unsigned)c->h);
-            double s = 0.0;
-            cidx[r].reserve(cands[r].size());
-            for (const Scored& x : cands[r]) {
-                auto it = c->id_count.find(x.id);
-                s += x.adjusted() * (double)(it == c->id_count.end() ? 0 : it->second);
-                auto jt = c->id_index.find(x.id);
-                cidx[r].push_back(jt == c->id_index.end() ? 0 : jt->second);
-            }
-            subfraction[r] = s;
+    const int h = c->h;
+    // candidates per read (hybrid-classifier.cc:55-66), in the nn order:
+    // their table indices in slot r of a [N][h] array, the subfraction, and
+    // the last (best) candidate for reads the HMM leaves unscored
+    if ((size_t)N * h > c->list_cap) {
+        hipHostFree(c->h_slots);
+        hipHostFree(c->h_packed);
+        c->h_slots = c->h_packed = nullptr;
+        c->list_cap = 0;
+        WP_HIP(hipHostMalloc((void**)&c->h_slots, sizeof(int) * (size_t)N * h, hipHostMallocDefault));
+        WP_HIP(hipHostMalloc((void**)&c->h_packed, sizeof(int) * (size_t)N * h, hipHostMallocDefault));
+        c->list_cap = (size_t)N * h;
+    }
+    int* slots = c->h_slots;
+    std::vector<int> ncand(N, 0), fb_id(N, -1);
+    std::vector<double> subfraction(N, 0.0), fb_adj(N, 0.0);
+    // the reads in pieces, searched (and their results copied to pinned
+    // memory) on two streams in turn: piece p + 2 is queued as soon as piece
+    // p is in, the host votes over piece p while the GPU searches the next
+    // two, and a piece's slowest queries overlap the next piece's search
+    const bool any = nn->n > 0;
+    const int pieces = N >= 16384 ? 8 : 1, per = (N + pieces - 1) / pieces;
+    int rc = any ? search_upload(nn, N, radiometries) : WP_OK;
+    for (int i = 0; i < 2 && !rc && any; i++)
+        if (!nn->st[i] && hipStreamCreate(&nn->st[i]) != hipSuccess) {
+            wp::set_error("cannot create the search streams");
+            rc = WP_ERR_HIP;
         }
-    });
+    auto queue_piece = [&](int p) -> int {
+        const int a0 = p * per, a1 = std::min(N, a0 + per);
+        if (a0 >= a1) return WP_OK;
+        int e = search_launch(nn, a0, a1, nn->st[p % 2], p % 2);
+        return e ? e : search_queue_fetch(nn, a0, a1, nn->st[p % 2]);
+    };
+    for (int p = 0; p < 2 && !rc && any; p++) rc = queue_piece(p);
+    std::vector<int> none(nn->k, -1);
+    std::vector<double> zero(nn->k, 0.0);
+    for (int p = 0; p < pieces && !rc; p++) {
+        const int r0 = p * per, r1 = std::min(N, r0 + per);
+        if (r0 >= r1) break;
+        if (any) {
+            if (hipStreamSynchronize(nn->st[p % 2]) != hipSuccess) {
+                wp::set_error("kNN search failed");
+                rc = WP_ERR_HIP;
+                break;
+            }
+            if (p + 2 < pieces) rc = queue_piece(p + 2);
+            for (int r = r0; r < r1 && !rc; r++)
+                if (nn->h_cnt[r] < 0) {
+                    wp::set_error("kNN search: k-best heap full (a point with no hits)");
+                    rc = WP_ERR_INTERNAL;
+                }
+            if (rc) break;
+        }
+        const int k = nn->k;
+        auto row_cnt = [&](int r) { return any ? nn->h_cnt[r] : 0; };
+        auto row_idx = [&](int r) { return any ? nn->h_idx + (size_t)r * k : none.data(); };
+        auto row_dist = [&](int r) { return any ? nn->h_dist + (size_t)r * k : zero.data(); };
+        parallel_for(r1 - r0, [&](int b, int e) {
+        for (int r = r0 + b; r < r0 + e; r++) {
+            const std::vector<Scored> cands = top_of(nn, row_cnt(r), row_idx(r), row_dist(r), (unsigned)h);
+            int* dst = &slots[(size_t)r * h];
+            double s = 0.0;
+            for (size_t i = 0; i < cands.size(); i++) {
+                int index, count;
+                c->lookup(cands[i].id, &index, &count);
+                s += cands[i].adjusted() * (double)count;
+                dst[i] = index;
+            }
+            ncand[r] = (int)cands.size();
+            subfraction[r] = s;
+            if (!cands.empty()) {
+                fb_id[r] = cands.back().id;
+                fb_adj[r] = cands.back().adjusted();
+            }
+        }
+        });
+    }
+    if (rc) {
+        hipDeviceSynchronize();  // a search still in flight writes into the classifier's buffers
+        return rc;
+    }
     std::vector<int> off(N + 1, 0);
-    for (int r = 0; r < N; r++) off[r + 1] = off[r] + (int)cidx[r].size();
-    std::vector<int> list(std::max(off[N], 1), 0);
-    parallel_for(N, [&](int b, int e) {
-        for (int r = b; r < e; r++) std::copy(cidx[r].begin(), cidx[r].end(), list.begin() + off[r]);
-    });
+    for (int r = 0; r < N; r++) off[r + 1] = off[r] + ncand[r];
+    const int* list = slots;
+    if ((size_t)off[N] != (size_t)N * h) {  // some reads hold fewer than h: pack the lists
+        int* packed = c->h_packed;
+        parallel_for(N, [&](int b, int e) {
+            for (int r = b; r < e; r++) std::copy(&slots[(size_t)r * h], &slots[(size_t)r * h] + ncand[r], &packed[off[r]]);
+        });
+        list = packed;
+    }
     std::vector<int> hid(N);
     std::vector<double> hadj(N);
-    rc = wp_hmm_classify_candidates(c->hmm, N, radiometries, off.data(), list.data(), hid.data(), hadj.data(),
-                                    nullptr, nullptr);
+    rc = wp_hmm_classify_candidates(c->hmm, N, radiometries, off.data(), list, hid.data(), hadj.data(), nullptr,
+                                    nullptr);
     if (rc) return rc;
     for (int r = 0; r < N; r++) {  // hybrid-classifier.cc:67-82
         double adj;
         int id;
-        if (hid[r] == -1) {
-            if (cands[r].empty()) {
-                id = -1;
-                adj = 0.0;
-            } else {
-                id = cands[r].back().id;
-                adj = cands[r].back().adjusted();
-            }
+        if (hid[r] == -1) {  // no scored candidate: the best of the kNN's (-1, 0 when it has none)
+            id = fb_id[r];
+            adj = fb_adj[r];
         } else {
             id = hid[r];
             adj = hadj[r] * subfraction[r];
