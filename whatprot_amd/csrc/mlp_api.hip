@@ -54,6 +54,9 @@ struct wp_mlp {
 
 namespace {
 
+// Logits per forward chunk of the shortlist calls (rows = kChunkBytes / 4 out_dim).
+constexpr long kChunkBytes = 2048l << 20;
+
 void free_mlp(wp_mlp* m) {
     for (auto* v : {&m->W, &m->b, &m->gW, &m->gb, &m->mW, &m->vW, &m->mb, &m->vb})
         for (float* p : *v) hipFree(p);
@@ -455,7 +458,7 @@ int wp_mlp_shortlist(wp_mlp* m, int N, const double* radiometries, int H, int* o
         wp::set_error("bad shortlist arguments");
         return WP_ERR_ARG;
     }
-    const int chunk = std::max(1, std::min(N, (int)std::max<long>(1, (512l << 20) / (4l * m->out_dim))));
+    const int chunk = std::max(1, std::min(N, (int)std::max<long>(1, kChunkBytes / (4l * m->out_dim))));
     int rc = ensure_batch(m, chunk);
     if (rc) return rc;
     double* dr = nullptr;
@@ -533,7 +536,7 @@ int shortlist_run(wp_hmm_classifier* clf, wp_mlp* m, int N, const double* d_rads
                   double min_posterior, int criterion, int* d_out_id, double* d_out_adj, int* out_num_fallback) {
     const int S = wp_hmm_classifier_num_dye_seqs(clf), d = m->in_dim;
     H = std::min(H, S);
-    const int chunk = std::max(1, std::min(N, (int)std::max<long>(1, (512l << 20) / (4l * m->out_dim))));
+    const int chunk = std::max(1, std::min(N, (int)std::max<long>(1, kChunkBytes / (4l * m->out_dim))));
     int rc = ensure_batch(m, chunk);
     if (!rc) rc = ensure_shortlist(m, N, H);
     if (rc) return rc;

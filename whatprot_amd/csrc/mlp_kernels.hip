@@ -351,7 +351,14 @@ __device__ inline void gemm_body(const GemmArgs& g) {
                 if (g.relu) v = fmaxf(v, 0.0f);
                 if (g.mask && !(from_bf16(g.mask[(long)m * g.ldm + n]) > 0.0f)) v = 0.0f;
                 acc[i][j][r] = v;
-                if (g.Cf) g.Cf[(long)m * g.ldc + n] = v;
+                if (g.Cf) {
+                    // the logits of the top-k path: streamed out (the top-k
+                    // reads back only the few groups that reach its threshold)
+                    if (kStats)
+                        __builtin_nontemporal_store(v, &g.Cf[(long)m * g.ldc + n]);
+                    else
+                        g.Cf[(long)m * g.ldc + n] = v;
+                }
                 if (g.Cb) g.Cb[(long)m * g.ldc + n] = to_bf16(v);
             }
         }
