@@ -256,3 +256,22 @@ def test_zero_reads_and_rejected_arguments(gpu, oracle, precision):
     err = log_err(got, want)
     err[np.isinf(want)] = 0.0
     assert err.max() <= (1e-12 if precision == 0 else FP32_LOGP_TOL)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_host_calls_reuse_staging(gpu, precision):
+    """The host-array entry points keep their device staging between calls
+    (grown when a call is larger): calls that grow, shrink and switch between
+    all-candidate and candidate-list scoring on one classifier give, bit for
+    bit, what a fresh classifier gives for each call."""
+    C, T, S = 3, 12, 200
+    m, strings, rads, _ = _workload(gpu, C, T, S, 120, (6, 30), (0, 5), seed=33)
+    rng = np.random.default_rng(4)
+    cands = [list(rng.choice(S, size=int(rng.integers(1, 30)), replace=False)) for _ in range(120)]
+    clf = make_clf(gpu, m, T, strings, 5.0, precision)
+    for lo, hi, with_cands in [(0, 30, False), (0, 120, True), (10, 25, True), (0, 90, False), (50, 51, True)]:
+        kw = {"candidates": cands[lo:hi]} if with_cands else {}
+        got = clf.classify_arrays(rads[lo:hi], **kw)
+        want = make_clf(gpu, m, T, strings, 5.0, precision).classify_arrays(rads[lo:hi], **kw)
+        for g, w_ in zip(got, want):
+            np.testing.assert_array_equal(g, w_)

@@ -146,8 +146,21 @@ struct BinSet {
 
 }  // namespace wp
 
+// Device staging of the host-array entry points, kept across calls and grown
+// as needed (no allocation on calls that fit).
+struct HostStage {
+    double* rads = nullptr;
+    int* id = nullptr;
+    double* out = nullptr;
+    double* pair = nullptr;
+    int* off = nullptr;
+    int* idx = nullptr;
+    size_t n_rads = 0, n_reads = 0, n_pair = 0, n_idx = 0;
+};
+
 struct wp_hmm_classifier {
     int T = 0, C = 0, S = 0, M = 0, precision = 0, G = 16, tri = 1;
+    HostStage hs;
     wp::DevModel dm;
     std::vector<int> ids, counts;
     uint8_t* dD = nullptr;
@@ -894,6 +907,9 @@ void wp_hmm_classifier_destroy(wp_hmm_classifier* c) {
     if (c->fork) hipEventDestroy(c->fork);
     if (c->r64) wp_hmm_classifier_destroy(c->r64);
     hipFree(c->wbuf);
+    for (void* p : {(void*)c->hs.rads, (void*)c->hs.id, (void*)c->hs.out, (void*)c->hs.pair, (void*)c->hs.off,
+                    (void*)c->hs.idx})
+        hipFree(p);
     if (!c->borrowed) {
         hipFree(c->dD);
         hipFree(c->dSeq);
@@ -1024,22 +1040,10 @@ static int classify_host(wp_hmm_classifier* c, int N, const double* rads, const 
         }
     }
     size_t tc = (size_t)N * c->T * c->C;
-    double* d_rads = nullptr;
-    int* d_id = nullptr;
-    double* d_out = nullptr;
-    double* d_pair = nullptr;
-    int* d_off = nullptr;
-    int* d_idx = nullptr;
+    HostStage& hs = c->hs;
     int rc = WP_OK;
     hipStream_t s = nullptr;
-    auto cleanup = [&]() {
-        hipFree(d_rads);
-        hipFree(d_id);
-        hipFree(d_out);
-        hipFree(d_pair);
-        hipFree(d_off);
-        hipFree(d_idx);
-    };
+    auto cleanup = [&]() {};
 #define WP_HIPC(expr)                                                              \
     do {                                                                           \
         hipError_t e_ = (expr);                                                    \
@@ -1049,16 +1053,41 @@ static int classify_host(wp_hmm_classifier* c, int N, const double* rads, const 
             return WP_ERR_HIP;                                                     \
         }                                                                          \
     } while (0)
-    WP_HIPC(hipMalloc(&d_rads, sizeof(double) * tc));
-    WP_HIPC(hipMalloc(&d_id, sizeof(int) * (size_t)N));
-    WP_HIPC(hipMalloc(&d_out, sizeof(double) * 3 * (size_t)N));
-    if (out_pair) WP_HIPC(hipMalloc(&d_pair, sizeof(double) * (size_t)N * std::max(c->S, 1)));
+    // grow-only staging buffers (freed with the classifier)
+    auto grow = [&](auto** p, size_t* have, size_t need, size_t elem) -> hipError_t {
+        if (need <= *have) return hipSuccess;
+        hipFree(*p);
+        *p = nullptr;
+        *have = 0;
+        hipError_t e = hipMalloc((void**)p, elem * std::max<size_t>(need, 1));
+        if (e == hipSuccess) *have = need;
+        return e;
+    };
+    {
+        size_t nr = hs.n_reads;  // id and out share the reads capacity
+        WP_HIPC(grow(&hs.rads, &hs.n_rads, tc, sizeof(double)));
+        if ((size_t)N > nr) {
+            size_t n1 = nr, n2 = nr, n3 = nr;
+            WP_HIPC(grow(&hs.id, &n1, (size_t)N, sizeof(int)));
+            WP_HIPC(grow(&hs.out, &n2, 3 * (size_t)N, sizeof(double)));
+            WP_HIPC(grow(&hs.off, &n3, (size_t)N + 1, sizeof(int)));
+            hs.n_reads = (size_t)N;
+        }
+    }
+    if (out_pair) WP_HIPC(grow(&hs.pair, &hs.n_pair, (size_t)N * std::max(c->S, 1), sizeof(double)));
+    double* d_rads = hs.rads;
+    int* d_id = hs.id;
+    double* d_out = hs.out;
+    double* d_pair = out_pair ? hs.pair : nullptr;
+    int* d_off = nullptr;
+    int* d_idx = nullptr;
     if (cand_off) {
         size_t nc = (size_t)(cand_off[N] - cand_off[0]);
         std::vector<int> off(N + 1);
         for (int r = 0; r <= N; r++) off[r] = cand_off[r] - cand_off[0];
-        WP_HIPC(hipMalloc(&d_off, sizeof(int) * (size_t)(N + 1)));
-        WP_HIPC(hipMalloc(&d_idx, sizeof(int) * std::max<size_t>(nc, 1)));
+        WP_HIPC(grow(&hs.idx, &hs.n_idx, std::max<size_t>(nc, 1), sizeof(int)));
+        d_off = hs.off;
+        d_idx = hs.idx;
         WP_HIPC(hipMemcpy(d_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
         if (nc) WP_HIPC(hipMemcpy(d_idx, cand_idx + cand_off[0], sizeof(int) * nc, hipMemcpyHostToDevice));
     }
