@@ -406,6 +406,8 @@ wp::ClassifyArgs base_args(const wp_hmm_classifier* c, const CallArgs& x) {
     a.list_count = nullptr;
     a.wide_list = nullptr;
     a.wide_count = nullptr;
+    a.over_list = nullptr;
+    a.over_count = nullptr;
     a.variant = 0;
     a.cap = 0;
     a.grid_cap = kListGrid;
@@ -461,12 +463,41 @@ hipError_t launch_bin_main(wp_hmm_classifier* c, const BinSet& bins, int b, cons
     // (the FP64 decision's lists are short in every bin)
     if (bin.hi <= 32 && x.tie_mode != 2) a.grid_cap = 0;
     if (bin.kernel == kRowPerLane) return wp::launch_classify_v3(a, c->precision, stream);
-    if (bin.kernel == kLaneGroup) return wp::launch_classify_v6(a, stream);
+    if (bin.kernel == kLaneGroup) {
+        a.over_list = c->ws.wide2_lists + (size_t)b * x.N;  // (unused by the lane-group bins otherwise)
+        a.over_count = c->ws.ctr + kCtrWide2 + b;
+        return wp::launch_classify_v6(a, stream);
+    }
     if (bin.kernel == kQuadRows) return wp::launch_classify_v5(a, stream);
     a.variant = 1;  // gather bins: separable Bleach passes; reads that outgrow them go to the wide pass
     a.cap = bin.cap_sep;
     a.ngroups = bin.ngroups_sep;
     return wp::launch_classify(a, c->precision, c->G, stream);
+}
+
+// Whether a lane-group bin's main launch caps its W block (its over list can
+// hold reads): 16-state boxes in the first variant.
+bool lane_group_capped(const Bin& bin, int variant) { return bin.kernel == kLaneGroup && bin.hi == 16 && !variant; }
+
+// The uncapped pass of a capped lane-group bin: the reads whose box classes
+// outgrew the main launch's W block, scored by the form that builds each
+// timestep's matrices just before its step (reads whose densities are too
+// wide still go on to the wide pass).
+hipError_t launch_bin_over(wp_hmm_classifier* c, const BinSet& bins, int b, const CallArgs& x, hipStream_t stream) {
+    const Bin& bin = bins[b];
+    wp::ClassifyArgs a = base_args(c, x);
+    a.nb_lo = bin.lo;
+    a.nb_hi = bin.hi;
+    a.NB = bin.hi;
+    a.cap = c->T * bin.hi;
+    a.ngroups = bin.ngroups;
+    a.list = c->ws.wide2_lists + (size_t)b * x.N;
+    a.list_count = c->ws.ctr + kCtrWide2 + b;
+    a.wide_list = c->ws.wide_lists + (size_t)b * x.N;
+    a.wide_count = c->ws.ctr + kCtrWide + b;
+    a.variant = 1;
+    if (x.tie_mode != 2) a.grid_cap = 0;
+    return wp::launch_classify_v6(a, stream);
 }
 
 // The wide pass of a bin: the reads its main launch handed over. Quad-rows
@@ -526,6 +557,7 @@ hipError_t launch_bins_branched(wp_hmm_classifier* c, wp_hmm_classifier* s, cons
         const int b = (int)bins.size() - 1 - pos;
         hipStream_t st = pos % nbr == 0 ? stream : c->branch[pos % nbr - 1];
         e = launch_bin_main(s, bins, b, x, s->variant[b], st);
+        if (e == hipSuccess && lane_group_capped(bins[b], s->variant[b])) e = launch_bin_over(s, bins, b, x, st);
         if (e == hipSuccess && bins[b].kernel != kRowPerLane) e = launch_bin_wide(s, bins, b, x, 1, st);
         if (e == hipSuccess && bins[b].kernel == kGather) e = launch_bin_wide(s, bins, b, x, 2, st);
     }

@@ -30,6 +30,11 @@ struct ClassifyArgs {
     const int* list_count;
     int* wide_list;           // quad-rows: reads handed to the wide pass are appended here
     int* wide_count;
+    // lane-group, capped W block: reads whose box classes outgrow the block
+    // (and whose densities fit) are appended here for the uncapped pass;
+    // null: they go to the wide list
+    int* over_list;
+    int* over_count;
     int variant;              // quad-rows: 1 = per-timestep box classes, 0 = bin-wide boxes
     int NB;                   // per-group state capacity in count-box entries
     int cap;                  // gather kernel: entries of each of a group's four state buffers

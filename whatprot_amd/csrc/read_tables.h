@@ -228,7 +228,7 @@ __device__ __forceinline__ bool read_prologue(const ClassifyArgs& a, const int r
         sH[k] = a.Hr[(size_t)r * T * C + k];
     }
     for (int k = tid; k < 2 * C * tri; k += kReadThreads) bin[k] = a.binom[k];
-    if (tid == 0) scan[0] = 0;
+    if (tid == 0) scan[0] = scan[1] = 0;
     __syncthreads();
     if (tid < T) {
         const int t = tid;
@@ -251,7 +251,7 @@ __device__ __forceinline__ bool read_prologue(const ClassifyArgs& a, const int r
             woff[t] = o;
             if (t >= 1) o += (1 + C) * (k * k + 4);
         }
-        if (kFillW && o > wcap) scan[0] = 1;  // the classes' matrices exceed the W block: wide pass
+        if (kFillW && o > wcap) scan[1] = 1;  // the classes' matrices exceed the W block
     }
     __syncthreads();
     // peptide-emission.cc:165-180
@@ -299,14 +299,20 @@ __device__ __forceinline__ bool read_prologue(const ClassifyArgs& a, const int r
     __syncthreads();
     // Densities spanning more than 2^kMaxDensitySpan in one box could fall below
     // FP32's range relative to the box maximum: such a read is flagged and
-    // left to the kernels that keep one exponent per density (as is a read
-    // whose box classes outgrow a capped W block).
-    const bool wide = scan[0] != 0;
+    // left to the kernels that keep one exponent per density. A read whose
+    // box classes outgrow a capped W block goes to the launch's over list
+    // (the uncapped pass, which builds each timestep's matrices before its
+    // step) or, without one, to the wide pass as well.
+    const bool span = scan[0] != 0, over = scan[1] != 0;
     if (tid == 0) {
-        a.wide[r] = wide ? 1 : 0;
-        if (wide) a.wide_list[atomicAdd(a.wide_count, 1)] = r;
+        const bool to_over = over && !span && a.over_list != nullptr;
+        a.wide[r] = (span || over) && !to_over ? 1 : 0;
+        if (to_over)
+            a.over_list[atomicAdd(a.over_count, 1)] = r;
+        else if (span || over)
+            a.wide_list[atomicAdd(a.wide_count, 1)] = r;
     }
-    if (wide) return false;
+    if (span || over) return false;
     for (int k = tid; k < T * NB; k += kReadThreads) etab[k] = (float)ldexp(ev[k], -xt[k / NB]);
     // Composite Bleach matrices of every timestep (unless the caller builds
     // each timestep's just before its step): timestep t's matrices are
