@@ -50,7 +50,7 @@ PEAK_FP32_TFLOPS = 157.3
 # decision), from two rocprofv3 --pmc runs of this bench (FETCH_SIZE, then WRITE_SIZE, per
 # dispatch, summed over the step), each scaled by what the same counter reads for a 512 MiB
 # device copy in the same run; refreshed whenever the classify kernels change.
-CLASSIFY_HBM_BYTES = 13_357_000_000
+CLASSIFY_HBM_BYTES = 13_452_000_000
 CLASSIFY_HBM_SOURCE = ("rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per step "
                        "(profiles/classify_fp32_100k_x_1k_cut5.md)")
 
