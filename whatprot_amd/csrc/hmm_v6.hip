@@ -156,14 +156,7 @@ __device__ __forceinline__ void v6_step(const V6Step& st, const int (&cix)[R], c
 // Timestep t's (1 + C) matrices of class K, built by the whole workgroup.
 template <int C, int NB>
 __device__ inline void fill_step_w(const ReadTables& s, int tri, int t, int K, float* dst) {
-    const int ms = K * K + 4;
-    const int total = (1 + C) * ms;
-    for (int k = threadIdx.x; k < total; k += kReadThreads) {
-        const int mat = k / ms;
-        const int rem = k - mat * ms;
-        if (rem >= K * K) continue;  // the 4-entry gap between matrices
-        dst[k] = (float)w_entry<C, NB>(s, tri, t, mat, rem / K, rem % K);
-    }
+    fill_w_block<C, NB>(s, tri, t, K, dst);
 }
 
 // One read: the shared prologue, then its candidates, L lanes each, in

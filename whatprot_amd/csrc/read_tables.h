@@ -171,25 +171,45 @@ __device__ inline bool in_bin(const ClassifyArgs& a, int r) {
 // Bleach matrices (per box class with kClass, else at NB) and the blocked /
 // detached backward tables. Returns false, uniformly over the workgroup, when
 // the read is handed to the wide pass instead. Block-wide.
-// Entry (q, q2) of matrix mat of timestep t >= 1 (0: the composite Bleach
-// Wt between the boxes of t - 1 and t; 1 + c: its Edman-shifted form W1[c],
-// which maps onto n'(q2) + 1_c and is zero at or above
-// U = min(Hr[t] + 1, Hr[t-1]), edman-transition.cc:28-60), from the read's
-// box coordinates and bleach tables.
+// Timestep t's (1 + C) matrices of class K, built by the whole workgroup.
+// Entry (q, q2) of matrix mat (0: the composite Bleach Wt between the boxes
+// of t - 1 and t; 1 + c: its Edman-shifted form W1[c], which maps onto
+// n'(q2) + 1_c and is zero at or above U = min(Hr[t] + 1, Hr[t-1]),
+// edman-transition.cc:28-60) is the product over channels of the bleach
+// table entries from the read's box coordinates. Each thread takes entries
+// (q, q2) of the K x K block (K a power of two: shifts, no divisions), loads
+// both states' coordinates once and writes the entry of every matrix;
+// matrices are K * K + 4 entries apart.
 template <int C, int NB>
-__device__ inline double w_entry(const ReadTables& s, int tri, int t, int mat, int q, int q2) {
-    if (!(q < s.nbt[t - 1] && q2 < s.nbt[t])) return 0.0;
+__device__ inline void fill_w_block(const ReadTables& s, int tri, int t, int K, float* dst) {
+    const int ms = K * K + 4;
+    const int lk = 31 - __builtin_clz(K);
+    const int n0 = s.nbt[t - 1], n1 = s.nbt[t];
     const double* ble = s.bin + C * tri;
-    double v = 1.0;
-    const int sc = mat - 1;  // shifted channel, -1 for Wt
-    for (int c = 0; c < C; c++) {
-        const int from = s.coord[((t - 1) * NB + q) * C + c];
-        const int to = s.coord[(t * NB + q2) * C + c] + (c == sc ? 1 : 0);
-        if (c == sc && to >= min(s.sH[t * C + c] + 1, s.sH[(t - 1) * C + c])) v = 0.0;
-        if (from < to) v = 0.0;
-        else v *= ble[c * tri + from * (from + 1) / 2 + to];
+    int lim[C];  // shifted channel c: target counts below U
+#pragma unroll
+    for (int c = 0; c < C; c++) lim[c] = min(s.sH[t * C + c] + 1, s.sH[(t - 1) * C + c]);
+    for (int e = threadIdx.x; e < K * K; e += kReadThreads) {
+        const int q = e >> lk, q2 = e & (K - 1);
+        const bool in = q < n0 && q2 < n1;
+        int from[C], to[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            from[c] = in ? s.coord[((t - 1) * NB + q) * C + c] : 0;
+            to[c] = in ? s.coord[(t * NB + q2) * C + c] : 0;
+        }
+#pragma unroll
+        for (int mat = 0; mat <= C; mat++) {
+            double v = in ? 1.0 : 0.0;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const int tc = to[c] + (c == mat - 1 ? 1 : 0);
+                if (c == mat - 1 && tc >= lim[c]) v = 0.0;
+                v = from[c] < tc ? 0.0 : v * ble[c * tri + from[c] * (from[c] + 1) / 2 + tc];
+            }
+            dst[mat * ms + e] = (float)v;
+        }
     }
-    return v;
 }
 
 // kFillW false: the W block is left to the caller (built per timestep), and
@@ -322,23 +342,9 @@ __device__ __forceinline__ bool read_prologue(const ClassifyArgs& a, const int r
     const int wtotal = !kFillW ? 0
                        : kClass ? woff[T - 1] + (T > 1 ? (1 + C) * (kcl[T - 1] * kcl[T - 1] + 4) : 0)
                                 : (T - 1) * (1 + C) * MSNB;
-    for (int k = tid; k < wtotal; k += kReadThreads) {
-        int t = 1, K = NB, rem;
-        if constexpr (kClass) {
-            while (t + 1 < T && woff[t + 1] <= k) t++;
-            K = kcl[t];
-            rem = k - woff[t];
-        } else {
-            t = 1 + k / ((1 + C) * MSNB);
-            rem = k - (t - 1) * (1 + C) * MSNB;
-        }
-        const int ms = K * K + 4;
-        const int mat = rem / ms;
-        rem -= mat * ms;
-        if (rem >= K * K) continue;  // the 4-entry gap between matrices
-        const int q = rem / K, q2 = rem % K;
-        Wall[(kClass ? woff[t] : t * (1 + C) * MSNB) + mat * ms + rem] = (float)w_entry<C, NB>(s, tri, t, mat, q, q2);
-    }
+    if (wtotal > 0)
+        for (int t = 1; t < T; t++)
+            fill_w_block<C, NB>(s, tri, t, kClass ? kcl[t] : NB, Wall + (kClass ? woff[t] : t * (1 + C) * MSNB));
     // Blocked and detached mass (one wave, lane q = box entry q): from level
     // t + 1 = u, beta_u = bq 2^eb and phi_(u+1) = ph 2^eb, to level t, both
     // relative to 2^(eb + X_u) (densities E = Ehat 2^X), in FP64:
