@@ -178,7 +178,7 @@ int wp_hmm_classify_device(wp_hmm_classifier* clf,
  * or graph is created. Launches go eagerly onto `stream` in order (no
  * parallel branches, no graph replay; the quad-rows kernel variants are the
  * defaults, or the autotuned ones once another entry point tuned them). A
- * call whose workspace exceeds 64 MiB (about 160,000 reads at C = 3, T = 12
+ * call whose workspace exceeds 64 MiB (about 85,000 reads at C = 3, T = 12
  * in FP32) returns WP_ERR_ARG before launching anything. Every call on one
  * GPU shares that GPU's array: calls that overlap on one GPU (another stream,
  * another classifier, another thread) race and must be serialised by the
@@ -262,7 +262,7 @@ int wp_hmm_set_refine(wp_hmm_classifier* clf, int enable, double abs, double rel
 
 /* After an FP32 call with the FP64 decision on (synchronises the device):
  * out2[0] = reads whose near-best list was rescored, out2[1] = those among
- * them rescored over every candidate (their window held more than 32). */
+ * them rescored over every candidate (their window held more than 128). */
 int wp_hmm_refine_stats(wp_hmm_classifier* clf, int num_reads, int* out2);
 
 /* Algorithmic work of classifying these reads against every dye-seq (no

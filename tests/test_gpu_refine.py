@@ -108,14 +108,15 @@ def test_fp32_ids_equal_oracle(gpu, oracle, name):
 
 
 def test_window_overflow_rescores_every_candidate(gpu, oracle):
-    """40 copies of one dye-seq tie exactly in FP32 and FP64: the window holds
-    more than 32 entries, the read is rescored over every candidate, and the
-    first copy wins as in the reference."""
+    """140 copies of one dye-seq tie exactly in FP32 and FP64: the window holds
+    more than 128 entries (the near-best list's capacity), the read is
+    rescored over every candidate, and the first copy wins as in the
+    reference."""
     w = gpu
     C, T, cut = 2, 8, 5.0
     m = w.SequencingModel.uniform(C)
     base = w.generate_dye_seqs(C, 30, 6, 16, 0, 3, 77)
-    strings = base[:10] + [base[10]] * 40 + base[11:]
+    strings = base[:10] + [base[10]] * 140 + base[11:]
     rads, idx = w.simulate_radiometries(m, T, [base[10]], 12, 78)
     S = len(strings)
     counts = np.ones(S, dtype=np.int32)

@@ -9,7 +9,7 @@
 namespace wp {
 
 // Near-best candidates per read handed from the FP32 pass to the FP64 pass.
-constexpr int kTieMax = 32;
+constexpr int kTieMax = 128;
 
 // Arguments of the fused classify kernel. Device pointers unless noted.
 struct ClassifyArgs {
