@@ -637,7 +637,7 @@ def bench_fit(args):
         "config": {"workload": "fit: 1 dye-seq L=20, 2 dyes x 3 channels, T=30, 1e6 reads, one EM iteration per step",
                    "reads": N, "timesteps": T, "channels": C, "dye_seq": ds, "precision": args.precision},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None, "kernel": "fit_estep_kernel",
+                     "frac": achieved / peak, "traffic": None, "kernel": "fit_rows_kernel",
                      "flops_per_launch": flops},
     }
     if prec == w.WP_PRECISION_FP32:

@@ -244,3 +244,26 @@ def test_batched_bootstrap_equals_sequential_rounds(gpu, precision):
     for ma, mb in zip(a["models"], b["models"]):
         assert _params(ma) == _params(mb)
     assert _params(a["ci_min"]) == _params(b["ci_min"]) and _params(a["ci_max"]) == _params(b["ci_max"])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("C,T,ds,n", CASES + [(3, 30, "..0..1..2..0..1..2..", 3000), (3, 16, ".0.1.2.0", 500)])
+def test_row_lane_estep_matches_workgroup_estep(gpu, monkeypatch, precision, C, T, ds, n):
+    """The row-lane E-step (one wavefront per read, count states in registers;
+    the kernel of every count shape it is compiled for) against the
+    workgroup-per-read E-step on the same reads and model: FP64 within 1e-12
+    relative, FP32 within 2e-5 relative of the largest statistic of its kind
+    (the two kernels sum the same FP32 products in different orders)."""
+    print(f"row-lane vs workgroup precision={precision} C={C} T={T} ds={ds} n={n}", flush=True)
+    model = w.SequencingModel.uniform(C, p_edman_failure=0.07, p_bleach=0.05, p_dud=0.08, p_detach=0.02)
+    rads, _ = w.simulate_radiometries(model, T, [ds], n, 300 + T)
+    got, got_lp = w.HMMFitter(T, C, ds, rads, precision).estep(model)
+    monkeypatch.setenv("WP_FIT_KERNEL", "generic")
+    want, want_lp = w.HMMFitter(T, C, ds, rads, precision).estep(model)
+    if precision == w.WP_PRECISION_FP64:
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-300)
+        assert got_lp == pytest.approx(want_lp, rel=1e-13)
+    else:
+        scale = np.max(np.abs(want))
+        assert np.all(np.abs(got - want) <= 2e-5 * np.abs(want) + 1e-7 * scale), (got, want)
+        assert got_lp == pytest.approx(want_lp, rel=1e-7)

@@ -7,6 +7,7 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -18,6 +19,7 @@
 
 struct wp_hmm_fitter {
     int T = 0, C = 0, N = 0, precision = 0, M = 0, NB = 1, tri = 1, nfib = 0, grid = 1, L = 0;
+    int rows = 0;  // row-lane E-step kernel
     int D[WP_MAX_CHANNELS] = {0};
     std::vector<short> seq;  // parsed dye-seq
     int len = 0;
@@ -139,6 +141,7 @@ int estep(wp_hmm_fitter* f, const wp_model* model, double* stats, double* sum_lo
     a.tri = f->tri;
     a.hist = f->d_hist;
     a.hist_stride = f->hist_stride;
+    a.rows = f->rows;
     a.partial = f->d_partial;
     a.L = f->L;
     a.weight = d_weight;
@@ -209,6 +212,7 @@ int estep_batch(wp_hmm_fitter* f, const std::vector<wp_model>& models, const std
     a.tri = f->tri;
     a.hist = f->d_hist;
     a.hist_stride = f->hist_stride;
+    a.rows = f->rows;
     a.partial = f->d_partial_b;
     a.L = f->L;
     a.weight = f->d_weight;
@@ -366,7 +370,12 @@ int wp_hmm_fitter_create(unsigned int num_timesteps, unsigned int num_channels, 
         }
         for (int e = 0; e < T; e++) hseq[e] = (signed char)(e < f->len ? f->seq[e] : -1);
     }
-    const size_t lds = wp::fit_lds_bytes(precision, T, C, f->NB, f->tri, f->nfib);
+    // the row-lane kernel where the count shape has one (WP_FIT_KERNEL=generic
+    // keeps the workgroup-per-read kernel, for comparisons)
+    const char* kern = std::getenv("WP_FIT_KERNEL");
+    f->rows = wp::fit_rows_supports(T, C, f->D) && !(kern && std::strcmp(kern, "generic") == 0);
+    const size_t lds = f->rows ? wp::fit_rows_lds_bytes(precision, T, C, f->NB, f->tri)
+                               : wp::fit_lds_bytes(precision, T, C, f->NB, f->tri, f->nfib);
     hipDeviceProp_t prop;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
@@ -380,10 +389,19 @@ int wp_hmm_fitter_create(unsigned int num_timesteps, unsigned int num_channels, 
         return WP_ERR_UNSUPPORTED;
     }
     // workgroups: as many as fit on the device at once, at most one per read
-    int per_cu = std::max(1, std::min(8, (int)(prop.maxSharedMemoryPerMultiProcessor / std::max<size_t>(lds, 1))));
-    f->grid = std::max(1, std::min(num_reads, per_cu * prop.multiProcessorCount));
     const size_t real = precision == WP_PRECISION_FP32 ? 4 : 8;
-    f->hist_stride = (size_t)T * 2 * T * f->NB;
+    if (f->rows) {
+        // two workgroups of four wavefronts per CU (two wavefronts per SIMD,
+        // the kernel's register bound), one read per wavefront at a time
+        const int waves = wp::fit_rows_threads() / 64;
+        f->grid = std::max(1, std::min((num_reads + waves - 1) / waves, 2 * prop.multiProcessorCount));
+        f->hist_stride = wp::fit_rows_hist_per_group(T, f->NB);
+    } else {
+        int per_cu =
+            std::max(1, std::min(8, (int)(prop.maxSharedMemoryPerMultiProcessor / std::max<size_t>(lds, 1))));
+        f->grid = std::max(1, std::min(num_reads, per_cu * prop.multiProcessorCount));
+        f->hist_stride = (size_t)T * 2 * T * f->NB;
+    }
     rc = put(&f->d_rads, radiometries, (size_t)num_reads * T * C);
     if (!rc) rc = put(&f->d_crd, crd.data(), crd.size());
     if (!rc) rc = put(&f->d_fbase, fbase.data(), fbase.size());

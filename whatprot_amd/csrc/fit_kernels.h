@@ -33,6 +33,7 @@ struct FitArgs {
     int nslots;                   // 1 for a plain E-step
     const DevModel* models;       // [nslots] device copies, or null: a.m
     const int* slot_round;        // [nslots] weight row of each slot, or null: row 0
+    int rows;                     // 1: row-lane kernel (fit_rows_supports), hist per wavefront
 };
 
 // Dynamic LDS of one workgroup ((size_t)-1 for an unsupported C).
@@ -43,5 +44,15 @@ int fit_threads();
 // d_out[L] the sum of ln P. grid workgroups, each owning hist and partial
 // slots.
 hipError_t launch_fit_estep(const FitArgs& a, int precision, int grid, double* d_out, hipStream_t stream);
+
+// Row-lane E-step (one wavefront per read, lane = tensor x Edman row, count
+// states in registers) for T <= 32 and the count shapes it is compiled for.
+bool fit_rows_supports(int T, int C, const int* D);
+int fit_rows_threads();
+size_t fit_rows_lds_bytes(int precision, int T, int C, int NB, int tri);
+// history elements of one workgroup (T x NB x 64 per wavefront)
+size_t fit_rows_hist_per_group(int T, int NB);
+// the E-step launch alone (per-workgroup totals into a.partial)
+hipError_t launch_fit_rows(const FitArgs& a, int precision, int grid, hipStream_t stream);
 
 }  // namespace wp

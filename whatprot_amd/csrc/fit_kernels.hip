@@ -615,7 +615,9 @@ int fit_threads() { return kThreads; }
 hipError_t launch_fit_estep(const FitArgs& a, int precision, int grid, double* d_out, hipStream_t stream) {
     hipError_t e = hipErrorInvalidValue;
     const int C = a.m.C;
-    if (precision == WP_PRECISION_FP32) {
+    if (a.rows) {
+        e = launch_fit_rows(a, precision, grid, stream);
+    } else if (precision == WP_PRECISION_FP32) {
         if (C == 1) e = launch_c<float, 1>(a, grid, stream);
         else if (C == 2) e = launch_c<float, 2>(a, grid, stream);
         else if (C == 3) e = launch_c<float, 3>(a, grid, stream);
