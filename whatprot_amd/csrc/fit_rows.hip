@@ -582,37 +582,32 @@ __device__ void rows_estep_one(const FitArgs& a, const DevModel& m, const double
                     xchg32<S, Real>(Y0, Y);  // the other tensor's alpha_{t-1}
                     Real ob[NB];
                     xchg32<S, Real>(ob, b);  // the other tensor's beta
+                    // one formula for both tensors (no divergent branch): A keeps
+                    // (1 - pcb) of its alpha and takes pcb of B's beta; B gains
+                    // pcb of A's alpha and keeps its beta
+                    const Real ca = tensor == 0 ? (1 - pcb) : (Real)1, co = tensor == 0 ? (Real)0 : pcb;
+                    const Real u1 = tensor == 0 ? pcb : (Real)0, u2 = tensor == 0 ? (1 - pcb) : (Real)1;
                     double fs = 0.0, fbs = 0.0, cn = 0.0, cdd = 0.0;
 #pragma unroll
                     for (int q = 0; q < NB; q++) {
-                        if (tensor == 0) {
-                            const Real x0 = Y[q];
-                            const Real a1A = (1 - pcb) * x0;
-                            const Real lA = (1 - pd) * b[q] + pd * d;
-                            const Real lB = (1 - pd) * ob[q] + pd * d;
-                            if (q != 0) {
-                                fs += (double)a1A;
-                                fbs += (double)(a1A * lA);
-                            }
-                            const Real cA2 = pcb * lB + (1 - pcb) * lA;
-                            cn += (double)(x0 * pcb * lB);
-                            cdd += (double)(x0 * cA2);
-                            b[q] = cA2;
-                        } else {
-                            const Real x0 = Y0[q];
-                            const Real a1B = Y[q] + pcb * x0;
-                            const Real lB = (1 - pd) * b[q] + pd * d;
-                            if (q != 0) {
-                                fs += (double)a1B;
-                                fbs += (double)(a1B * lB);
-                            }
-                            b[q] = lB;
+                        const Real x0 = Y[q];
+                        const Real a1 = ca * x0 + co * Y0[q];
+                        const Real l = (1 - pd) * b[q] + pd * d;
+                        const Real lo = (1 - pd) * ob[q] + pd * d;
+                        if (q != 0) {
+                            fs += (double)a1;
+                            fbs += (double)(a1 * l);
                         }
+                        const Real nb = u1 * lo + u2 * l;
+                        cn += (double)(x0 * pcb * lo);
+                        cdd += (double)(x0 * nb);
+                        b[q] = nb;
                     }
-                    if (live) {
+                    if (live && tensor == 0) {
                         A(kAccCyclic) += cn * F;
                         A(kAccCyclic + 1) += cdd * F;
-                    } else {
+                    }
+                    if (!live) {
                         fs = 0.0;
                         fbs = 0.0;
                     }
