@@ -247,7 +247,10 @@ def test_batched_bootstrap_equals_sequential_rounds(gpu, precision):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("C,T,ds,n", CASES + [(3, 30, "..0..1..2..0..1..2..", 3000), (3, 16, ".0.1.2.0", 500)])
+@pytest.mark.parametrize("C,T,ds,n", CASES + [
+    (3, 30, "..0..1..2..0..1..2..", 3000), (3, 16, ".0.1.2.0", 500),
+    (1, 10, "..0..", 200), (2, 9, "..0.1..", 200), (2, 12, ".0.1..1", 200),
+    (3, 8, "0.1.2", 200), (3, 14, ".0..1.1.2", 200), (3, 32, "..2..0..1..2..1..0.", 300)])
 def test_row_lane_estep_matches_workgroup_estep(gpu, monkeypatch, precision, C, T, ds, n):
     """The row-lane E-step (one wavefront per read, count states in registers;
     the kernel of every count shape it is compiled for) against the

@@ -696,30 +696,46 @@ hipError_t launch_s(const FitArgs& a, int grid, hipStream_t stream) {
     return hipGetLastError();
 }
 
-// Count shapes with a row-lane kernel: 1 <= D_c <= 2 per channel, C <= 3,
-// listed explicitly (each is its own code object).
+// Count shapes with a row-lane kernel: every shape of C <= 3 channels with
+// 1 or 2 dyes in each (each is its own code object).
+template <typename Real, int C, int D0, int D1 = 1, int D2 = 1>
+hipError_t launch_d(const FitArgs& a, int grid, hipStream_t stream) {
+    if constexpr (C == 1) {
+        return launch_s<Shape<1, D0, 0, 0>, Real>(a, grid, stream);
+    } else if constexpr (C == 2) {
+        return a.D[1] == 1 ? launch_s<Shape<2, D0, 1, 0>, Real>(a, grid, stream)
+                           : launch_s<Shape<2, D0, 2, 0>, Real>(a, grid, stream);
+    } else {
+        if (a.D[1] == 1)
+            return a.D[2] == 1 ? launch_s<Shape<3, D0, 1, 1>, Real>(a, grid, stream)
+                               : launch_s<Shape<3, D0, 1, 2>, Real>(a, grid, stream);
+        return a.D[2] == 1 ? launch_s<Shape<3, D0, 2, 1>, Real>(a, grid, stream)
+                           : launch_s<Shape<3, D0, 2, 2>, Real>(a, grid, stream);
+    }
+}
+
+template <typename Real, int C>
+hipError_t launch_c(const FitArgs& a, int grid, hipStream_t stream) {
+    return a.D[0] == 1 ? launch_d<Real, C, 1>(a, grid, stream) : launch_d<Real, C, 2>(a, grid, stream);
+}
+
 template <typename Real>
 hipError_t launch_r(const FitArgs& a, int grid, hipStream_t stream) {
-    const int C = a.m.C;
-    const int* D = a.D;
-    if (C == 1 && D[0] == 1) return launch_s<Shape<1, 1, 0, 0>, Real>(a, grid, stream);
-    if (C == 1 && D[0] == 2) return launch_s<Shape<1, 2, 0, 0>, Real>(a, grid, stream);
-    if (C == 2 && D[0] == 2 && D[1] == 2) return launch_s<Shape<2, 2, 2, 0>, Real>(a, grid, stream);
-    if (C == 3 && D[0] == 2 && D[1] == 2 && D[2] == 2) return launch_s<Shape<3, 2, 2, 2>, Real>(a, grid, stream);
-    if (C == 3 && D[0] == 2 && D[1] == 2 && D[2] == 1) return launch_s<Shape<3, 2, 2, 1>, Real>(a, grid, stream);
-    if (C == 3 && D[0] == 2 && D[1] == 1 && D[2] == 1) return launch_s<Shape<3, 2, 1, 1>, Real>(a, grid, stream);
-    return hipErrorInvalidValue;
+    switch (a.m.C) {
+    case 1: return launch_c<Real, 1>(a, grid, stream);
+    case 2: return launch_c<Real, 2>(a, grid, stream);
+    case 3: return launch_c<Real, 3>(a, grid, stream);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
 
 bool fit_rows_supports(int T, int C, const int* D) {
-    if (T < 1 || T > 32) return false;
-    if (C == 1) return D[0] == 1 || D[0] == 2;
-    if (C == 2) return D[0] == 2 && D[1] == 2;
-    if (C == 3)
-        return D[0] == 2 && ((D[1] == 2 && (D[2] == 2 || D[2] == 1)) || (D[1] == 1 && D[2] == 1));
-    return false;
+    if (T < 1 || T > 32 || C < 1 || C > 3) return false;
+    for (int c = 0; c < C; c++)
+        if (D[c] < 1 || D[c] > 2) return false;
+    return true;
 }
 
 int fit_rows_threads() { return kRowThreads; }
