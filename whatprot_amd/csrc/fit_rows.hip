@@ -26,7 +26,9 @@
 //
 // Scaling is fit_estep_kernel's: a power of two after every emission, the
 // forward exponents K_t and the backward exponent J, statistics of timestep
-// t scaled by 2^(K_{t-1} + J - log2 P) and accumulated in FP64.
+// t scaled by 2^(K_{t-1} + J - log2 P). A lane sums the terms of one step of
+// one timestep (at most NB of them) in the state type and adds that sum to
+// its FP64 totals, which run over timesteps and reads.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -145,15 +147,16 @@ __device__ __attribute__((always_inline)) inline void bin_fwd(Real (&x)[S::NB], 
 
 // Backward of the same step in place on b (binomial-transition.cc:138-183)
 // with its statistics against the forward state f (:185-256); returns the
-// lane's numerator and denominator sums.
+// lane's numerator and denominator sums (in the state type: one timestep's
+// terms of one lane, added to the FP64 totals by the caller).
 template <class S, int c, typename Real>
 __device__ __attribute__((always_inline)) inline void bin_bwd(Real (&b)[S::NB], const Real (&f)[S::NB],
-                                                              const Real* P, double& num, double& den) {
+                                                              const Real* P, Real& num, Real& den) {
     constexpr int len = S::d(c) + 1, s = S::st(c);
     Real p[len * (len + 1) / 2];
 #pragma unroll
     for (int i = 0; i < len * (len + 1) / 2; i++) p[i] = P[i];
-    double n = 0.0, d = 0.0;
+    Real n = 0, d = 0;
 #pragma unroll
     for (int q0 = 0; q0 < S::NB; q0++) {
         if (S::coord(q0, c) != 0) continue;
@@ -164,10 +167,10 @@ __device__ __attribute__((always_inline)) inline void bin_bwd(Real (&b)[S::NB], 
             for (int to = 0; to <= from; to++) v += p[from * (from + 1) / 2 + to] * b[q0 + to * s];
             if (from >= 1) {
                 const Real af = f[q0 + from * s];
-                d += (double)(af * v) * from;
+                d += (af * v) * (Real)from;
 #pragma unroll
                 for (int to = 0; to < from; to++)
-                    n += (double)(af * p[from * (from + 1) / 2 + to] * b[q0 + to * s]) * (from - to);
+                    n += (af * p[from * (from + 1) / 2 + to] * b[q0 + to * s]) * (Real)(from - to);
             }
             b[q0 + from * s] = v;
         }
@@ -197,11 +200,11 @@ __device__ __attribute__((always_inline)) inline void bleach_bwd_all(Real (&b)[S
 #pragma unroll
         for (int q = 0; q < S::NB; q++) Y[q] = Y0[q];
         bleach_upto<S, c, Real>(Y, Pb, tri);
-        double n, d;
+        Real n, d;
         bin_bwd<S, c, Real>(b, Y, Pb + (size_t)c * tri, n, d);
         if (live) {
-            acc[kAccStride * 4 * c] += n * F;
-            acc[kAccStride * (4 * c + 1)] += d * F;
+            acc[kAccStride * 4 * c] += (double)n * F;
+            acc[kAccStride * (4 * c + 1)] += (double)d * F;
         }
         bleach_bwd_all<S, c, Real>(b, Y0, Y, Pb, tri, live, F, acc);
     }
@@ -225,11 +228,11 @@ __device__ __attribute__((always_inline)) inline void dud_bwd_all(Real (&b)[S::N
 #pragma unroll
         for (int q = 0; q < S::NB; q++) Y[q] = Y0[q];
         dud_upto<S, c, Real>(Y, Pd, tri);
-        double n, d;
+        Real n, d;
         bin_bwd<S, c, Real>(b, Y, Pd + (size_t)c * tri, n, d);
         if (live) {
-            acc[kAccStride * 4 * c] += n * F;
-            acc[kAccStride * (4 * c + 1)] += d * F;
+            acc[kAccStride * 4 * c] += (double)n * F;
+            acc[kAccStride * (4 * c + 1)] += (double)d * F;
         }
         dud_bwd_all<S, c, Real>(b, Y0, Y, Pd, tri, live, F, acc);
     }
@@ -348,7 +351,7 @@ template <class S, typename Real, int q>
 __device__ __attribute__((always_inline)) inline void edman_bwd_q(Real (&b)[S::NB], const Real (&nxt)[S::NB],
                                                                   const Real (&f)[S::NB],
                                                                   int c, const EdmanCoef<S, Real>& ec, Real pf,
-                                                                  double& num, double& den) {
+                                                                  Real& num, Real& den) {
     Real v = pf * b[q];
     if (c < 0) {
         v += (1 - pf) * nxt[q];
@@ -383,8 +386,8 @@ __device__ __attribute__((always_inline)) inline void edman_bwd_q(Real (&b)[S::N
         v += cs * nxt[q];
     }
     if constexpr (q != 0) {
-        num += (double)(f[q] * pf * b[q]);
-        den += (double)(f[q] * v);
+        num += f[q] * pf * b[q];
+        den += f[q] * v;
     }
     b[q] = v;
 }
@@ -393,7 +396,7 @@ template <class S, typename Real, int q = 0>
 __device__ __attribute__((always_inline)) inline void edman_bwd_all(Real (&b)[S::NB], const Real (&nxt)[S::NB],
                                                                     const Real (&f)[S::NB],
                                                                     int c, const EdmanCoef<S, Real>& ec, Real pf,
-                                                                    double& num, double& den) {
+                                                                    Real& num, Real& den) {
     if constexpr (q < S::NB) {
         edman_bwd_q<S, Real, q>(b, nxt, f, c, ec, pf, num, den);
         edman_bwd_all<S, Real, q + 1>(b, nxt, f, c, ec, pf, num, den);
@@ -565,12 +568,12 @@ __device__ void rows_estep_one(const FitArgs& a, const DevModel& m, const double
                     const int c = e < T ? (int)L.seq[e] : -1;
                     EdmanCoef<S, Real> ec;
                     edman_coef<S, Real>(ec, pf, c >= 0 ? (int)L.track[e * C + c] : 1);
-                    double num = 0.0, den = 0.0;
+                    Real num = 0, den = 0;
                     if (tensor == 0) {
                         edman_bwd_all<S, Real>(b, nxt, Y, c, ec, pf, num, den);
                         if (live) {
-                            A(kAccEdman) += num * F;
-                            A(kAccEdman + 1) += den * F;
+                            A(kAccEdman) += (double)num * F;
+                            A(kAccEdman + 1) += (double)den * F;
                         }
                     }
                 }
@@ -587,7 +590,7 @@ __device__ void rows_estep_one(const FitArgs& a, const DevModel& m, const double
                     // pcb of A's alpha and keeps its beta
                     const Real ca = tensor == 0 ? (1 - pcb) : (Real)1, co = tensor == 0 ? (Real)0 : pcb;
                     const Real u1 = tensor == 0 ? pcb : (Real)0, u2 = tensor == 0 ? (1 - pcb) : (Real)1;
-                    double fs = 0.0, fbs = 0.0, cn = 0.0, cdd = 0.0;
+                    Real fs = 0, fbs = 0, cn = 0, cdd = 0;
 #pragma unroll
                     for (int q = 0; q < NB; q++) {
                         const Real x0 = Y[q];
@@ -595,27 +598,23 @@ __device__ void rows_estep_one(const FitArgs& a, const DevModel& m, const double
                         const Real l = (1 - pd) * b[q] + pd * d;
                         const Real lo = (1 - pd) * ob[q] + pd * d;
                         if (q != 0) {
-                            fs += (double)a1;
-                            fbs += (double)(a1 * l);
+                            fs += a1;
+                            fbs += a1 * l;
                         }
                         const Real nb = u1 * lo + u2 * l;
-                        cn += (double)(x0 * pcb * lo);
-                        cdd += (double)(x0 * nb);
+                        cn += x0 * pcb * lo;
+                        cdd += x0 * nb;
                         b[q] = nb;
                     }
                     if (live && tensor == 0) {
-                        A(kAccCyclic) += cn * F;
-                        A(kAccCyclic + 1) += cdd * F;
+                        A(kAccCyclic) += (double)cn * F;
+                        A(kAccCyclic + 1) += (double)cdd * F;
                     }
-                    if (!live) {
-                        fs = 0.0;
-                        fbs = 0.0;
-                    }
-                    fs = wave_sum(fs);
-                    fbs = wave_sum(fbs);
+                    const double wfs = wave_sum(live ? (double)fs : 0.0);
+                    const double wfbs = wave_sum(live ? (double)fbs : 0.0);
                     if (lane == t - 1) {
-                        A(acc_stats(C) - 2) += fs * (double)pd * (double)d * F;
-                        A(acc_stats(C) - 1) += fbs * F;
+                        A(acc_stats(C) - 2) += wfs * (double)pd * (double)d * F;
+                        A(acc_stats(C) - 1) += wfbs * F;
                     }
                 }
             } else {
