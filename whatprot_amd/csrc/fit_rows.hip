@@ -46,6 +46,8 @@ constexpr int kRowWaves = kRowThreads / 64;
 constexpr int kAccStride = 64;
 constexpr int kAccEdman = 0, kAccInitial = 2, kAccCyclic = 4, kAccChannel = 6;
 __host__ __device__ constexpr int acc_stats(int C) { return 8 + 4 * C; }
+// dye counts 0..2 of one channel: the Edman coefficients of a row
+constexpr int kCoefK = 3;
 
 // Count shape: C channels with D0..D2 dyes (C <= 3).
 template <int C_, int D0, int D1, int D2>
@@ -101,6 +103,7 @@ struct RowSmem {
     int* Ee;              // [waves][T][NB] exponent (FP32 only)
     double* red;          // [waves][64] final reductions
     double* acc;          // [waves][acc_stats(C)][64] per-lane statistics
+    Real* ecoef;          // [T][2][kCoefK] Edman coefficients of each row: stay[k], then up[k]
 };
 
 template <typename Real>
@@ -119,6 +122,7 @@ __host__ __device__ inline size_t row_smem(int T, int C, int NB, int tri, RowSme
     l.Ee = (int*)take(sizeof(Real) == 4 ? sizeof(int) * kRowWaves * T * NB : 0);
     l.red = (double*)take(sizeof(double) * kRowWaves * 64);
     l.acc = (double*)take(sizeof(double) * kRowWaves * acc_stats(C) * kAccStride);
+    l.ecoef = (Real*)take(sizeof(Real) * T * 2 * kCoefK);
     if (s) *s = l;
     return o;
 }
@@ -272,12 +276,22 @@ struct EdmanCoef {
     Real stay[S::dmax() + 1], up[S::dmax() + 1];
 };
 
+template <typename Real>
+__device__ inline Real coef_stay(Real pf, int k, int Kc) {
+    return k < Kc ? (1 - pf) * (1 - (Real)k / (Real)Kc) : (Real)0;
+}
+template <typename Real>
+__device__ inline Real coef_up(Real pf, int k, int Kc) {
+    return (1 - pf) * ((Real)(k + 1) / (Real)Kc);
+}
+
+// The coefficients of row e from the workgroup's table (built once per model).
 template <class S, typename Real>
-__device__ inline void edman_coef(EdmanCoef<S, Real>& ec, Real pf, int Kc) {
+__device__ inline void edman_coef(EdmanCoef<S, Real>& ec, const Real* table, int e) {
 #pragma unroll
     for (int k = 0; k <= S::dmax(); k++) {
-        ec.stay[k] = k < Kc ? (1 - pf) * (1 - (Real)k / (Real)Kc) : (Real)0;
-        ec.up[k] = (1 - pf) * ((Real)(k + 1) / (Real)Kc);
+        ec.stay[k] = table[(size_t)e * 2 * kCoefK + k];
+        ec.up[k] = table[(size_t)e * 2 * kCoefK + kCoefK + k];
     }
 }
 
@@ -425,6 +439,13 @@ __device__ void rows_estep_one(const FitArgs& a, const DevModel& m, const double
     for (int i = tid; i < T; i += kRowThreads) L.seq[i] = a.seq[i];
     for (int i = tid; i < T * C; i += kRowThreads) L.track[i] = a.track[i];
     __syncthreads();
+    for (int i = tid; i < T * kCoefK; i += kRowThreads) {
+        const int e = i / kCoefK, k = i - e * kCoefK, c = L.seq[e];
+        const int Kc = c >= 0 ? (int)L.track[e * C + c] : 1;
+        L.ecoef[(size_t)e * 2 * kCoefK + k] = coef_stay<Real>((Real)m.pf, k, Kc);
+        L.ecoef[(size_t)e * 2 * kCoefK + kCoefK + k] = coef_up<Real>((Real)m.pf, k, Kc);
+    }
+    __syncthreads();
     const Real* Pd = L.bin;
     const Real* Pb = L.bin + (size_t)C * tri;
     Real* Em = L.Em + (size_t)wave * T * NB;
@@ -505,7 +526,7 @@ __device__ void rows_estep_one(const FitArgs& a, const DevModel& m, const double
                 for (int q = 0; q < NB; q++) o[q] = __shfl_up(x[q], 1, 64);
                 const int c = e >= 1 && e - 1 < T ? (int)L.seq[e - 1] : -1;
                 EdmanCoef<S, Real> ec;
-                edman_coef<S, Real>(ec, pf, c >= 0 ? (int)L.track[(e - 1) * C + c] : 1);
+                edman_coef<S, Real>(ec, L.ecoef, c >= 0 ? e - 1 : 0);
                 if (tensor == 0) {
                     edman_fwd_all<S, Real>(x, o, c, ec, pf, e >= 1 && e <= t, e < t);
                 } else {
@@ -567,7 +588,7 @@ __device__ void rows_estep_one(const FitArgs& a, const DevModel& m, const double
                     for (int q = 0; q < NB; q++) nxt[q] = __shfl_down(b[q], 1, 64);
                     const int c = e < T ? (int)L.seq[e] : -1;
                     EdmanCoef<S, Real> ec;
-                    edman_coef<S, Real>(ec, pf, c >= 0 ? (int)L.track[e * C + c] : 1);
+                    edman_coef<S, Real>(ec, L.ecoef, c >= 0 ? e : 0);
                     Real num = 0, den = 0;
                     if (tensor == 0) {
                         edman_bwd_all<S, Real>(b, nxt, Y, c, ec, pf, num, den);
@@ -685,6 +706,7 @@ __global__ void __launch_bounds__(kRowThreads, 2) fit_rows_kernel(const FitArgs 
 
 template <class S, typename Real>
 hipError_t launch_s(const FitArgs& a, int grid, hipStream_t stream) {
+    static_assert(S::dmax() < kCoefK, "Edman coefficient table holds counts 0..kCoefK-1");
     const size_t lds = row_smem<Real>(a.m.T, S::C, S::NB, a.tri, nullptr, nullptr);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)fit_rows_kernel<S, Real>,
