@@ -1069,6 +1069,10 @@ __global__ void __launch_bounds__(64 * kWaveRows) topk_wave_kernel(int rows, int
 
 constexpr int kStatWords = 8;    // 16-byte words of group keys per lane: 2048 groups, 32768 logits
 constexpr int kStatCand = 1024;  // largest candidate list (LDS), 16 keys per lane
+// Entries of a row's list of groups at or above tau' (LDS). tau' leaves at
+// most H + 32 groups (cap >= 2H) unless group keys tie at it; a row with more
+// is searched over all its logits instead.
+__host__ __device__ inline int stat_group_list(int G, int cap) { return G < cap + 64 ? G : cap + 64; }
 
 // Radix selection over keys held in registers, NR per lane (absent keys 0,
 // never counted): n keys count, n >= H >= 1, all in [lo, hi]. Two bits a round,
@@ -1170,9 +1174,10 @@ __global__ void __launch_bounds__(64 * kWaveRows) topk_stats_kernel(int rows, in
     const int r = blockIdx.x * kWaveRows + wv;
     if (r >= rows) return;  // whole wavefront
     const int G = (N + kWaveGroup - 1) / kWaveGroup, G4 = (G + 3) / 4;
-    const size_t per_wave = (size_t)G * 4 + (size_t)cap * 8;
+    const int GL = stat_group_list(G, cap);
+    const size_t per_wave = (size_t)GL * 4 + (size_t)cap * 8;
     int* glist = (int*)(smem + wv * per_wave);
-    unsigned int* ckey = (unsigned int*)(glist + G);
+    unsigned int* ckey = (unsigned int*)(glist + GL);
     int* cidx = (int*)(ckey + cap);
     const float* z = Z + (long)r * ldz;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -1244,10 +1249,16 @@ __global__ void __launch_bounds__(64 * kWaveRows) topk_stats_kernel(int rows, in
         int pos = NG + __popcll(b0 & below) + 2 * __popcll(b1 & below) + 4 * __popcll(b2 & below);
 #pragma unroll
         for (int c = 0; c < 4; c++)
-            if (f[c]) glist[pos++] = i0 + c;
+            if (f[c]) {
+                if (pos < GL) glist[pos] = i0 + c;
+                pos++;
+            }
         NG += __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
     }
     wsync();
+    // more groups than the list holds (ties at tau'): the search over the whole row
+    const bool glisted = NG <= GL;
+    if (!glisted) NG = 0;
 
     // the logits >= tau' of those groups, four groups per load (16 lanes each)
     int M = 0;
@@ -1277,7 +1288,7 @@ __global__ void __launch_bounds__(64 * kWaveRows) topk_stats_kernel(int rows, in
         }
     }
     wsync();
-    const bool listed = M <= cap;
+    const bool listed = glisted && M <= cap;
     unsigned int K = 0u;
     if (listed) {
         // the exact selection among the candidates (all in [tau', khi]), held in registers
@@ -1326,7 +1337,7 @@ hipError_t topk_softmax_stats(int rows, int N, int H, const float* Z, long ldz, 
     // candidate lists of 2H (at least 512) logits
     const int cap = std::min(kStatCand, std::max(512, 2 * H));
     const int G = (N + kWaveGroup - 1) / kWaveGroup;
-    const size_t lds = (size_t)kWaveRows * ((size_t)G * 4 + (size_t)cap * 8);
+    const size_t lds = (size_t)kWaveRows * ((size_t)stat_group_list(G, cap) * 4 + (size_t)cap * 8);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)topk_stats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)lds);
