@@ -391,10 +391,11 @@ int wp_hmm_fitter_create(unsigned int num_timesteps, unsigned int num_channels, 
     // workgroups: as many as fit on the device at once, at most one per read
     const size_t real = precision == WP_PRECISION_FP32 ? 4 : 8;
     if (f->rows) {
-        // two workgroups of four wavefronts per CU (two wavefronts per SIMD,
-        // the kernel's register bound), one read per wavefront at a time
+        // as many workgroups of four wavefronts per CU as its register bound
+        // keeps resident, one read per wavefront at a time
         const int waves = wp::fit_rows_threads() / 64;
-        f->grid = std::max(1, std::min((num_reads + waves - 1) / waves, 2 * prop.multiProcessorCount));
+        f->grid = std::max(1, std::min((num_reads + waves - 1) / waves,
+                                       wp::fit_rows_groups_per_cu(precision) * prop.multiProcessorCount));
         f->hist_stride = wp::fit_rows_hist_per_group(T, f->NB);
     } else {
         int per_cu =

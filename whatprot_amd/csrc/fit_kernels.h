@@ -52,6 +52,8 @@ int fit_rows_threads();
 size_t fit_rows_lds_bytes(int precision, int T, int C, int NB, int tri);
 // history elements of one workgroup (T x NB x 64 per wavefront)
 size_t fit_rows_hist_per_group(int T, int NB);
+// workgroups of the row-lane kernel resident per CU (its register bound)
+int fit_rows_groups_per_cu(int precision);
 // the E-step launch alone (per-workgroup totals into a.partial)
 hipError_t launch_fit_rows(const FitArgs& a, int precision, int grid, hipStream_t stream);
 

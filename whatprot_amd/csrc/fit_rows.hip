@@ -692,8 +692,13 @@ __device__ void rows_estep_one(const FitArgs& a, const DevModel& m, const double
 
 // The E-steps of a.nslots models in one launch, each slot exactly as a
 // launch of that slot alone (fit_estep_kernel's slot contract).
+// FP32: two wavefronts per SIMD (256 VGPRs, a few spilled); FP64: one,
+// with the state arrays in 512 registers instead of scratch
+template <typename Real>
+constexpr int rows_waves_per_simd() { return sizeof(Real) == 4 ? 2 : 1; }
+
 template <class S, typename Real>
-__global__ void __launch_bounds__(kRowThreads, 2) fit_rows_kernel(const FitArgs a) {
+__global__ void __launch_bounds__(kRowThreads, rows_waves_per_simd<Real>()) fit_rows_kernel(const FitArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     for (int k = 0; k < a.nslots; k++) {
         const int row = a.slot_round ? a.slot_round[k] : 0;
@@ -767,6 +772,10 @@ size_t fit_rows_lds_bytes(int precision, int T, int C, int NB, int tri) {
 }
 
 size_t fit_rows_hist_per_group(int T, int NB) { return (size_t)kRowWaves * T * NB * 64; }
+
+int fit_rows_groups_per_cu(int precision) {
+    return precision == WP_PRECISION_FP32 ? rows_waves_per_simd<float>() : rows_waves_per_simd<double>();
+}
 
 hipError_t launch_fit_rows(const FitArgs& a, int precision, int grid, hipStream_t stream) {
     return precision == WP_PRECISION_FP32 ? launch_r<float>(a, grid, stream) : launch_r<double>(a, grid, stream);
