@@ -54,7 +54,9 @@
 #include "classify_common.h"
 #include "hmm_kernels.h"
 
-// Register budget of the gather kernel: 3 waves per SIMD (no spills) unless overridden.
+// Register budget of the gather kernel: 3 waves per SIMD unless overridden.
+// The strided C = 3 forms spill 78-93 dwords per lane at this bound; at 2
+// waves per SIMD (FP64, 21 dwords) the near-tie decision was no faster.
 #ifndef WP_WAVES_PER_EU
 #define WP_WAVES_PER_EU 3
 #endif
