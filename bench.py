@@ -113,6 +113,8 @@ def main():
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--no-north-star", action="store_true",
                     help="leave out the learned-shortlist leg (10^5 reads x 20,000 dye-seqs) of the default line")
+    ap.add_argument("--no-fit-leg", action="store_true",
+                    help="leave out the Baum-Welch E-step leg (10^6 reads, T = 30) of the default line")
     ap.add_argument("--shortlist-reads", type=int, default=100000)
     ap.add_argument("--labels", choices=["hmm", "truth"], default="hmm",
                     help="shortlist training labels: the HMM's pick among same-dye-count dye-seqs, or the source")
@@ -277,6 +279,16 @@ def main():
     if not args.no_north_star:
         log("north star: learned shortlist at 10^5 reads x 20,000 dye-seqs")
         result["north_star"] = shortlist_leg(args, steps=3)
+    if not args.no_fit_leg:
+        log("fit: Baum-Welch E-step of 10^6 reads at T = 30")
+        fa = argparse.Namespace(**vars(args))
+        fa.reads, fa.precision, fa.steps, fa.warmup, fa.no_cpu_baseline = 100000, "fp32", 2, 1, True
+        r = fit_result(fa)
+        result["fit"] = {"workload": r["config"]["workload"], "reads_per_s": r["value"],
+                         "ms_per_em_iteration": r["ms_per_step"], "roofline": r["roofline"],
+                         "fp32_vs_fp64_max_rel_stat": r.get("fp32_vs_fp64_max_rel_stat"),
+                         "timing": "1 untimed EM iteration, then 2 (GPU E-step over every read, device "
+                                   "reduction, host M-step) between synchronizations"}
     print(json.dumps(result), flush=True)
 
 
@@ -589,6 +601,10 @@ def fit_flops_per_read(T, D):
 
 
 def bench_fit(args):
+    print(json.dumps(fit_result(args)), flush=True)
+
+
+def fit_result(args):
     import numpy as np
     import torch
 
@@ -658,7 +674,7 @@ def bench_fit(args):
             result["cpu_baseline"] = {"value": n / d1, "unit": "reads/s", "cores": 1, "kind": "reference",
                                       "sample": f"{n} reads of the same workload, one serial E-step "
                                                 f"(HMMFitter::fit's loop), {d1:.1f} s wall"}
-    print(json.dumps(result), flush=True)
+    return result
 
 
 if __name__ == "__main__":
